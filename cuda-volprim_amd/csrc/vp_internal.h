@@ -1,0 +1,165 @@
+// vp_internal.h — device-side data layout and host-side object definitions
+// shared by the build kernels (vp_bvh.hip), the render kernels
+// (vp_render.hip) and the C ABI (vp_api.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/vp_api.h"
+#include "vp_core.h"
+
+namespace vp {
+
+// ---------------------------------------------------------------------------
+// BVH: 2N-1 nodes in depth-first preorder, 32 bytes each (two float4):
+//   a = (min.x, min.y, min.z, skip)   skip = index of the next node when the
+//                                     subtree is skipped (2N-1 ends the walk)
+//   b = (max.x, max.y, max.z, prim)   prim = stored primitive index of a leaf,
+//                                     0xffffffff for an inner node
+// A hit inner node continues at i + 1 (its left child).  Leaves appear in
+// stored-primitive order, so a walk meets primitives in ascending index.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kInnerNode = 0xffffffffu;
+
+struct SceneView {
+    const PrimRecord* prims;
+    const float4* nodes;
+    uint32_t n_prims;
+    uint32_t n_nodes;
+};
+
+struct EnvDev {
+    const float* tex;
+    const AliasEntry* alias;
+    const float* joint;
+    uint32_t w, h;
+    float total;
+    uint32_t has_table;
+};
+
+// One job slice of the persistent renderer: samples [s_begin, s_begin+s_count)
+// of one image.  Path q of the slice (0 <= q < path_count) is sample
+// s_begin + q / npad of the pixel at tile position q % npad (8x8 tiles).
+struct JobDesc {
+    SceneView scene;
+    EnvDev env;
+    CameraPod cam;
+    PhaseParams hg;
+    uint32_t width, height, spp, seed;
+    uint32_t max_depth, rr_depth;
+    float rr_max_survival, firefly_clamp, filter_stddev;
+    uint32_t use_ris, ris_candidates;
+    uint32_t tiles_x, npad;
+    uint32_t s_begin, s_count;
+    uint64_t path_begin, path_count;
+    float4* samples;  // s_count * width * height radiance values of this slice
+};
+
+// Counter slots of the render kernel.
+enum CounterSlot : int {
+    kCntActiveOverflow = 0,
+    kCntMaxHits = 1,
+    kCntMaxOverlap = 2,
+    kCntPaths = 3,
+    kCntTrace = 4,         // collect traversals (one per bounce)
+    kCntScatter = 5,
+    kCntShadowRays = 6,
+    kCntArgHit = 7,        // entered primitives evaluated by the argmin
+    kCntArgActive = 8,     // origin-set primitives evaluated by the argmin
+    kCntArgAccept = 9,     // candidates that beat the running minimum
+    kCntRbActive = 10,     // rebuild: origin-set members tested
+    kCntRbHit = 11,        // rebuild: point-query candidates tested as entries
+    kCntAlb = 12,          // albedo terms
+    kCntShActive = 13,     // shadow: origin-set chords integrated
+    kCntShHit = 14,        // shadow: entered chords integrated
+    kCntLeafTests = 15,    // entry tests at BVH leaves (all traversals)
+    kCntNodeVisits = 16,   // BVH nodes loaded (all traversals)
+    kCntContain = 17,      // containment tests (point queries)
+    kNumCounters = 18,
+};
+
+}  // namespace vp
+
+// ---------------------------------------------------------------------------
+// Host objects behind the opaque C handles.
+// ---------------------------------------------------------------------------
+struct vp_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t prop{};
+    // persistent render resources, grown on demand
+    uint32_t* slab = nullptr;
+    size_t slab_words = 0;
+    unsigned long long* counters = nullptr;  // vp::kNumCounters
+    unsigned long long* path_counter = nullptr;
+    vp::JobDesc* jobs_dev = nullptr;
+    uint32_t jobs_cap = 0;
+    uint64_t* job_begin_dev = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::mutex mu;
+    vp_launch_config tuned{};
+    bool have_tuned = false;
+};
+
+struct vp_scene {
+    vp_context* ctx = nullptr;
+    uint32_t n = 0;
+    uint32_t depth = 0;
+    vp::PrimRecord* prims = nullptr;  // stored (Morton) order
+    float4* nodes = nullptr;           // 2 * (2n - 1)
+    uint32_t* order = nullptr;         // order[k] = input index
+    uint32_t* morton = nullptr;        // codes in input order
+    vp::SceneView view() const { return vp::SceneView{prims, nodes, n, n ? 2 * n - 1 : 0}; }
+};
+
+struct vp_env {
+    vp_context* ctx = nullptr;
+    uint32_t w = 0, h = 0;
+    float total = 0.0f;
+    float* tex = nullptr;
+    float* joint = nullptr;
+    vp::AliasEntry* alias = nullptr;
+    std::vector<float> joint_host;
+    std::vector<vp::AliasEntry> alias_host;
+    vp::EnvDev view() const {
+        return vp::EnvDev{tex, alias, joint, w, h, total, (alias != nullptr && total > 0.0f) ? 1u : 0u};
+    }
+};
+
+// error plumbing (vp_api.cpp)
+void vp_set_error(const std::string& msg);
+#define VP_HIP_TRY(expr)                                                                   \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            vp_set_error(std::string(#expr) + ": " + hipGetErrorString(e_));               \
+            return VP_ERR_HIP;                                                             \
+        }                                                                                  \
+    } while (0)
+
+// launchers (vp_bvh.hip)
+vp_status vp_build_scene_device(vp_context* ctx, const vp_primitive* host_prims, uint32_t n, vp_scene* scene);
+
+// launchers (vp_render.hip)
+struct RenderLaunch {
+    uint32_t block;
+    uint32_t grid;
+    uint32_t active_cap;
+    int count_events;
+    int measure;
+};
+hipError_t vp_launch_render(vp_context* ctx, const vp::JobDesc* jobs_dev, const uint64_t* job_begin_dev,
+                            uint32_t njobs, uint64_t total_paths, const RenderLaunch& L, hipStream_t s);
+hipError_t vp_launch_welford(const vp::JobDesc& job, float4* mean, uint32_t sample_base, hipStream_t s);
+hipError_t vp_launch_normalize(float4* out, const float4* in, size_t n, float scale, hipStream_t s);
+hipError_t vp_launch_transmittance(vp::SceneView sc, const float* rays, uint32_t m, float* T, hipStream_t s);
+hipError_t vp_launch_overlaps(vp::SceneView sc, const float* rays, uint32_t m, uint32_t cap, uint32_t* counts,
+                              float* t, uint32_t* idx, hipStream_t s);
+hipError_t vp_launch_math(int fn, const float* x, const float* y, uint32_t n, float* out, hipStream_t s);
+hipError_t vp_launch_prim(int fn, const vp::PrimRecord* rec, const float* args, uint32_t m, float* out,
+                          hipStream_t s);
+int vp_render_occupancy(uint32_t block, uint32_t active_cap, int count_events, int* blocks_per_cu);

@@ -1,0 +1,720 @@
+// vp_render.hip — the render path on gfx950.
+//
+// One persistent megakernel traces camera paths; each 64-lane wavefront
+// claims new paths with one atomic per wave (ballot + popcount rank) as soon
+// as any of its lanes finishes, so lanes stay busy across uneven path
+// lengths.  Per bounce a lane
+//   1. takes the argmin of closed-form per-primitive free flights over the
+//      primitives containing its origin and, in one stackless walk of the
+//      BVH, over every primitive the ray enters (no hit buffer: each
+//      primitive's uniform is a counter hash, so the minimum can be taken in
+//      any order and the walk prunes boxes beyond the running minimum);
+//   2. rebuilds the active set at the scatter point with one BVH point query,
+//      applying the reference's per-primitive exit tests;
+//   3. evaluates albedo, samples the HG phase, and runs NEE (balance-MIS of
+//      phase and environment sampling, or product RIS) with shadow rays whose
+//      optical depth is accumulated in fixed point during a single walk;
+//   4. applies Russian roulette and the throughput guards.
+// Each path writes its radiance to a per-sample slot; a second kernel folds
+// slots into the per-pixel Welford mean in sample order, so the image does
+// not depend on which lane ran which path or on what else shared the launch.
+//
+// Reference (kfernandez31/cuda-volprim): device/entry/raygen.cuh:21-243,
+// device/entry/path_stages.cuh:25-226, device/core/sampling.cuh:257-583,
+// device/entry/anyhit.cuh:53-86, device/kernels/normalize_accumulator.cu:16-35.
+#include "vp_internal.h"
+
+namespace vp {
+namespace {
+
+constexpr float kTauInvScale = 1.0f / 1099511627776.0f;
+
+__device__ __forceinline__ f3 safe_inv(f3 d) {
+    const float x = fabsf_(d.x) < 1e-20f ? copysignf_(1e-20f, d.x) : d.x;
+    const float y = fabsf_(d.y) < 1e-20f ? copysignf_(1e-20f, d.y) : d.y;
+    const float z = fabsf_(d.z) < 1e-20f ? copysignf_(1e-20f, d.z) : d.z;
+    return f3{1.0f / x, 1.0f / y, 1.0f / z};
+}
+
+__device__ __forceinline__ PrimRecord load_prim(const PrimRecord* prims, uint32_t i) {
+    const float4* p = reinterpret_cast<const float4*>(prims + i);
+    const float4 a = p[0], b = p[1], c = p[2], d = p[3];
+    PrimRecord r;
+    r.cx = a.x; r.cy = a.y; r.cz = a.z; r.qw = a.w;
+    r.qx = b.x; r.qy = b.y; r.qz = b.z; r.rsx = b.w;
+    r.rsy = c.x; r.rsz = c.y; r.dnf = c.z; r.icf = c.w;
+    r.ar = d.x; r.ag = d.y; r.ab = d.z; r.mass = d.w;
+    return r;
+}
+
+// Stackless walk of the rope layout over [0, tmax]; leaf(prim) returns false
+// to stop.  tmax may be lowered by the caller between leaves.
+template <typename Leaf>
+__device__ __forceinline__ void walk_ray(const SceneView& s, f3 o, f3 inv, const float& tmax, uint32_t& visits,
+                                         Leaf&& leaf) {
+    uint32_t i = 0;
+    const uint32_t nn = s.n_nodes;
+    while (i < nn) {
+        const float4 a = s.nodes[2 * i];
+        const float4 b = s.nodes[2 * i + 1];
+        ++visits;
+        const float tx0 = (a.x - o.x) * inv.x, tx1 = (b.x - o.x) * inv.x;
+        const float ty0 = (a.y - o.y) * inv.y, ty1 = (b.y - o.y) * inv.y;
+        const float tz0 = (a.z - o.z) * inv.z, tz1 = (b.z - o.z) * inv.z;
+        const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+        const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+        const uint32_t skip = __float_as_uint(a.w);
+        if (tnear <= tfar) {
+            const uint32_t prim = __float_as_uint(b.w);
+            if (prim != kInnerNode) {
+                if (!leaf(prim)) return;
+                i = skip;
+            } else {
+                i = i + 1;
+            }
+        } else {
+            i = skip;
+        }
+    }
+}
+
+// Stackless walk visiting every leaf whose box contains x, in ascending
+// stored-primitive order.
+template <typename Leaf>
+__device__ __forceinline__ void walk_point(const SceneView& s, f3 x, uint32_t& visits, Leaf&& leaf) {
+    uint32_t i = 0;
+    const uint32_t nn = s.n_nodes;
+    while (i < nn) {
+        const float4 a = s.nodes[2 * i];
+        const float4 b = s.nodes[2 * i + 1];
+        ++visits;
+        const bool in = x.x >= a.x && x.x <= b.x && x.y >= a.y && x.y <= b.y && x.z >= a.z && x.z <= b.z;
+        const uint32_t skip = __float_as_uint(a.w);
+        if (in) {
+            const uint32_t prim = __float_as_uint(b.w);
+            if (prim != kInnerNode) {
+                leaf(prim);
+                i = skip;
+            } else {
+                i = i + 1;
+            }
+        } else {
+            i = skip;
+        }
+    }
+}
+
+// Add one chord to the fixed-point optical depth; true once the total
+// reaches 88 (exp(-88) underflows), which is order independent because every
+// term is non-negative.
+__device__ __forceinline__ bool tau_add(uint64_t& acc, float od) {
+    const float c = od > 0.0f ? od : 0.0f;
+    if (c >= kMaxTau) return true;
+    acc += static_cast<uint64_t>(c * kTauScaleF);
+    return acc >= kTauLimitFixed;
+}
+
+__device__ __forceinline__ float tau_to_T(uint64_t acc) {
+    const float tau = static_cast<float>(acc) * kTauInvScale;
+    return tau >= kMaxTau ? 0.0f : expf_(-tau);
+}
+
+struct EvCount {
+    uint32_t v[kNumCounters];
+};
+
+template <bool kCount>
+struct Ev {
+    EvCount c;
+    __device__ __forceinline__ void reset() {
+        if constexpr (kCount)
+            for (int i = 0; i < kNumCounters; ++i) c.v[i] = 0;
+    }
+    __device__ __forceinline__ void add(int slot, uint32_t n = 1) {
+        if constexpr (kCount) c.v[slot] += n;
+    }
+    __device__ __forceinline__ void flush(unsigned long long* g) {
+        if constexpr (kCount) {
+            for (int i = kCntPaths; i < kNumCounters; ++i)
+                if (c.v[i]) atomicAdd(&g[i], static_cast<unsigned long long>(c.v[i]));
+            reset();
+        }
+    }
+};
+
+// compute_transmittance_to_env (sampling.cuh:548-583) with the fused
+// any-hit integration (anyhit.cuh:53-66).
+template <bool kCount>
+__device__ float transmittance(const SceneView& s, f3 o, f3 d, const uint32_t* act, uint32_t na, uint32_t stride,
+                               Ev<kCount>& ev) {
+    uint64_t acc = 0;
+    for (uint32_t k = 0; k < na; ++k) {
+        const PrimRecord pr = load_prim(s.prims, act[k * stride]);
+        const float te = exit_from_inside(pr, o, d);
+        ev.add(kCntShActive);
+        if (te > 0.0f && te < kTMax)
+            if (tau_add(acc, optical_depth(pr, o, d, 0.0f, te))) return 0.0f;
+    }
+    const f3 inv = safe_inv(d);
+    const float tmax = kTMax;
+    bool full = false;
+    uint32_t visits = 0, leaves = 0, sh = 0;
+    walk_ray(s, o, inv, tmax, visits, [&](uint32_t prim) -> bool {
+        const PrimRecord pr = load_prim(s.prims, prim);
+        ++leaves;
+        float wl2;
+        const float tj = entry_t(pr, o, d, &wl2);
+        if (tj < 0.0f) return true;
+        const float te = exit_from_entry(pr, o, d, tj, wl2);
+        ++sh;
+        if (te > tj && te < kTMax) {
+            if (tau_add(acc, optical_depth(pr, o, d, tj, te))) {
+                full = true;
+                return false;
+            }
+        }
+        return true;
+    });
+    ev.add(kCntNodeVisits, visits);
+    ev.add(kCntLeafTests, leaves);
+    ev.add(kCntShHit, sh);
+    if (full) return 0.0f;
+    return tau_to_T(acc);
+}
+
+__device__ __forceinline__ EnvView env_view(const EnvDev& e) {
+    EnvView v;
+    v.tex = e.tex;
+    v.alias = e.has_table ? e.alias : nullptr;
+    v.joint = e.joint;
+    v.w = e.w;
+    v.h = e.h;
+    v.total = e.total;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t find_job(const uint64_t* begin, uint32_t njobs, uint64_t p) {
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (begin[mid] <= p) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+struct PathState {
+    f3 o, d, beta, L;
+    Pcg32 rng;
+    uint64_t key;
+    uint32_t bounce, job, na, cur, slot;
+};
+
+template <bool kCount>
+__global__ void __launch_bounds__(256) k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begin,
+                                                uint32_t njobs, uint64_t total_paths,
+                                                unsigned long long* __restrict__ path_counter,
+                                                uint32_t* __restrict__ slab, uint32_t cap, uint32_t stride,
+                                                unsigned long long* __restrict__ counters, int measure) {
+    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t* my = slab + gl;  // entry k of buffer b at my[(b * cap + k) * stride]
+    PathState st;
+    bool alive = false;
+    bool exhausted = false;
+    Ev<kCount> ev;
+    ev.reset();
+    uint32_t overflow = 0;
+
+    while (true) {
+        // ---- claim new paths for idle lanes: one atomic per wavefront ----
+        const bool want = !alive && !exhausted;
+        const unsigned long long wmask = __ballot(want);
+        if (wmask) {
+            const uint32_t leader = __ffsll(static_cast<long long>(wmask)) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(path_counter, static_cast<unsigned long long>(__popcll(wmask)));
+            base = __shfl(base, leader);
+            if (want) {
+                const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+                const uint64_t p = base + __popcll(wmask & lt);
+                if (p >= total_paths) {
+                    exhausted = true;
+                } else {
+                    // ---- path setup (raygen.cuh:72-113) ----
+                    const uint32_t j = find_job(job_begin, njobs, p);
+                    const JobDesc& J = jobs[j];
+                    const uint64_t q = p - J.path_begin;
+                    const uint32_t sl = static_cast<uint32_t>(q / J.npad);
+                    const uint32_t r = static_cast<uint32_t>(q - static_cast<uint64_t>(sl) * J.npad);
+                    const uint32_t tile = r >> 6, w = r & 63u;
+                    const uint32_t px = (tile % J.tiles_x) * 8u + (w & 7u);
+                    const uint32_t py = (tile / J.tiles_x) * 8u + (w >> 3);
+                    if (px < J.width && py < J.height) {
+                        const uint32_t sample = J.s_begin + sl;
+                        const uint64_t pix = static_cast<uint64_t>(py) * J.width + px;
+                        const uint64_t seq = hash64(pix * J.spp + sample);
+                        st.rng = pcg_init(J.seed, seq);
+                        st.key = path_key(J.seed, seq);
+                        float jx, jy;
+                        if (J.filter_stddev > 0.0f) {
+                            gaussian_jitter(st.rng, J.filter_stddev, &jx, &jy);
+                        } else {
+                            jx = next_uniform(st.rng) - 0.5f;
+                            jy = next_uniform(st.rng) - 0.5f;
+                        }
+                        camera_ray(J.cam, px, py, jx, jy, &st.o, &st.d);
+                        st.beta = mk(1.0f);
+                        st.L = mk(0.0f);
+                        st.bounce = 0;
+                        st.job = j;
+                        st.cur = 0;
+                        st.slot = static_cast<uint32_t>(static_cast<uint64_t>(sl) * J.width * J.height + pix);
+                        // origin-inside set (sampling.cuh:322-341) by point query
+                        uint32_t na = 0, ov = 0, visits = 0, tests = 0;
+                        const SceneView sc = J.scene;
+                        const f3 o = st.o;
+                        walk_point(sc, o, visits, [&](uint32_t prim) {
+                            ++tests;
+                            if (point_inside(load_prim(sc.prims, prim), o)) {
+                                ++ov;
+                                if (na < cap) my[na * stride] = prim, ++na;
+                                else ++overflow;
+                            }
+                        });
+                        st.na = na;
+                        ev.add(kCntNodeVisits, visits);
+                        ev.add(kCntContain, tests);
+                        ev.add(kCntPaths);
+                        if (measure) atomicMax(&counters[kCntMaxOverlap], static_cast<unsigned long long>(ov));
+                        alive = true;
+                    }
+                }
+            }
+        }
+        if (__ballot(alive) == 0) {
+            if (__ballot(!exhausted) == 0) break;
+            continue;
+        }
+        if (!alive) continue;
+
+        // ---- one bounce (raygen.cuh:115-208) ----
+        const JobDesc& J = jobs[st.job];
+        const SceneView sc = J.scene;
+        const f3 o = st.o, d = st.d;
+        const uint32_t b = st.bounce;
+        uint32_t* act = my + (st.cur * cap) * stride;
+        uint32_t* nxt = my + ((st.cur ^ 1u) * cap) * stride;
+        const uint32_t na = st.na;
+        bool finish = false;
+
+        // argmin over the origin set (sampling.cuh:356-370)
+        float ts = kTMax;
+        for (uint32_t k = 0; k < na; ++k) {
+            const uint32_t i = act[k * stride];
+            const PrimRecord pr = load_prim(sc.prims, i);
+            const float te = exit_from_inside(pr, o, d);
+            const float tau = free_flight_tau(prim_uniform(st.key, b, 0u, i));
+            const float t = inv_cdf(pr, o, d, tau);
+            if (t >= 0.0f && t < ts && t <= te) ts = t;
+        }
+        ev.add(kCntArgActive, na);
+        ev.add(kCntTrace);
+        // argmin over entered primitives (sampling.cuh:372-412) in one walk
+        {
+            const f3 inv = safe_inv(d);
+            float tmax = measure ? kTMax : ts;
+            uint32_t hits = 0, visits = 0, leaves = 0, accept = 0;
+            walk_ray(sc, o, inv, tmax, visits, [&](uint32_t prim) -> bool {
+                const PrimRecord pr = load_prim(sc.prims, prim);
+                ++leaves;
+                float wl2;
+                const float tj = entry_t(pr, o, d, &wl2);
+                if (tj < 0.0f) return true;
+                ++hits;
+                const float tau = free_flight_tau(prim_uniform(st.key, b, 1u, prim));
+                float t = inv_cdf_span(pr, o, d, tj, tau);
+                if (t >= 0.0f && t < tj) t = tj;
+                if (t >= 0.0f && t < ts) {
+                    ++accept;
+                    const float te = exit_from_entry(pr, o, d, tj, wl2);
+                    if (t <= te) {
+                        ts = t;
+                        if (!measure) tmax = ts;
+                    }
+                }
+                return true;
+            });
+            ev.add(kCntArgHit, hits);
+            ev.add(kCntArgAccept, accept);
+            ev.add(kCntNodeVisits, visits);
+            ev.add(kCntLeafTests, leaves);
+            if (measure) atomicMax(&counters[kCntMaxHits], static_cast<unsigned long long>(hits));
+        }
+
+        const EnvView env = env_view(J.env);
+        // analytic direct term at bounce 0 (raygen.cuh:148-160)
+        if (b == 0) {
+            ev.add(kCntShadowRays);
+            const float T = transmittance(sc, o, d, act, na, stride, ev);
+            st.L = st.L + (st.beta * T) * env_lookup(env, d);
+        }
+
+        if (ts >= kTMax) {
+            finish = true;  // escape: with NEE + analytic direct nothing more is added
+        } else {
+            // rebuild the active set at the scatter point (sampling.cuh:422-458)
+            const f3 x = o + ts * d;
+            uint32_t nn = 0, ia = 0, visits = 0, rb_a = 0, rb_h = 0;
+            auto push = [&](uint32_t v) {
+                if (nn < cap) nxt[nn * stride] = v, ++nn;
+                else ++overflow;
+            };
+            walk_point(sc, x, visits, [&](uint32_t c) {
+                while (ia < na) {
+                    const uint32_t a = act[ia * stride];
+                    if (a >= c) break;
+                    ++rb_a;
+                    if (ts <= exit_from_inside(load_prim(sc.prims, a), o, d)) push(a);
+                    ++ia;
+                }
+                const PrimRecord pr = load_prim(sc.prims, c);
+                bool keep = false;
+                if (ia < na && act[ia * stride] == c) {
+                    ++rb_a;
+                    keep = ts <= exit_from_inside(pr, o, d);
+                    ++ia;
+                }
+                if (!keep) {
+                    float wl2;
+                    const float tj = entry_t(pr, o, d, &wl2);
+                    ++rb_h;
+                    keep = tj >= 0.0f && tj <= ts && ts <= exit_from_entry(pr, o, d, tj, wl2);
+                }
+                if (keep) push(c);
+            });
+            while (ia < na) {
+                const uint32_t a = act[ia * stride];
+                ++rb_a;
+                if (ts <= exit_from_inside(load_prim(sc.prims, a), o, d)) push(a);
+                ++ia;
+            }
+            ev.add(kCntNodeVisits, visits);
+            ev.add(kCntRbActive, rb_a);
+            ev.add(kCntRbHit, rb_h);
+            st.cur ^= 1u;
+            st.na = nn;
+            act = nxt;
+            if (measure) {
+                uint32_t ovl = 0, v2 = 0;
+                walk_point(sc, x, v2, [&](uint32_t c) { ovl += point_inside(load_prim(sc.prims, c), x) ? 1u : 0u; });
+                atomicMax(&counters[kCntMaxOverlap], static_cast<unsigned long long>(ovl));
+            }
+
+            float phase_pdf;
+            const f3 wo = phase_sample(J.hg, d, st.rng, &phase_pdf);
+            ev.add(kCntScatter);
+
+            // evaluate_albedo (sampling.cuh:257-283)
+            f3 aa = mk(0.0f);
+            float aw = 0.0f;
+            for (uint32_t k = 0; k < nn; ++k) {
+                const PrimRecord pr = load_prim(sc.prims, act[k * stride]);
+                const float wgt = pr.mass * prim_pdf(pr, x);
+                aa = aa + f3{pr.ar, pr.ag, pr.ab} * wgt;
+                aw += wgt;
+            }
+            ev.add(kCntAlb, nn);
+            const f3 albedo = aw <= 0.0f ? mk(0.0f) : aa * rcpf(aw);
+
+            // accumulate_nee_direct (path_stages.cuh:148-226)
+            const f3 base = st.beta * albedo;
+            const f3 luma = f3{kLumR, kLumG, kLumB};
+            if (J.use_ris) {
+                float wsum = 0.0f;
+                f3 ydir = mk(0.0f), yenv = mk(0.0f);
+                for (uint32_t c = 0; c < J.ris_candidates; ++c) {
+                    float cpdf;
+                    const f3 cwo = env_sample(env, st.rng, &cpdf);
+                    if (cpdf <= 0.0f) continue;
+                    const f3 envc = env_lookup(env, cwo);
+                    const float lumc = dot(envc, luma);
+                    const float wgt = phase_eval(J.hg, d, cwo) * lumc * rcpf(cpdf);
+                    if (!(wgt > 0.0f)) continue;
+                    wsum += wgt;
+                    if (next_uniform(st.rng) * wsum < wgt) {
+                        ydir = cwo;
+                        yenv = envc;
+                    }
+                }
+                if (wsum > 0.0f) {
+                    const float lumy = dot(yenv, luma);
+                    if (lumy > 0.0f) {
+                        ev.add(kCntShadowRays);
+                        const float T = transmittance(sc, x, ydir, act, nn, stride, ev);
+                        const float Wr = wsum * rcpf(static_cast<float>(J.ris_candidates));
+                        st.L = st.L + ((base * (yenv * rcpf(lumy))) * T) * Wr;
+                    }
+                }
+            } else {
+                float apdf;
+                const f3 awo = phase_sample(J.hg, d, st.rng, &apdf);
+                const float pb_a = env_pdf(env, awo);
+                const float sa = apdf + pb_a;
+                const float wa = sa > 0.0f ? apdf / sa : 0.0f;
+                const f3 enva = env_lookup(env, awo);
+                ev.add(kCntShadowRays);
+                const float Ta = transmittance(sc, x, awo, act, nn, stride, ev);
+                st.L = st.L + ((base * enva) * Ta) * wa;
+
+                float bpdf;
+                const f3 bwo = env_sample(env, st.rng, &bpdf);
+                const float ph_b = phase_eval(J.hg, d, bwo);
+                const float sb = bpdf + ph_b;
+                const float wb = sb > 0.0f ? bpdf / sb : 0.0f;
+                const f3 envb = env_lookup(env, bwo);
+                ev.add(kCntShadowRays);
+                const float Tb = transmittance(sc, x, bwo, act, nn, stride, ev);
+                st.L = st.L + (((((base * envb) * Tb) * wb) * ph_b) * rcpf(bpdf));
+            }
+            st.beta = st.beta * albedo;
+
+            // russian_roulette (path_stages.cuh:25-37) and guards (raygen.cuh:191-204)
+            if (b >= J.rr_depth) {
+                const float ps = minf(J.rr_max_survival, max3(st.beta));
+                if (next_uniform(st.rng) > ps || ps <= 0.0f) finish = true;
+                else st.beta = st.beta * rcpf(ps);
+            }
+            if (!finish && max3(st.beta) < kMinThroughput) finish = true;
+            if (!finish && !isfinitef_(sum3(st.beta))) finish = true;
+            if (!finish) {
+                st.o = x;
+                st.d = wo;
+                st.bounce = b + 1;
+                if (st.bounce >= J.max_depth) finish = true;
+            }
+        }
+
+        if (finish) {
+            // finalize_sample (path_stages.cuh:52-62)
+            f3 L = st.L;
+            if (J.firefly_clamp > 0.0f) {
+                const float lum = dot(L, f3{kLumR, kLumG, kLumB});
+                if (lum > J.firefly_clamp) L = L * (J.firefly_clamp * rcpf(lum));
+            }
+            if (!isfinitef_(sum3(L))) L = mk(0.0f);
+            J.samples[st.slot] = make_float4(L.x, L.y, L.z, 1.0f);
+            alive = false;
+            ev.flush(counters);
+        }
+    }
+    if (overflow) atomicAdd(&counters[kCntActiveOverflow], static_cast<unsigned long long>(overflow));
+}
+
+// Fold a slice of per-sample radiance into the Welford mean, in sample order
+// (raygen.cuh:214-238): n_inv = 1 / (samples before + 1).
+__global__ void k_welford(const float4* __restrict__ samples, uint32_t npix, uint32_t s_count, uint32_t sample_base,
+                          float4* __restrict__ mean) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    float4 m = sample_base == 0 ? make_float4(0.0f, 0.0f, 0.0f, 1.0f) : mean[i];
+    f3 mu = f3{m.x, m.y, m.z};
+    for (uint32_t s = 0; s < s_count; ++s) {
+        const float4 v = samples[static_cast<size_t>(s) * npix + i];
+        const float ninv = rcpf(static_cast<float>(sample_base + s + 1));
+        mu = mu + (f3{v.x, v.y, v.z} - mu) * ninv;
+    }
+    mean[i] = make_float4(mu.x, mu.y, mu.z, 1.0f);
+}
+
+// normalize_accumulator_kernel (device/kernels/normalize_accumulator.cu:16-35)
+__global__ void k_normalize(float4* __restrict__ out, const float4* __restrict__ in, size_t n, float scale) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const float4 v = in[i];
+        out[i] = make_float4(v.x * scale, v.y * scale, v.z * scale, v.w * scale);
+    }
+}
+
+// Transmittance along fixed rays, origin set = every primitive containing the
+// origin (the bounce-0 scan), no capacity limit.
+__global__ void k_transmittance_rays(SceneView sc, const float* __restrict__ rays, uint32_t m, float* __restrict__ T) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    const f3 o = f3{rays[6 * r], rays[6 * r + 1], rays[6 * r + 2]};
+    const f3 d = f3{rays[6 * r + 3], rays[6 * r + 4], rays[6 * r + 5]};
+    uint64_t acc = 0;
+    bool full = false;
+    uint32_t visits = 0;
+    walk_point(sc, o, visits, [&](uint32_t prim) {
+        const PrimRecord pr = load_prim(sc.prims, prim);
+        if (full || !point_inside(pr, o)) return;
+        const float te = exit_from_inside(pr, o, d);
+        if (te > 0.0f && te < kTMax)
+            if (tau_add(acc, optical_depth(pr, o, d, 0.0f, te))) full = true;
+    });
+    if (!full) {
+        const f3 inv = safe_inv(d);
+        const float tmax = kTMax;
+        walk_ray(sc, o, inv, tmax, visits, [&](uint32_t prim) -> bool {
+            const PrimRecord pr = load_prim(sc.prims, prim);
+            float wl2;
+            const float tj = entry_t(pr, o, d, &wl2);
+            if (tj < 0.0f) return true;
+            const float te = exit_from_entry(pr, o, d, tj, wl2);
+            if (te > tj && te < kTMax) {
+                if (tau_add(acc, optical_depth(pr, o, d, tj, te))) {
+                    full = true;
+                    return false;
+                }
+            }
+            return true;
+        });
+    }
+    T[r] = full ? 0.0f : tau_to_T(acc);
+}
+
+// Entry list of each ray (collect_hits, sampling.cuh:287-305) sorted by (t, index).
+__global__ void k_overlaps(SceneView sc, const float* __restrict__ rays, uint32_t m, uint32_t cap,
+                           uint32_t* __restrict__ counts, float* __restrict__ tout, uint32_t* __restrict__ iout) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    const f3 o = f3{rays[6 * r], rays[6 * r + 1], rays[6 * r + 2]};
+    const f3 d = f3{rays[6 * r + 3], rays[6 * r + 4], rays[6 * r + 5]};
+    float* tr = tout + static_cast<size_t>(r) * cap;
+    uint32_t* ir = iout + static_cast<size_t>(r) * cap;
+    uint32_t c = 0, visits = 0;
+    const f3 inv = safe_inv(d);
+    const float tmax = kTMax;
+    walk_ray(sc, o, inv, tmax, visits, [&](uint32_t prim) -> bool {
+        float wl2;
+        const float tj = entry_t(load_prim(sc.prims, prim), o, d, &wl2);
+        if (tj < 0.0f) return true;
+        if (c < cap) {
+            uint32_t k = c;
+            while (k > 0 && (tr[k - 1] > tj || (tr[k - 1] == tj && ir[k - 1] > prim))) {
+                tr[k] = tr[k - 1];
+                ir[k] = ir[k - 1];
+                --k;
+            }
+            tr[k] = tj;
+            ir[k] = prim;
+        }
+        ++c;
+        return true;
+    });
+    counts[r] = c;
+}
+
+__global__ void k_math(int fn, const float* __restrict__ x, const float* __restrict__ y, uint32_t n,
+                       float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float a = x[i], b = y[i];
+    float r;
+    switch (fn) {
+        case 0: r = expf_(a); break;
+        case 1: r = logf_(a); break;
+        case 2: r = sinf_(a); break;
+        case 3: r = cosf_(a); break;
+        case 4: r = atan2f_(a, b); break;
+        case 5: r = acosf_(a); break;
+        case 6: r = erff_(a); break;
+        case 7: r = erfcf_(a); break;
+        case 8: r = erfinvf_(a); break;
+        case 9: r = erf_diff_(a, b); break;
+        case 10: r = sqrtf_(a); break;
+        case 11: r = a / b; break;
+        default: r = __builtin_nanf("");
+    }
+    out[i] = r;
+}
+
+__global__ void k_prim(int fn, const PrimRecord* __restrict__ rec, const float* __restrict__ args, uint32_t m,
+                       float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const PrimRecord p = *rec;
+    const float* a = args + 9 * i;
+    const f3 o = f3{a[0], a[1], a[2]}, d = f3{a[3], a[4], a[5]};
+    float wl2, r;
+    switch (fn) {
+        case 0: r = inv_cdf(p, o, d, a[8]); break;
+        case 1: r = inv_cdf_span(p, o, d, a[6], a[8]); break;
+        case 2: r = optical_depth(p, o, d, a[6], a[7]); break;
+        case 3: r = exit_from_inside(p, o, d); break;
+        case 4: r = entry_t(p, o, d, &wl2); break;
+        case 5: {
+            const float t = entry_t(p, o, d, &wl2);
+            r = t >= 0.0f ? exit_from_entry(p, o, d, t, wl2) : -1.0f;
+        } break;
+        case 6: r = point_inside(p, o) ? 1.0f : 0.0f; break;
+        case 7: r = prim_pdf(p, o); break;
+        default: r = __builtin_nanf("");
+    }
+    out[i] = r;
+}
+
+inline uint32_t cdiv(uint64_t a, uint32_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
+
+}  // namespace
+}  // namespace vp
+
+using namespace vp;
+
+hipError_t vp_launch_render(vp_context* ctx, const JobDesc* jobs_dev, const uint64_t* job_begin_dev, uint32_t njobs,
+                            uint64_t total_paths, const RenderLaunch& L, hipStream_t s) {
+    const uint32_t stride = L.grid * L.block;
+    if (L.count_events)
+        hipLaunchKernelGGL(k_render<true>, dim3(L.grid), dim3(L.block), 0, s, jobs_dev, job_begin_dev, njobs,
+                           total_paths, ctx->path_counter, ctx->slab, L.active_cap, stride, ctx->counters, L.measure);
+    else
+        hipLaunchKernelGGL(k_render<false>, dim3(L.grid), dim3(L.block), 0, s, jobs_dev, job_begin_dev, njobs,
+                           total_paths, ctx->path_counter, ctx->slab, L.active_cap, stride, ctx->counters, L.measure);
+    return hipGetLastError();
+}
+
+int vp_render_occupancy(uint32_t block, uint32_t, int count_events, int* blocks_per_cu) {
+    int nb = 0;
+    hipError_t e;
+    if (count_events)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(k_render<true>), block, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(k_render<false>), block, 0);
+    *blocks_per_cu = nb > 0 ? nb : 1;
+    return e == hipSuccess ? 0 : -1;
+}
+
+hipError_t vp_launch_welford(const JobDesc& job, float4* mean, uint32_t sample_base, hipStream_t s) {
+    const uint32_t npix = job.width * job.height;
+    hipLaunchKernelGGL(k_welford, dim3(cdiv(npix, 256)), dim3(256), 0, s, job.samples, npix, job.s_count,
+                       sample_base, mean);
+    return hipGetLastError();
+}
+
+hipError_t vp_launch_normalize(float4* out, const float4* in, size_t n, float scale, hipStream_t s) {
+    const uint32_t grid = std::min<uint32_t>(cdiv(n, 256), 1024u);
+    hipLaunchKernelGGL(k_normalize, dim3(grid ? grid : 1), dim3(256), 0, s, out, in, n, scale);
+    return hipGetLastError();
+}
+
+hipError_t vp_launch_transmittance(SceneView sc, const float* rays, uint32_t m, float* T, hipStream_t s) {
+    hipLaunchKernelGGL(k_transmittance_rays, dim3(cdiv(m, 128)), dim3(128), 0, s, sc, rays, m, T);
+    return hipGetLastError();
+}
+
+hipError_t vp_launch_overlaps(SceneView sc, const float* rays, uint32_t m, uint32_t cap, uint32_t* counts, float* t,
+                              uint32_t* idx, hipStream_t s) {
+    hipLaunchKernelGGL(k_overlaps, dim3(cdiv(m, 128)), dim3(128), 0, s, sc, rays, m, cap, counts, t, idx);
+    return hipGetLastError();
+}
+
+hipError_t vp_launch_math(int fn, const float* x, const float* y, uint32_t n, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_math, dim3(cdiv(n, 256)), dim3(256), 0, s, fn, x, y, n, out);
+    return hipGetLastError();
+}
+
+hipError_t vp_launch_prim(int fn, const PrimRecord* rec, const float* args, uint32_t m, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_prim, dim3(cdiv(m, 256)), dim3(256), 0, s, fn, rec, args, m, out);
+    return hipGetLastError();
+}

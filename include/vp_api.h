@@ -1,0 +1,203 @@
+/* vp_api.h -- C ABI of the MI355X Gaussian-mixture volumetric path tracer.
+ *
+ * Plain C types only (pointers, sizes, PODs).  Every call returns a
+ * vp_status; vp_last_error() gives the message of the last failure on the
+ * calling thread.  Calls on one context are externally synchronised (one
+ * host thread drives a context, as the reference's Renderer does); the
+ * server (vp_server_*) owns its own thread.
+ *
+ * Reference interfaces each entry point replaces (kfernandez31/cuda-volprim):
+ *   vp_scene_create      Renderer::initPrimsAndGAS: Morton sort + GAS/IAS build
+ *                        (src/thesis/host/app/renderer.cpp:85-173)
+ *   vp_env_create        EnvironmentMap ctor: texture + importance CDF
+ *                        (include/thesis/host/params/environment_map.h:45-151)
+ *   vp_camera_*          host::params::Camera::createPerspective/createOrthographic
+ *                        (include/thesis/host/params/camera.h:40-150)
+ *   vp_render            Renderer::render batch loop + optixLaunch(W x H)
+ *                        (src/thesis/host/app/renderer.cpp:341-439)
+ *   vp_normalize_accumulator
+ *                        launch_normalize_accumulator_kernel
+ *                        (device/kernels/normalize_accumulator.h:11-13)
+ *   vp_render_jobs       (no counterpart: the batched persistent form)
+ *   vp_server_*          (no counterpart: the serving form)
+ *   vp_transmittance_rays, vp_collect_overlaps, vp_math_eval, vp_prim_eval
+ *                        test entry points exposing compute_transmittance_to_env
+ *                        (device/core/sampling.cuh:548-583), collect_hits
+ *                        (sampling.cuh:287-305) and the primitive math
+ *                        (include/thesis/device/params/primitive.h:127-318)
+ */
+#ifndef VP_API_H
+#define VP_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    VP_OK = 0,
+    VP_ERR_INVALID = 1,   /* bad argument (null pointer, size, range) */
+    VP_ERR_HIP = 2,       /* HIP runtime failure */
+    VP_ERR_OOM = 3,       /* device allocation failed or admission refused */
+    VP_ERR_NOT_READY = 4, /* server request still running */
+    VP_ERR_STATE = 5      /* object in the wrong state for the call */
+} vp_status;
+
+typedef struct vp_context vp_context;
+typedef struct vp_scene vp_scene;
+typedef struct vp_env vp_env;
+typedef struct vp_server vp_server;
+
+/* One Gaussian: centre, FORWARD rotation quaternion (w, x, y, z; normalised by
+ * the library), per-axis scale (linear, > 0), albedo, total mass M
+ * (PLY sigma_t_0 x sigma multiplier, src/thesis/host/utils/io/ply.cpp:107-116). */
+typedef struct {
+    float center[3];
+    float quat_wxyz[4];
+    float scale[3];
+    float albedo[3];
+    float mass;
+} vp_primitive;
+
+/* Camera as the device consumes it (include/thesis/device/params/camera.h:17-26). */
+typedef struct {
+    float eye[3];
+    float pixel00_rel[3];
+    float pixel_du[3];
+    float pixel_dv[3];
+    float view_dir[3];
+    float orthographic; /* 0 = perspective, 1 = orthographic */
+} vp_camera;
+
+/* Render settings: the test_runner flag surface (test/test_runner.cpp:101-129). */
+typedef struct {
+    uint32_t width, height;
+    uint32_t spp;             /* --spp */
+    uint32_t seed;            /* --seed */
+    uint32_t max_depth;       /* --max-depth (default 128) */
+    uint32_t rr_depth;        /* --rr-depth (default 12) */
+    float rr_max_survival;    /* --rr-max-survival (default 0.99) */
+    float firefly_clamp;      /* --firefly-clamp (0 = off) */
+    float filter_stddev;      /* --filter-width (0 = box) */
+    float phase_g;            /* --phase-g (default 0.85) */
+    uint32_t use_ris;         /* --ris */
+    uint32_t ris_candidates;  /* --ris-candidates (default 6) */
+    uint32_t measure_caps;    /* --measure-caps */
+    uint32_t active_cap;      /* origin/scatter active-set capacity (0 = 128) */
+} vp_render_params;
+
+typedef struct {
+    uint64_t active_overflow;   /* active-set inserts dropped at capacity */
+    uint64_t hit_overflow;      /* always 0: entry hits are streamed, never dropped */
+    uint32_t max_hits_per_ray;  /* --measure-caps: max entries on one traced ray */
+    uint32_t max_point_overlap; /* --measure-caps: max primitives containing a vertex */
+    double kernel_ms;           /* GPU time of the render kernels */
+    double wall_ms;             /* host wall time of the call */
+    uint64_t paths;             /* camera paths traced */
+} vp_render_stats;
+
+/* Launch configuration chosen by the autotuner or forced by the caller. */
+typedef struct {
+    uint32_t block;        /* threads per workgroup (64..1024, multiple of 64) */
+    uint32_t blocks_per_cu;/* persistent workgroups per CU (0 = occupancy max) */
+    uint32_t active_cap;   /* active-set capacity (entries per path) */
+    uint32_t batch_spp;    /* samples per pixel per launch (0 = all) */
+} vp_launch_config;
+
+/* ---- context ---------------------------------------------------------- */
+vp_status vp_context_create(int device, vp_context** out);
+vp_status vp_context_destroy(vp_context* ctx);
+const char* vp_last_error(void);
+vp_status vp_device_info(vp_context* ctx, char* name, size_t name_len, uint32_t* num_cus, double* clock_mhz);
+
+/* ---- scene / environment / camera -------------------------------------- */
+vp_status vp_scene_create(vp_context* ctx, const vp_primitive* prims, uint32_t n, vp_scene** out);
+vp_status vp_scene_destroy(vp_scene* scene);
+/* Order the scene stores primitives in (order[k] = input index of the k-th),
+ * their 30-bit Morton codes (input order) and the 64-byte device records
+ * (16 floats each, stored order).  Any pointer may be null. */
+vp_status vp_scene_export(vp_scene* scene, uint32_t* order, uint32_t* morton, float* records16, uint32_t* depth);
+vp_status vp_env_create(vp_context* ctx, const float* rgba, uint32_t width, uint32_t height, vp_env** out);
+vp_status vp_env_destroy(vp_env* env);
+/* joint weights (W*H), alias thresholds (W*H), alias indices (W*H), total. */
+vp_status vp_env_export(vp_env* env, float* joint, float* prob, uint32_t* alias, float* total);
+vp_status vp_camera_perspective(uint32_t width, uint32_t height, const float eye[3], const float target[3],
+                                const float up[3], float vfov_deg, vp_camera* out);
+vp_status vp_camera_orthographic(uint32_t width, uint32_t height, const float eye[3], const float target[3],
+                                 const float up[3], float ortho_height, vp_camera* out);
+void vp_default_params(vp_render_params* p);
+
+/* ---- plain renderer ---------------------------------------------------- */
+/* Render to a host RGBA float image (width*height*4). */
+vp_status vp_render(vp_context* ctx, vp_scene* scene, vp_env* env, const vp_camera* cam,
+                    const vp_render_params* params, float* out_rgba, vp_render_stats* stats);
+/* Same, output stays on the device: out_rgba_dev (W*H*4 floats), and, if not
+ * null, out_samples_dev (spp*W*H*4 floats, sample-major) receives every
+ * sample's radiance.  stream is a hipStream_t (null = the context stream). */
+vp_status vp_render_device(vp_context* ctx, vp_scene* scene, vp_env* env, const vp_camera* cam,
+                           const vp_render_params* params, const vp_launch_config* launch,
+                           float* out_rgba_dev, float* out_samples_dev, void* stream, vp_render_stats* stats);
+/* out[i] = mean[i] * scale over n RGBA pixels, device pointers. */
+vp_status vp_normalize_accumulator(float* out_rgba_dev, const float* mean_rgba_dev, size_t n, float scale,
+                                   void* stream);
+
+/* ---- batched persistent form ------------------------------------------- */
+typedef struct {
+    vp_scene* scene;
+    vp_env* env;
+    vp_camera camera;
+    vp_render_params params;
+    float* out_rgba; /* host, width*height*4 */
+} vp_job;
+vp_status vp_render_jobs(vp_context* ctx, vp_job* jobs, uint32_t njobs, const vp_launch_config* launch,
+                         vp_render_stats* stats);
+/* Search launch configurations on a job mix, cache and return the best. */
+vp_status vp_autotune(vp_context* ctx, vp_job* jobs, uint32_t njobs, vp_launch_config* best);
+
+/* ---- serving form ------------------------------------------------------ */
+typedef struct {
+    uint64_t mem_budget_bytes; /* admission limit on per-path state + image bytes in flight */
+    uint32_t slice_spp;        /* samples per pixel advanced per request per step */
+    uint32_t max_inflight;     /* requests on the GPU at once */
+} vp_server_config;
+typedef struct {
+    uint64_t completed;
+    uint64_t samples;
+    double p50_ms, p99_ms, mean_ms;
+    double busy_ms;
+    uint64_t steps;
+} vp_server_stats;
+vp_status vp_server_create(vp_context* ctx, const vp_server_config* cfg, vp_server** out);
+/* Queue a request; *ticket identifies it.  The output buffer must stay valid
+ * until vp_server_poll reports completion. */
+vp_status vp_server_submit(vp_server* srv, const vp_job* job, uint64_t* ticket);
+/* VP_OK when done (latency_ms filled), VP_ERR_NOT_READY otherwise. */
+vp_status vp_server_poll(vp_server* srv, uint64_t ticket, double* latency_ms);
+vp_status vp_server_drain(vp_server* srv);
+vp_status vp_server_stats_get(vp_server* srv, vp_server_stats* out);
+vp_status vp_server_destroy(vp_server* srv);
+
+/* ---- test entry points -------------------------------------------------- */
+/* rays: m x (ox, oy, oz, dx, dy, dz), host.  T[m]: exp(-tau) with the origin
+ * set = primitives whose 3-sigma bound contains the origin. */
+vp_status vp_transmittance_rays(vp_context* ctx, vp_scene* scene, const float* rays, uint32_t m, float* T);
+/* Entry hits of each ray sorted by (t, stored index): counts[m], t[m*cap],
+ * idx[m*cap] (stored indices). */
+vp_status vp_collect_overlaps(vp_context* ctx, vp_scene* scene, const float* rays, uint32_t m, uint32_t cap,
+                              uint32_t* counts, float* t, uint32_t* idx);
+/* Evaluate math function fn (0 exp 1 log 2 sin 3 cos 4 atan2 5 acos 6 erf
+ * 7 erfc 8 erfinv 9 erf_diff 10 sqrt 11 div) on the device. */
+vp_status vp_math_eval(vp_context* ctx, int fn, const float* x, const float* y, uint32_t n, float* out);
+/* Evaluate primitive function fn (0 inv_cdf 1 inv_cdf_span 2 optical_depth
+ * 3 exit_inside 4 entry_t 5 exit_entry 6 point_inside 7 pdf) for one record
+ * over m argument rows (o3 d3 t0 t1 tau). */
+vp_status vp_prim_eval(vp_context* ctx, int fn, const float* record16, const float* args9, uint32_t m,
+                       float* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VP_API_H */

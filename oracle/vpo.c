@@ -1,0 +1,1076 @@
+/* vpo.c -- CPU oracle for the Gaussian-mixture volumetric path tracer.
+ *
+ * TEST INFRASTRUCTURE, NOT THE PRODUCT.  Only tests/, __graft_entry__.smoke()
+ * and the cpu_baseline leg of bench.py load this library, and only as the
+ * checker (or the timed CPU baseline).  The GPU renderer never calls it.
+ *
+ * A plain-C restatement of kfernandez31/cuda-volprim's render path, brute
+ * force over all primitives (no acceleration structure), multi-threaded over
+ * pixel rows.  Every function cites the reference lines it follows.  The
+ * deliberate, documented differences from the reference are the ones the
+ * GPU build makes too:
+ *   - the per-primitive free-flight uniform is a counter hash of
+ *     (path, bounce, role, primitive) instead of the next PCG draw
+ *     (the reference's own cap-free experiment, latex/chapters/06-optimization.tex:470-476),
+ *     so the argmin does not depend on the order primitives are visited in;
+ *   - active sets are kept sorted by primitive index (set semantics as
+ *     device/include CompactSet, capacity cap_active);
+ *   - optical depth along a shadow ray is summed in 24.40 fixed point and the
+ *     ray returns 0 as soon as the total reaches 88;
+ *   - erf(b) - erf(a) is evaluated through erfc on the tails;
+ *   - the environment is importance-sampled through an alias table over the
+ *     same texel distribution (joint/total) as the reference's 2D CDF; the
+ *     reference CDF is restated too (vpo_env_ref_cdf) so tests can pin the
+ *     distribution;
+ *   - bilinear environment lookup with exact float weights;
+ *   - the entry hit list is unbounded (the reference drops beyond 128).
+ * Parity status of each function is recorded in DESIGN.md.
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off).
+ */
+#include "vpo_math.h"
+
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* vectors                                                                   */
+/* ------------------------------------------------------------------------ */
+typedef struct { float x, y, z; } v3;
+static inline v3 V(float x, float y, float z) { v3 r = {x, y, z}; return r; }
+static inline v3 vadd(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 vsub(v3 a, v3 b) { return V(a.x + (-b.x), a.y + (-b.y), a.z + (-b.z)); }
+static inline v3 vmul(v3 a, v3 b) { return V(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline v3 vscale(v3 a, float s) { return V(a.x * s, a.y * s, a.z * s); }
+static inline v3 vfma(float a, v3 b, v3 c) { return V(fmaf(a, b.x, c.x), fmaf(a, b.y, c.y), fmaf(a, b.z, c.z)); }
+/* math.h:254-256 */
+static inline float vdot(v3 a, v3 b) { return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z)); }
+/* math.h:306-310 */
+static inline v3 vcross(v3 a, v3 b) {
+    return V(fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)), fmaf(a.x, b.y, -(a.y * b.x)));
+}
+static inline float vmax3(v3 v) { return vpo_max(vpo_max(v.x, v.y), v.z); }
+static inline v3 vnormalize(v3 v) { return vscale(v, vpo_rsqrt(vdot(v, v))); }
+
+/* ------------------------------------------------------------------------ */
+/* primitive record (primitive.h:27-87): 16 floats                           */
+/* c.xyz, qconj.wxyz, rcp_scale.xyz, dnf, icf, albedo.rgb, mass              */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    float cx, cy, cz, qw, qx, qy, qz, rsx, rsy, rsz, dnf, icf, ar, ag, ab, mass;
+} prim_t;
+
+#define ONE_OVER_TWO_PI_POW_3_2 0.0634936359f
+#define ROOT_TWO_PI 2.5066282746f
+#define ROOT_HALF_PI (ROOT_TWO_PI * 0.5f)
+#define ROOT_TWO 1.41421356237309504880f
+#define ONE_OVER_ROOT_TWO (1.0f / ROOT_TWO)
+#define ONE_OVER_FOUR_PI (1.0f / (4.0f * VPO_PI))
+#define INF_F 1e20f
+#define EXTENT2 9.0f
+
+/* Primitive(center, q.conjugate(), scale, albedo, M) with q = UnitQuaternion::from
+ * (quat.h:54-57, primitive.h:70-87,108-111). */
+void vpo_make_prims(const float* in, long n, float* out) {
+    for (long i = 0; i < n; ++i) {
+        const float* r = in + 14 * i;
+        prim_t* p = (prim_t*)(out + 16 * i);
+        float w = r[3], x = r[4], y = r[5], z = r[6];
+        float l2 = fmaf(w, w, fmaf(x, x, fmaf(y, y, z * z)));
+        float s = 1.0f / sqrtf(l2);
+        p->cx = r[0]; p->cy = r[1]; p->cz = r[2];
+        p->qw = w * s; p->qx = -(x * s); p->qy = -(y * s); p->qz = -(z * s);
+        p->rsx = 1.0f / r[7]; p->rsy = 1.0f / r[8]; p->rsz = 1.0f / r[9];
+        p->dnf = ONE_OVER_TWO_PI_POW_3_2 * (p->rsx * p->rsy * p->rsz);
+        p->icf = r[13] * p->dnf * ROOT_HALF_PI;
+        p->ar = r[10]; p->ag = r[11]; p->ab = r[12];
+        p->mass = r[13];
+    }
+}
+
+/* quat.h:67-71 */
+static inline v3 qrot(const prim_t* p, v3 v) {
+    v3 u = V(p->qx, p->qy, p->qz);
+    v3 uv = vcross(u, v);
+    v3 uuv = vcross(u, uv);
+    return vfma(2.0f, vfma(p->qw, uv, uuv), v);
+}
+/* primitive.h:98-104 */
+static inline v3 local_pos(const prim_t* p, v3 x) { return vmul(qrot(p, vsub(x, V(p->cx, p->cy, p->cz))), V(p->rsx, p->rsy, p->rsz)); }
+static inline v3 local_dir(const prim_t* p, v3 d) { return vmul(qrot(p, d), V(p->rsx, p->rsy, p->rsz)); }
+
+/* primitive.h:127-135 */
+static float prim_pdf(const prim_t* p, v3 x) {
+    v3 l = local_pos(p, x);
+    return vpo_expf(-0.5f * vdot(l, l)) * p->dnf;
+}
+
+/* primitive.h:140-186 */
+static float inv_cdf(const prim_t* pr, v3 o, v3 d, float tau) {
+    v3 w = local_dir(pr, d), p = local_pos(pr, o);
+    float wl2 = vdot(w, w);
+    float wil = vpo_rsqrt(wl2);
+    float wp = vdot(w, p) * wil;
+    float pp = vdot(p, p);
+    float diff = fmaf(-wp, wp, pp);
+    float K = wil * vpo_expf(-0.5f * diff) * pr->icf;
+    float raw = vpo_erff(wp * ONE_OVER_ROOT_TWO) + tau * vpo_rcp(K);
+    if (raw > 1.0f) return INF_F;
+    float v = vpo_erfinvf(vpo_clamp(raw, -1.0f, 1.0f));
+    return (ROOT_TWO * v - wp) * wil;
+}
+
+/* primitive.h:203-246 */
+static float inv_cdf_span(const prim_t* pr, v3 o, v3 d, float toff, float tau) {
+    v3 w = local_dir(pr, d), p = local_pos(pr, o);
+    float wl2 = vdot(w, w);
+    float wil = vpo_rsqrt(wl2);
+    float wl = wl2 * wil;
+    float wp = vdot(w, p) * wil;
+    float pp = vdot(p, p);
+    float diff = fmaf(-wp, wp, pp);
+    float K = wil * vpo_expf(-0.5f * diff) * pr->icf;
+    float wpo = fmaf(toff, wl, wp);
+    float raw = vpo_erff(wpo * ONE_OVER_ROOT_TWO) + tau * vpo_rcp(K);
+    if (raw > 1.0f) return INF_F;
+    float v = vpo_erfinvf(vpo_clamp(raw, -1.0f, 1.0f));
+    return (ROOT_TWO * v - wp) * wil;
+}
+
+/* primitive.h:249-318 (erf difference through erfc on the tails) */
+static float optical_depth(const prim_t* pr, v3 o, v3 d, float t0, float t1) {
+    if (t1 - t0 <= 1e-6f) return 0.0f;
+    v3 w = local_dir(pr, d), p = local_pos(pr, o);
+    float wil = vpo_rsqrt(vdot(w, w));
+    float t0s = vpo_max(0.0f, t0);
+    float t1s = vpo_max(t0s + 1e-6f, t1);
+    v3 wn = vscale(w, wil);
+    v3 ps = vadd(p, vscale(w, t0s));
+    float tl = vpo_clamp((t1s - t0s) * vpo_rcp(wil), 0.0f, 6.0f);
+    float B = vdot(wn, ps);
+    float C = vdot(ps, ps);
+    float perp = fmaf(-B, B, C);
+    float e = vpo_expf(-0.5f * perp);
+    float G = ROOT_TWO_PI * wil;
+    float bs = B * ONE_OVER_ROOT_TWO;
+    float term = vpo_erf_diff(bs, fmaf(tl, ONE_OVER_ROOT_TWO, bs)) * 0.5f;
+    return pr->mass * G * e * term * pr->dnf;
+}
+
+/* intersection.h:115-126 */
+static float exit_inside(const prim_t* pr, v3 o, v3 d) {
+    v3 w = local_dir(pr, d), p = local_pos(pr, o);
+    float a = vdot(w, w);
+    float b = vdot(p, w);
+    float c = vdot(p, p) - EXTENT2;
+    float disc = fmaf(-a, c, b * b);
+    return (-b + sqrtf(disc)) * vpo_rcp(a);
+}
+
+/* intersection.h:142-173 */
+static float exit_entry(const prim_t* pr, v3 o, v3 d, float te, float wl2) {
+    v3 pe = vadd(o, vscale(d, te));
+    v3 p = local_pos(pr, pe);
+    v3 w = local_dir(pr, d);
+    float pdw = vdot(p, w);
+    float ed = -2.0f * pdw * vpo_rcp(wl2);
+    return te + ed;
+}
+
+/* Entry into the 3-sigma bound over [0, 1e20): the built-in sphere contract
+ * (renderer.cpp:142-157 instance transform T R S(3 s); anyhit.cuh:68-86;
+ * intersection.h:47-55 front face only), solved with the stable quadratic of
+ * intersection.h:66-101 at radius^2 = 9.  Negative = no entry. */
+static float entry_t(const prim_t* pr, v3 o, v3 d, float* wl2) {
+    v3 w = local_dir(pr, d), p = local_pos(pr, o);
+    float a = vdot(w, w);
+    *wl2 = a;
+    if (a < 1e-12f) return -1.0f;
+    float c = vdot(p, p) - EXTENT2;
+    if (!(c > 0.0f)) return -1.0f;
+    float b = -vdot(p, w);
+    if (!(b > 0.0f)) return -1.0f;
+    float ai = vpo_rcp(a);
+    v3 q0 = vfma(b * ai, w, p);
+    float delta = EXTENT2 - vdot(q0, q0);
+    if (delta < 1e-12f) return -1.0f;
+    float q = b + sqrtf(a * delta);
+    float t = c * vpo_rcp(q);
+    if (!(t >= 0.0f && t < INF_F)) return -1.0f;
+    return t;
+}
+
+/* intersection.h:56-61 */
+static int point_inside(const prim_t* pr, v3 x) {
+    v3 l = local_pos(pr, x);
+    return vdot(l, l) <= EXTENT2;
+}
+
+/* test access to the primitive functions: fn selects, args per row */
+void vpo_prim_eval(int fn, const float* prim16, const float* args, long m, float* out) {
+    const prim_t* p = (const prim_t*)prim16;
+    for (long i = 0; i < m; ++i) {
+        const float* a = args + 9 * i; /* o(3) d(3) t0 t1 tau */
+        v3 o = V(a[0], a[1], a[2]), d = V(a[3], a[4], a[5]);
+        float wl2;
+        switch (fn) {
+            case 0: out[i] = inv_cdf(p, o, d, a[8]); break;
+            case 1: out[i] = inv_cdf_span(p, o, d, a[6], a[8]); break;
+            case 2: out[i] = optical_depth(p, o, d, a[6], a[7]); break;
+            case 3: out[i] = exit_inside(p, o, d); break;
+            case 4: out[i] = entry_t(p, o, d, &wl2); break;
+            case 5: { float t = entry_t(p, o, d, &wl2); out[i] = t >= 0.0f ? exit_entry(p, o, d, t, wl2) : -1.0f; } break;
+            case 6: out[i] = (float)point_inside(p, o); break;
+            case 7: out[i] = prim_pdf(p, o); break;
+            default: out[i] = NAN;
+        }
+    }
+}
+
+void vpo_math_eval(int fn, const float* x, const float* y, long n, float* out) {
+    for (long i = 0; i < n; ++i) {
+        switch (fn) {
+            case 0: out[i] = vpo_expf(x[i]); break;
+            case 1: out[i] = vpo_logf(x[i]); break;
+            case 2: out[i] = vpo_sinf(x[i]); break;
+            case 3: out[i] = vpo_cosf(x[i]); break;
+            case 4: out[i] = vpo_atan2f(x[i], y[i]); break;
+            case 5: out[i] = vpo_acosf(x[i]); break;
+            case 6: out[i] = vpo_erff(x[i]); break;
+            case 7: out[i] = vpo_erfcf(x[i]); break;
+            case 8: out[i] = vpo_erfinvf(x[i]); break;
+            case 9: out[i] = vpo_erf_diff(x[i], y[i]); break;
+            case 10: out[i] = sqrtf(x[i]); break;
+            case 11: out[i] = x[i] / y[i]; break;
+            default: out[i] = NAN;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Morton codes (host/utils/math.h:103-160) and stable sort by code           */
+/* ------------------------------------------------------------------------ */
+static uint32_t expand_bits(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+uint32_t vpo_expand_bits(uint32_t v) { return expand_bits(v); }
+
+/* renderer.cpp:90-104: bounds of the centres, inv_extent = 1/max(ext, 1e-30) */
+void vpo_morton_codes(const float* centers, long n, uint32_t* codes) {
+    float mn[3] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f};
+    float mx[3] = {-3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
+    for (long i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            float c = centers[3 * i + k];
+            mn[k] = vpo_min(mn[k], c);
+            mx[k] = vpo_max(mx[k], c);
+        }
+    float inv[3];
+    for (int k = 0; k < 3; ++k) inv[k] = 1.0f / vpo_max(mx[k] + (-mn[k]), 1e-30f);
+    for (long i = 0; i < n; ++i) {
+        uint32_t q[3];
+        for (int k = 0; k < 3; ++k) {
+            float nrm = (centers[3 * i + k] + (-mn[k])) * inv[k];
+            q[k] = (uint32_t)(vpo_clamp(nrm, 0.0f, 1.0f) * 1023.0f);
+        }
+        codes[i] = (expand_bits(q[2]) << 2) | (expand_bits(q[1]) << 1) | expand_bits(q[0]);
+    }
+}
+
+static const uint32_t* g_sort_codes;
+static int cmp_code_idx(const void* a, const void* b) {
+    uint32_t ia = *(const uint32_t*)a, ib = *(const uint32_t*)b;
+    uint32_t ca = g_sort_codes[ia], cb = g_sort_codes[ib];
+    if (ca != cb) return ca < cb ? -1 : 1;
+    return ia < ib ? -1 : (ia > ib ? 1 : 0);
+}
+/* order[k] = original index of the k-th primitive by (code, index) */
+void vpo_sort_order(const uint32_t* codes, long n, uint32_t* order) {
+    for (long i = 0; i < n; ++i) order[i] = (uint32_t)i;
+    g_sort_codes = codes;
+    qsort(order, (size_t)n, sizeof(uint32_t), cmp_code_idx);
+}
+
+/* ------------------------------------------------------------------------ */
+/* RNG (random.cuh:16-53) and the per-primitive counter draw                  */
+/* ------------------------------------------------------------------------ */
+typedef struct { uint64_t state, inc; } pcg_t;
+static inline uint32_t pcg_next(pcg_t* r) {
+    uint64_t old = r->state;
+    r->state = old * 6364136223846793005ull + r->inc;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((0u - rot) & 31u));
+}
+static inline uint64_t hash64(uint64_t x) {
+    x = x * 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x = x * 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return x;
+}
+static inline pcg_t pcg_init(uint64_t seed, uint64_t seq) {
+    pcg_t r;
+    r.state = 0;
+    r.inc = (seq << 1u) | 1u;
+    pcg_next(&r);
+    r.state += seed;
+    pcg_next(&r);
+    return r;
+}
+static inline float u01(uint32_t v) { return (float)v / 4294967296.0f; }
+static inline float next_u(pcg_t* r) { return u01(pcg_next(r)); }
+static inline uint64_t path_key(uint64_t seed, uint64_t seq) { return hash64(seq ^ (seed * 0x9e3779b97f4a7c15ull) ^ 0x243f6a8885a308d3ull); }
+/* role 0 = primitive containing the ray origin, 1 = primitive entered */
+static inline float prim_u(uint64_t key, uint32_t bounce, uint32_t role, uint32_t prim) {
+    uint64_t x = key + 0x9e3779b97f4a7c15ull * (((uint64_t)((bounce << 1) | role) << 32) | prim);
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return u01((uint32_t)(x >> 32));
+}
+/* sampling.cuh:310-313 */
+static inline float ff_tau(float xi) { return -vpo_logf(vpo_max(1.0f - xi, 1e-10f)); }
+
+void vpo_pcg_stream(uint64_t seed, uint64_t seq, long n, uint32_t* out) {
+    pcg_t r = pcg_init(seed, seq);
+    for (long i = 0; i < n; ++i) out[i] = pcg_next(&r);
+}
+void vpo_pcg_raw(uint64_t state, uint64_t inc, long n, uint32_t* out) {
+    pcg_t r = {state, inc};
+    for (long i = 0; i < n; ++i) out[i] = pcg_next(&r);
+}
+uint64_t vpo_hash64(uint64_t x) { return hash64(x); }
+
+/* ------------------------------------------------------------------------ */
+/* shadow-ray transmittance (sampling.cuh:548-583, anyhit.cuh:53-66)          */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    uint64_t n_sh_active, n_sh_hit, n_arg_active, n_arg_hit, n_arg_accept;
+    uint64_t n_rb_active, n_rb_hit, n_alb, n_scatter, n_sample, n_shadow_rays, n_trace;
+    uint64_t active_overflow, max_hits, max_overlap;
+} counters_t;
+
+#define TAU_LIMIT_FIXED 96757023244288ull /* 88 * 2^40 */
+
+static inline int tau_add(uint64_t* acc, float od) {
+    float c = od > 0.0f ? od : 0.0f;
+    if (c >= 88.0f) return 1;
+    *acc += (uint64_t)(c * 1099511627776.0f);
+    return *acc >= TAU_LIMIT_FIXED;
+}
+
+static float transmittance(const prim_t* prims, long n, v3 o, v3 d, const uint32_t* set, int nset, counters_t* k) {
+    uint64_t acc = 0;
+    for (int s = 0; s < nset; ++s) {
+        const prim_t* pr = prims + set[s];
+        float te = exit_inside(pr, o, d);
+        if (k) k->n_sh_active++;
+        if (te > 0.0f && te < INF_F) {
+            if (tau_add(&acc, optical_depth(pr, o, d, 0.0f, te))) return 0.0f;
+        }
+    }
+    for (long j = 0; j < n; ++j) {
+        const prim_t* pr = prims + j;
+        float wl2;
+        float t = entry_t(pr, o, d, &wl2);
+        if (t < 0.0f) continue;
+        float te = exit_entry(pr, o, d, t, wl2);
+        if (k) k->n_sh_hit++;
+        if (te > t && te < INF_F) {
+            if (tau_add(&acc, optical_depth(pr, o, d, t, te))) return 0.0f;
+        }
+    }
+    float tau = (float)acc * (1.0f / 1099511627776.0f);
+    return tau >= 88.0f ? 0.0f : vpo_expf(-tau);
+}
+
+/* Rays with origin-inside set = every primitive whose bound contains the
+ * origin (the bounce-0 scan, sampling.cuh:322-341). */
+void vpo_transmittance_rays(const float* prims16, long n, const float* rays, long m, float* T) {
+    const prim_t* prims = (const prim_t*)prims16;
+    uint32_t* set = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+    for (long r = 0; r < m; ++r) {
+        v3 o = V(rays[6 * r], rays[6 * r + 1], rays[6 * r + 2]);
+        v3 d = V(rays[6 * r + 3], rays[6 * r + 4], rays[6 * r + 5]);
+        int ns = 0;
+        for (long i = 0; i < n; ++i)
+            if (point_inside(prims + i, o)) set[ns++] = (uint32_t)i;
+        T[r] = transmittance(prims, n, o, d, set, ns, NULL);
+    }
+    free(set);
+}
+
+/* Entry list of each ray sorted by (t, index): counts[r], then t/idx rows of
+ * width cap.  Returns the largest count seen. */
+long vpo_overlaps(const float* prims16, long n, const float* rays, long m, long cap, uint32_t* counts, float* tout, uint32_t* iout) {
+    const prim_t* prims = (const prim_t*)prims16;
+    long maxc = 0;
+    for (long r = 0; r < m; ++r) {
+        v3 o = V(rays[6 * r], rays[6 * r + 1], rays[6 * r + 2]);
+        v3 d = V(rays[6 * r + 3], rays[6 * r + 4], rays[6 * r + 5]);
+        long c = 0;
+        float* tr = tout + r * cap;
+        uint32_t* ir = iout + r * cap;
+        for (long j = 0; j < n; ++j) {
+            float wl2;
+            float t = entry_t(prims + j, o, d, &wl2);
+            if (t < 0.0f) continue;
+            if (c < cap) {
+                /* insertion in (t, index) order */
+                long k = c;
+                while (k > 0 && (tr[k - 1] > t)) { tr[k] = tr[k - 1]; ir[k] = ir[k - 1]; --k; }
+                tr[k] = t;
+                ir[k] = (uint32_t)j;
+            }
+            ++c;
+        }
+        counts[r] = (uint32_t)c;
+        if (c > maxc) maxc = c;
+    }
+    return maxc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* environment                                                               */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    const float* tex;
+    const float* prob;
+    const uint32_t* alias;
+    const float* joint;
+    uint32_t w, h;
+    float total;
+} env_t;
+
+/* joint = lum709 * sin(theta_texel_centre), total = sum of row sums
+ * (host/params/environment_map.h:45-101). */
+static float env_joint(const float* rgba, int w, int h, float* joint) {
+    float total = 0.0f;
+    for (int v = 0; v < h; ++v) {
+        float theta = ((float)v + 0.5f) / (float)h * VPO_PI;
+        float st = vpo_sinf(theta);
+        float row = 0.0f;
+        for (int u = 0; u < w; ++u) {
+            const float* t = rgba + 4 * ((size_t)v * w + u);
+            float lum = 0.2126f * t[0] + 0.7152f * t[1] + 0.0722f * t[2];
+            float wt = lum * st;
+            joint[(size_t)v * w + u] = wt;
+            row += wt;
+        }
+        total += row;
+    }
+    return total;
+}
+
+/* Vose alias table over p_i = joint_i / sum(joint) in double precision,
+ * stacks popped from the back. */
+void vpo_env_tables(const float* rgba, int w, int h, float* joint, float* prob, uint32_t* alias, float* total_out) {
+    size_t n = (size_t)w * h;
+    float total = env_joint(rgba, w, h, joint);
+    *total_out = total;
+    double sum = 0.0;
+    for (size_t i = 0; i < n; ++i) sum += (double)joint[i];
+    double* q = (double*)malloc(sizeof(double) * n);
+    uint32_t* small = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    uint32_t* large = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    size_t ns = 0, nl = 0;
+    for (size_t i = 0; i < n; ++i) {
+        q[i] = sum > 0.0 ? (double)joint[i] / sum * (double)n : 1.0;
+        if (q[i] < 1.0) small[ns++] = (uint32_t)i;
+        else large[nl++] = (uint32_t)i;
+    }
+    while (ns > 0 && nl > 0) {
+        uint32_t s = small[--ns];
+        uint32_t l = large[--nl];
+        prob[s] = (float)q[s];
+        alias[s] = l;
+        q[l] = (q[l] + q[s]) - 1.0;
+        if (q[l] < 1.0) small[ns++] = l;
+        else large[nl++] = l;
+    }
+    while (nl > 0) { uint32_t l = large[--nl]; prob[l] = 1.0f; alias[l] = l; }
+    while (ns > 0) { uint32_t s = small[--ns]; prob[s] = 1.0f; alias[s] = s; }
+    free(q);
+    free(small);
+    free(large);
+}
+
+/* The reference's marginal/conditional CDFs, restated to pin the texel
+ * distribution (host/params/environment_map.h:45-101). */
+void vpo_env_ref_cdf(const float* rgba, int w, int h, float* marginal, float* conditional, float* joint, float* total_out) {
+    for (int v = 0; v < h; ++v) {
+        float theta = ((float)v + 0.5f) / (float)h * VPO_PI;
+        float st = vpo_sinf(theta);
+        float row = 0.0f;
+        for (int u = 0; u < w; ++u) {
+            const float* t = rgba + 4 * ((size_t)v * w + u);
+            float lum = 0.2126f * t[0] + 0.7152f * t[1] + 0.0722f * t[2];
+            float wt = lum * st;
+            joint[(size_t)v * w + u] = wt;
+            row += wt;
+            conditional[(size_t)v * w + u] = row;
+        }
+        if (row > 0.0f) {
+            float inv = 1.0f / row;
+            for (int u = 0; u < w; ++u) conditional[(size_t)v * w + u] *= inv;
+        } else {
+            for (int u = 0; u < w; ++u) conditional[(size_t)v * w + u] = ((float)u + 1.0f) / (float)w;
+        }
+        marginal[v] = row;
+    }
+    float total = 0.0f;
+    for (int v = 0; v < h; ++v) { total += marginal[v]; marginal[v] = total; }
+    *total_out = total;
+    if (total > 0.0f) {
+        float inv = 1.0f / total;
+        for (int v = 0; v < h; ++v) marginal[v] *= inv;
+    } else {
+        for (int v = 0; v < h; ++v) marginal[v] = ((float)v + 1.0f) / (float)h;
+    }
+}
+
+static inline float lerp1(float a, float b, float t) { return fmaf(t, b - a, a); }
+
+/* environment_map.h:36-61 with software bilinear (wrap u, clamp v) */
+static v3 env_lookup(const env_t* e, v3 d) {
+    float theta = vpo_atan2f(d.z, d.x);
+    float phi = vpo_acosf(vpo_clamp(d.y, -1.0f, 1.0f));
+    float u = fmaf(theta, 1.0f / (2.0f * VPO_PI), 0.5f);
+    float v = phi * (1.0f / VPO_PI);
+    float x = u * (float)e->w - 0.5f;
+    float y = v * (float)e->h - 0.5f;
+    float x0 = floorf(x), y0 = floorf(y);
+    float fx = x - x0, fy = y - y0;
+    int W = (int)e->w, H = (int)e->h;
+    int i0 = (int)x0 % W;
+    if (i0 < 0) i0 += W;
+    int i1 = i0 + 1;
+    if (i1 >= W) i1 -= W;
+    int j0 = (int)y0, j1 = j0 + 1;
+    j0 = j0 < 0 ? 0 : (j0 >= H ? H - 1 : j0);
+    j1 = j1 < 0 ? 0 : (j1 >= H ? H - 1 : j1);
+    const float* t00 = e->tex + 4 * ((size_t)j0 * W + i0);
+    const float* t10 = e->tex + 4 * ((size_t)j0 * W + i1);
+    const float* t01 = e->tex + 4 * ((size_t)j1 * W + i0);
+    const float* t11 = e->tex + 4 * ((size_t)j1 * W + i1);
+    float r[3];
+    for (int c = 0; c < 3; ++c) {
+        float a = lerp1(t00[c], t10[c], fx);
+        float b = lerp1(t01[c], t11[c], fx);
+        r[c] = lerp1(a, b, fy);
+    }
+    return V(r[0], r[1], r[2]);
+}
+
+static int env_table(const env_t* e) { return e->prob != NULL && e->total > 0.0f && e->w != 0 && e->h != 0; }
+
+/* sampling.cuh:218-246 */
+static float env_pdf(const env_t* e, v3 wo) {
+    if (!env_table(e)) return ONE_OVER_FOUR_PI;
+    int W = (int)e->w, H = (int)e->h;
+    float theta = vpo_atan2f(wo.z, wo.x);
+    float polar = vpo_acosf(vpo_clamp(wo.y, -1.0f, 1.0f));
+    float un = fmaf(theta, 1.0f / (2.0f * VPO_PI), 0.5f);
+    float vn = polar * (1.0f / VPO_PI);
+    int u = (int)(un * (float)W);
+    int v = (int)(vn * (float)H);
+    u = u < 0 ? 0 : (u >= W ? W - 1 : u);
+    v = v < 0 ? 0 : (v >= H ? H - 1 : v);
+    float joint = e->joint[v * W + u];
+    float sp = vpo_sinf(polar);
+    if (joint <= 0.0f || sp <= 0.0f) return 0.0f;
+    float p_uv = (joint * vpo_rcp(e->total)) * (float)(W * H);
+    float jac = 2.0f * VPO_PI * VPO_PI * sp;
+    return p_uv * vpo_rcp(jac);
+}
+
+/* sampling.cuh:173-216, texel through the alias table */
+static v3 env_sample(const env_t* e, pcg_t* rng, float* pdf) {
+    float ux = next_u(rng), uy = next_u(rng);
+    if (!env_table(e)) {
+        float z = fmaf(-2.0f, ux, 1.0f);
+        float r = sqrtf(vpo_max(0.0f, fmaf(-z, z, 1.0f)));
+        float sp, cp;
+        vpo_sincosf(2.0f * VPO_PI * uy, &sp, &cp);
+        *pdf = ONE_OVER_FOUR_PI;
+        return V(r * cp, r * sp, z);
+    }
+    uint32_t n = e->w * e->h;
+    uint32_t col = (uint32_t)(ux * (float)n);
+    if (col >= n) col = n - 1;
+    uint32_t texel = uy < e->prob[col] ? col : e->alias[col];
+    uint32_t v = texel / e->w;
+    uint32_t u = texel - v * e->w;
+    float jx = next_u(rng), jy = next_u(rng);
+    float un = ((float)u + jx) * vpo_rcp((float)e->w);
+    float vn = ((float)v + jy) * vpo_rcp((float)e->h);
+    float theta = fmaf(2.0f * VPO_PI, un, -VPO_PI);
+    float polar = vn * VPO_PI;
+    float sp, cpol, st, ct;
+    vpo_sincosf(polar, &sp, &cpol);
+    vpo_sincosf(theta, &st, &ct);
+    v3 wo = V(sp * ct, cpol, sp * st);
+    float joint = e->joint[texel];
+    float p_uv = (joint * vpo_rcp(e->total)) * (float)(e->w * e->h);
+    float jac = 2.0f * VPO_PI * VPO_PI * sp;
+    *pdf = sp > 0.0f ? p_uv * vpo_rcp(jac) : 0.0f;
+    return wo;
+}
+
+/* ------------------------------------------------------------------------ */
+/* phase (sampling.cuh:74-134; host fold renderer.cpp:224-235)               */
+/* ------------------------------------------------------------------------ */
+typedef struct { int iso; float g, geff, opg2, omg2, ni2g, coeff; } hg_t;
+static hg_t hg_make(float g) {
+    hg_t h;
+    float ag = g < 0.0f ? -g : g;
+    h.iso = ag < 1e-3f;
+    h.g = g;
+    h.geff = -g;
+    float g2 = g * g;
+    h.opg2 = 1.0f + g2;
+    h.omg2 = 1.0f - g2;
+    h.ni2g = h.iso ? 0.0f : (-0.5f / h.geff);
+    h.coeff = ONE_OVER_FOUR_PI * h.omg2;
+    return h;
+}
+static float hg_eval_cos(const hg_t* h, float c) {
+    if (h->iso) return ONE_OVER_FOUR_PI;
+    float den = fmaf(-2.0f * h->g, c, h->opg2);
+    float is = vpo_rsqrt(den);
+    return h->coeff * (is * is * is);
+}
+static void onb(v3 n, v3* tx, v3* ty) {
+    float sign = copysignf(1.0f, n.z);
+    float a = -1.0f / (sign + n.z);
+    float b = n.x * n.y * a;
+    *tx = V(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    *ty = V(b, sign + n.y * n.y * a, -n.y);
+}
+static v3 hg_sample(const hg_t* h, v3 wi, pcg_t* rng, float* pdf) {
+    float ux = next_u(rng), uy = next_u(rng), c;
+    if (h->iso) {
+        c = fmaf(-2.0f, ux, 1.0f);
+    } else {
+        float inner = fmaf(-2.0f, ux, 1.0f);
+        float den = fmaf(h->geff, inner, 1.0f);
+        float sq = h->omg2 * vpo_rcp(den);
+        c = h->ni2g * fmaf(-sq, sq, h->opg2);
+    }
+    float s = sqrtf(vpo_max(0.0f, 1.0f - c * c));
+    float sp, cp;
+    vpo_sincosf(2.0f * VPO_PI * uy, &sp, &cp);
+    v3 ex, ey;
+    onb(wi, &ex, &ey);
+    v3 wo = vadd(vadd(vscale(ex, s * cp), vscale(ey, s * sp)), vscale(wi, c));
+    *pdf = hg_eval_cos(h, c);
+    return wo;
+}
+
+/* ------------------------------------------------------------------------ */
+/* camera (device/params/camera.h:33-57; host/params/camera.h:40-111)         */
+/* POD: eye[3] p00[3] du[3] dv[3] view_dir[3] ortho                            */
+/* ------------------------------------------------------------------------ */
+static v3 vld(const float* p) { return V(p[0], p[1], p[2]); }
+static void vst(float* p, v3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+
+void vpo_camera_perspective(int w, int h, const float* from, const float* at, const float* up, float vfov, float* pod) {
+    float theta = vfov * (VPO_PI / 180.0f);
+    float hh = tanf(0.5f * theta);
+    v3 lf = vld(from), la = vld(at), vu = vld(up);
+    v3 W = vnormalize(vsub(lf, la));
+    v3 U = vnormalize(vcross(vu, W));
+    v3 Vv = vcross(W, U);
+    v3 diff = vsub(lf, la);
+    float focal = sqrtf(vdot(diff, diff));
+    float aspect = (float)w * vpo_rcp((float)h);
+    float vh = 2.0f * hh * focal;
+    float vw = vh * aspect;
+    v3 vpu = vscale(U, vw);
+    v3 vpv = vscale(V(-Vv.x, -Vv.y, -Vv.z), vh);
+    v3 pdu = vscale(vpu, vpo_rcp((float)w));
+    v3 pdv = vscale(vpv, vpo_rcp((float)h));
+    /* midpoint(a, b) = lerp(a, b, 0.5) = fma(0.5, b - a, a) (math.h:350-366) */
+    v3 mid_uv = vfma(0.5f, vsub(vpv, vpu), vpu);
+    v3 ul = vsub(vsub(lf, vscale(W, focal)), mid_uv);
+    v3 mid_d = vfma(0.5f, vsub(pdv, pdu), pdu);
+    v3 p00 = vadd(ul, mid_d);
+    memset(pod, 0, 16 * sizeof(float));
+    vst(pod, lf);
+    vst(pod + 3, vsub(p00, lf));
+    vst(pod + 6, pdu);
+    vst(pod + 9, pdv);
+    vst(pod + 12, V(0.0f, 0.0f, 1.0f));
+    pod[15] = 0.0f;
+}
+
+void vpo_camera_orthographic(int w, int h, const float* from, const float* at, const float* up, float ortho_h, float* pod) {
+    v3 lf = vld(from), la = vld(at), vu = vld(up);
+    v3 vd = vnormalize(vsub(la, lf));
+    v3 right = vnormalize(vcross(vd, vu));
+    v3 upv = vcross(right, vd);
+    float aspect = (float)w * vpo_rcp((float)h);
+    float vh = ortho_h;
+    float vw = vh * aspect;
+    v3 vpu = vscale(right, vw);
+    v3 vpv = vscale(upv, vh);
+    v3 pdu = vscale(vpu, vpo_rcp((float)w));
+    v3 pdv = vscale(vpv, vpo_rcp((float)h));
+    v3 ul = vsub(lf, vfma(0.5f, vsub(vpv, vpu), vpu));
+    v3 p00 = vadd(ul, vfma(0.5f, vsub(pdv, pdu), pdu));
+    memset(pod, 0, 16 * sizeof(float));
+    vst(pod, lf);
+    vst(pod + 3, vsub(p00, lf));
+    vst(pod + 6, pdu);
+    vst(pod + 9, pdv);
+    vst(pod + 12, vd);
+    pod[15] = 1.0f;
+}
+
+static void cam_ray(const float* pod, uint32_t px, uint32_t py, float jx, float jy, v3* o, v3* d) {
+    float x = (float)px + jx, y = (float)py + jy;
+    v3 r = vfma(x, vld(pod + 6), vfma(y, vld(pod + 9), vld(pod + 3)));
+    if (pod[15] != 0.0f) {
+        *o = vadd(vld(pod), r);
+        *d = vnormalize(vld(pod + 12));
+    } else {
+        *o = vld(pod);
+        *d = vnormalize(r);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* the estimator (raygen.cuh:21-243, path_stages.cuh:25-226,                  */
+/* sampling.cuh:322-583)                                                      */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    uint32_t width, height, spp, seed;
+    uint32_t max_depth, rr_depth;
+    float rr_max_survival, firefly_clamp, filter_stddev, phase_g;
+    uint32_t use_ris, ris_candidates, active_cap, measure;
+    uint32_t x0, y0, x1, y1;         /* pixel crop [x0,x1) x [y0,y1) */
+    uint32_t sample_begin, sample_end; /* samples [begin, end) of spp */
+} vpo_params;
+
+typedef struct {
+    const prim_t* prims;
+    long n;
+    env_t env;
+    const float* cam;
+    const vpo_params* p;
+    hg_t hg;
+    float* out_mean;
+    float* out_samples;
+    uint32_t next_row;
+    pthread_mutex_t mu;
+    counters_t total;
+} job_t;
+
+/* set insert with the reference's CompactSet contract (drop at capacity) */
+static inline void set_push(uint32_t* s, int* ns, int cap, uint32_t v, counters_t* k) {
+    if (*ns < cap) s[(*ns)++] = v;
+    else k->active_overflow++;
+}
+
+static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, uint32_t* buf, counters_t* k) {
+    const vpo_params* P = J->p;
+    const prim_t* prims = J->prims;
+    const long n = J->n;
+    const int cap = (int)P->active_cap;
+    uint32_t* act = buf;
+    uint32_t* nxt = buf + cap;
+
+    /* raygen.cuh:82-99 */
+    uint64_t pix = (uint64_t)py * P->width + px;
+    uint64_t seq = hash64(pix * P->spp + sidx);
+    pcg_t rng = pcg_init(P->seed, seq);
+    uint64_t key = path_key(P->seed, seq);
+    float jx, jy;
+    if (P->filter_stddev > 0.0f) {
+        float u1 = vpo_max(next_u(&rng), 1e-7f);
+        float u2 = next_u(&rng);
+        float r = P->filter_stddev * sqrtf(-2.0f * vpo_logf(u1));
+        float s, c;
+        vpo_sincosf(2.0f * VPO_PI * u2, &s, &c);
+        jx = vpo_clamp(r * c, -2.0f, 2.0f);
+        jy = vpo_clamp(r * s, -2.0f, 2.0f);
+    } else {
+        jx = next_u(&rng) - 0.5f;
+        jy = next_u(&rng) - 0.5f;
+    }
+    v3 o, d;
+    cam_ray(J->cam, px, py, jx, jy, &o, &d);
+    k->n_sample++;
+
+    v3 beta = V(1.0f, 1.0f, 1.0f), L = V(0.0f, 0.0f, 0.0f);
+
+    /* bounce-0 origin-inside set (sampling.cuh:322-341) */
+    int na = 0;
+    uint32_t overlap = 0;
+    for (long i = 0; i < n; ++i)
+        if (point_inside(prims + i, o)) {
+            ++overlap;
+            set_push(act, &na, cap, (uint32_t)i, k);
+        }
+    if (P->measure && overlap > k->max_overlap) k->max_overlap = overlap;
+
+    for (uint32_t bounce = 0; bounce < P->max_depth; ++bounce) {
+        /* argmin over entered primitives and the origin set (sampling.cuh:350-415) */
+        float ts_min = INF_F;
+        uint32_t hits = 0;
+        k->n_trace++;
+        for (long j = 0; j < n; ++j) {
+            const prim_t* pr = prims + j;
+            float wl2;
+            float tj = entry_t(pr, o, d, &wl2);
+            if (tj < 0.0f) continue;
+            ++hits;
+            k->n_arg_hit++;
+            float tau = ff_tau(prim_u(key, bounce, 1u, (uint32_t)j));
+            float ts = inv_cdf_span(pr, o, d, tj, tau);
+            if (ts >= 0.0f && ts < tj) ts = tj;
+            if (ts >= 0.0f && ts < ts_min) {
+                k->n_arg_accept++;
+                float te = exit_entry(pr, o, d, tj, wl2);
+                if (ts <= te) ts_min = ts;
+            }
+        }
+        for (int s = 0; s < na; ++s) {
+            const prim_t* pr = prims + act[s];
+            k->n_arg_active++;
+            float te = exit_inside(pr, o, d);
+            float tau = ff_tau(prim_u(key, bounce, 0u, act[s]));
+            float ts = inv_cdf(pr, o, d, tau);
+            if (ts >= 0.0f && ts < ts_min && ts <= te) ts_min = ts;
+        }
+        if (P->measure && hits > k->max_hits) k->max_hits = hits;
+
+        /* analytic direct term at bounce 0 (raygen.cuh:148-160) */
+        if (bounce == 0) {
+            k->n_shadow_rays++;
+            float T = transmittance(prims, n, o, d, act, na, k);
+            v3 env = env_lookup(&J->env, d);
+            L = vadd(L, vmul(vscale(beta, T), env));
+        }
+        if (ts_min >= INF_F) break; /* escape: nothing to add with NEE + analytic direct */
+
+        /* rebuild the active set at the scatter point (sampling.cuh:422-458) */
+        int nn = 0;
+        {
+            int ia = 0;
+            for (long j = 0; j < n; ++j) {
+                int in_act = 0;
+                while (ia < na && act[ia] < (uint32_t)j) ++ia;
+                if (ia < na && act[ia] == (uint32_t)j) in_act = 1;
+                const prim_t* pr = prims + j;
+                int keep = 0;
+                if (in_act) {
+                    k->n_rb_active++;
+                    if (ts_min <= exit_inside(pr, o, d)) keep = 1;
+                }
+                if (!keep) {
+                    float wl2;
+                    float tj = entry_t(pr, o, d, &wl2);
+                    if (tj >= 0.0f && tj <= ts_min) {
+                        k->n_rb_hit++;
+                        if (ts_min <= exit_entry(pr, o, d, tj, wl2)) keep = 1;
+                    }
+                }
+                if (keep) set_push(nxt, &nn, cap, (uint32_t)j, k);
+            }
+        }
+        { uint32_t* t = act; act = nxt; nxt = t; na = nn; }
+
+        v3 x = vadd(o, vscale(d, ts_min));
+        if (P->measure) {
+            uint32_t ov = 0;
+            for (long i = 0; i < n; ++i) ov += (uint32_t)point_inside(prims + i, x);
+            if (ov > k->max_overlap) k->max_overlap = ov;
+        }
+        float phase_pdf;
+        v3 wo = hg_sample(&J->hg, d, &rng, &phase_pdf);
+        k->n_scatter++;
+
+        /* evaluate_albedo (sampling.cuh:257-283) */
+        v3 aa = V(0.0f, 0.0f, 0.0f);
+        float aw = 0.0f;
+        for (int s = 0; s < na; ++s) {
+            const prim_t* pr = prims + act[s];
+            k->n_alb++;
+            float wgt = pr->mass * prim_pdf(pr, x);
+            aa = vadd(aa, vscale(V(pr->ar, pr->ag, pr->ab), wgt));
+            aw += wgt;
+        }
+        v3 albedo = aw <= 0.0f ? V(0.0f, 0.0f, 0.0f) : vscale(aa, vpo_rcp(aw));
+
+        /* accumulate_nee_direct (path_stages.cuh:148-226) */
+        v3 base = vmul(beta, albedo);
+        if (P->use_ris) {
+            float wsum = 0.0f;
+            v3 ydir = V(0, 0, 0), yenv = V(0, 0, 0);
+            for (uint32_t c = 0; c < P->ris_candidates; ++c) {
+                float cpdf;
+                v3 cwo = env_sample(&J->env, &rng, &cpdf);
+                if (cpdf <= 0.0f) continue;
+                v3 envc = env_lookup(&J->env, cwo);
+                float lumc = vdot(envc, V(0.2126f, 0.7152f, 0.0722f));
+                float wgt = hg_eval_cos(&J->hg, vdot(d, cwo)) * lumc * vpo_rcp(cpdf);
+                if (!(wgt > 0.0f)) continue;
+                wsum += wgt;
+                if (next_u(&rng) * wsum < wgt) { ydir = cwo; yenv = envc; }
+            }
+            if (wsum > 0.0f) {
+                float lumy = vdot(yenv, V(0.2126f, 0.7152f, 0.0722f));
+                if (lumy > 0.0f) {
+                    k->n_shadow_rays++;
+                    float T = transmittance(prims, n, x, ydir, act, na, k);
+                    float Wr = wsum * vpo_rcp((float)P->ris_candidates);
+                    L = vadd(L, vscale(vscale(vmul(base, vscale(yenv, vpo_rcp(lumy))), T), Wr));
+                }
+            }
+        } else {
+            float apdf;
+            v3 awo = hg_sample(&J->hg, d, &rng, &apdf);
+            float pb_a = env_pdf(&J->env, awo);
+            float sa = apdf + pb_a;
+            float wa = sa > 0.0f ? apdf / sa : 0.0f;
+            v3 enva = env_lookup(&J->env, awo);
+            k->n_shadow_rays++;
+            float Ta = transmittance(prims, n, x, awo, act, na, k);
+            L = vadd(L, vscale(vscale(vmul(base, enva), Ta), wa));
+
+            float bpdf;
+            v3 bwo = env_sample(&J->env, &rng, &bpdf);
+            float ph_b = hg_eval_cos(&J->hg, vdot(d, bwo));
+            float sb = bpdf + ph_b;
+            float wb = sb > 0.0f ? bpdf / sb : 0.0f;
+            v3 envb = env_lookup(&J->env, bwo);
+            k->n_shadow_rays++;
+            float Tb = transmittance(prims, n, x, bwo, act, na, k);
+            L = vadd(L, vscale(vscale(vscale(vscale(vmul(base, envb), Tb), wb), ph_b), vpo_rcp(bpdf)));
+        }
+        beta = vmul(beta, albedo);
+
+        /* russian_roulette (path_stages.cuh:25-37) */
+        if (bounce >= P->rr_depth) {
+            float ps = vpo_min(P->rr_max_survival, vmax3(beta));
+            if (next_u(&rng) > ps || ps <= 0.0f) break;
+            beta = vscale(beta, vpo_rcp(ps));
+        }
+        if (vmax3(beta) < 1e-4f) break;
+        {
+            float sb = beta.x + beta.y + beta.z;
+            if (!((vpo_f2u(sb) & 0x7f800000u) != 0x7f800000u)) break;
+        }
+        o = x;
+        d = wo;
+    }
+
+    /* finalize_sample (path_stages.cuh:52-62) */
+    if (P->firefly_clamp > 0.0f) {
+        float lum = vdot(L, V(0.2126f, 0.7152f, 0.0722f));
+        if (lum > P->firefly_clamp) L = vscale(L, P->firefly_clamp * vpo_rcp(lum));
+    }
+    {
+        float s = L.x + L.y + L.z;
+        if (!((vpo_f2u(s) & 0x7f800000u) != 0x7f800000u)) L = V(0.0f, 0.0f, 0.0f);
+    }
+    return L;
+}
+
+static void counters_add(counters_t* a, const counters_t* b) {
+    uint64_t* pa = (uint64_t*)a;
+    const uint64_t* pb = (const uint64_t*)b;
+    const int nsum = 13; /* fields before max_hits */
+    for (int i = 0; i < nsum; ++i) pa[i] += pb[i];
+    if (b->max_hits > a->max_hits) a->max_hits = b->max_hits;
+    if (b->max_overlap > a->max_overlap) a->max_overlap = b->max_overlap;
+}
+
+static void* worker(void* arg) {
+    job_t* J = (job_t*)arg;
+    const vpo_params* P = J->p;
+    uint32_t* buf = (uint32_t*)malloc(sizeof(uint32_t) * 2 * (P->active_cap ? P->active_cap : 1));
+    counters_t k;
+    memset(&k, 0, sizeof(k));
+    for (;;) {
+        pthread_mutex_lock(&J->mu);
+        uint32_t y = J->next_row++;
+        pthread_mutex_unlock(&J->mu);
+        if (y >= P->y1) break;
+        for (uint32_t x = P->x0; x < P->x1; ++x) {
+            /* Welford mean over samples in order (raygen.cuh:214-219) */
+            v3 mean = V(0, 0, 0);
+            for (uint32_t s = P->sample_begin; s < P->sample_end; ++s) {
+                v3 Ls = trace_sample(J, x, y, s, buf, &k);
+                if (J->out_samples) {
+                    size_t idx = (((size_t)(s - P->sample_begin) * (P->y1 - P->y0) + (y - P->y0)) * (P->x1 - P->x0) + (x - P->x0)) * 4;
+                    J->out_samples[idx] = Ls.x;
+                    J->out_samples[idx + 1] = Ls.y;
+                    J->out_samples[idx + 2] = Ls.z;
+                    J->out_samples[idx + 3] = 1.0f;
+                }
+                float ninv = vpo_rcp((float)(s - P->sample_begin + 1));
+                mean = vadd(mean, vscale(vsub(Ls, mean), ninv));
+            }
+            if (J->out_mean) {
+                size_t idx = ((size_t)(y - P->y0) * (P->x1 - P->x0) + (x - P->x0)) * 4;
+                J->out_mean[idx] = mean.x;
+                J->out_mean[idx + 1] = mean.y;
+                J->out_mean[idx + 2] = mean.z;
+                J->out_mean[idx + 3] = 1.0f;
+            }
+        }
+    }
+    pthread_mutex_lock(&J->mu);
+    counters_add(&J->total, &k);
+    pthread_mutex_unlock(&J->mu);
+    free(buf);
+    return NULL;
+}
+
+/* prims16: records in scene order (the GPU's sorted order); env arrays from
+ * vpo_env_tables (prob may be NULL for the uniform fallback); counters: 15
+ * uint64 in counters_t order.  Returns 0 on success. */
+int vpo_render(const float* prims16, long n, const float* env_rgba, const float* joint, const float* prob,
+               const uint32_t* alias, float env_total, uint32_t ew, uint32_t eh, const float* cam_pod,
+               const vpo_params* p, float* out_mean, float* out_samples, uint64_t* counters, int threads) {
+    if (p->active_cap == 0 || p->x1 > p->width || p->y1 > p->height || p->x0 >= p->x1 || p->y0 >= p->y1 ||
+        p->sample_end > p->spp || p->sample_begin >= p->sample_end || ew == 0 || eh == 0)
+        return 1;
+    job_t J;
+    memset(&J, 0, sizeof(J));
+    J.prims = (const prim_t*)prims16;
+    J.n = n;
+    J.env.tex = env_rgba;
+    J.env.joint = joint;
+    J.env.prob = prob;
+    J.env.alias = alias;
+    J.env.total = env_total;
+    J.env.w = ew;
+    J.env.h = eh;
+    J.cam = cam_pod;
+    J.p = p;
+    J.hg = hg_make(p->phase_g);
+    J.out_mean = out_mean;
+    J.out_samples = out_samples;
+    J.next_row = p->y0;
+    pthread_mutex_init(&J.mu, NULL);
+    if (threads < 1) threads = 1;
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)threads);
+    for (int t = 0; t < threads; ++t) pthread_create(&th[t], NULL, worker, &J);
+    for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+    free(th);
+    pthread_mutex_destroy(&J.mu);
+    if (counters) memcpy(counters, &J.total, sizeof(counters_t));
+    return 0;
+}

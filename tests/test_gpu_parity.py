@@ -1,0 +1,205 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Integer and index work (Morton codes, order, overlap lists) and, thanks to
+the deterministic FP32 math on both sides, the floating-point results too are
+compared bit for bit.  Each case prints its name and sizes before touching
+the GPU.
+"""
+import numpy as np
+import pytest
+
+import vpo
+from volprim_amd import scenes, volprim
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def assert_bitwise(gpu, ref, what):
+    g, r = np.asarray(gpu, np.float32), np.asarray(ref, np.float32)
+    assert g.shape == r.shape, what
+    both_nan = np.isnan(g) & np.isnan(r)
+    diff = (bits(g) != bits(r)) & ~both_nan
+    if diff.any():
+        i = np.flatnonzero(diff.reshape(-1))[:5]
+        raise AssertionError(f"{what}: {int(diff.sum())} of {g.size} differ; first {i}: gpu={g.reshape(-1)[i]} "
+                             f"ref={r.reshape(-1)[i]}")
+
+
+MATH_CASES = {
+    "exp": lambda rng, n: (rng.uniform(-110, 90, n), None),
+    "log": lambda rng, n: (np.exp(rng.uniform(-103, 88, n)), None),
+    "sin": lambda rng, n: (rng.uniform(-30, 30, n), None),
+    "cos": lambda rng, n: (rng.uniform(-30, 30, n), None),
+    "atan2": lambda rng, n: (rng.normal(size=n), rng.normal(size=n)),
+    "acos": lambda rng, n: (rng.uniform(-1, 1, n), None),
+    "erf": lambda rng, n: (rng.uniform(-6, 6, n), None),
+    "erfc": lambda rng, n: (rng.uniform(-4, 11, n), None),
+    "erfinv": lambda rng, n: (rng.uniform(-1, 1, n), None),
+    "erf_diff": lambda rng, n: tuple(np.sort(rng.uniform(-6, 6, (2, n)), axis=0)),
+    "sqrt": lambda rng, n: (np.exp(rng.uniform(-80, 80, n)), None),
+    "div": lambda rng, n: (rng.normal(size=n), rng.normal(size=n)),
+}
+SPECIALS = np.array([0.0, -0.0, 1.0, -1.0, 0.5, 1e-30, -1e-30, 1e-40, 88.7, -87.5, -103.9, 3.92, 10.1, 10.2,
+                     0.9999999, -0.9999999, np.inf, -np.inf, np.nan], np.float32)
+
+
+@pytest.mark.parametrize("name", sorted(MATH_CASES))
+def test_math_bitexact(ctx, name):
+    rng = np.random.default_rng(abs(hash(name)) % 2**32)
+    x, y = MATH_CASES[name](rng, 200_000)
+    x = np.concatenate([np.float32(x), SPECIALS]).astype(np.float32)
+    y = np.concatenate([np.float32(y), np.flip(SPECIALS)]).astype(np.float32) if y is not None else None
+    print(f"[case] math {name} n={x.size}", flush=True)
+    assert_bitwise(ctx.math_eval(name, x, y), vpo.math_eval(name, x, y), f"math {name}")
+
+
+def _random_rays(rng, m, spread=2.0):
+    o = rng.uniform(-spread, spread, (m, 3))
+    d = rng.normal(size=(m, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.concatenate([o, d], 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_prim_functions_bitexact(ctx, seed):
+    rng = np.random.default_rng(seed)
+    rows = scenes.random_scene(1, seed + 10)
+    rec = vpo.make_prims(rows)[0]
+    m = 50_000
+    rays = _random_rays(rng, m, 1.0)
+    t0 = rng.uniform(0, 2, m).astype(np.float32)
+    t1 = (t0 + rng.uniform(0, 2, m)).astype(np.float32)
+    tau = rng.exponential(1.0, m).astype(np.float32)
+    args = np.concatenate([rays, t0[:, None], t1[:, None], tau[:, None]], 1).astype(np.float32)
+    for fn in volprim.PRIM_FNS:
+        print(f"[case] prim {fn} seed={seed} m={m}", flush=True)
+        assert_bitwise(ctx.prim_eval(fn, rec, args), vpo.prim_eval(fn, rec, args), f"prim {fn}")
+
+
+@pytest.mark.parametrize("n,seed", [(1, 0), (2, 1), (7, 2), (300, 3), (4096, 4)])
+def test_scene_build_matches_oracle(ctx, n, seed):
+    rows = scenes.random_scene(n, seed)
+    if n == 4096:
+        rows = scenes.c1_scattering()
+    print(f"[case] scene build n={n}", flush=True)
+    sc = ctx.scene(rows)
+    order, morton, rec, depth = sc.export()
+    sc.close()
+    codes = vpo.morton_codes(rows[:, 0:3])
+    assert np.array_equal(morton, codes), "Morton codes differ from the CPU restatement"
+    assert np.array_equal(order, vpo.sort_order(codes)), "stable sort order differs"
+    assert_bitwise(rec, vpo.make_prims(rows)[order], "primitive records")
+    assert depth < 64
+
+
+def test_scene_build_duplicate_codes(ctx):
+    # collinear stack of identical Gaussians (SG_CLUSTER_MODE=stress): all Morton codes tie
+    rows = np.repeat(scenes.c0_single_gaussian(albedo=0.5), 33, axis=0)
+    print("[case] scene build duplicates n=33", flush=True)
+    sc = ctx.scene(rows)
+    order, morton, rec, depth = sc.export()
+    sc.close()
+    assert np.array_equal(order, np.arange(33, dtype=np.uint32))
+
+
+@pytest.mark.parametrize("n,seed", [(1, 0), (60, 1), (500, 2)])
+def test_overlap_lists_bitexact(ctx, n, seed):
+    rng = np.random.default_rng(100 + seed)
+    rows = scenes.random_scene(n, seed)
+    sc = ctx.scene(rows)
+    order, _, rec, _ = sc.export()
+    rays = _random_rays(rng, 4000, 1.5)
+    cap = 512
+    print(f"[case] overlaps n={n} rays={rays.shape[0]}", flush=True)
+    cg, tg, ig = ctx.collect_overlaps(sc, rays, cap)
+    sc.close()
+    co, to, io = vpo.overlaps(rec, rays, cap)
+    assert np.array_equal(cg, co), "entry counts differ"
+    assert cg.max() < cap
+    for r in range(rays.shape[0]):
+        k = int(cg[r])
+        assert np.array_equal(ig[r, :k], io[r, :k]), f"ray {r}: indices differ"
+        assert_bitwise(tg[r, :k], to[r, :k], f"ray {r} entry t")
+
+
+@pytest.mark.parametrize("n,seed", [(1, 0), (60, 1), (500, 2)])
+def test_transmittance_bitexact(ctx, n, seed):
+    rng = np.random.default_rng(200 + seed)
+    rows = scenes.random_scene(n, seed)
+    sc = ctx.scene(rows)
+    _, _, rec, _ = sc.export()
+    rays = _random_rays(rng, 4000, 1.5)
+    print(f"[case] transmittance n={n} rays={rays.shape[0]}", flush=True)
+    Tg = ctx.transmittance_rays(sc, rays)
+    sc.close()
+    assert_bitwise(Tg, vpo.transmittance_rays(rec, rays), "transmittance")
+
+
+def _render_both(ctx, rows, env, cam, **kw):
+    sc = ctx.scene(rows)
+    ev = ctx.env(env)
+    _, _, rec, _ = sc.export()
+    p = volprim.default_params(**kw)
+    img, st = ctx.render(sc, ev, cam, p)
+    sc.close()
+    ev.close()
+    ref, _, k = vpo.render(rec, env, cam.pod(), p.width, p.height, p.spp, seed=p.seed, max_depth=p.max_depth,
+                           rr_depth=p.rr_depth, rr_max_survival=p.rr_max_survival, firefly_clamp=p.firefly_clamp,
+                           filter_stddev=p.filter_stddev, phase_g=p.phase_g, use_ris=bool(p.use_ris),
+                           ris_candidates=p.ris_candidates, active_cap=p.active_cap)
+    return img, ref, st, k
+
+
+RENDER_CASES = {
+    "c0_absorb": dict(rows=lambda: scenes.c0_single_gaussian(), env="const", w=32, h=32, spp=4, kw={}),
+    "c0_scatter": dict(rows=lambda: scenes.c0_single_gaussian(albedo=0.9), env="const", w=24, h=24, spp=4, kw={}),
+    "mis_sky": dict(rows=lambda: scenes.random_scene(48, 5), env="sky", w=24, h=20, spp=3, kw={}),
+    "ris_sky": dict(rows=lambda: scenes.random_scene(48, 6), env="sky", w=20, h=20, spp=3,
+                    kw=dict(use_ris=1, ris_candidates=4)),
+    "iso_filter_clamp": dict(rows=lambda: scenes.random_scene(40, 7), env="sky", w=20, h=16, spp=3,
+                             kw=dict(phase_g=0.0, filter_stddev=0.7, firefly_clamp=5.0, rr_depth=2)),
+    "small_cap": dict(rows=lambda: scenes.random_scene(80, 8, smax=0.5), env="const", w=16, h=16, spp=2,
+                      kw=dict(active_cap=2)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(RENDER_CASES))
+def test_render_bitexact(ctx, case):
+    c = RENDER_CASES[case]
+    env = scenes.constant_env() if c["env"] == "const" else scenes.sky_env(64, 32, sun_radius_deg=4.0, sun_scale=50.0)
+    cam = volprim.camera_perspective(**scenes.default_camera_args(c["w"], c["h"]))
+    print(f"[case] render {case} {c['w']}x{c['h']} spp={c['spp']}", flush=True)
+    img, ref, st, k = _render_both(ctx, c["rows"](), env, cam, width=c["w"], height=c["h"], spp=c["spp"], **c["kw"])
+    assert_bitwise(img, ref, f"render {case}")
+    assert st["active_overflow"] == k["active_overflow"]
+
+
+def test_render_orthographic_bitexact(ctx):
+    rows = scenes.random_scene(30, 9)
+    cam = volprim.camera_orthographic(20, 20, np.array([0, 0, -5], np.float32), np.zeros(3, np.float32),
+                                      np.array([0, 1, 0], np.float32), 3.0)
+    print("[case] render ortho 20x20", flush=True)
+    img, ref, _, _ = _render_both(ctx, rows, scenes.constant_env(), cam, width=20, height=20, spp=2)
+    assert_bitwise(img, ref, "render ortho")
+
+
+def test_furnace_gpu(ctx):
+    # albedo 1 medium in a constant white environment reproduces the field exactly:
+    # |mean - 1| < max(5 SEM, 2e-3)  (experiments/mitsuba-reference/furnace_check.py:36-40)
+    rows = scenes.random_scene(64, 11, albedo=1.0)
+    rows[:, 10:13] = 1.0
+    sc = ctx.scene(rows)
+    ev = ctx.env(scenes.constant_env())
+    cam = volprim.camera_perspective(**scenes.default_camera_args(64, 64))
+    p = volprim.default_params(width=64, height=64, spp=64, max_depth=512, rr_depth=512, phase_g=0.3)
+    print("[case] furnace 64x64 spp=64", flush=True)
+    img, _ = ctx.render(sc, ev, cam, p)
+    sc.close()
+    ev.close()
+    m = img[..., 0:3].mean()
+    sem = img[..., 0:3].std() / np.sqrt(img[..., 0:3].size)
+    assert abs(m - 1.0) < max(5 * sem, 2e-3), (m, sem)
