@@ -165,8 +165,9 @@ vp_status run_jobs(vp_context* ctx, std::vector<JobRun>& runs, const vp_launch_c
             return fail(VP_ERR_OOM, "cannot allocate " + std::to_string(max_slice_bytes) + " bytes of sample slots");
     }
 
-    VP_HIP_TRY(hipEventRecord(ctx->ev0, s));
     uint64_t total_paths_all = 0;
+    double render_ms = 0.0;
+    uint32_t launches = 0;
     std::vector<vp::JobDesc> descs;
     std::vector<uint64_t> begins;
     std::vector<size_t> idx;
@@ -203,8 +204,10 @@ vp_status run_jobs(vp_context* ctx, std::vector<JobRun>& runs, const vp_launch_c
                                   hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctx->path_counter, 1u, 0ull);
         VP_HIP_TRY(hipGetLastError());
+        VP_HIP_TRY(hipEventRecord(ctx->ev0, s));
         VP_HIP_TRY(vp_launch_render(ctx, ctx->jobs_dev, ctx->job_begin_dev, static_cast<uint32_t>(descs.size()),
                                     total, L, s));
+        VP_HIP_TRY(hipEventRecord(ctx->ev1, s));
         for (size_t k = 0; k < descs.size(); ++k) {
             VP_HIP_TRY(vp_launch_welford(descs[k], runs[idx[k]].mean, descs[k].s_begin, s));
             runs[idx[k]].done += descs[k].s_count;
@@ -212,12 +215,13 @@ vp_status run_jobs(vp_context* ctx, std::vector<JobRun>& runs, const vp_launch_c
         total_paths_all += total;
         // descs/begins are reused next round: wait for the copies to be consumed
         VP_HIP_TRY(hipStreamSynchronize(s));
+        float ms = 0.0f;
+        VP_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        render_ms += ms;
+        ++launches;
     }
-    VP_HIP_TRY(hipEventRecord(ctx->ev1, s));
-    VP_HIP_TRY(hipEventSynchronize(ctx->ev1));
-    float ms = 0.0f;
-    VP_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    if (kernel_ms) *kernel_ms = ms;
+    if (kernel_ms) *kernel_ms = render_ms;
+    ctx->last_launches = launches;
     if (paths) *paths = total_paths_all;
     return VP_OK;
 }
@@ -560,9 +564,35 @@ vp_status vp_render_device(vp_context* ctx, vp_scene* sc, vp_env* env, const vp_
     if (st != VP_OK) return st;
     if (stats) {
         stats->kernel_ms = kms;
+        stats->launches = ctx->last_launches;
         stats->paths = static_cast<uint64_t>(params->width) * params->height * params->spp;
         stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
+    return VP_OK;
+}
+
+vp_status vp_render_count_events(vp_context* ctx, vp_scene* sc, vp_env* env, const vp_camera* cam,
+                                 const vp_render_params* params, uint64_t* events, uint32_t n_events) {
+    if (!ctx || !sc || !env || !cam || !events) return fail(VP_ERR_INVALID, "null argument");
+    vp_status st = check_params(params);
+    if (st != VP_OK) return st;
+    VP_HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const size_t bytes = static_cast<size_t>(params->width) * params->height * sizeof(float4);
+    Dev mean;
+    if (hipMalloc(&mean.p, bytes) != hipSuccess) return fail(VP_ERR_OOM, "cannot allocate the image");
+    std::vector<JobRun> runs(1);
+    runs[0].desc = make_job(sc, env, cam, params);
+    runs[0].mean = static_cast<float4*>(mean.p);
+    st = reset_counters(ctx, s);
+    if (st != VP_OK) return st;
+    st = run_jobs(ctx, runs, nullptr, params->active_cap ? params->active_cap : 128u, 1,
+                  params->measure_caps ? 1 : 0, s, nullptr, nullptr);
+    if (st != VP_OK) return st;
+    unsigned long long c[vp::kNumCounters];
+    VP_HIP_TRY(hipMemcpyAsync(c, ctx->counters, sizeof(c), hipMemcpyDeviceToHost, s));
+    VP_HIP_TRY(hipStreamSynchronize(s));
+    for (uint32_t i = 0; i < n_events && i < static_cast<uint32_t>(vp::kNumCounters); ++i) events[i] = c[i];
     return VP_OK;
 }
 
@@ -631,6 +661,7 @@ vp_status vp_render_jobs(vp_context* ctx, vp_job* jobs, uint32_t njobs, const vp
     if (st != VP_OK) return st;
     if (stats) {
         stats->kernel_ms = kms;
+        stats->launches = ctx->last_launches;
         stats->paths = samples;
         stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }

@@ -101,6 +101,7 @@ struct vp_context {
     uint64_t* job_begin_dev = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::mutex mu;
+    uint32_t last_launches = 0;
     vp_launch_config tuned{};
     bool have_tuned = false;
 };

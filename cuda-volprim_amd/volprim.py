@@ -53,7 +53,8 @@ class RenderParams(ctypes.Structure):
 class RenderStats(ctypes.Structure):
     _fields_ = [("active_overflow", ctypes.c_uint64), ("hit_overflow", ctypes.c_uint64),
                 ("max_hits_per_ray", ctypes.c_uint32), ("max_point_overlap", ctypes.c_uint32),
-                ("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double), ("paths", ctypes.c_uint64)]
+                ("kernel_ms", ctypes.c_double), ("launches", ctypes.c_uint32), ("wall_ms", ctypes.c_double),
+                ("paths", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -90,8 +91,11 @@ EXPORTED = [
     "vp_camera_perspective", "vp_camera_orthographic", "vp_default_params", "vp_render",
     "vp_render_device", "vp_normalize_accumulator", "vp_render_jobs", "vp_autotune", "vp_server_create",
     "vp_server_submit", "vp_server_poll", "vp_server_drain", "vp_server_stats_get", "vp_server_destroy",
-    "vp_transmittance_rays", "vp_collect_overlaps", "vp_math_eval", "vp_prim_eval",
+    "vp_transmittance_rays", "vp_collect_overlaps", "vp_render_count_events", "vp_math_eval", "vp_prim_eval",
 ]
+EVENT_NAMES = ["active_overflow", "max_hits", "max_overlap", "paths", "traces", "scatters", "shadow_rays",
+               "arg_hit", "arg_active", "arg_accept", "rb_active", "rb_hit", "alb", "sh_active", "sh_hit",
+               "leaf_tests", "node_visits", "contain"]
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -139,6 +143,8 @@ def load():
     lib.vp_server_destroy.argtypes = [_vp]
     lib.vp_transmittance_rays.argtypes = [_vp, _vp, _f32p, ctypes.c_uint32, _f32p]
     lib.vp_collect_overlaps.argtypes = [_vp, _vp, _f32p, ctypes.c_uint32, ctypes.c_uint32, _u32p, _f32p, _u32p]
+    lib.vp_render_count_events.argtypes = [_vp, _vp, _vp, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams),
+                                           np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS"), ctypes.c_uint32]
     lib.vp_math_eval.argtypes = [_vp, ctypes.c_int, _f32p, _f32p, ctypes.c_uint32, _f32p]
     lib.vp_prim_eval.argtypes = [_vp, ctypes.c_int, _f32p, _f32p, ctypes.c_uint32, _f32p]
     _LIB = lib
@@ -300,6 +306,12 @@ class Context:
         _check(load().vp_render_jobs(self.handle, arr, len(jobs), ctypes.byref(launch) if launch else None,
                                      ctypes.byref(st)), "vp_render_jobs")
         return outs, st.as_dict()
+
+    def count_events(self, scene, env, cam, params):
+        ev = np.zeros(len(EVENT_NAMES), np.uint64)
+        _check(load().vp_render_count_events(self.handle, scene.handle, env.handle, ctypes.byref(cam),
+                                             ctypes.byref(params), ev, len(EVENT_NAMES)), "vp_render_count_events")
+        return dict(zip(EVENT_NAMES, [int(v) for v in ev]))
 
     def transmittance_rays(self, scene, rays6):
         rays6 = f32(rays6).reshape(-1, 6)

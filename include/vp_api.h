@@ -93,7 +93,8 @@ typedef struct {
     uint64_t hit_overflow;      /* always 0: entry hits are streamed, never dropped */
     uint32_t max_hits_per_ray;  /* --measure-caps: max entries on one traced ray */
     uint32_t max_point_overlap; /* --measure-caps: max primitives containing a vertex */
-    double kernel_ms;           /* GPU time of the render kernels */
+    double kernel_ms;           /* GPU time of the render-kernel launches (HIP events) */
+    uint32_t launches;          /* render-kernel launches the call made */
     double wall_ms;             /* host wall time of the call */
     uint64_t paths;             /* camera paths traced */
 } vp_render_stats;
@@ -187,6 +188,14 @@ vp_status vp_transmittance_rays(vp_context* ctx, vp_scene* scene, const float* r
  * idx[m*cap] (stored indices). */
 vp_status vp_collect_overlaps(vp_context* ctx, vp_scene* scene, const float* rays, uint32_t m, uint32_t cap,
                               uint32_t* counts, float* t, uint32_t* idx);
+/* Render once with the event-counting kernel variant and return the event
+ * counters (n_events <= 18, order of the kernel's counter slots: active
+ * overflow, max hits, max overlap, paths, traces, scatters, shadow rays,
+ * argmin hits, argmin origin-set terms, argmin accepts, rebuild origin-set
+ * tests, rebuild entry tests, albedo terms, shadow origin-set chords, shadow
+ * entered chords, leaf entry tests, node visits, containment tests). */
+vp_status vp_render_count_events(vp_context* ctx, vp_scene* scene, vp_env* env, const vp_camera* cam,
+                                 const vp_render_params* params, uint64_t* events, uint32_t n_events);
 /* Evaluate math function fn (0 exp 1 log 2 sin 3 cos 4 atan2 5 acos 6 erf
  * 7 erfc 8 erfinv 9 erf_diff 10 sqrt 11 div) on the device. */
 vp_status vp_math_eval(vp_context* ctx, int fn, const float* x, const float* y, uint32_t n, float* out);

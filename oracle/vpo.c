@@ -759,6 +759,7 @@ typedef struct {
     uint32_t use_ris, ris_candidates, active_cap, measure;
     uint32_t x0, y0, x1, y1;         /* pixel crop [x0,x1) x [y0,y1) */
     uint32_t sample_begin, sample_end; /* samples [begin, end) of spp */
+    uint32_t stride;                   /* render every stride-th row and column of the crop (0 = 1) */
 } vpo_params;
 
 typedef struct {
@@ -1007,7 +1008,9 @@ static void* worker(void* arg) {
         uint32_t y = J->next_row++;
         pthread_mutex_unlock(&J->mu);
         if (y >= P->y1) break;
-        for (uint32_t x = P->x0; x < P->x1; ++x) {
+        const uint32_t st = P->stride ? P->stride : 1u;
+        if ((y - P->y0) % st) continue;
+        for (uint32_t x = P->x0; x < P->x1; x += st) {
             /* Welford mean over samples in order (raygen.cuh:214-219) */
             v3 mean = V(0, 0, 0);
             for (uint32_t s = P->sample_begin; s < P->sample_end; ++s) {
