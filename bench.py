@@ -93,6 +93,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--block", type=int, default=0, help="threads per workgroup (0 = library default)")
+    ap.add_argument("--bpc", type=int, default=0, help="persistent workgroups per CU (0 = occupancy limit)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is in DESIGN.md",
@@ -111,15 +113,18 @@ def main():
     cam = volprim.camera_perspective(**cam_args)
     params = volprim.default_params(**pp)
     out = torch.empty((pp["height"], pp["width"], 4), dtype=torch.float32, device="cuda")
+    launch = None
+    if args.block or args.bpc:
+        launch = volprim.LaunchConfig(args.block, args.bpc, 0, 0)
     torch.cuda.synchronize()
 
     for _ in range(args.warmup):
-        ctx.render_device(scene, envh, cam, params, out.data_ptr())
+        ctx.render_device(scene, envh, cam, params, out.data_ptr(), launch=launch)
     torch.cuda.synchronize()
     kernel_ms, launches = 0.0, 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        st = ctx.render_device(scene, envh, cam, params, out.data_ptr())
+        st = ctx.render_device(scene, envh, cam, params, out.data_ptr(), launch=launch)
         kernel_ms += st["kernel_ms"]
         launches += st["launches"]
     torch.cuda.synchronize()

@@ -104,6 +104,88 @@ __device__ __forceinline__ void walk_point(const SceneView& s, f3 x, uint32_t& v
     }
 }
 
+// Leaf queue depth per lane.  The walks below postpone leaves into a per-lane
+// LDS queue (column `lane` of a [kLeafQ][blockDim] array) and keep walking
+// until every lane of the wavefront has a full queue or has finished; then all
+// lanes process their queued leaves together.  Node steps stay convergent and
+// the expensive leaf work runs with most lanes active (the "while-while"
+// scheme of Aila and Laine 2009, with speculative postponing).
+#ifndef VP_LEAF_Q
+#define VP_LEAF_Q 16
+#endif
+constexpr uint32_t kLeafQ = VP_LEAF_Q;
+
+struct LeafQueue {
+    uint32_t* col;  // &lds[0][lane]
+    uint32_t stride;  // blockDim.x
+    __device__ __forceinline__ uint32_t& at(uint32_t k) const { return col[k * stride]; }
+};
+
+template <typename Leaf>
+__device__ __forceinline__ void walk_ray_q(const SceneView& s, f3 o, f3 inv, const float& tmax, uint32_t& visits,
+                                           const LeafQueue& q, Leaf&& leaf) {
+    uint32_t i = 0;
+    const uint32_t nn = s.n_nodes;
+    while (true) {
+        uint32_t cnt = 0;
+        while (i < nn && cnt < kLeafQ) {
+            const float4 a = s.nodes[2 * i];
+            const float4 b = s.nodes[2 * i + 1];
+            ++visits;
+            const float tx0 = (a.x - o.x) * inv.x, tx1 = (b.x - o.x) * inv.x;
+            const float ty0 = (a.y - o.y) * inv.y, ty1 = (b.y - o.y) * inv.y;
+            const float tz0 = (a.z - o.z) * inv.z, tz1 = (b.z - o.z) * inv.z;
+            const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+            const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+            const uint32_t skip = __float_as_uint(a.w);
+            const uint32_t prim = __float_as_uint(b.w);
+            if (tnear <= tfar) {
+                if (prim != kInnerNode) {
+                    q.at(cnt++) = prim;
+                    i = skip;
+                } else {
+                    i = i + 1;
+                }
+            } else {
+                i = skip;
+            }
+        }
+        for (uint32_t k = 0; k < cnt; ++k)
+            if (!leaf(q.at(k))) return;
+        if (i >= nn) return;
+    }
+}
+
+template <typename Leaf>
+__device__ __forceinline__ void walk_point_q(const SceneView& s, f3 x, uint32_t& visits, const LeafQueue& q,
+                                             Leaf&& leaf) {
+    uint32_t i = 0;
+    const uint32_t nn = s.n_nodes;
+    while (true) {
+        uint32_t cnt = 0;
+        while (i < nn && cnt < kLeafQ) {
+            const float4 a = s.nodes[2 * i];
+            const float4 b = s.nodes[2 * i + 1];
+            ++visits;
+            const bool in = x.x >= a.x && x.x <= b.x && x.y >= a.y && x.y <= b.y && x.z >= a.z && x.z <= b.z;
+            const uint32_t skip = __float_as_uint(a.w);
+            const uint32_t prim = __float_as_uint(b.w);
+            if (in) {
+                if (prim != kInnerNode) {
+                    q.at(cnt++) = prim;
+                    i = skip;
+                } else {
+                    i = i + 1;
+                }
+            } else {
+                i = skip;
+            }
+        }
+        for (uint32_t k = 0; k < cnt; ++k) leaf(q.at(k));
+        if (i >= nn) return;
+    }
+}
+
 // Add one chord to the fixed-point optical depth; true once the total
 // reaches 88 (exp(-88) underflows), which is order independent because every
 // term is non-negative.
@@ -146,7 +228,7 @@ struct Ev {
 // any-hit integration (anyhit.cuh:53-66).
 template <bool kCount>
 __device__ float transmittance(const SceneView& s, f3 o, f3 d, const uint32_t* act, uint32_t na, uint32_t stride,
-                               Ev<kCount>& ev) {
+                               const LeafQueue& lq, Ev<kCount>& ev) {
     uint64_t acc = 0;
     for (uint32_t k = 0; k < na; ++k) {
         const PrimRecord pr = load_prim(s.prims, act[k * stride]);
@@ -159,7 +241,7 @@ __device__ float transmittance(const SceneView& s, f3 o, f3 d, const uint32_t* a
     const float tmax = kTMax;
     bool full = false;
     uint32_t visits = 0, leaves = 0, sh = 0;
-    walk_ray(s, o, inv, tmax, visits, [&](uint32_t prim) -> bool {
+    walk_ray_q(s, o, inv, tmax, visits, lq, [&](uint32_t prim) -> bool {
         const PrimRecord pr = load_prim(s.prims, prim);
         ++leaves;
         float wl2;
@@ -210,15 +292,22 @@ struct PathState {
     uint32_t bounce, job, na, cur, slot;
 };
 
+// Minimum waves per SIMD the register allocator targets for the megakernel.
+#ifndef VP_RENDER_WAVES_PER_EU
+#define VP_RENDER_WAVES_PER_EU 6
+#endif
+
 template <bool kCount>
-__global__ void __launch_bounds__(256) k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begin,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VP_RENDER_WAVES_PER_EU))) k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begin,
                                                 uint32_t njobs, uint64_t total_paths,
                                                 unsigned long long* __restrict__ path_counter,
                                                 uint32_t* __restrict__ slab, uint32_t cap, uint32_t stride,
                                                 unsigned long long* __restrict__ counters, int measure) {
+    extern __shared__ uint32_t s_leafq[];  // [kLeafQ][blockDim.x]
     const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t* my = slab + gl;  // entry k of buffer b at my[(b * cap + k) * stride]
+    const LeafQueue lq{s_leafq + threadIdx.x, blockDim.x};
     PathState st;
     bool alive = false;
     bool exhausted = false;
@@ -325,7 +414,7 @@ __global__ void __launch_bounds__(256) k_render(const JobDesc* __restrict__ jobs
             const f3 inv = safe_inv(d);
             float tmax = measure ? kTMax : ts;
             uint32_t hits = 0, visits = 0, leaves = 0, accept = 0;
-            walk_ray(sc, o, inv, tmax, visits, [&](uint32_t prim) -> bool {
+            walk_ray_q(sc, o, inv, tmax, visits, lq, [&](uint32_t prim) -> bool {
                 const PrimRecord pr = load_prim(sc.prims, prim);
                 ++leaves;
                 float wl2;
@@ -356,7 +445,7 @@ __global__ void __launch_bounds__(256) k_render(const JobDesc* __restrict__ jobs
         // analytic direct term at bounce 0 (raygen.cuh:148-160)
         if (b == 0) {
             ev.add(kCntShadowRays);
-            const float T = transmittance(sc, o, d, act, na, stride, ev);
+            const float T = transmittance(sc, o, d, act, na, stride, lq, ev);
             st.L = st.L + (st.beta * T) * env_lookup(env, d);
         }
 
@@ -370,7 +459,7 @@ __global__ void __launch_bounds__(256) k_render(const JobDesc* __restrict__ jobs
                 if (nn < cap) nxt[nn * stride] = v, ++nn;
                 else ++overflow;
             };
-            walk_point(sc, x, visits, [&](uint32_t c) {
+            walk_point_q(sc, x, visits, lq, [&](uint32_t c) {
                 while (ia < na) {
                     const uint32_t a = act[ia * stride];
                     if (a >= c) break;
@@ -451,7 +540,7 @@ __global__ void __launch_bounds__(256) k_render(const JobDesc* __restrict__ jobs
                     const float lumy = dot(yenv, luma);
                     if (lumy > 0.0f) {
                         ev.add(kCntShadowRays);
-                        const float T = transmittance(sc, x, ydir, act, nn, stride, ev);
+                        const float T = transmittance(sc, x, ydir, act, nn, stride, lq, ev);
                         const float Wr = wsum * rcpf(static_cast<float>(J.ris_candidates));
                         st.L = st.L + ((base * (yenv * rcpf(lumy))) * T) * Wr;
                     }
@@ -464,7 +553,7 @@ __global__ void __launch_bounds__(256) k_render(const JobDesc* __restrict__ jobs
                 const float wa = sa > 0.0f ? apdf / sa : 0.0f;
                 const f3 enva = env_lookup(env, awo);
                 ev.add(kCntShadowRays);
-                const float Ta = transmittance(sc, x, awo, act, nn, stride, ev);
+                const float Ta = transmittance(sc, x, awo, act, nn, stride, lq, ev);
                 st.L = st.L + ((base * enva) * Ta) * wa;
 
                 float bpdf;
@@ -474,7 +563,7 @@ __global__ void __launch_bounds__(256) k_render(const JobDesc* __restrict__ jobs
                 const float wb = sb > 0.0f ? bpdf / sb : 0.0f;
                 const f3 envb = env_lookup(env, bwo);
                 ev.add(kCntShadowRays);
-                const float Tb = transmittance(sc, x, bwo, act, nn, stride, ev);
+                const float Tb = transmittance(sc, x, bwo, act, nn, stride, lq, ev);
                 st.L = st.L + (((((base * envb) * Tb) * wb) * ph_b) * rcpf(bpdf));
             }
             st.beta = st.beta * albedo;
@@ -665,11 +754,12 @@ using namespace vp;
 hipError_t vp_launch_render(vp_context* ctx, const JobDesc* jobs_dev, const uint64_t* job_begin_dev, uint32_t njobs,
                             uint64_t total_paths, const RenderLaunch& L, hipStream_t s) {
     const uint32_t stride = L.grid * L.block;
+    const size_t lds = sizeof(uint32_t) * kLeafQ * L.block;
     if (L.count_events)
-        hipLaunchKernelGGL(k_render<true>, dim3(L.grid), dim3(L.block), 0, s, jobs_dev, job_begin_dev, njobs,
+        hipLaunchKernelGGL(k_render<true>, dim3(L.grid), dim3(L.block), lds, s, jobs_dev, job_begin_dev, njobs,
                            total_paths, ctx->path_counter, ctx->slab, L.active_cap, stride, ctx->counters, L.measure);
     else
-        hipLaunchKernelGGL(k_render<false>, dim3(L.grid), dim3(L.block), 0, s, jobs_dev, job_begin_dev, njobs,
+        hipLaunchKernelGGL(k_render<false>, dim3(L.grid), dim3(L.block), lds, s, jobs_dev, job_begin_dev, njobs,
                            total_paths, ctx->path_counter, ctx->slab, L.active_cap, stride, ctx->counters, L.measure);
     return hipGetLastError();
 }
@@ -677,10 +767,11 @@ hipError_t vp_launch_render(vp_context* ctx, const JobDesc* jobs_dev, const uint
 int vp_render_occupancy(uint32_t block, uint32_t, int count_events, int* blocks_per_cu) {
     int nb = 0;
     hipError_t e;
+    const size_t lds = sizeof(uint32_t) * kLeafQ * block;
     if (count_events)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(k_render<true>), block, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(k_render<true>), block, lds);
     else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(k_render<false>), block, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(k_render<false>), block, lds);
     *blocks_per_cu = nb > 0 ? nb : 1;
     return e == hipSuccess ? 0 : -1;
 }

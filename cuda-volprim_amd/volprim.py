@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libvolprim.so")
+# VP_LIBRARY selects another build of the same library (used for sweeps).
+LIB_PATH = os.environ.get("VP_LIBRARY", os.path.join(HERE, "libvolprim.so"))
 
 VP_OK = 0
 _STATUS = {0: "VP_OK", 1: "VP_ERR_INVALID", 2: "VP_ERR_HIP", 3: "VP_ERR_OOM", 4: "VP_ERR_NOT_READY",
