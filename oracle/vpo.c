@@ -1077,3 +1077,41 @@ int vpo_render(const float* prims16, long n, const float* env_rgba, const float*
     if (counters) memcpy(counters, &J.total, sizeof(counters_t));
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* test access to the phase function and the environment sampler            */
+/* ------------------------------------------------------------------------ */
+void vpo_phase_eval(float g, const float* wi, const float* wo, long n, float* out) {
+    hg_t h = hg_make(g);
+    for (long i = 0; i < n; ++i) out[i] = hg_eval_cos(&h, vdot(vld(wi + 3 * i), vld(wo + 3 * i)));
+}
+void vpo_phase_sample(float g, const float* wi, uint64_t seed, long n, float* wo, float* pdf) {
+    hg_t h = hg_make(g);
+    pcg_t r = pcg_init(seed, 7u);
+    for (long i = 0; i < n; ++i) {
+        v3 o = hg_sample(&h, vld(wi + 3 * i), &r, pdf + i);
+        vst(wo + 3 * i, o);
+    }
+}
+void vpo_env_sample(const float* rgba, const float* joint, const float* prob, const uint32_t* alias, float total,
+                    uint32_t w, uint32_t h, uint64_t seed, long n, float* wo, float* pdf) {
+    env_t e = {rgba, prob, alias, joint, w, h, total};
+    pcg_t r = pcg_init(seed, 11u);
+    for (long i = 0; i < n; ++i) vst(wo + 3 * i, env_sample(&e, &r, pdf + i));
+}
+void vpo_env_pdf(const float* rgba, const float* joint, const float* prob, const uint32_t* alias, float total,
+                 uint32_t w, uint32_t h, const float* wo, long n, float* pdf) {
+    env_t e = {rgba, prob, alias, joint, w, h, total};
+    for (long i = 0; i < n; ++i) pdf[i] = env_pdf(&e, vld(wo + 3 * i));
+}
+void vpo_env_lookup(const float* rgba, uint32_t w, uint32_t h, const float* dirs, long n, float* rgb) {
+    env_t e = {rgba, NULL, NULL, NULL, w, h, 0.0f};
+    for (long i = 0; i < n; ++i) vst(rgb + 3 * i, env_lookup(&e, vld(dirs + 3 * i)));
+}
+void vpo_camera_ray(const float* pod, uint32_t px, uint32_t py, float jx, float jy, float* o, float* d) {
+    v3 oo, dd;
+    cam_ray(pod, px, py, jx, jy, &oo, &dd);
+    vst(o, oo);
+    vst(d, dd);
+}
+float vpo_prim_uniform(uint64_t key, uint32_t bounce, uint32_t role, uint32_t prim) { return prim_u(key, bounce, role, prim); }

@@ -76,6 +76,17 @@ def load():
                                _f32p, ctypes.POINTER(Params), ctypes.c_void_p, ctypes.c_void_p,
                                _u64p, ctypes.c_int]
     lib.vpo_render.restype = ctypes.c_int
+    lib.vpo_phase_eval.argtypes = [ctypes.c_float, _f32p, _f32p, ctypes.c_long, _f32p]
+    lib.vpo_phase_sample.argtypes = [ctypes.c_float, _f32p, ctypes.c_uint64, ctypes.c_long, _f32p, _f32p]
+    lib.vpo_env_sample.argtypes = [_f32p, _f32p, _f32p, _u32p, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
+                                   ctypes.c_uint64, ctypes.c_long, _f32p, _f32p]
+    lib.vpo_env_pdf.argtypes = [_f32p, _f32p, _f32p, _u32p, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
+                                _f32p, ctypes.c_long, _f32p]
+    lib.vpo_env_lookup.argtypes = [_f32p, ctypes.c_uint32, ctypes.c_uint32, _f32p, ctypes.c_long, _f32p]
+    lib.vpo_camera_ray.argtypes = [_f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
+                                   _f32p, _f32p]
+    lib.vpo_prim_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+    lib.vpo_prim_uniform.restype = ctypes.c_float
     return lib
 
 
@@ -228,3 +239,51 @@ def render(prims16, env_rgba, cam_pod, width, height, spp, seed=42, max_depth=12
     if rc != 0:
         raise ValueError("vpo_render rejected its arguments")
     return mean, smp, dict(zip(COUNTER_NAMES, counters.tolist()))
+
+
+def phase_eval(g, wi, wo):
+    wi, wo = f32(wi).reshape(-1, 3), f32(wo).reshape(-1, 3)
+    out = np.empty(wi.shape[0], np.float32)
+    lib().vpo_phase_eval(g, wi, wo, wi.shape[0], out)
+    return out
+
+
+def phase_sample(g, wi, seed=1):
+    wi = f32(wi).reshape(-1, 3)
+    wo = np.empty_like(wi)
+    pdf = np.empty(wi.shape[0], np.float32)
+    lib().vpo_phase_sample(g, wi, seed, wi.shape[0], wo, pdf)
+    return wo, pdf
+
+
+def env_sample(rgba, tables, n, seed=1):
+    rgba = f32(rgba)
+    joint, prob, alias, total = tables
+    wo = np.empty((n, 3), np.float32)
+    pdf = np.empty(n, np.float32)
+    lib().vpo_env_sample(rgba.reshape(-1), joint, prob, alias, total, rgba.shape[1], rgba.shape[0], seed, n, wo, pdf)
+    return wo, pdf
+
+
+def env_pdf(rgba, tables, wo):
+    rgba = f32(rgba)
+    joint, prob, alias, total = tables
+    wo = f32(wo).reshape(-1, 3)
+    pdf = np.empty(wo.shape[0], np.float32)
+    lib().vpo_env_pdf(rgba.reshape(-1), joint, prob, alias, total, rgba.shape[1], rgba.shape[0], wo, wo.shape[0], pdf)
+    return pdf
+
+
+def env_lookup(rgba, dirs):
+    rgba = f32(rgba)
+    dirs = f32(dirs).reshape(-1, 3)
+    out = np.empty_like(dirs)
+    lib().vpo_env_lookup(rgba.reshape(-1), rgba.shape[1], rgba.shape[0], dirs, dirs.shape[0], out)
+    return out
+
+
+def camera_ray(pod, px, py, jx, jy):
+    o = np.empty(3, np.float32)
+    d = np.empty(3, np.float32)
+    lib().vpo_camera_ray(f32(pod), px, py, jx, jy, o, d)
+    return o, d

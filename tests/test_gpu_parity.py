@@ -203,3 +203,16 @@ def test_furnace_gpu(ctx):
     m = img[..., 0:3].mean()
     sem = img[..., 0:3].std() / np.sqrt(img[..., 0:3].size)
     assert abs(m - 1.0) < max(5 * sem, 2e-3), (m, sem)
+
+
+@pytest.mark.parametrize("case", ["a", "b"])
+def test_gpu_transmittance_vs_fp64_golden(ctx, case):
+    import os
+    G = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "golden_fp64.npz"))
+    rows, rays, Tg = G[f"rows_{case}"], G[f"rays_{case}"], G[f"T_{case}"]
+    print(f"[case] golden transmittance {case} n={rows.shape[0]} rays={rays.shape[0]}", flush=True)
+    sc = ctx.scene(rows)
+    T = ctx.transmittance_rays(sc, rays).astype(np.float64)
+    sc.close()
+    mask = Tg > 1e-30
+    assert (np.abs(T - Tg) / Tg)[mask].max() < 1e-5
