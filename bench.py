@@ -95,6 +95,7 @@ def main():
     ap.add_argument("--no-count", action="store_true")
     ap.add_argument("--block", type=int, default=0, help="threads per workgroup (0 = library default)")
     ap.add_argument("--bpc", type=int, default=0, help="persistent workgroups per CU (0 = occupancy limit)")
+    ap.add_argument("--regen", type=int, default=0, help="idle lanes per wavefront before claiming new paths")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is in DESIGN.md",
@@ -114,8 +115,8 @@ def main():
     params = volprim.default_params(**pp)
     out = torch.empty((pp["height"], pp["width"], 4), dtype=torch.float32, device="cuda")
     launch = None
-    if args.block or args.bpc:
-        launch = volprim.LaunchConfig(args.block, args.bpc, 0, 0)
+    if args.block or args.bpc or args.regen:
+        launch = volprim.LaunchConfig(args.block, args.bpc, 0, 0, args.regen)
     torch.cuda.synchronize()
 
     for _ in range(args.warmup):

@@ -25,8 +25,6 @@ struct Dev {
     }
 };
 
-constexpr size_t kSliceBudgetBytes = size_t(1) << 30;  // per-sample radiance bytes per launch
-
 vp_status fail(vp_status code, const std::string& msg) {
     g_error = msg;
     return code;
@@ -34,7 +32,7 @@ vp_status fail(vp_status code, const std::string& msg) {
 
 bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
 
-vp_status check_params(const vp_render_params* p) {
+vp_status check_params_impl(const vp_render_params* p) {
     if (!p) return fail(VP_ERR_INVALID, "params is null");
     if (p->width == 0 || p->height == 0) return fail(VP_ERR_INVALID, "width and height must be positive");
     if (static_cast<uint64_t>(p->width) * p->height > (1ull << 28))
@@ -50,7 +48,7 @@ vp_status check_params(const vp_render_params* p) {
     return VP_OK;
 }
 
-vp::JobDesc make_job(const vp_scene* sc, const vp_env* env, const vp_camera* cam, const vp_render_params* p) {
+vp::JobDesc make_job_impl(const vp_scene* sc, const vp_env* env, const vp_camera* cam, const vp_render_params* p) {
     vp::JobDesc J{};
     J.scene = sc->view();
     J.env = env->view();
@@ -78,171 +76,6 @@ vp::JobDesc make_job(const vp_scene* sc, const vp_env* env, const vp_camera* cam
     const uint32_t tiles_y = (p->height + 7) / 8;
     J.npad = J.tiles_x * tiles_y * 64u;
     return J;
-}
-
-__global__ void k_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-vp_status ensure_slab(vp_context* ctx, size_t words) {
-    if (ctx->slab_words >= words) return VP_OK;
-    if (ctx->slab) (void)hipFree(ctx->slab);
-    ctx->slab = nullptr;
-    ctx->slab_words = 0;
-    if (hipMalloc(&ctx->slab, words * sizeof(uint32_t)) != hipSuccess)
-        return fail(VP_ERR_OOM, "cannot allocate the active-set slab (" + std::to_string(words * 4) + " bytes)");
-    ctx->slab_words = words;
-    return VP_OK;
-}
-
-vp_status ensure_jobs(vp_context* ctx, uint32_t n) {
-    if (ctx->jobs_cap >= n) return VP_OK;
-    if (ctx->jobs_dev) (void)hipFree(ctx->jobs_dev);
-    if (ctx->job_begin_dev) (void)hipFree(ctx->job_begin_dev);
-    ctx->jobs_dev = nullptr;
-    ctx->job_begin_dev = nullptr;
-    ctx->jobs_cap = 0;
-    VP_HIP_TRY(hipMalloc(&ctx->jobs_dev, sizeof(vp::JobDesc) * n));
-    VP_HIP_TRY(hipMalloc(&ctx->job_begin_dev, sizeof(uint64_t) * n));
-    ctx->jobs_cap = n;
-    return VP_OK;
-}
-
-RenderLaunch resolve_launch(vp_context* ctx, const vp_launch_config* cfg, uint32_t active_cap, int count, int measure) {
-    vp_launch_config c{};
-    if (cfg) c = *cfg;
-    else if (ctx->have_tuned) c = ctx->tuned;
-    RenderLaunch L{};
-    L.block = c.block ? c.block : 256u;
-    L.block = std::max<uint32_t>(64u, std::min<uint32_t>(1024u, L.block / 64u * 64u));
-    int occ = 1;
-    vp_render_occupancy(L.block, active_cap, count, &occ);
-    uint32_t bpc = c.blocks_per_cu ? std::min<uint32_t>(c.blocks_per_cu, static_cast<uint32_t>(occ))
-                                   : static_cast<uint32_t>(occ);
-    if (bpc == 0) bpc = 1;
-    L.grid = static_cast<uint32_t>(ctx->prop.multiProcessorCount) * bpc;
-    L.active_cap = active_cap;
-    L.count_events = count;
-    L.measure = measure;
-    return L;
-}
-
-// Render the job list in rounds.  Each round gives every unfinished job its
-// next slice of samples (bounded by kSliceBudgetBytes in total), runs one
-// persistent launch over all slices and folds every slice into its job's
-// Welford mean.  Used by vp_render_device (one job) and vp_render_jobs.
-struct JobRun {
-    vp::JobDesc desc;
-    float4* mean = nullptr;  // device W*H
-    float4* samples_all = nullptr;  // optional caller buffer spp*W*H
-    uint32_t done = 0;       // samples folded so far
-    uint32_t slice_spp = 0;  // samples per round
-};
-
-vp_status run_jobs(vp_context* ctx, std::vector<JobRun>& runs, const vp_launch_config* cfg, uint32_t active_cap,
-                   int count, int measure, hipStream_t s, double* kernel_ms, uint64_t* paths) {
-    const RenderLaunch L = resolve_launch(ctx, cfg, active_cap, count, measure);
-    const size_t lanes = static_cast<size_t>(L.grid) * L.block;
-    vp_status st = ensure_slab(ctx, 2 * static_cast<size_t>(active_cap) * lanes);
-    if (st != VP_OK) return st;
-    st = ensure_jobs(ctx, static_cast<uint32_t>(runs.size()));
-    if (st != VP_OK) return st;
-
-    // slice sizes: spread the per-launch sample budget over the jobs
-    size_t total_pix = 0;
-    for (auto& r : runs) total_pix += static_cast<size_t>(r.desc.width) * r.desc.height;
-    const size_t budget_samples = std::max<size_t>(1, kSliceBudgetBytes / (16 * std::max<size_t>(total_pix, 1)));
-    size_t max_slice_bytes = 0;
-    for (auto& r : runs) {
-        if (r.slice_spp == 0) r.slice_spp = static_cast<uint32_t>(std::min<size_t>(r.desc.spp, budget_samples));
-        if (!r.samples_all)
-            max_slice_bytes += static_cast<size_t>(r.slice_spp) * r.desc.width * r.desc.height * sizeof(float4);
-    }
-    Dev slice_buf;
-    if (max_slice_bytes) {
-        if (hipMalloc(&slice_buf.p, max_slice_bytes) != hipSuccess)
-            return fail(VP_ERR_OOM, "cannot allocate " + std::to_string(max_slice_bytes) + " bytes of sample slots");
-    }
-
-    uint64_t total_paths_all = 0;
-    double render_ms = 0.0;
-    uint32_t launches = 0;
-    std::vector<vp::JobDesc> descs;
-    std::vector<uint64_t> begins;
-    std::vector<size_t> idx;
-    while (true) {
-        descs.clear();
-        begins.clear();
-        idx.clear();
-        uint64_t total = 0;
-        size_t off = 0;
-        for (size_t i = 0; i < runs.size(); ++i) {
-            JobRun& r = runs[i];
-            if (r.done >= r.desc.spp) continue;
-            vp::JobDesc J = r.desc;
-            J.s_begin = r.done;
-            J.s_count = std::min(r.slice_spp, r.desc.spp - r.done);
-            J.path_begin = total;
-            J.path_count = static_cast<uint64_t>(J.npad) * J.s_count;
-            const size_t npix = static_cast<size_t>(J.width) * J.height;
-            if (r.samples_all) {
-                J.samples = r.samples_all + static_cast<size_t>(J.s_begin) * npix;
-            } else {
-                J.samples = reinterpret_cast<float4*>(static_cast<char*>(slice_buf.p) + off);
-                off += static_cast<size_t>(J.s_count) * npix * sizeof(float4);
-            }
-            total += J.path_count;
-            descs.push_back(J);
-            begins.push_back(J.path_begin);
-            idx.push_back(i);
-        }
-        if (descs.empty()) break;
-        VP_HIP_TRY(hipMemcpyAsync(ctx->jobs_dev, descs.data(), sizeof(vp::JobDesc) * descs.size(),
-                                  hipMemcpyHostToDevice, s));
-        VP_HIP_TRY(hipMemcpyAsync(ctx->job_begin_dev, begins.data(), sizeof(uint64_t) * begins.size(),
-                                  hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctx->path_counter, 1u, 0ull);
-        VP_HIP_TRY(hipGetLastError());
-        VP_HIP_TRY(hipEventRecord(ctx->ev0, s));
-        VP_HIP_TRY(vp_launch_render(ctx, ctx->jobs_dev, ctx->job_begin_dev, static_cast<uint32_t>(descs.size()),
-                                    total, L, s));
-        VP_HIP_TRY(hipEventRecord(ctx->ev1, s));
-        for (size_t k = 0; k < descs.size(); ++k) {
-            VP_HIP_TRY(vp_launch_welford(descs[k], runs[idx[k]].mean, descs[k].s_begin, s));
-            runs[idx[k]].done += descs[k].s_count;
-        }
-        total_paths_all += total;
-        // descs/begins are reused next round: wait for the copies to be consumed
-        VP_HIP_TRY(hipStreamSynchronize(s));
-        float ms = 0.0f;
-        VP_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        render_ms += ms;
-        ++launches;
-    }
-    if (kernel_ms) *kernel_ms = render_ms;
-    ctx->last_launches = launches;
-    if (paths) *paths = total_paths_all;
-    return VP_OK;
-}
-
-vp_status read_stats(vp_context* ctx, hipStream_t s, vp_render_stats* stats) {
-    unsigned long long c[vp::kNumCounters];
-    VP_HIP_TRY(hipMemcpyAsync(c, ctx->counters, sizeof(c), hipMemcpyDeviceToHost, s));
-    VP_HIP_TRY(hipStreamSynchronize(s));
-    if (stats) {
-        stats->active_overflow = c[vp::kCntActiveOverflow];
-        stats->hit_overflow = 0;
-        stats->max_hits_per_ray = static_cast<uint32_t>(c[vp::kCntMaxHits]);
-        stats->max_point_overlap = static_cast<uint32_t>(c[vp::kCntMaxOverlap]);
-    }
-    return VP_OK;
-}
-
-vp_status reset_counters(vp_context* ctx, hipStream_t s) {
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctx->counters, static_cast<uint32_t>(vp::kNumCounters), 0ull);
-    VP_HIP_TRY(hipGetLastError());
-    return VP_OK;
 }
 
 // Vose alias table over p_i = joint_i / sum(joint), double precision,
@@ -287,6 +120,10 @@ void build_alias(const std::vector<float>& joint, std::vector<vp::AliasEntry>& o
 }  // namespace
 
 void vp_set_error(const std::string& msg) { g_error = msg; }
+vp_status vp_check_params(const vp_render_params* p) { return check_params_impl(p); }
+vp::JobDesc vp_make_job(const vp_scene* sc, const vp_env* env, const vp_camera* cam, const vp_render_params* p) {
+    return make_job_impl(sc, env, cam, p);
+}
 
 extern "C" {
 
@@ -303,6 +140,8 @@ vp_status vp_context_create(int device, vp_context** out) {
     if (hipGetDeviceProperties(&ctx->prop, device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
         hipMalloc(&ctx->counters, sizeof(unsigned long long) * vp::kNumCounters) != hipSuccess ||
         hipMalloc(&ctx->path_counter, sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&ctx->launch_desc, sizeof(uint64_t) * 2) != hipSuccess ||
+        hipHostMalloc(&ctx->desc_pinned, sizeof(uint64_t) * 2, hipHostMallocDefault) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
         vp_context_destroy(ctx);
         return fail(VP_ERR_HIP, "context setup failed");
@@ -318,8 +157,13 @@ vp_status vp_context_destroy(vp_context* ctx) {
     if (ctx->slab) (void)hipFree(ctx->slab);
     if (ctx->counters) (void)hipFree(ctx->counters);
     if (ctx->path_counter) (void)hipFree(ctx->path_counter);
+    vp_drop_graph(ctx);
     if (ctx->jobs_dev) (void)hipFree(ctx->jobs_dev);
     if (ctx->job_begin_dev) (void)hipFree(ctx->job_begin_dev);
+    if (ctx->launch_desc) (void)hipFree(ctx->launch_desc);
+    if (ctx->jobs_pinned) (void)hipHostFree(ctx->jobs_pinned);
+    if (ctx->begin_pinned) (void)hipHostFree(ctx->begin_pinned);
+    if (ctx->desc_pinned) (void)hipHostFree(ctx->desc_pinned);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -543,24 +387,24 @@ vp_status vp_render_device(vp_context* ctx, vp_scene* sc, vp_env* env, const vp_
                            const vp_render_params* params, const vp_launch_config* launch, float* out_rgba_dev,
                            float* out_samples_dev, void* stream, vp_render_stats* stats) {
     if (!ctx || !sc || !env || !cam || !out_rgba_dev) return fail(VP_ERR_INVALID, "null argument");
-    vp_status st = check_params(params);
+    vp_status st = vp_check_params(params);
     if (st != VP_OK) return st;
     const auto t0 = std::chrono::steady_clock::now();
     VP_HIP_TRY(hipSetDevice(ctx->device));
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
     const uint32_t cap = params->active_cap ? params->active_cap : 128u;
     std::vector<JobRun> runs(1);
-    runs[0].desc = make_job(sc, env, cam, params);
+    runs[0].desc = vp_make_job(sc, env, cam, params);
     runs[0].mean = reinterpret_cast<float4*>(out_rgba_dev);
     runs[0].samples_all = reinterpret_cast<float4*>(out_samples_dev);
     if (launch && launch->batch_spp) runs[0].slice_spp = launch->batch_spp;
-    st = reset_counters(ctx, s);
+    st = vp_reset_counters(ctx, s);
     if (st != VP_OK) return st;
     double kms = 0.0;
     uint64_t paths = 0;
-    st = run_jobs(ctx, runs, launch, cap, 0, params->measure_caps ? 1 : 0, s, &kms, &paths);
+    st = vp_run_jobs(ctx, runs, launch, cap, 0, params->measure_caps ? 1 : 0, s, &kms, &paths);
     if (st != VP_OK) return st;
-    st = read_stats(ctx, s, stats);
+    st = vp_read_stats(ctx, s, stats);
     if (st != VP_OK) return st;
     if (stats) {
         stats->kernel_ms = kms;
@@ -574,7 +418,7 @@ vp_status vp_render_device(vp_context* ctx, vp_scene* sc, vp_env* env, const vp_
 vp_status vp_render_count_events(vp_context* ctx, vp_scene* sc, vp_env* env, const vp_camera* cam,
                                  const vp_render_params* params, uint64_t* events, uint32_t n_events) {
     if (!ctx || !sc || !env || !cam || !events) return fail(VP_ERR_INVALID, "null argument");
-    vp_status st = check_params(params);
+    vp_status st = vp_check_params(params);
     if (st != VP_OK) return st;
     VP_HIP_TRY(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
@@ -582,11 +426,11 @@ vp_status vp_render_count_events(vp_context* ctx, vp_scene* sc, vp_env* env, con
     Dev mean;
     if (hipMalloc(&mean.p, bytes) != hipSuccess) return fail(VP_ERR_OOM, "cannot allocate the image");
     std::vector<JobRun> runs(1);
-    runs[0].desc = make_job(sc, env, cam, params);
+    runs[0].desc = vp_make_job(sc, env, cam, params);
     runs[0].mean = static_cast<float4*>(mean.p);
-    st = reset_counters(ctx, s);
+    st = vp_reset_counters(ctx, s);
     if (st != VP_OK) return st;
-    st = run_jobs(ctx, runs, nullptr, params->active_cap ? params->active_cap : 128u, 1,
+    st = vp_run_jobs(ctx, runs, nullptr, params->active_cap ? params->active_cap : 128u, 1,
                   params->measure_caps ? 1 : 0, s, nullptr, nullptr);
     if (st != VP_OK) return st;
     unsigned long long c[vp::kNumCounters];
@@ -599,7 +443,7 @@ vp_status vp_render_count_events(vp_context* ctx, vp_scene* sc, vp_env* env, con
 vp_status vp_render(vp_context* ctx, vp_scene* sc, vp_env* env, const vp_camera* cam, const vp_render_params* params,
                     float* out_rgba, vp_render_stats* stats) {
     if (!ctx || !out_rgba) return fail(VP_ERR_INVALID, "null argument");
-    vp_status st = check_params(params);
+    vp_status st = vp_check_params(params);
     if (st != VP_OK) return st;
     const auto t0 = std::chrono::steady_clock::now();
     VP_HIP_TRY(hipSetDevice(ctx->device));
@@ -637,27 +481,27 @@ vp_status vp_render_jobs(vp_context* ctx, vp_job* jobs, uint32_t njobs, const vp
     for (uint32_t i = 0; i < njobs; ++i) {
         const vp_job& j = jobs[i];
         if (!j.scene || !j.env || !j.out_rgba) return fail(VP_ERR_INVALID, "job " + std::to_string(i) + ": null field");
-        vp_status st = check_params(&j.params);
+        vp_status st = vp_check_params(&j.params);
         if (st != VP_OK) return st;
         cap = std::max(cap, j.params.active_cap ? j.params.active_cap : 128u);
         measure |= j.params.measure_caps ? 1 : 0;
-        runs[i].desc = make_job(j.scene, j.env, &j.camera, &j.params);
+        runs[i].desc = vp_make_job(j.scene, j.env, &j.camera, &j.params);
         const size_t bytes = static_cast<size_t>(j.params.width) * j.params.height * sizeof(float4);
         if (hipMalloc(&means[i].p, bytes) != hipSuccess) return fail(VP_ERR_OOM, "cannot allocate job images");
         runs[i].mean = static_cast<float4*>(means[i].p);
         if (launch && launch->batch_spp) runs[i].slice_spp = launch->batch_spp;
         samples += static_cast<uint64_t>(j.params.width) * j.params.height * j.params.spp;
     }
-    vp_status st = reset_counters(ctx, ctx->stream);
+    vp_status st = vp_reset_counters(ctx, ctx->stream);
     if (st != VP_OK) return st;
     double kms = 0.0;
-    st = run_jobs(ctx, runs, launch, cap, 0, measure, ctx->stream, &kms, nullptr);
+    st = vp_run_jobs(ctx, runs, launch, cap, 0, measure, ctx->stream, &kms, nullptr);
     if (st != VP_OK) return st;
     for (uint32_t i = 0; i < njobs; ++i) {
         const size_t bytes = static_cast<size_t>(jobs[i].params.width) * jobs[i].params.height * sizeof(float4);
         VP_HIP_TRY(hipMemcpyAsync(jobs[i].out_rgba, means[i].p, bytes, hipMemcpyDeviceToHost, ctx->stream));
     }
-    st = read_stats(ctx, ctx->stream, stats);
+    st = vp_read_stats(ctx, ctx->stream, stats);
     if (st != VP_OK) return st;
     if (stats) {
         stats->kernel_ms = kms;

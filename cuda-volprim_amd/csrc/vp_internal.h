@@ -57,6 +57,7 @@ struct JobDesc {
     uint32_t s_begin, s_count;
     uint64_t path_begin, path_count;
     float4* samples;  // s_count * width * height radiance values of this slice
+    float4* mean;     // width * height Welford mean of the job
 };
 
 // Counter slots of the render kernel.
@@ -99,6 +100,15 @@ struct vp_context {
     vp::JobDesc* jobs_dev = nullptr;
     uint32_t jobs_cap = 0;
     uint64_t* job_begin_dev = nullptr;
+    uint64_t* launch_desc = nullptr;  // [0] job slices, [1] total paths
+    // pinned staging of the job table, the slice begins and the launch descriptor
+    vp::JobDesc* jobs_pinned = nullptr;
+    uint64_t* begin_pinned = nullptr;
+    uint64_t* desc_pinned = nullptr;
+    uint32_t pinned_cap = 0;
+    // captured step graph and the shape it was captured for
+    hipGraphExec_t graph_exec = nullptr;
+    uint64_t graph_key[6] = {0, 0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::mutex mu;
     uint32_t last_launches = 0;
@@ -150,12 +160,38 @@ struct RenderLaunch {
     uint32_t block;
     uint32_t grid;
     uint32_t active_cap;
+    uint32_t regen_min;
     int count_events;
     int measure;
 };
-hipError_t vp_launch_render(vp_context* ctx, const vp::JobDesc* jobs_dev, const uint64_t* job_begin_dev,
-                            uint32_t njobs, uint64_t total_paths, const RenderLaunch& L, hipStream_t s);
-hipError_t vp_launch_welford(const vp::JobDesc& job, float4* mean, uint32_t sample_base, hipStream_t s);
+hipError_t vp_launch_step(vp_context* ctx, const RenderLaunch& L, uint32_t max_jobs, uint32_t max_npix,
+                          hipStream_t s);
+
+// execution of job lists (vp_exec.cpp)
+constexpr size_t kSliceBudgetBytes = size_t(1) << 30;  // per-sample radiance bytes per step
+
+struct JobRun {
+    vp::JobDesc desc;
+    float4* mean = nullptr;         // device W*H Welford mean
+    float4* samples_all = nullptr;  // optional caller buffer spp*W*H
+    uint32_t done = 0;              // samples folded so far
+    uint32_t slice_spp = 0;         // samples per step (0 = chosen from the budget)
+};
+vp_status vp_ensure_slab(vp_context* ctx, size_t words);
+vp_status vp_ensure_jobs(vp_context* ctx, uint32_t n);
+void vp_drop_graph(vp_context* ctx);
+RenderLaunch vp_resolve_launch(vp_context* ctx, const vp_launch_config* cfg, uint32_t active_cap, int count,
+                               int measure);
+// One step over the first nslices entries of the pinned job table (capacity max_jobs).
+vp_status vp_run_step(vp_context* ctx, const RenderLaunch& L, uint32_t nslices, uint64_t total_paths,
+                      uint32_t max_jobs, uint32_t max_npix, bool use_graph, hipStream_t s);
+vp_status vp_run_jobs(vp_context* ctx, std::vector<JobRun>& runs, const vp_launch_config* cfg, uint32_t active_cap,
+                      int count, int measure, hipStream_t s, double* kernel_ms, uint64_t* paths);
+vp_status vp_read_stats(vp_context* ctx, hipStream_t s, vp_render_stats* stats);
+vp_status vp_reset_counters(vp_context* ctx, hipStream_t s);
+vp::JobDesc vp_make_job(const vp_scene* sc, const vp_env* env, const vp_camera* cam, const vp_render_params* p);
+vp_status vp_check_params(const vp_render_params* p);
+hipError_t vp_launch_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v, hipStream_t s);
 hipError_t vp_launch_normalize(float4* out, const float4* in, size_t n, float scale, hipStream_t s);
 hipError_t vp_launch_transmittance(vp::SceneView sc, const float* rays, uint32_t m, float* T, hipStream_t s);
 hipError_t vp_launch_overlaps(vp::SceneView sc, const float* rays, uint32_t m, uint32_t cap, uint32_t* counts,

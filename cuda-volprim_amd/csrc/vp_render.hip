@@ -297,13 +297,19 @@ struct PathState {
 #define VP_RENDER_WAVES_PER_EU 6
 #endif
 
+// launch_desc[0] = number of job slices, launch_desc[1] = total paths; read
+// from device memory so one captured graph serves every step of a job mix.
+// A wavefront claims new paths once at least regen_min of its lanes are idle
+// (or none is busy).
 template <bool kCount>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VP_RENDER_WAVES_PER_EU))) k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begin,
-                                                uint32_t njobs, uint64_t total_paths,
-                                                unsigned long long* __restrict__ path_counter,
-                                                uint32_t* __restrict__ slab, uint32_t cap, uint32_t stride,
-                                                unsigned long long* __restrict__ counters, int measure) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VP_RENDER_WAVES_PER_EU)))
+k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begin,
+         const uint64_t* __restrict__ launch_desc, unsigned long long* __restrict__ path_counter,
+         uint32_t* __restrict__ slab, uint32_t cap, uint32_t stride, uint32_t regen_min,
+         unsigned long long* __restrict__ counters, int measure) {
     extern __shared__ uint32_t s_leafq[];  // [kLeafQ][blockDim.x]
+    const uint32_t njobs = static_cast<uint32_t>(launch_desc[0]);
+    const uint64_t total_paths = launch_desc[1];
     const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t* my = slab + gl;  // entry k of buffer b at my[(b * cap + k) * stride]
@@ -318,7 +324,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VP_REN
     while (true) {
         // ---- claim new paths for idle lanes: one atomic per wavefront ----
         const bool want = !alive && !exhausted;
-        const unsigned long long wmask = __ballot(want);
+        unsigned long long wmask = __ballot(want);
+        if (wmask && __popcll(wmask) < regen_min && __ballot(alive) != 0) wmask = 0;
         if (wmask) {
             const uint32_t leader = __ffsll(static_cast<long long>(wmask)) - 1;
             unsigned long long base = 0;
@@ -600,20 +607,30 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VP_REN
     if (overflow) atomicAdd(&counters[kCntActiveOverflow], static_cast<unsigned long long>(overflow));
 }
 
-// Fold a slice of per-sample radiance into the Welford mean, in sample order
-// (raygen.cuh:214-238): n_inv = 1 / (samples before + 1).
-__global__ void k_welford(const float4* __restrict__ samples, uint32_t npix, uint32_t s_count, uint32_t sample_base,
-                          float4* __restrict__ mean) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npix) return;
-    float4 m = sample_base == 0 ? make_float4(0.0f, 0.0f, 0.0f, 1.0f) : mean[i];
-    f3 mu = f3{m.x, m.y, m.z};
-    for (uint32_t s = 0; s < s_count; ++s) {
-        const float4 v = samples[static_cast<size_t>(s) * npix + i];
-        const float ninv = rcpf(static_cast<float>(sample_base + s + 1));
-        mu = mu + (f3{v.x, v.y, v.z} - mu) * ninv;
+// Fold every job slice of the launch into its Welford mean, in sample order
+// (raygen.cuh:214-238): n_inv = 1 / (samples before + 1).  blockIdx.y picks
+// the job; blocks beyond the launch's job count or the job's pixels exit.
+__global__ void k_welford_all(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ launch_desc) {
+    const uint32_t j = blockIdx.y;
+    if (j >= static_cast<uint32_t>(launch_desc[0])) return;
+    const JobDesc& J = jobs[j];
+    const uint32_t npix = J.width * J.height;
+    float4* mean = J.mean;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x) {
+        const float4 m = J.s_begin == 0 ? make_float4(0.0f, 0.0f, 0.0f, 1.0f) : mean[i];
+        f3 mu = f3{m.x, m.y, m.z};
+        for (uint32_t s = 0; s < J.s_count; ++s) {
+            const float4 v = J.samples[static_cast<size_t>(s) * npix + i];
+            const float ninv = rcpf(static_cast<float>(J.s_begin + s + 1));
+            mu = mu + (f3{v.x, v.y, v.z} - mu) * ninv;
+        }
+        mean[i] = make_float4(mu.x, mu.y, mu.z, 1.0f);
     }
-    mean[i] = make_float4(mu.x, mu.y, mu.z, 1.0f);
+}
+
+__global__ void k_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // normalize_accumulator_kernel (device/kernels/normalize_accumulator.cu:16-35)
@@ -751,16 +768,30 @@ inline uint32_t cdiv(uint64_t a, uint32_t b) { return static_cast<uint32_t>((a +
 
 using namespace vp;
 
-hipError_t vp_launch_render(vp_context* ctx, const JobDesc* jobs_dev, const uint64_t* job_begin_dev, uint32_t njobs,
-                            uint64_t total_paths, const RenderLaunch& L, hipStream_t s) {
+// One step of the persistent renderer over the job table in ctx->jobs_dev:
+// reset the path counter (a fill kernel, so the sequence can be captured in a
+// graph), trace every path of every job slice, fold the slices into the
+// Welford means.  Job count and path total come from ctx->launch_desc.
+hipError_t vp_launch_step(vp_context* ctx, const RenderLaunch& L, uint32_t max_jobs, uint32_t max_npix,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctx->path_counter, 1u, 0ull);
     const uint32_t stride = L.grid * L.block;
     const size_t lds = sizeof(uint32_t) * kLeafQ * L.block;
     if (L.count_events)
-        hipLaunchKernelGGL(k_render<true>, dim3(L.grid), dim3(L.block), lds, s, jobs_dev, job_begin_dev, njobs,
-                           total_paths, ctx->path_counter, ctx->slab, L.active_cap, stride, ctx->counters, L.measure);
+        hipLaunchKernelGGL(k_render<true>, dim3(L.grid), dim3(L.block), lds, s, ctx->jobs_dev, ctx->job_begin_dev,
+                           ctx->launch_desc, ctx->path_counter, ctx->slab, L.active_cap, stride, L.regen_min,
+                           ctx->counters, L.measure);
     else
-        hipLaunchKernelGGL(k_render<false>, dim3(L.grid), dim3(L.block), lds, s, jobs_dev, job_begin_dev, njobs,
-                           total_paths, ctx->path_counter, ctx->slab, L.active_cap, stride, ctx->counters, L.measure);
+        hipLaunchKernelGGL(k_render<false>, dim3(L.grid), dim3(L.block), lds, s, ctx->jobs_dev, ctx->job_begin_dev,
+                           ctx->launch_desc, ctx->path_counter, ctx->slab, L.active_cap, stride, L.regen_min,
+                           ctx->counters, L.measure);
+    const uint32_t gx = std::max<uint32_t>(1u, std::min<uint32_t>(cdiv(max_npix, 256), 512u));
+    hipLaunchKernelGGL(k_welford_all, dim3(gx, max_jobs), dim3(256), 0, s, ctx->jobs_dev, ctx->launch_desc);
+    return hipGetLastError();
+}
+
+hipError_t vp_launch_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(n, 64)), dim3(64), 0, s, p, n, v);
     return hipGetLastError();
 }
 
@@ -774,13 +805,6 @@ int vp_render_occupancy(uint32_t block, uint32_t, int count_events, int* blocks_
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(k_render<false>), block, lds);
     *blocks_per_cu = nb > 0 ? nb : 1;
     return e == hipSuccess ? 0 : -1;
-}
-
-hipError_t vp_launch_welford(const JobDesc& job, float4* mean, uint32_t sample_base, hipStream_t s) {
-    const uint32_t npix = job.width * job.height;
-    hipLaunchKernelGGL(k_welford, dim3(cdiv(npix, 256)), dim3(256), 0, s, job.samples, npix, job.s_count,
-                       sample_base, mean);
-    return hipGetLastError();
 }
 
 hipError_t vp_launch_normalize(float4* out, const float4* in, size_t n, float scale, hipStream_t s) {

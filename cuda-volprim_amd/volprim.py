@@ -63,7 +63,7 @@ class RenderStats(ctypes.Structure):
 
 class LaunchConfig(ctypes.Structure):
     _fields_ = [("block", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
-                ("active_cap", ctypes.c_uint32), ("batch_spp", ctypes.c_uint32)]
+                ("active_cap", ctypes.c_uint32), ("batch_spp", ctypes.c_uint32), ("regen_min", ctypes.c_uint32)]
 
 
 class Job(ctypes.Structure):
@@ -314,6 +314,18 @@ class Context:
                                              ctypes.byref(params), ev, len(EVENT_NAMES)), "vp_render_count_events")
         return dict(zip(EVENT_NAMES, [int(v) for v in ev]))
 
+    def autotune(self, jobs):
+        """jobs: list of (scene, env, cam, params).  Returns the chosen LaunchConfig."""
+        arr = (Job * len(jobs))()
+        for i, (sc, env, cam, p) in enumerate(jobs):
+            arr[i].scene, arr[i].env, arr[i].camera, arr[i].params = sc.handle, env.handle, cam, p
+        best = LaunchConfig()
+        _check(load().vp_autotune(self.handle, arr, len(jobs), ctypes.byref(best)), "vp_autotune")
+        return best
+
+    def server(self, mem_budget_bytes=8 << 30, slice_spp=4, max_inflight=64):
+        return Server(self, mem_budget_bytes, slice_spp, max_inflight)
+
     def transmittance_rays(self, scene, rays6):
         rays6 = f32(rays6).reshape(-1, 6)
         T = np.empty(rays6.shape[0], np.float32)
@@ -345,3 +357,49 @@ class Context:
         _check(load().vp_prim_eval(self.handle, PRIM_FNS[name], f32(rec16).reshape(16), args9.reshape(-1),
                                    args9.shape[0], out), "vp_prim_eval")
         return out
+
+
+class Server:
+    """Continuous-batching request loop (vp_server_*).  submit() returns a
+    ticket and the host image the result lands in; poll() reports completion."""
+
+    def __init__(self, ctx, mem_budget_bytes, slice_spp, max_inflight):
+        self.ctx = ctx
+        cfg = ServerConfig(mem_budget_bytes, slice_spp, max_inflight)
+        h = _vp()
+        _check(load().vp_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)), "vp_server_create")
+        self.handle = h
+        self._keep = {}
+
+    def submit(self, scene, env, cam, params):
+        out = np.empty((params.height, params.width, 4), np.float32)
+        job = Job()
+        job.scene, job.env, job.camera, job.params, job.out_rgba = scene.handle, env.handle, cam, params, \
+            out.ctypes.data
+        t = ctypes.c_uint64(0)
+        _check(load().vp_server_submit(self.handle, ctypes.byref(job), ctypes.byref(t)), "vp_server_submit")
+        self._keep[t.value] = out
+        return t.value, out
+
+    def poll(self, ticket):
+        """None while running, else the latency in ms."""
+        lat = ctypes.c_double(0)
+        rc = load().vp_server_poll(self.handle, ticket, ctypes.byref(lat))
+        if rc == 4:
+            return None
+        _check(rc, "vp_server_poll")
+        self._keep.pop(ticket, None)
+        return lat.value
+
+    def drain(self):
+        _check(load().vp_server_drain(self.handle), "vp_server_drain")
+
+    def stats(self):
+        st = ServerStats()
+        _check(load().vp_server_stats_get(self.handle, ctypes.byref(st)), "vp_server_stats_get")
+        return st.as_dict()
+
+    def close(self):
+        if self.handle:
+            load().vp_server_destroy(self.handle)
+            self.handle = None

@@ -105,6 +105,7 @@ typedef struct {
     uint32_t blocks_per_cu;/* persistent workgroups per CU (0 = occupancy max) */
     uint32_t active_cap;   /* active-set capacity (entries per path) */
     uint32_t batch_spp;    /* samples per pixel per launch (0 = all) */
+    uint32_t regen_min;    /* idle lanes a wavefront waits for before claiming new paths (0 = 1) */
 } vp_launch_config;
 
 /* ---- context ---------------------------------------------------------- */
