@@ -58,32 +58,144 @@ def workload(scenes):
     return rows, env, cam_args, params
 
 
-def cpu_baseline(rows, env, cam_pod, params, threads, budget_s=20.0):
+def cpu_baseline(rows, env, cam_pod, params, threads, budget_s=20.0, repeats=5):
     """The oracle (brute force over all primitives, pthreads over rows) on a
-    strided pixel sample of the same image; Msamples/s is spp-invariant."""
+    strided pixel sample of the same image; Msamples/s is spp-invariant.
+    One warm-up render sizes the sample, then `repeats` timed renders
+    (median and spread reported)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import vpo
     order = vpo.scene_order(rows)
     rec = vpo.make_prims(rows)[order]
     tabs = vpo.env_tables(env)
-    stride, spp_used = 16, 1
-    t0 = time.perf_counter()
-    vpo.render(rec, env, cam_pod, params["width"], params["height"], params["spp"], seed=params["seed"],
-               max_depth=params["max_depth"], rr_depth=params["rr_depth"], phase_g=params["phase_g"],
-               stride=stride, samples=(0, spp_used), threads=threads, env_tables_=tabs)
-    probe = time.perf_counter() - t0
+    stride = 16
+
+    def run(spp_used):
+        t0 = time.perf_counter()
+        vpo.render(rec, env, cam_pod, params["width"], params["height"], params["spp"], seed=params["seed"],
+                   max_depth=params["max_depth"], rr_depth=params["rr_depth"], phase_g=params["phase_g"],
+                   stride=stride, samples=(0, spp_used), threads=threads, env_tables_=tabs)
+        return time.perf_counter() - t0
+
+    probe = run(1)
     npx = ((params["width"] + stride - 1) // stride) * ((params["height"] + stride - 1) // stride)
-    # grow the sample (more samples per pixel) to about budget_s of work
-    spp_used = int(max(1, min(params["spp"], budget_s / max(probe, 1e-3))))
-    t0 = time.perf_counter()
-    vpo.render(rec, env, cam_pod, params["width"], params["height"], params["spp"], seed=params["seed"],
-               max_depth=params["max_depth"], rr_depth=params["rr_depth"], phase_g=params["phase_g"],
-               stride=stride, samples=(0, spp_used), threads=threads, env_tables_=tabs)
-    dt = time.perf_counter() - t0
+    # samples per pixel so that each timed repeat is about budget_s / repeats
+    spp_used = int(max(1, min(params["spp"], budget_s / repeats / max(probe, 1e-3))))
+    times = [run(spp_used) for _ in range(repeats)]
     n = npx * spp_used
-    return {"value": n / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
-            "sample": f"{npx} pixels (every {stride}th row and column of 512x512) x {spp_used} spp = {n} paths "
-                      f"in {dt:.1f} s; brute force over all primitives"}
+    rates = sorted(n / t / 1e6 for t in times)
+    return {"value": float(np.median(rates)), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "spread": [rates[0], rates[-1]],
+            "sample": f"{npx} pixels (every {stride}th row and column of 512x512) x {spp_used} spp = {n} paths per "
+                      f"repeat, 1 warm-up + {repeats} timed repeats (median); brute force over all primitives"}
+
+
+MIX_ACTIVE_CAP = {"scattering": 128, "cloud": 384, "torus": 384}
+
+
+def run_mix(args):
+    """The job mix three ways on resident scenes: plain (one vp_render per
+    request, in order), batched (every job in one vp_render_jobs call with
+    the autotuned launch), served (a Poisson request stream into vp_server).
+    All three must return bit-identical images."""
+    import threading
+    from volprim_amd import scenes, volprim
+
+    t0 = time.perf_counter()
+    gens = {"scattering": lambda: scenes.c1_scattering(n=4096, seed=1, sigma_multiplier=10.0, albedo=0.9),
+            "cloud": lambda: scenes.c2_cloud(n=1_000_000, seed=2, sigma_multiplier=7.5, albedo=0.99),
+            "torus": lambda: scenes.torus_shell(n=250_000, seed=3, sigma_multiplier=10.0, albedo=0.95)}
+    jobs = scenes.mix_jobs(n_jobs=args.mix_jobs, seed=4, res_div=args.mix_res_div, spp_div=args.mix_spp_div)
+    ctx = volprim.Context(0)
+    dev_name = ctx.device_info()[0]
+    handles = {k: ctx.scene(g()) for k, g in gens.items() if any(j["scene"] == k for j in jobs)}
+    envh = ctx.env(scenes.sky_env(2048, 1024))
+    print(f"[mix] {len(jobs)} jobs, scenes and environment resident after {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    reqs = []
+    for j in jobs:
+        p = volprim.default_params(width=j["width"], height=j["height"], spp=j["spp"], seed=j["seed"], max_depth=64,
+                                   rr_depth=12, phase_g=j["phase_g"], active_cap=MIX_ACTIVE_CAP[j["scene"]])
+        reqs.append((handles[j["scene"]], envh, volprim.camera_perspective(**j["cam"]), p))
+    samples = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
+
+    # plain: one request at a time, one render call each
+    t0 = time.perf_counter()
+    plain_imgs, plain_lat = [], []
+    for r in reqs:
+        s = time.perf_counter()
+        img, st = ctx.render(*r)
+        if st["active_overflow"] or st["hit_overflow"]:
+            raise RuntimeError(f"active-set overflow in the plain run: {st}")
+        plain_imgs.append(img)
+        plain_lat.append((time.perf_counter() - s) * 1e3)
+    t_plain = time.perf_counter() - t0
+    print(f"[mix] plain {t_plain:.2f} s", file=sys.stderr, flush=True)
+
+    # batched: all jobs in one call, launch shape from the autotuner
+    t0 = time.perf_counter()
+    tuned = ctx.autotune(reqs)
+    t_tune = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batch_imgs, bst = ctx.render_jobs(reqs, launch=tuned)
+    t_batch = time.perf_counter() - t0
+    print(f"[mix] batched {t_batch:.2f} s (autotune {t_tune:.1f} s)", file=sys.stderr, flush=True)
+
+    # served: Poisson arrivals at an offered load of lam_factor x plain capacity
+    rate = args.mix_load * len(reqs) / t_plain
+    arr = np.cumsum(np.random.default_rng(5).exponential(1.0 / rate, size=len(reqs)))
+    srv = ctx.server(mem_budget_bytes=args.mix_mem_gib << 30, slice_spp=args.mix_slice, max_inflight=64)
+    tickets = [None] * len(reqs)
+    t0 = time.perf_counter()
+
+    def feeder():
+        for i, r in enumerate(reqs):
+            d = t0 + arr[i] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets[i] = srv.submit(*r)
+
+    th = threading.Thread(target=feeder)
+    th.start()
+    th.join()
+    srv.drain()
+    t_served = time.perf_counter() - t0
+    lat = [srv.poll(t[0]) for t in tickets]
+    sst = srv.stats()
+    srv.close()
+    served_imgs = [t[1] for t in tickets]
+    print(f"[mix] served {t_served:.2f} s", file=sys.stderr, flush=True)
+
+    same = all(np.array_equal(a, b) and np.array_equal(a, c) for a, b, c in zip(plain_imgs, batch_imgs, served_imgs))
+    ms = lambda v: round(float(v), 2)
+    line = {
+        "metric": "Msamples/s on the mixed request stream (plain vs batched vs served); p50/p99 request latency",
+        "value": round(samples / t_served / 1e6, 4), "unit": "Msamples/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic", "device": dev_name,
+        "config": {"workload": f"{len(jobs)} jobs (seed 4): resolution U{{64..1024}}^2 / {args.mix_res_div}, spp "
+                               f"round(exp U[ln4, ln256]) / {args.mix_spp_div}, scenes U{{scattering 4096, cloud 1M, "
+                               f"torus 250k}}, max depth 64, RR 12, sky environment 2048x1024",
+                   "samples": samples, "jobs": len(jobs)},
+        "plain": {"seconds": round(t_plain, 3), "msamples_s": round(samples / t_plain / 1e6, 4),
+                  "p50_ms": ms(np.percentile(plain_lat, 50)), "p99_ms": ms(np.percentile(plain_lat, 99))},
+        "batched": {"seconds": round(t_batch, 3), "msamples_s": round(samples / t_batch / 1e6, 4),
+                    "launch": {"block": tuned.block, "blocks_per_cu": tuned.blocks_per_cu,
+                               "active_cap": tuned.active_cap, "regen_min": tuned.regen_min},
+                    "autotune_s": round(t_tune, 2), "launches": bst["launches"],
+                    "speedup_vs_plain": round(t_plain / t_batch, 3)},
+        "served": {"seconds": round(t_served, 3), "msamples_s": round(samples / t_served / 1e6, 4),
+                   "lambda_req_s": round(rate, 3), "requests": len(reqs),
+                   "p50_ms": ms(np.percentile(lat, 50)), "p99_ms": ms(np.percentile(lat, 99)),
+                   "speedup_vs_plain": round(t_plain / t_served, 3), "server_stats": sst},
+        "images_identical": bool(same),
+    }
+    print(json.dumps(line), flush=True)
+    for h in handles.values():
+        h.close()
+    envh.close()
+    ctx.close()
+    if not same:
+        sys.exit(1)
 
 
 def main():
@@ -93,9 +205,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--no-quality", action="store_true", help="skip the image-error estimate")
+    ap.add_argument("--rmse-ref-spp", type=int, default=0, help="also measure relative RMSE against a render "
+                                                                   "with this many spp (untimed)")
     ap.add_argument("--block", type=int, default=0, help="threads per workgroup (0 = library default)")
     ap.add_argument("--bpc", type=int, default=0, help="persistent workgroups per CU (0 = occupancy limit)")
     ap.add_argument("--regen", type=int, default=0, help="idle lanes per wavefront before claiming new paths")
+    ap.add_argument("--mix", action="store_true", help="run the job mix (plain vs batched vs served) instead")
+    ap.add_argument("--mix-jobs", type=int, default=64)
+    ap.add_argument("--mix-res-div", type=int, default=4)
+    ap.add_argument("--mix-spp-div", type=int, default=4)
+    ap.add_argument("--mix-load", type=float, default=2.0, help="offered load as a multiple of plain capacity")
+    ap.add_argument("--mix-mem-gib", type=int, default=16)
+    ap.add_argument("--mix-slice", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is in DESIGN.md",
@@ -104,6 +226,9 @@ def main():
 
     import torch
     pkg = load_pkg()
+    if args.mix:
+        run_mix(args)
+        return
     from volprim_amd import scenes, volprim
 
     rows, env, cam_args, pp = workload(scenes)
@@ -149,6 +274,37 @@ def main():
                     "peak_nonpacked": PEAK_FP32_NONPACKED_TFLOPS, "kernel": "k_render",
                     "avg_launch_ms": round(avg_launch_s * 1e3, 3), "flops_per_launch": flops_per_launch}
 
+    # image error: relative RMSE of the timed image from its per-pixel sample
+    # variance (an unbiased estimate of the MSE against the converged image),
+    # and optionally measured against a high-spp render of the same view
+    quality = None
+    if not args.no_quality:
+        smp = torch.empty((pp["spp"], pp["height"], pp["width"], 4), dtype=torch.float32, device="cuda")
+        ctx.render_device(scene, envh, cam, params, out.data_ptr(), samples_ptr=smp.data_ptr(), launch=launch)
+        torch.cuda.synchronize()
+        rgb = smp[..., :3].double()
+        var_mean = rgb.var(dim=0, unbiased=True) / pp["spp"]
+        mean_img = out[..., :3].double()
+        rel_est = float(var_mean.mean().sqrt() / mean_img.mean())
+        quality = {"rel_rmse_est": round(rel_est, 5), "spp": pp["spp"],
+                   "spp_for_1pct_est": int(np.ceil(pp["spp"] * (rel_est / 0.01) ** 2)),
+                   "method": "per-pixel sample variance / spp"}
+        del smp, rgb, var_mean
+        if args.rmse_ref_spp:
+            ref = torch.empty_like(out)
+            pr = volprim.default_params(**dict(pp, spp=args.rmse_ref_spp, seed=pp["seed"] + 1))
+            t_ref = time.perf_counter()
+            ctx.render_device(scene, envh, cam, pr, ref.data_ptr(), launch=launch)
+            torch.cuda.synchronize()
+            t_ref = time.perf_counter() - t_ref
+            R = ref[..., :3].double()
+            mse = float(((mean_img - R) ** 2).mean())
+            quality.update({"rel_rmse_vs_ref": round(float(np.sqrt(mse)) / float(R.mean()), 5),
+                            "ref_spp": args.rmse_ref_spp, "ref_seconds": round(t_ref, 2),
+                            "ref_note": "reference rendered with seed+1 (independent samples); its own noise "
+                                        "adds spp/ref_spp of the variance"})
+        del mean_img
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(rows, env, cam.pod(), pp, threads=min(os.cpu_count() or 1, 16))
@@ -167,6 +323,7 @@ def main():
         "device": dev_name,
         "kernel_ms_per_step": round(kernel_ms / args.steps, 3),
         "events_per_step": events,
+        "quality": quality,
         "image_mean": float(img[..., 0:3].mean()),
     }
     print(json.dumps(line), flush=True)

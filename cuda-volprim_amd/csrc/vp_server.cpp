@@ -16,7 +16,8 @@
 // launch shapes (workgroup size, persistent workgroups per CU, regeneration
 // threshold) and sizes the active-set capacity from the measured maximum
 // point overlap (ceil(1.125 v) rounded up to 16, the rule of the reference's
-// --measure-caps, src/thesis/host/app/renderer.cpp:405-418); the best shape
+// --measure-caps, src/thesis/host/app/renderer.cpp:405-418, and at least the
+// largest capacity the jobs request); the best shape
 // is kept in the context and used by every later launch that passes no
 // explicit configuration.
 #include <algorithm>
@@ -296,14 +297,30 @@ vp_status vp_autotune(vp_context* ctx, vp_job* jobs, uint32_t njobs, vp_launch_c
     if (!ctx || !jobs || njobs == 0) return fail(VP_ERR_INVALID, "no jobs");
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(VP_ERR_HIP, "hipSetDevice failed");
     hipStream_t s = ctx->stream;
-    // reduced copy of the mix: at most 4 spp per job
+    // reduced copy of the mix: every view downscaled by an integer factor k
+    // to at most kTuneRes pixels a side (same field of view, k-times wider
+    // pixels) at one sample per pixel
+    constexpr uint32_t kTuneRes = 64;
     std::vector<vp_render_params> params(njobs);
+    std::vector<vp_camera> cams(njobs);
     for (uint32_t i = 0; i < njobs; ++i) {
         vp_status st = vp_check_params(&jobs[i].params);
         if (st != VP_OK) return st;
         params[i] = jobs[i].params;
-        params[i].spp = std::min<uint32_t>(params[i].spp, 4u);
+        params[i].spp = 1;
         params[i].measure_caps = 0;
+        const uint32_t side = std::max(params[i].width, params[i].height);
+        const uint32_t k = (side + kTuneRes - 1) / kTuneRes;
+        params[i].width = (params[i].width + k - 1) / k;
+        params[i].height = (params[i].height + k - 1) / k;
+        vp_camera c = jobs[i].camera;
+        const float kf = static_cast<float>(k), h = 0.5f * (kf - 1.0f);
+        for (int a = 0; a < 3; ++a) {
+            c.pixel00_rel[a] += h * (c.pixel_du[a] + c.pixel_dv[a]);
+            c.pixel_du[a] *= kf;
+            c.pixel_dv[a] *= kf;
+        }
+        cams[i] = c;
     }
     std::vector<void*> means(njobs, nullptr);
     struct Free {
@@ -319,7 +336,7 @@ vp_status vp_autotune(vp_context* ctx, vp_job* jobs, uint32_t njobs, vp_launch_c
     auto make_runs = [&](std::vector<JobRun>& runs) {
         runs.assign(njobs, JobRun{});
         for (uint32_t i = 0; i < njobs; ++i) {
-            runs[i].desc = vp_make_job(jobs[i].scene, jobs[i].env, &jobs[i].camera, &params[i]);
+            runs[i].desc = vp_make_job(jobs[i].scene, jobs[i].env, &cams[i], &params[i]);
             runs[i].mean = static_cast<float4*>(means[i]);
         }
     };
@@ -335,6 +352,9 @@ vp_status vp_autotune(vp_context* ctx, vp_job* jobs, uint32_t njobs, vp_launch_c
         if (st != VP_OK) return st;
         const uint32_t v = rs.max_point_overlap;
         cap = std::max<uint32_t>(16u, ((v * 9 + 7) / 8 + 15) / 16 * 16);
+        // the reduced mix sees fewer vertices than the full one: never go
+        // below the capacities the jobs ask for
+        for (uint32_t i = 0; i < njobs; ++i) cap = std::max(cap, jobs[i].params.active_cap);
     }
     vp_launch_config best_cfg{};
     double best_rate = -1.0;

@@ -206,3 +206,32 @@ def default_camera_args(width, height, dist=3.5, fov=40.0, view="negz"):
            "diag": (dist * 0.6, dist * 0.5, -dist * 0.6)}[view]
     return dict(w=width, h=height, eye=np.array(eye, np.float32), target=np.zeros(3, np.float32),
                 up=np.array([0, 1, 0], np.float32), vfov=fov)
+
+
+# Per-scene render settings of the job mix: (phase g, generator)
+MIX_SCENES = ("scattering", "cloud", "torus")
+MIX_PHASE_G = {"scattering": 0.0, "cloud": 0.85, "torus": 0.0}
+
+
+def mix_jobs(n_jobs=64, seed=4, res_div=1, spp_div=1):
+    """The batched job mix: resolution uniform over {64,...,1024}^2, spp =
+    round(exp U[ln 4, ln 256]), scene uniform over (scattering 4096,
+    cloud 1M, torus 250k), camera on a sphere of radius 3.5 (uniform
+    direction) looking at the origin with a 40 degree vertical FOV.
+    res_div / spp_div shrink every job by the same factor (spp >= 1).
+    Returns dicts: scene, width, height, spp, seed, phase_g, camera args."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n_jobs):
+        res = int(rng.choice([64, 128, 256, 512, 1024]))
+        spp = int(round(np.exp(rng.uniform(np.log(4.0), np.log(256.0)))))
+        sc = MIX_SCENES[int(rng.integers(0, 3))]
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        up = np.array([0, 1, 0], np.float32) if abs(v[1]) < 0.99 else np.array([1, 0, 0], np.float32)
+        w = max(8, res // res_div)
+        jobs.append(dict(scene=sc, width=w, height=w, spp=max(1, spp // spp_div), seed=i,
+                         phase_g=MIX_PHASE_G[sc],
+                         cam=dict(w=w, h=w, eye=(3.5 * v).astype(np.float32), target=np.zeros(3, np.float32),
+                                  up=up, vfov=40.0)))
+    return jobs

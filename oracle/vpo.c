@@ -782,6 +782,61 @@ static inline void set_push(uint32_t* s, int* ns, int cap, uint32_t v, counters_
     else k->active_overflow++;
 }
 
+/* First scatter distance along (o, d): minimum over every primitive the ray
+ * enters of inv_cdf_span from its entry, clamped up to the entry and
+ * accepted if inside the exit, and over the origin set of inv_cdf, accepted
+ * if inside the exit (sampling.cuh:310-313,350-415).  INF = escape. */
+static float argmin_scatter(const prim_t* prims, long n, v3 o, v3 d, const uint32_t* act, int na, uint64_t key,
+                            uint32_t bounce, uint32_t* hits_out, counters_t* k) {
+    float ts_min = INF_F;
+    uint32_t hits = 0;
+    for (long j = 0; j < n; ++j) {
+        const prim_t* pr = prims + j;
+        float wl2;
+        float tj = entry_t(pr, o, d, &wl2);
+        if (tj < 0.0f) continue;
+        ++hits;
+        k->n_arg_hit++;
+        float tau = ff_tau(prim_u(key, bounce, 1u, (uint32_t)j));
+        float ts = inv_cdf_span(pr, o, d, tj, tau);
+        if (ts >= 0.0f && ts < tj) ts = tj;
+        if (ts >= 0.0f && ts < ts_min) {
+            k->n_arg_accept++;
+            float te = exit_entry(pr, o, d, tj, wl2);
+            if (ts <= te) ts_min = ts;
+        }
+    }
+    for (int s = 0; s < na; ++s) {
+        const prim_t* pr = prims + act[s];
+        k->n_arg_active++;
+        float te = exit_inside(pr, o, d);
+        float tau = ff_tau(prim_u(key, bounce, 0u, act[s]));
+        float ts = inv_cdf(pr, o, d, tau);
+        if (ts >= 0.0f && ts < ts_min && ts <= te) ts_min = ts;
+    }
+    *hits_out = hits;
+    return ts_min;
+}
+
+/* First scatter distance of m independent paths (path keys keys[i], bounce 0)
+ * along one ray; the origin set is every primitive containing o.  A test
+ * entry point for the distribution P[t* > t] = exp(-tau(0, t)). */
+void vpo_first_scatter(const float* prims16, long n, const float* ray6, const uint64_t* keys, long m, float* out) {
+    const prim_t* prims = (const prim_t*)prims16;
+    v3 o = V(ray6[0], ray6[1], ray6[2]), d = V(ray6[3], ray6[4], ray6[5]);
+    uint32_t* act = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+    int na = 0;
+    for (long i = 0; i < n; ++i)
+        if (point_inside(prims + i, o)) act[na++] = (uint32_t)i;
+    counters_t k;
+    memset(&k, 0, sizeof(k));
+    for (long i = 0; i < m; ++i) {
+        uint32_t hits;
+        out[i] = argmin_scatter(prims, n, o, d, act, na, keys[i], 0u, &hits, &k);
+    }
+    free(act);
+}
+
 static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, uint32_t* buf, counters_t* k) {
     const vpo_params* P = J->p;
     const prim_t* prims = J->prims;
@@ -826,33 +881,9 @@ static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, 
 
     for (uint32_t bounce = 0; bounce < P->max_depth; ++bounce) {
         /* argmin over entered primitives and the origin set (sampling.cuh:350-415) */
-        float ts_min = INF_F;
         uint32_t hits = 0;
         k->n_trace++;
-        for (long j = 0; j < n; ++j) {
-            const prim_t* pr = prims + j;
-            float wl2;
-            float tj = entry_t(pr, o, d, &wl2);
-            if (tj < 0.0f) continue;
-            ++hits;
-            k->n_arg_hit++;
-            float tau = ff_tau(prim_u(key, bounce, 1u, (uint32_t)j));
-            float ts = inv_cdf_span(pr, o, d, tj, tau);
-            if (ts >= 0.0f && ts < tj) ts = tj;
-            if (ts >= 0.0f && ts < ts_min) {
-                k->n_arg_accept++;
-                float te = exit_entry(pr, o, d, tj, wl2);
-                if (ts <= te) ts_min = ts;
-            }
-        }
-        for (int s = 0; s < na; ++s) {
-            const prim_t* pr = prims + act[s];
-            k->n_arg_active++;
-            float te = exit_inside(pr, o, d);
-            float tau = ff_tau(prim_u(key, bounce, 0u, act[s]));
-            float ts = inv_cdf(pr, o, d, tau);
-            if (ts >= 0.0f && ts < ts_min && ts <= te) ts_min = ts;
-        }
+        float ts_min = argmin_scatter(prims, n, o, d, act, na, key, bounce, &hits, k);
         if (P->measure && hits > k->max_hits) k->max_hits = hits;
 
         /* analytic direct term at bounce 0 (raygen.cuh:148-160) */

@@ -86,6 +86,8 @@ def load():
     lib.vpo_camera_ray.argtypes = [_f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
                                    _f32p, _f32p]
     lib.vpo_prim_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+    lib.vpo_first_scatter.argtypes = [_f32p, ctypes.c_long, _f32p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_long,
+                                      _f32p]
     lib.vpo_prim_uniform.restype = ctypes.c_float
     return lib
 
@@ -155,6 +157,16 @@ def pcg_stream(seed, seq, n):
 def pcg_raw(state, inc, n):
     out = np.empty(n, np.uint32)
     lib().vpo_pcg_raw(state, inc, n, out)
+    return out
+
+
+def first_scatter(prims16, ray6, keys):
+    """Bounce-0 argmin free-flight distance of independent path keys along one ray (inf = escape)."""
+    prims16 = f32(prims16).reshape(-1, 16)
+    keys = np.ascontiguousarray(keys, np.uint64)
+    out = np.empty(keys.size, np.float32)
+    lib().vpo_first_scatter(prims16.reshape(-1), prims16.shape[0], f32(ray6).reshape(6),
+                            keys.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), keys.size, out)
     return out
 
 

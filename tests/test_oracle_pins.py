@@ -390,3 +390,45 @@ def test_mis_and_ris_agree_in_expectation():
     ma, mb = sa[..., 1].astype(np.float64).mean(), sb[..., 1].astype(np.float64).mean()
     se = math.sqrt(sa[..., 1].var() / sa[..., 1].size + sb[..., 1].var() / sb[..., 1].size)
     assert abs(ma - mb) < 4 * se + 1e-4
+
+
+def test_argmin_free_flight_distribution():
+    # The argmin of per-primitive closed-form free flights must sample the
+    # first collision of the whole mixture: P[t* > t] = exp(-tau(0, t)) with
+    # tau the summed FP64 optical depth, and P[escape] = exp(-tau_total).
+    # Four overlapping Gaussians, the ray starting inside the first one.
+    from golden.make_golden import chord, od_closed
+    rng = np.random.default_rng(21)
+    rows = scenes.random_scene(4, 77, smin=0.25, smax=0.45)
+    rows[:, 0:3] = [[0.0, 0.0, 0.0], [0.1, -0.05, 0.9], [-0.1, 0.1, 1.7], [0.05, 0.0, 2.6]]
+    rows[:, 13] = [0.35, 0.3, 0.25, 0.3]
+    rec = vpo.make_prims(rows)
+    o = np.array([0.05, 0.02, -0.2])
+    d = np.array([0.02, -0.01, 1.0])
+    d /= np.linalg.norm(d)
+    ray = np.concatenate([o, d]).astype(np.float32)
+    o64, d64 = ray[:3].astype(np.float64), ray[3:].astype(np.float64)
+    chords = [chord(r, o64, d64)[0] for r in rows]
+    assert chord(rows[0], o64, d64)[1] and all(c is not None for c in chords)
+
+    def tau(t):
+        return sum(od_closed(r, o64, d64, c[0], min(max(t, c[0]), c[1])) for r, c in zip(rows, chords))
+
+    m = 200_000
+    keys = rng.integers(0, 2**63, size=m, dtype=np.uint64)
+    ts = vpo.first_scatter(rec, ray, keys).astype(np.float64)
+    esc = ts >= 1e20
+    tau_tot = tau(1e9)
+    p_esc = math.exp(-tau_tot)
+    assert 0.1 < p_esc < 0.5
+    assert abs(esc.mean() - p_esc) < 4 * math.sqrt(p_esc * (1 - p_esc) / m)
+    grid = np.linspace(0.0, max(c[1] for c in chords), 400)
+    F = np.array([1.0 - math.exp(-tau(t)) for t in grid])
+    F_emp = np.searchsorted(np.sort(ts[~esc]), grid, side="right") / m
+    # Kolmogorov-Smirnov distance on the grid, 99.9 % critical value 1.95/sqrt(m)
+    assert np.abs(F_emp - F).max() < 1.95 / math.sqrt(m), np.abs(F_emp - F).max()
+    # every sampled collision lies inside some primitive's chord
+    inside_any = np.zeros(m, bool)
+    for c in chords:
+        inside_any |= (ts >= c[0] - 1e-5) & (ts <= c[1] + 1e-5)
+    assert inside_any[~esc].all()
