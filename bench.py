@@ -211,6 +211,11 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="threads per workgroup (0 = library default)")
     ap.add_argument("--bpc", type=int, default=0, help="persistent workgroups per CU (0 = occupancy limit)")
     ap.add_argument("--regen", type=int, default=0, help="idle lanes per wavefront before claiming new paths")
+    ap.add_argument("--pipeline", choices=["mega", "wave"], default="mega",
+                    help="persistent megakernel or wavefront-queue pipeline")
+    ap.add_argument("--pool", type=int, default=0, help="wavefront path slots (0 = library default)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 256 MiB elementwise copy kernel first (HBM counter calibration)")
     ap.add_argument("--mix", action="store_true", help="run the job mix (plain vs batched vs served) instead")
     ap.add_argument("--mix-jobs", type=int, default=64)
     ap.add_argument("--mix-res-div", type=int, default=4)
@@ -240,10 +245,19 @@ def main():
     params = volprim.default_params(**pp)
     out = torch.empty((pp["height"], pp["width"], 4), dtype=torch.float32, device="cuda")
     launch = None
-    if args.block or args.bpc or args.regen:
-        launch = volprim.LaunchConfig(args.block, args.bpc, 0, 0, args.regen)
+    if args.block or args.bpc or args.regen or args.pipeline != "mega" or args.pool:
+        launch = volprim.LaunchConfig(args.block, args.bpc, 0, 0, args.regen,
+                                      volprim.PIPELINE_WAVEFRONT if args.pipeline == "wave" else 0, args.pool)
     torch.cuda.synchronize()
 
+    if args.copy_calibration:
+        # a plain elementwise kernel of known traffic (reads and writes
+        # 256 MiB each) for scaling the HBM counters of a --pmc run
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
     for _ in range(args.warmup):
         ctx.render_device(scene, envh, cam, params, out.data_ptr(), launch=launch)
     torch.cuda.synchronize()
@@ -271,7 +285,8 @@ def main():
         achieved = flops_per_launch / avg_launch_s / 1e12
         roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                    "peak_nonpacked": PEAK_FP32_NONPACKED_TFLOPS, "kernel": "k_render",
+                    "peak_nonpacked": PEAK_FP32_NONPACKED_TFLOPS,
+                    "kernel": "k_render" if args.pipeline == "mega" else "k_wf_* (one render step)",
                     "avg_launch_ms": round(avg_launch_s * 1e3, 3), "flops_per_launch": flops_per_launch}
 
     # image error: relative RMSE of the timed image from its per-pixel sample
@@ -322,6 +337,7 @@ def main():
         "cpu_baseline": cpu,
         "device": dev_name,
         "kernel_ms_per_step": round(kernel_ms / args.steps, 3),
+        "pipeline": args.pipeline,
         "events_per_step": events,
         "quality": quality,
         "image_mean": float(img[..., 0:3].mean()),

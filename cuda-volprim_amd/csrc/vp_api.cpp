@@ -158,6 +158,7 @@ vp_status vp_context_destroy(vp_context* ctx) {
     if (ctx->counters) (void)hipFree(ctx->counters);
     if (ctx->path_counter) (void)hipFree(ctx->path_counter);
     vp_drop_graph(ctx);
+    vp_free_wf(ctx);
     if (ctx->jobs_dev) (void)hipFree(ctx->jobs_dev);
     if (ctx->job_begin_dev) (void)hipFree(ctx->job_begin_dev);
     if (ctx->launch_desc) (void)hipFree(ctx->launch_desc);
@@ -492,6 +493,9 @@ vp_status vp_render_jobs(vp_context* ctx, vp_job* jobs, uint32_t njobs, const vp
         if (launch && launch->batch_spp) runs[i].slice_spp = launch->batch_spp;
         samples += static_cast<uint64_t>(j.params.width) * j.params.height * j.params.spp;
     }
+    // one capacity serves the whole launch: the largest any job or the
+    // launch configuration asks for
+    if (launch && launch->active_cap) cap = std::max(cap, launch->active_cap);
     vp_status st = vp_reset_counters(ctx, ctx->stream);
     if (st != VP_OK) return st;
     double kms = 0.0;

@@ -1,0 +1,313 @@
+// vp_device.h — device building blocks shared by the render kernels:
+// record loads, the stackless BVH walks (plain and with the LDS leaf queue),
+// fixed-point optical depth, shadow-ray transmittance, event counters and
+// the job lookup.  Included by vp_render.hip (megakernel, test kernels) and
+// vp_wavefront.hip (wavefront pipeline).
+#pragma once
+
+#include "vp_internal.h"
+
+namespace vp {
+namespace {
+
+constexpr float kTauInvScale = 1.0f / 1099511627776.0f;
+
+__device__ __forceinline__ f3 safe_inv(f3 d) {
+    const float x = fabsf_(d.x) < 1e-20f ? copysignf_(1e-20f, d.x) : d.x;
+    const float y = fabsf_(d.y) < 1e-20f ? copysignf_(1e-20f, d.y) : d.y;
+    const float z = fabsf_(d.z) < 1e-20f ? copysignf_(1e-20f, d.z) : d.z;
+    return f3{1.0f / x, 1.0f / y, 1.0f / z};
+}
+
+__device__ __forceinline__ PrimRecord load_prim(const PrimRecord* prims, uint32_t i) {
+    const float4* p = reinterpret_cast<const float4*>(prims + i);
+    const float4 a = p[0], b = p[1], c = p[2], d = p[3];
+    PrimRecord r;
+    r.cx = a.x; r.cy = a.y; r.cz = a.z; r.qw = a.w;
+    r.qx = b.x; r.qy = b.y; r.qz = b.z; r.rsx = b.w;
+    r.rsy = c.x; r.rsz = c.y; r.dnf = c.z; r.icf = c.w;
+    r.ar = d.x; r.ag = d.y; r.ab = d.z; r.mass = d.w;
+    return r;
+}
+
+// Stackless walk of the rope layout over [0, tmax]; leaf(prim) returns false
+// to stop.  tmax may be lowered by the caller between leaves.
+template <typename Leaf>
+__device__ __forceinline__ void walk_ray(const SceneView& s, f3 o, f3 inv, const float& tmax, uint32_t& visits,
+                                         Leaf&& leaf) {
+    uint32_t i = 0;
+    const uint32_t nn = s.n_nodes;
+    while (i < nn) {
+        const float4 a = s.nodes[2 * i];
+        const float4 b = s.nodes[2 * i + 1];
+        ++visits;
+        const float tx0 = (a.x - o.x) * inv.x, tx1 = (b.x - o.x) * inv.x;
+        const float ty0 = (a.y - o.y) * inv.y, ty1 = (b.y - o.y) * inv.y;
+        const float tz0 = (a.z - o.z) * inv.z, tz1 = (b.z - o.z) * inv.z;
+        const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+        const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+        const uint32_t skip = __float_as_uint(a.w);
+        if (tnear <= tfar) {
+            const uint32_t prim = __float_as_uint(b.w);
+            if (prim != kInnerNode) {
+                if (!leaf(prim)) return;
+                i = skip;
+            } else {
+                i = i + 1;
+            }
+        } else {
+            i = skip;
+        }
+    }
+}
+
+// Stackless walk visiting every leaf whose box contains x, in ascending
+// stored-primitive order.
+template <typename Leaf>
+__device__ __forceinline__ void walk_point(const SceneView& s, f3 x, uint32_t& visits, Leaf&& leaf) {
+    uint32_t i = 0;
+    const uint32_t nn = s.n_nodes;
+    while (i < nn) {
+        const float4 a = s.nodes[2 * i];
+        const float4 b = s.nodes[2 * i + 1];
+        ++visits;
+        const bool in = x.x >= a.x && x.x <= b.x && x.y >= a.y && x.y <= b.y && x.z >= a.z && x.z <= b.z;
+        const uint32_t skip = __float_as_uint(a.w);
+        if (in) {
+            const uint32_t prim = __float_as_uint(b.w);
+            if (prim != kInnerNode) {
+                leaf(prim);
+                i = skip;
+            } else {
+                i = i + 1;
+            }
+        } else {
+            i = skip;
+        }
+    }
+}
+
+// Leaf queue depth per lane.  The walks below postpone leaves into a per-lane
+// LDS queue (column `lane` of a [kLeafQ][blockDim] array) and keep walking
+// until every lane of the wavefront has a full queue or has finished; then all
+// lanes process their queued leaves together.  Node steps stay convergent and
+// the expensive leaf work runs with most lanes active (the "while-while"
+// scheme of Aila and Laine 2009, with speculative postponing).
+#ifndef VP_LEAF_Q
+#define VP_LEAF_Q 16
+#endif
+constexpr uint32_t kLeafQ = VP_LEAF_Q;
+
+struct LeafQueue {
+    uint32_t* col;  // &lds[0][lane]
+    uint32_t stride;  // blockDim.x
+    __device__ __forceinline__ uint32_t& at(uint32_t k) const { return col[k * stride]; }
+};
+
+template <typename Leaf>
+__device__ __forceinline__ void walk_ray_q(const SceneView& s, f3 o, f3 inv, const float& tmax, uint32_t& visits,
+                                           const LeafQueue& q, Leaf&& leaf) {
+    uint32_t i = 0;
+    const uint32_t nn = s.n_nodes;
+    while (true) {
+        uint32_t cnt = 0;
+        while (i < nn && cnt < kLeafQ) {
+            const float4 a = s.nodes[2 * i];
+            const float4 b = s.nodes[2 * i + 1];
+            ++visits;
+            const float tx0 = (a.x - o.x) * inv.x, tx1 = (b.x - o.x) * inv.x;
+            const float ty0 = (a.y - o.y) * inv.y, ty1 = (b.y - o.y) * inv.y;
+            const float tz0 = (a.z - o.z) * inv.z, tz1 = (b.z - o.z) * inv.z;
+            const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+            const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+            const uint32_t skip = __float_as_uint(a.w);
+            const uint32_t prim = __float_as_uint(b.w);
+            if (tnear <= tfar) {
+                if (prim != kInnerNode) {
+                    q.at(cnt++) = prim;
+                    i = skip;
+                } else {
+                    i = i + 1;
+                }
+            } else {
+                i = skip;
+            }
+        }
+        for (uint32_t k = 0; k < cnt; ++k)
+            if (!leaf(q.at(k))) return;
+        if (i >= nn) return;
+    }
+}
+
+template <typename Leaf>
+__device__ __forceinline__ void walk_point_q(const SceneView& s, f3 x, uint32_t& visits, const LeafQueue& q,
+                                             Leaf&& leaf) {
+    uint32_t i = 0;
+    const uint32_t nn = s.n_nodes;
+    while (true) {
+        uint32_t cnt = 0;
+        while (i < nn && cnt < kLeafQ) {
+            const float4 a = s.nodes[2 * i];
+            const float4 b = s.nodes[2 * i + 1];
+            ++visits;
+            const bool in = x.x >= a.x && x.x <= b.x && x.y >= a.y && x.y <= b.y && x.z >= a.z && x.z <= b.z;
+            const uint32_t skip = __float_as_uint(a.w);
+            const uint32_t prim = __float_as_uint(b.w);
+            if (in) {
+                if (prim != kInnerNode) {
+                    q.at(cnt++) = prim;
+                    i = skip;
+                } else {
+                    i = i + 1;
+                }
+            } else {
+                i = skip;
+            }
+        }
+        for (uint32_t k = 0; k < cnt; ++k) leaf(q.at(k));
+        if (i >= nn) return;
+    }
+}
+
+// Add one chord to the fixed-point optical depth; true once the total
+// reaches 88 (exp(-88) underflows), which is order independent because every
+// term is non-negative.
+__device__ __forceinline__ bool tau_add(uint64_t& acc, float od) {
+    const float c = od > 0.0f ? od : 0.0f;
+    if (c >= kMaxTau) return true;
+    acc += static_cast<uint64_t>(c * kTauScaleF);
+    return acc >= kTauLimitFixed;
+}
+
+__device__ __forceinline__ float tau_to_T(uint64_t acc) {
+    const float tau = static_cast<float>(acc) * kTauInvScale;
+    return tau >= kMaxTau ? 0.0f : expf_(-tau);
+}
+
+struct EvCount {
+    uint32_t v[kNumCounters];
+};
+
+template <bool kCount>
+struct Ev {
+    EvCount c;
+    __device__ __forceinline__ void reset() {
+        if constexpr (kCount)
+            for (int i = 0; i < kNumCounters; ++i) c.v[i] = 0;
+    }
+    __device__ __forceinline__ void add(int slot, uint32_t n = 1) {
+        if constexpr (kCount) c.v[slot] += n;
+    }
+    __device__ __forceinline__ void flush(unsigned long long* g) {
+        if constexpr (kCount) {
+            for (int i = kCntPaths; i < kNumCounters; ++i)
+                if (c.v[i]) atomicAdd(&g[i], static_cast<unsigned long long>(c.v[i]));
+            reset();
+        }
+    }
+};
+
+// compute_transmittance_to_env (sampling.cuh:548-583) with the fused
+// any-hit integration (anyhit.cuh:53-66).
+template <bool kCount>
+__device__ float transmittance(const SceneView& s, f3 o, f3 d, const uint32_t* act, uint32_t na, uint32_t stride,
+                               const LeafQueue& lq, Ev<kCount>& ev) {
+    uint64_t acc = 0;
+    for (uint32_t k = 0; k < na; ++k) {
+        const PrimRecord pr = load_prim(s.prims, act[k * stride]);
+        const float te = exit_from_inside(pr, o, d);
+        ev.add(kCntShActive);
+        if (te > 0.0f && te < kTMax)
+            if (tau_add(acc, optical_depth(pr, o, d, 0.0f, te))) return 0.0f;
+    }
+    const f3 inv = safe_inv(d);
+    const float tmax = kTMax;
+    bool full = false;
+    uint32_t visits = 0, leaves = 0, sh = 0;
+    walk_ray_q(s, o, inv, tmax, visits, lq, [&](uint32_t prim) -> bool {
+        const PrimRecord pr = load_prim(s.prims, prim);
+        ++leaves;
+        float wl2;
+        const float tj = entry_t(pr, o, d, &wl2);
+        if (tj < 0.0f) return true;
+        const float te = exit_from_entry(pr, o, d, tj, wl2);
+        ++sh;
+        if (te > tj && te < kTMax) {
+            if (tau_add(acc, optical_depth(pr, o, d, tj, te))) {
+                full = true;
+                return false;
+            }
+        }
+        return true;
+    });
+    ev.add(kCntNodeVisits, visits);
+    ev.add(kCntLeafTests, leaves);
+    ev.add(kCntShHit, sh);
+    if (full) return 0.0f;
+    return tau_to_T(acc);
+}
+
+__device__ __forceinline__ EnvView env_view(const EnvDev& e) {
+    EnvView v;
+    v.tex = e.tex;
+    v.alias = e.has_table ? e.alias : nullptr;
+    v.joint = e.joint;
+    v.w = e.w;
+    v.h = e.h;
+    v.total = e.total;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t find_job(const uint64_t* begin, uint32_t njobs, uint64_t p) {
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (begin[mid] <= p) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+
+// Path setup (raygen.cuh:72-113) for path q of job J's current slice: pixel
+// and sample from the 8x8-tile numbering, RNG stream, counter-hash key,
+// jittered camera ray and the per-sample output slot.  False for the padding
+// paths of partial tiles.
+__device__ __forceinline__ bool path_begin(const JobDesc& J, uint64_t q, f3& o, f3& d, Pcg32& rng, uint64_t& key,
+                                           uint32_t& slot) {
+    const uint32_t sl = static_cast<uint32_t>(q / J.npad);
+    const uint32_t r = static_cast<uint32_t>(q - static_cast<uint64_t>(sl) * J.npad);
+    const uint32_t tile = r >> 6, w = r & 63u;
+    const uint32_t px = (tile % J.tiles_x) * 8u + (w & 7u);
+    const uint32_t py = (tile / J.tiles_x) * 8u + (w >> 3);
+    if (px >= J.width || py >= J.height) return false;
+    const uint32_t sample = J.s_begin + sl;
+    const uint64_t pix = static_cast<uint64_t>(py) * J.width + px;
+    const uint64_t seq = hash64(pix * J.spp + sample);
+    rng = pcg_init(J.seed, seq);
+    key = path_key(J.seed, seq);
+    float jx, jy;
+    if (J.filter_stddev > 0.0f) {
+        gaussian_jitter(rng, J.filter_stddev, &jx, &jy);
+    } else {
+        jx = next_uniform(rng) - 0.5f;
+        jy = next_uniform(rng) - 0.5f;
+    }
+    camera_ray(J.cam, px, py, jx, jy, &o, &d);
+    slot = static_cast<uint32_t>(static_cast<uint64_t>(sl) * J.width * J.height + pix);
+    return true;
+}
+
+// finalize_sample (path_stages.cuh:52-62): optional luminance clamp,
+// non-finite radiance to zero.
+__device__ __forceinline__ float4 finalize_radiance(const JobDesc& J, f3 L) {
+    if (J.firefly_clamp > 0.0f) {
+        const float lum = dot(L, f3{kLumR, kLumG, kLumB});
+        if (lum > J.firefly_clamp) L = L * (J.firefly_clamp * rcpf(lum));
+    }
+    if (!isfinitef_(sum3(L))) L = mk(0.0f);
+    return make_float4(L.x, L.y, L.z, 1.0f);
+}
+
+}  // namespace
+}  // namespace vp

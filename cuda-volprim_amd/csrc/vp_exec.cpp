@@ -35,6 +35,70 @@ void vp_drop_graph(vp_context* ctx) {
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     ctx->graph_exec = nullptr;
     for (auto& k : ctx->graph_key) k = 0;
+    if (ctx->wf_graph) (void)hipGraphExecDestroy(ctx->wf_graph);
+    ctx->wf_graph = nullptr;
+    for (auto& k : ctx->wf_graph_key) k = 0;
+}
+
+void vp_free_wf(vp_context* ctx) {
+    if (ctx->wf_graph) (void)hipGraphExecDestroy(ctx->wf_graph);
+    ctx->wf_graph = nullptr;
+    for (auto& k : ctx->wf_graph_key) k = 0;
+    if (ctx->wf_mem) (void)hipFree(ctx->wf_mem);
+    ctx->wf_mem = nullptr;
+    ctx->wf = vp::WfPool{};
+    if (ctx->wf_done_pinned) (void)hipHostFree(ctx->wf_done_pinned);
+    ctx->wf_done_pinned = nullptr;
+    for (auto& e : ctx->wf_ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
+}
+
+// Path pool of P slots with active-set capacity cap (one allocation, carved).
+static vp_status vp_ensure_wf(vp_context* ctx, uint32_t P, uint32_t cap) {
+    if (ctx->wf_mem && ctx->wf.P == P && ctx->wf.cap == cap) return VP_OK;
+    if (static_cast<uint64_t>(P) * cap * 2 >= (uint64_t(1) << 32))
+        return fail(VP_ERR_INVALID, "wavefront pool: pool_paths x active_cap too large");
+    vp_free_wf(ctx);
+    const size_t p = P;
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) / 256 * 256;
+        return o;
+    };
+    const size_t o_ob = carve(16 * p), o_dj = carve(16 * p), o_bt = carve(16 * p), o_lr = carve(16 * p);
+    const size_t o_rng = carve(16 * p), o_misc = carve(16 * p);
+    const size_t o_pw = carve(48 * p), o_pv = carve(48 * p), o_po = carve(48 * p), o_pd = carve(48 * p);
+    const size_t o_live = carve(4 * p), o_sh = carve(12 * p), o_qc = carve(16), o_fin = carve(8);
+    const size_t o_slab = carve(static_cast<size_t>(2) * cap * p * 4);
+    if (hipMalloc(&ctx->wf_mem, off) != hipSuccess) {
+        ctx->wf_mem = nullptr;
+        return fail(VP_ERR_OOM, "cannot allocate the wavefront path pool (" + std::to_string(off) + " bytes)");
+    }
+    char* b = static_cast<char*>(ctx->wf_mem);
+    vp::WfPool& w = ctx->wf;
+    w.ob = reinterpret_cast<float4*>(b + o_ob);
+    w.dj = reinterpret_cast<float4*>(b + o_dj);
+    w.bt = reinterpret_cast<float4*>(b + o_bt);
+    w.lr = reinterpret_cast<float4*>(b + o_lr);
+    w.rng = reinterpret_cast<uint4*>(b + o_rng);
+    w.misc = reinterpret_cast<uint4*>(b + o_misc);
+    w.pw = reinterpret_cast<float4*>(b + o_pw);
+    w.pv = reinterpret_cast<float4*>(b + o_pv);
+    w.po = reinterpret_cast<float4*>(b + o_po);
+    w.pd = reinterpret_cast<float4*>(b + o_pd);
+    w.live = reinterpret_cast<uint32_t*>(b + o_live);
+    w.shadow = reinterpret_cast<uint32_t*>(b + o_sh);
+    w.qc = reinterpret_cast<unsigned int*>(b + o_qc);
+    w.finished = reinterpret_cast<unsigned long long*>(b + o_fin);
+    w.slab = reinterpret_cast<uint32_t*>(b + o_slab);
+    w.P = P;
+    w.cap = cap;
+    VP_HIP_TRY(hipHostMalloc(&ctx->wf_done_pinned, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+    for (auto& e : ctx->wf_ev) VP_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return VP_OK;
 }
 
 vp_status vp_ensure_jobs(vp_context* ctx, uint32_t n) {
@@ -76,13 +140,83 @@ RenderLaunch vp_resolve_launch(vp_context* ctx, const vp_launch_config* cfg, uin
     L.regen_min = std::max<uint32_t>(1u, std::min<uint32_t>(64u, c.regen_min ? c.regen_min : 1u));
     L.count_events = count;
     L.measure = measure;
+    // the wavefront pipeline has no event-counting or measuring variant
+    L.pipeline = (c.pipeline == VP_PIPELINE_WAVEFRONT && !count && !measure) ? VP_PIPELINE_WAVEFRONT
+                                                                              : VP_PIPELINE_MEGAKERNEL;
+    L.pool = c.pool_paths ? std::max<uint32_t>(4096u, c.pool_paths / 64u * 64u) : (1u << 20);
+    if (L.pipeline == VP_PIPELINE_WAVEFRONT) {
+        const uint32_t cus = static_cast<uint32_t>(ctx->prop.multiProcessorCount);
+        int oe = 1, os = 1, osh = 1;
+        vp_wf_occupancy(0, &oe);
+        vp_wf_occupancy(1, &os);
+        vp_wf_occupancy(2, &osh);
+        L.wf.grid_extend = cus * static_cast<uint32_t>(oe);
+        L.wf.grid_scatter = cus * static_cast<uint32_t>(os);
+        L.wf.grid_shadow = cus * static_cast<uint32_t>(osh);
+    }
     return L;
+}
+
+// Wavefront step: stage the tables, reset the pool, replay the captured
+// two-iteration graph until every path of the step has retired (checked one
+// replay behind, so the device never waits on the host), fold the slices.
+static vp_status vp_run_step_wf(vp_context* ctx, const RenderLaunch& L, uint64_t total_paths, uint32_t max_jobs,
+                                uint32_t max_npix, hipStream_t s) {
+    vp_status st = vp_ensure_wf(ctx, L.pool, L.active_cap);
+    if (st != VP_OK) return st;
+    VP_HIP_TRY(hipMemcpyAsync(ctx->jobs_dev, ctx->jobs_pinned, sizeof(vp::JobDesc) * max_jobs, hipMemcpyHostToDevice,
+                              s));
+    VP_HIP_TRY(hipMemcpyAsync(ctx->job_begin_dev, ctx->begin_pinned, sizeof(uint64_t) * max_jobs,
+                              hipMemcpyHostToDevice, s));
+    VP_HIP_TRY(hipMemcpyAsync(ctx->launch_desc, ctx->desc_pinned, sizeof(uint64_t) * 2, hipMemcpyHostToDevice, s));
+    VP_HIP_TRY(vp_launch_fill_u64(ctx->path_counter, 1u, 0ull, s));
+    VP_HIP_TRY(vp_wf_launch_init(ctx->wf, s));
+    const uint64_t key[6] = {L.pool, L.active_cap, L.wf.grid_extend, L.wf.grid_scatter, L.wf.grid_shadow,
+                             reinterpret_cast<uint64_t>(s)};
+    if (!ctx->wf_graph || !std::equal(key, key + 6, ctx->wf_graph_key)) {
+        if (ctx->wf_graph) (void)hipGraphExecDestroy(ctx->wf_graph);
+        ctx->wf_graph = nullptr;
+        VP_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const hipError_t e = vp_wf_launch_iterations(ctx, ctx->wf, L.wf, s);
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(s, &g);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return fail(VP_ERR_HIP, std::string("wavefront graph capture failed: ") +
+                                        hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        const hipError_t e3 = hipGraphInstantiate(&ctx->wf_graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e3 != hipSuccess) {
+            ctx->wf_graph = nullptr;
+            return fail(VP_ERR_HIP, std::string("wavefront graph instantiation failed: ") + hipGetErrorString(e3));
+        }
+        std::copy(key, key + 6, ctx->wf_graph_key);
+    }
+    // Every replay advances each live path by two bounces or retires it, and
+    // refills free slots, so the loop ends; the bound below only guards
+    // against a broken invariant.
+    const uint64_t max_replays = (total_paths / L.pool + 2) * 1024 + 1024;
+    for (uint64_t k = 0;; ++k) {
+        if (k > max_replays) return fail(VP_ERR_HIP, "wavefront pipeline did not drain");
+        VP_HIP_TRY(hipGraphLaunch(ctx->wf_graph, s));
+        VP_HIP_TRY(hipMemcpyAsync(&ctx->wf_done_pinned[k & 1], ctx->wf.finished, sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, s));
+        VP_HIP_TRY(hipEventRecord(ctx->wf_ev[k & 1], s));
+        if (k > 0) {
+            VP_HIP_TRY(hipEventSynchronize(ctx->wf_ev[(k - 1) & 1]));
+            if (ctx->wf_done_pinned[(k - 1) & 1] >= total_paths) break;
+        }
+    }
+    VP_HIP_TRY(vp_launch_welford(ctx, max_jobs, max_npix, s));
+    return VP_OK;
 }
 
 vp_status vp_run_step(vp_context* ctx, const RenderLaunch& L, uint32_t nslices, uint64_t total_paths,
                       uint32_t max_jobs, uint32_t max_npix, bool use_graph, hipStream_t s) {
     ctx->desc_pinned[0] = nslices;
     ctx->desc_pinned[1] = total_paths;
+    if (L.pipeline == VP_PIPELINE_WAVEFRONT) return vp_run_step_wf(ctx, L, total_paths, max_jobs, max_npix, s);
     auto enqueue = [&]() -> hipError_t {
         hipError_t e = hipMemcpyAsync(ctx->jobs_dev, ctx->jobs_pinned, sizeof(vp::JobDesc) * max_jobs,
                                       hipMemcpyHostToDevice, s);
@@ -127,7 +261,8 @@ vp_status vp_run_jobs(vp_context* ctx, std::vector<JobRun>& runs, const vp_launc
                       int count, int measure, hipStream_t s, double* kernel_ms, uint64_t* paths) {
     const RenderLaunch L = vp_resolve_launch(ctx, cfg, active_cap, count, measure);
     const size_t lanes = static_cast<size_t>(L.grid) * L.block;
-    vp_status st = vp_ensure_slab(ctx, 2 * static_cast<size_t>(active_cap) * lanes);
+    vp_status st = VP_OK;
+    if (L.pipeline == VP_PIPELINE_MEGAKERNEL) st = vp_ensure_slab(ctx, 2 * static_cast<size_t>(active_cap) * lanes);
     if (st != VP_OK) return st;
     st = vp_ensure_jobs(ctx, static_cast<uint32_t>(runs.size()));
     if (st != VP_OK) return st;

@@ -60,6 +60,29 @@ struct JobDesc {
     float4* mean;     // width * height Welford mean of the job
 };
 
+// Path pool of the wavefront pipeline (structure of arrays over P slots).
+struct WfPool {
+    float4* ob;    // origin.xyz, bounce
+    float4* dj;    // direction.xyz, job
+    float4* bt;    // throughput.xyz, scatter distance
+    float4* lr;    // radiance.xyz, active-set size
+    uint4* rng;    // PCG32 state, increment
+    uint4* misc;   // key (2 words), sample slot, flags (set buffer, state, pending terms)
+    float4* pw;    // [3][P] pending term weight.xyz, s1
+    float4* pv;    // [3][P] pending term factor.xyz, s2
+    float4* po;    // [3][P] shadow-ray origin.xyz, set buffer | count << 1
+    float4* pd;    // [3][P] shadow-ray direction.xyz, transmittance
+    uint32_t* live;      // [P] live-slot queue
+    uint32_t* shadow;    // [3P] shadow-ray queue (slot << 2 | term)
+    unsigned int* qc;    // [4] live / shadow counts, two parity sets
+    unsigned long long* finished;  // paths retired (or padding) so far
+    uint32_t* slab;      // [2][cap][P] active sets
+    uint32_t P, cap;
+};
+struct WfLaunch {
+    uint32_t grid_extend, grid_scatter, grid_shadow;
+};
+
 // Counter slots of the render kernel.
 enum CounterSlot : int {
     kCntActiveOverflow = 0,
@@ -110,6 +133,14 @@ struct vp_context {
     hipGraphExec_t graph_exec = nullptr;
     uint64_t graph_key[6] = {0, 0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // wavefront pipeline: path pool, its allocation, the captured
+    // two-iteration graph and the pinned completion counters
+    vp::WfPool wf{};
+    void* wf_mem = nullptr;
+    hipGraphExec_t wf_graph = nullptr;
+    uint64_t wf_graph_key[6] = {0, 0, 0, 0, 0, 0};
+    unsigned long long* wf_done_pinned = nullptr;  // [2]
+    hipEvent_t wf_ev[2] = {nullptr, nullptr};
     std::mutex mu;
     uint32_t last_launches = 0;
     vp_launch_config tuned{};
@@ -163,6 +194,9 @@ struct RenderLaunch {
     uint32_t regen_min;
     int count_events;
     int measure;
+    uint32_t pipeline;  // VP_PIPELINE_MEGAKERNEL or VP_PIPELINE_WAVEFRONT
+    uint32_t pool;      // wavefront path slots
+    vp::WfLaunch wf;
 };
 hipError_t vp_launch_step(vp_context* ctx, const RenderLaunch& L, uint32_t max_jobs, uint32_t max_npix,
                           hipStream_t s);
@@ -200,3 +234,9 @@ hipError_t vp_launch_math(int fn, const float* x, const float* y, uint32_t n, fl
 hipError_t vp_launch_prim(int fn, const vp::PrimRecord* rec, const float* args, uint32_t m, float* out,
                           hipStream_t s);
 int vp_render_occupancy(uint32_t block, uint32_t active_cap, int count_events, int* blocks_per_cu);
+// wavefront pipeline launchers (vp_wavefront.hip)
+hipError_t vp_wf_launch_init(const vp::WfPool& w, hipStream_t s);
+hipError_t vp_wf_launch_iterations(vp_context* ctx, const vp::WfPool& w, const vp::WfLaunch& L, hipStream_t s);
+int vp_wf_occupancy(int which, int* blocks_per_cu);
+hipError_t vp_launch_welford(vp_context* ctx, uint32_t max_jobs, uint32_t max_npix, hipStream_t s);
+void vp_free_wf(vp_context* ctx);

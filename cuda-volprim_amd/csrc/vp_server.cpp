@@ -13,8 +13,8 @@
 // any mix.
 //
 // Autotuning: vp_autotune times a reduced copy of a job mix for a grid of
-// launch shapes (workgroup size, persistent workgroups per CU, regeneration
-// threshold) and sizes the active-set capacity from the measured maximum
+// launch shapes (megakernel: workgroup size, persistent workgroups per CU,
+// regeneration threshold; wavefront pipeline: path-pool size) and sizes the active-set capacity from the measured maximum
 // point overlap (ceil(1.125 v) rounded up to 16, the rule of the reference's
 // --measure-caps, src/thesis/host/app/renderer.cpp:405-418, and at least the
 // largest capacity the jobs request); the best shape
@@ -196,7 +196,9 @@ vp_status vp_server_create(vp_context* ctx, const vp_server_config* cfg, vp_serv
     }
     const uint32_t cap = ctx->have_tuned && ctx->tuned.active_cap ? ctx->tuned.active_cap : 128u;
     S->L = vp_resolve_launch(ctx, nullptr, cap, 0, 0);
-    vp_status st = vp_ensure_slab(ctx, 2 * static_cast<size_t>(cap) * S->L.grid * S->L.block);
+    vp_status st = S->L.pipeline == VP_PIPELINE_MEGAKERNEL
+                       ? vp_ensure_slab(ctx, 2 * static_cast<size_t>(cap) * S->L.grid * S->L.block)
+                       : VP_OK;
     if (st == VP_OK) st = vp_ensure_jobs(ctx, S->cfg.max_inflight);
     if (st == VP_OK) st = vp_reset_counters(ctx, ctx->stream);
     if (st != VP_OK) {
@@ -367,7 +369,7 @@ vp_status vp_autotune(vp_context* ctx, vp_job* jobs, uint32_t njobs, vp_launch_c
         for (uint32_t bi = 0; bi < 2; ++bi) {
             if (bi == 1 && bpcs[1] == bpcs[0]) continue;
             for (uint32_t rg : regens) {
-                vp_launch_config c{b, bpcs[bi], cap, 0, rg};
+                vp_launch_config c{b, bpcs[bi], cap, 0, rg, VP_PIPELINE_MEGAKERNEL, 0};
                 std::vector<JobRun> runs;
                 make_runs(runs);
                 double ms = 0.0;
@@ -382,6 +384,23 @@ vp_status vp_autotune(vp_context* ctx, vp_job* jobs, uint32_t njobs, vp_launch_c
                     best_cfg = c;
                 }
             }
+        }
+    }
+    // the wavefront pipeline at three pool sizes
+    for (uint32_t pool : {1u << 19, 1u << 20, 1u << 21}) {
+        if (static_cast<uint64_t>(pool) * cap * 2 >= (uint64_t(1) << 32)) continue;
+        vp_launch_config c{256u, 0u, cap, 0u, 1u, VP_PIPELINE_WAVEFRONT, pool};
+        std::vector<JobRun> runs;
+        make_runs(runs);
+        uint64_t paths = 0;
+        const auto t0 = clk::now();
+        vp_status st = vp_run_jobs(ctx, runs, &c, cap, 0, 0, s, nullptr, &paths);
+        if (st != VP_OK) return st;
+        const double wall = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+        const double rate = static_cast<double>(paths) / std::max(wall, 1e-3);
+        if (rate > best_rate) {
+            best_rate = rate;
+            best_cfg = c;
         }
     }
     ctx->tuned = best_cfg;

@@ -1,0 +1,457 @@
+// vp_wavefront.hip — the wavefront-queue variant of the renderer.
+//
+// A pool of P path slots lives in HBM (structure of arrays).  One iteration
+// advances every live path by one bounce with four kernels, each over a
+// compacted queue so its wavefronts do one kind of work:
+//   k_wf_shade_regen  per slot: add the previous bounce's shadow-ray terms to
+//                     L in the estimator's order, retire finished paths to
+//                     their sample slots, claim new paths for free slots
+//                     (one atomic per wavefront, ballot + popcount ranks),
+//                     trace new paths' origin sets, and append live slots to
+//                     the live queue (ballot prefix compaction);
+//   k_wf_extend       per live path: argmin free flight (origin set + one
+//                     pruned BVH walk);
+//   k_wf_scatter      per live path: bounce-0 analytic direct ray, active-set
+//                     rebuild at the scatter vertex, albedo, phase sampling,
+//                     NEE direction sampling, Russian roulette; shadow rays
+//                     are appended to the shadow queue;
+//   k_wf_shadow       per shadow ray: fixed-point transmittance.
+// Each path draws its random numbers in the same order as the megakernel
+// (camera jitter, phase, NEE, roulette) and adds its radiance terms in the
+// same order, so both variants write identical per-sample radiance.
+//
+// Queue counters come in two parity sets; the first kernel of an iteration
+// clears the set the next iteration will use, so a two-iteration sequence
+// can be captured once as a HIP graph and replayed (no memset nodes).
+//
+// Reference (kfernandez31/cuda-volprim): device/entry/raygen.cuh:21-243,
+// device/entry/path_stages.cuh:25-226, device/core/sampling.cuh:257-583.
+#include "vp_device.h"
+
+namespace vp {
+namespace {
+
+enum : uint32_t { kSlotFree = 0, kSlotAlive = 1, kSlotDone = 2 };
+
+// misc.w bit fields
+__device__ __forceinline__ uint32_t f_cur(uint32_t w) { return w & 1u; }
+__device__ __forceinline__ uint32_t f_state(uint32_t w) { return (w >> 1) & 3u; }
+__device__ __forceinline__ uint32_t f_npend(uint32_t w) { return (w >> 3) & 3u; }
+__device__ __forceinline__ uint32_t f_pack(uint32_t cur, uint32_t state, uint32_t npend) {
+    return cur | (state << 1) | (npend << 3);
+}
+
+// Wave-aggregated append: lanes with `want` get consecutive indices from one
+// atomicAdd on *counter per wavefront.
+__device__ __forceinline__ uint32_t wave_append(unsigned int* counter, bool want) {
+    const unsigned long long m = __ballot(want);
+    if (m == 0) return 0;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = __ffsll(static_cast<long long>(m)) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, static_cast<unsigned int>(__popcll(m)));
+    base = __shfl(base, leader);
+    const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    return base + static_cast<uint32_t>(__popcll(m & lt));
+}
+
+__device__ __forceinline__ void wave_count(unsigned long long* counter, bool flag) {
+    const unsigned long long m = __ballot(flag);
+    if (m != 0 && __lane_id() == static_cast<uint32_t>(__ffsll(static_cast<long long>(m)) - 1))
+        atomicAdd(counter, static_cast<unsigned long long>(__popcll(m)));
+}
+
+#define VP_WF_LB __launch_bounds__(256)
+
+__global__ void VP_WF_LB k_wf_init(WfPool w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < w.P) w.misc[i] = make_uint4(0u, 0u, 0u, f_pack(0u, kSlotFree, 0u));
+    if (i < 4) w.qc[i] = 0u;
+    if (i == 0) *w.finished = 0ull;
+}
+
+__global__ void VP_WF_LB k_wf_shade_regen(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begin,
+                                          const uint64_t* __restrict__ launch_desc,
+                                          unsigned long long* __restrict__ path_counter, WfPool w, uint32_t par,
+                                          unsigned long long* __restrict__ counters) {
+    const uint32_t njobs = static_cast<uint32_t>(launch_desc[0]);
+    const uint64_t total_paths = launch_desc[1];
+    const uint32_t P = w.P;
+    if (blockIdx.x == 0 && threadIdx.x < 2) w.qc[2 * (par ^ 1u) + threadIdx.x] = 0u;
+    // one slot per thread; the grid covers the pool
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = i < P;
+    uint4 m = in ? w.misc[i] : make_uint4(0u, 0u, 0u, f_pack(0u, kSlotFree, 0u));
+    uint32_t cur = f_cur(m.w), state = f_state(m.w);
+    const uint32_t npend = f_npend(m.w);
+    bool retired = false;
+    if (in && state != kSlotFree) {
+        float4 lr = w.lr[i];
+        f3 L = f3{lr.x, lr.y, lr.z};
+        for (uint32_t k = 0; k < npend; ++k) {
+            const float T = w.pd[k * P + i].w;
+            const float4 a = w.pw[k * P + i], b = w.pv[k * P + i];
+            L = L + ((((f3{a.x, a.y, a.z} * T) * f3{b.x, b.y, b.z}) * a.w) * b.w);
+        }
+        if (state == kSlotDone) {
+            const JobDesc& J = jobs[__float_as_uint(w.dj[i].w)];
+            J.samples[m.z] = finalize_radiance(J, L);
+            state = kSlotFree;
+            retired = true;
+        } else {
+            w.lr[i] = make_float4(L.x, L.y, L.z, lr.w);
+        }
+    }
+    wave_count(w.finished, retired);
+
+    // claim new paths for free slots
+    const bool want = in && state == kSlotFree;
+    const unsigned long long wmask = __ballot(want);
+    bool padded = false;
+    uint32_t overflow = 0;
+    if (wmask) {
+        const uint32_t lane = __lane_id();
+        const uint32_t leader = __ffsll(static_cast<long long>(wmask)) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(path_counter, static_cast<unsigned long long>(__popcll(wmask)));
+        base = __shfl(base, leader);
+        if (want) {
+            const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+            const uint64_t p = base + __popcll(wmask & lt);
+            if (p < total_paths) {
+                const uint32_t j = find_job(job_begin, njobs, p);
+                const JobDesc& J = jobs[j];
+                f3 o, d;
+                Pcg32 rng;
+                uint64_t key;
+                uint32_t slot;
+                if (path_begin(J, p - J.path_begin, o, d, rng, key, slot)) {
+                    // origin-inside set (sampling.cuh:322-341) by point query, into buffer 0
+                    uint32_t na = 0, visits = 0;
+                    uint32_t* act = w.slab + i;
+                    const SceneView sc = J.scene;
+                    walk_point(sc, o, visits, [&](uint32_t prim) {
+                        if (point_inside(load_prim(sc.prims, prim), o)) {
+                            if (na < w.cap) act[static_cast<size_t>(na) * P] = prim, ++na;
+                            else ++overflow;
+                        }
+                    });
+                    w.ob[i] = make_float4(o.x, o.y, o.z, __uint_as_float(0u));
+                    w.dj[i] = make_float4(d.x, d.y, d.z, __uint_as_float(j));
+                    w.bt[i] = make_float4(1.0f, 1.0f, 1.0f, kTMax);
+                    w.lr[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(na));
+                    w.rng[i] = make_uint4(static_cast<uint32_t>(rng.state), static_cast<uint32_t>(rng.state >> 32),
+                                          static_cast<uint32_t>(rng.inc), static_cast<uint32_t>(rng.inc >> 32));
+                    m = make_uint4(static_cast<uint32_t>(key), static_cast<uint32_t>(key >> 32), slot, 0u);
+                    cur = 0;
+                    state = kSlotAlive;
+                } else {
+                    padded = true;  // a partial tile's padding path: done at once
+                }
+            }
+        }
+    }
+    wave_count(w.finished, padded);
+    if (overflow) atomicAdd(&counters[kCntActiveOverflow], static_cast<unsigned long long>(overflow));
+    if (in) {
+        m.w = f_pack(cur, state, 0u);
+        w.misc[i] = m;
+    }
+    const bool live = in && state == kSlotAlive;
+    const uint32_t q = wave_append(&w.qc[2 * par], live);
+    if (live) w.live[q] = i;
+}
+
+__global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par) {
+    extern __shared__ uint32_t s_leafq[];
+    const LeafQueue lq{s_leafq + threadIdx.x, blockDim.x};
+    const uint32_t n = w.qc[2 * par];
+    const uint32_t P = w.P;
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
+        const uint32_t i = w.live[idx];
+        const float4 ob = w.ob[i], dj = w.dj[i];
+        const f3 o = f3{ob.x, ob.y, ob.z}, d = f3{dj.x, dj.y, dj.z};
+        const uint32_t b = __float_as_uint(ob.w);
+        const JobDesc& J = jobs[__float_as_uint(dj.w)];
+        const SceneView sc = J.scene;
+        const uint4 m = w.misc[i];
+        const uint64_t key = static_cast<uint64_t>(m.x) | (static_cast<uint64_t>(m.y) << 32);
+        const uint32_t na = __float_as_uint(w.lr[i].w);
+        const uint32_t* act = w.slab + static_cast<size_t>(f_cur(m.w)) * w.cap * P + i;
+        // argmin over the origin set (sampling.cuh:356-370)
+        float ts = kTMax;
+        for (uint32_t k = 0; k < na; ++k) {
+            const uint32_t pi = act[static_cast<size_t>(k) * P];
+            const PrimRecord pr = load_prim(sc.prims, pi);
+            const float te = exit_from_inside(pr, o, d);
+            const float tau = free_flight_tau(prim_uniform(key, b, 0u, pi));
+            const float t = inv_cdf(pr, o, d, tau);
+            if (t >= 0.0f && t < ts && t <= te) ts = t;
+        }
+        // argmin over entered primitives (sampling.cuh:372-412) in one pruned walk
+        const f3 inv = safe_inv(d);
+        float tmax = ts;
+        uint32_t visits = 0;
+        walk_ray_q(sc, o, inv, tmax, visits, lq, [&](uint32_t prim) -> bool {
+            const PrimRecord pr = load_prim(sc.prims, prim);
+            float wl2;
+            const float tj = entry_t(pr, o, d, &wl2);
+            if (tj < 0.0f) return true;
+            const float tau = free_flight_tau(prim_uniform(key, b, 1u, prim));
+            float t = inv_cdf_span(pr, o, d, tj, tau);
+            if (t >= 0.0f && t < tj) t = tj;
+            if (t >= 0.0f && t < ts) {
+                const float te = exit_from_entry(pr, o, d, tj, wl2);
+                if (t <= te) {
+                    ts = t;
+                    tmax = ts;
+                }
+            }
+            return true;
+        });
+        w.bt[i].w = ts;
+    }
+}
+
+// Pending radiance term k of slot i: contribution (((w*T)*v)*s1)*s2 once the
+// shadow kernel has written T.
+__device__ __forceinline__ void put_pending(const WfPool& w, uint32_t i, uint32_t k, f3 org, uint32_t set,
+                                            uint32_t count, f3 dir, f3 wt, f3 v, float s1, float s2) {
+    const uint32_t e = k * w.P + i;
+    w.po[e] = make_float4(org.x, org.y, org.z, __uint_as_float(set | (count << 1)));
+    w.pd[e] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+    w.pw[e] = make_float4(wt.x, wt.y, wt.z, s1);
+    w.pv[e] = make_float4(v.x, v.y, v.z, s2);
+}
+
+__global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par,
+                                      unsigned long long* __restrict__ counters) {
+    extern __shared__ uint32_t s_leafq[];
+    const LeafQueue lq{s_leafq + threadIdx.x, blockDim.x};
+    const uint32_t n = w.qc[2 * par];
+    const uint32_t P = w.P;
+    const size_t plane = static_cast<size_t>(w.cap) * P;
+    uint32_t overflow = 0;
+    // every lane of a wavefront runs the same trip count so the queue
+    // appends below stay wave-uniform
+    const uint32_t span = gridDim.x * blockDim.x;
+    const uint32_t first = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    for (uint32_t base = first; base < n; base += span) {
+        const uint32_t idx = base + __lane_id();
+        const bool active = idx < n;
+        uint32_t npend = 0;
+        uint32_t i = 0;
+        if (active) {
+            i = w.live[idx];
+            const float4 ob = w.ob[i], dj = w.dj[i], bt = w.bt[i];
+            const f3 o = f3{ob.x, ob.y, ob.z}, d = f3{dj.x, dj.y, dj.z};
+            const uint32_t b = __float_as_uint(ob.w);
+            const JobDesc& J = jobs[__float_as_uint(dj.w)];
+            const SceneView sc = J.scene;
+            const EnvView env = env_view(J.env);
+            uint4 m = w.misc[i];
+            uint32_t cur = f_cur(m.w);
+            const uint32_t na = __float_as_uint(w.lr[i].w);
+            f3 beta = f3{bt.x, bt.y, bt.z};
+            const float ts = bt.w;
+            uint4 rs = w.rng[i];
+            Pcg32 rng;
+            rng.state = static_cast<uint64_t>(rs.x) | (static_cast<uint64_t>(rs.y) << 32);
+            rng.inc = static_cast<uint64_t>(rs.z) | (static_cast<uint64_t>(rs.w) << 32);
+            bool finish = false;
+            uint32_t nn = na;
+
+            // analytic direct term at bounce 0 (raygen.cuh:148-160): T along the
+            // camera ray with the origin set
+            if (b == 0) put_pending(w, i, npend++, o, cur, na, d, beta, env_lookup(env, d), 1.0f, 1.0f);
+
+            if (ts >= kTMax) {
+                finish = true;
+            } else {
+                // rebuild the active set at the scatter point (sampling.cuh:422-458)
+                const uint32_t* act = w.slab + cur * plane + i;
+                uint32_t* nxt = w.slab + (cur ^ 1u) * plane + i;
+                const f3 x = o + ts * d;
+                uint32_t cnt = 0, ia = 0, visits = 0;
+                auto push = [&](uint32_t v) {
+                    if (cnt < w.cap) nxt[static_cast<size_t>(cnt) * P] = v, ++cnt;
+                    else ++overflow;
+                };
+                walk_point_q(sc, x, visits, lq, [&](uint32_t c) {
+                    while (ia < na) {
+                        const uint32_t a = act[static_cast<size_t>(ia) * P];
+                        if (a >= c) break;
+                        if (ts <= exit_from_inside(load_prim(sc.prims, a), o, d)) push(a);
+                        ++ia;
+                    }
+                    const PrimRecord pr = load_prim(sc.prims, c);
+                    bool keep = false;
+                    if (ia < na && act[static_cast<size_t>(ia) * P] == c) {
+                        keep = ts <= exit_from_inside(pr, o, d);
+                        ++ia;
+                    }
+                    if (!keep) {
+                        float wl2;
+                        const float tj = entry_t(pr, o, d, &wl2);
+                        keep = tj >= 0.0f && tj <= ts && ts <= exit_from_entry(pr, o, d, tj, wl2);
+                    }
+                    if (keep) push(c);
+                });
+                while (ia < na) {
+                    const uint32_t a = act[static_cast<size_t>(ia) * P];
+                    if (ts <= exit_from_inside(load_prim(sc.prims, a), o, d)) push(a);
+                    ++ia;
+                }
+                cur ^= 1u;
+                nn = cnt;
+                const uint32_t* nact = nxt;
+
+                float phase_pdf;
+                const f3 wo = phase_sample(J.hg, d, rng, &phase_pdf);
+
+                // evaluate_albedo (sampling.cuh:257-283)
+                f3 aa = mk(0.0f);
+                float aw = 0.0f;
+                for (uint32_t k = 0; k < nn; ++k) {
+                    const PrimRecord pr = load_prim(sc.prims, nact[static_cast<size_t>(k) * P]);
+                    const float wgt = pr.mass * prim_pdf(pr, x);
+                    aa = aa + f3{pr.ar, pr.ag, pr.ab} * wgt;
+                    aw += wgt;
+                }
+                const f3 albedo = aw <= 0.0f ? mk(0.0f) : aa * rcpf(aw);
+
+                // accumulate_nee_direct (path_stages.cuh:148-226): directions and
+                // weights now, transmittance in the shadow kernel
+                const f3 base3 = beta * albedo;
+                const f3 luma = f3{kLumR, kLumG, kLumB};
+                if (J.use_ris) {
+                    float wsum = 0.0f;
+                    f3 ydir = mk(0.0f), yenv = mk(0.0f);
+                    for (uint32_t c = 0; c < J.ris_candidates; ++c) {
+                        float cpdf;
+                        const f3 cwo = env_sample(env, rng, &cpdf);
+                        if (cpdf <= 0.0f) continue;
+                        const f3 envc = env_lookup(env, cwo);
+                        const float lumc = dot(envc, luma);
+                        const float wgt = phase_eval(J.hg, d, cwo) * lumc * rcpf(cpdf);
+                        if (!(wgt > 0.0f)) continue;
+                        wsum += wgt;
+                        if (next_uniform(rng) * wsum < wgt) {
+                            ydir = cwo;
+                            yenv = envc;
+                        }
+                    }
+                    if (wsum > 0.0f) {
+                        const float lumy = dot(yenv, luma);
+                        if (lumy > 0.0f) {
+                            const float Wr = wsum * rcpf(static_cast<float>(J.ris_candidates));
+                            put_pending(w, i, npend++, x, cur, nn, ydir, base3 * (yenv * rcpf(lumy)), mk(Wr), 1.0f,
+                                        1.0f);
+                        }
+                    }
+                } else {
+                    float apdf;
+                    const f3 awo = phase_sample(J.hg, d, rng, &apdf);
+                    const float pb_a = env_pdf(env, awo);
+                    const float sa = apdf + pb_a;
+                    const float wa = sa > 0.0f ? apdf / sa : 0.0f;
+                    const f3 enva = env_lookup(env, awo);
+                    put_pending(w, i, npend++, x, cur, nn, awo, base3 * enva, mk(wa), 1.0f, 1.0f);
+
+                    float bpdf;
+                    const f3 bwo = env_sample(env, rng, &bpdf);
+                    const float ph_b = phase_eval(J.hg, d, bwo);
+                    const float sb = bpdf + ph_b;
+                    const float wb = sb > 0.0f ? bpdf / sb : 0.0f;
+                    const f3 envb = env_lookup(env, bwo);
+                    put_pending(w, i, npend++, x, cur, nn, bwo, base3 * envb, mk(wb), ph_b, rcpf(bpdf));
+                }
+                beta = beta * albedo;
+
+                // russian_roulette (path_stages.cuh:25-37) and guards (raygen.cuh:191-204)
+                if (b >= J.rr_depth) {
+                    const float ps = minf(J.rr_max_survival, max3(beta));
+                    if (next_uniform(rng) > ps || ps <= 0.0f) finish = true;
+                    else beta = beta * rcpf(ps);
+                }
+                if (!finish && max3(beta) < kMinThroughput) finish = true;
+                if (!finish && !isfinitef_(sum3(beta))) finish = true;
+                if (!finish) {
+                    w.ob[i] = make_float4(x.x, x.y, x.z, __uint_as_float(b + 1));
+                    w.dj[i] = make_float4(wo.x, wo.y, wo.z, dj.w);
+                    if (b + 1 >= J.max_depth) finish = true;
+                }
+            }
+            w.bt[i] = make_float4(beta.x, beta.y, beta.z, ts);
+            w.lr[i].w = __uint_as_float(nn);
+            w.rng[i] = make_uint4(static_cast<uint32_t>(rng.state), static_cast<uint32_t>(rng.state >> 32),
+                                  static_cast<uint32_t>(rng.inc), static_cast<uint32_t>(rng.inc >> 32));
+            m.w = f_pack(cur, finish ? kSlotDone : kSlotAlive, npend);
+            w.misc[i].w = m.w;
+        }
+        // shadow-queue appends, one atomic per wavefront and term index
+        for (uint32_t k = 0; k < 3; ++k) {
+            const bool has = active && npend > k;
+            const uint32_t q = wave_append(&w.qc[2 * par + 1], has);
+            if (has) w.shadow[q] = (i << 2) | k;
+        }
+    }
+    if (overflow) atomicAdd(&counters[kCntActiveOverflow], static_cast<unsigned long long>(overflow));
+}
+
+__global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par) {
+    extern __shared__ uint32_t s_leafq[];
+    const LeafQueue lq{s_leafq + threadIdx.x, blockDim.x};
+    const uint32_t n = w.qc[2 * par + 1];
+    const uint32_t P = w.P;
+    Ev<false> ev;
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
+        const uint32_t e = w.shadow[idx];
+        const uint32_t i = e >> 2, k = e & 3u;
+        const float4 po = w.po[k * P + i];
+        const float4 pd = w.pd[k * P + i];
+        const uint32_t sc_bits = __float_as_uint(po.w);
+        const uint32_t set = sc_bits & 1u, count = sc_bits >> 1;
+        const JobDesc& J = jobs[__float_as_uint(w.dj[i].w)];
+        const uint32_t* act = w.slab + static_cast<size_t>(set) * w.cap * P + i;
+        const float T = transmittance(J.scene, f3{po.x, po.y, po.z}, f3{pd.x, pd.y, pd.z}, act, count, P, lq, ev);
+        w.pd[k * P + i].w = T;
+    }
+}
+
+inline uint32_t cdiv(uint64_t a, uint32_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
+
+}  // namespace
+}  // namespace vp
+
+using namespace vp;
+
+hipError_t vp_wf_launch_init(const WfPool& w, hipStream_t s) {
+    hipLaunchKernelGGL(k_wf_init, dim3(cdiv(w.P, 256)), dim3(256), 0, s, w);
+    return hipGetLastError();
+}
+
+// Two iterations (parities 0 and 1) of the pipeline.
+hipError_t vp_wf_launch_iterations(vp_context* ctx, const WfPool& w, const WfLaunch& L, hipStream_t s) {
+    const size_t lds = sizeof(uint32_t) * kLeafQ * 256;
+    for (uint32_t par = 0; par < 2; ++par) {
+        hipLaunchKernelGGL(k_wf_shade_regen, dim3(cdiv(w.P, 256)), dim3(256), 0, s, ctx->jobs_dev,
+                           ctx->job_begin_dev, ctx->launch_desc, ctx->path_counter, w, par, ctx->counters);
+        hipLaunchKernelGGL(k_wf_extend, dim3(L.grid_extend), dim3(256), lds, s, ctx->jobs_dev, w, par);
+        hipLaunchKernelGGL(k_wf_scatter, dim3(L.grid_scatter), dim3(256), lds, s, ctx->jobs_dev, w, par,
+                           ctx->counters);
+        hipLaunchKernelGGL(k_wf_shadow, dim3(L.grid_shadow), dim3(256), lds, s, ctx->jobs_dev, w, par);
+    }
+    return hipGetLastError();
+}
+
+int vp_wf_occupancy(int which, int* blocks_per_cu) {
+    const size_t lds = sizeof(uint32_t) * kLeafQ * 256;
+    const void* f = which == 0   ? reinterpret_cast<const void*>(k_wf_extend)
+                    : which == 1 ? reinterpret_cast<const void*>(k_wf_scatter)
+                                 : reinterpret_cast<const void*>(k_wf_shadow);
+    int nb = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 256, lds);
+    *blocks_per_cu = nb > 0 ? nb : 1;
+    return e == hipSuccess ? 0 : -1;
+}

@@ -63,7 +63,12 @@ class RenderStats(ctypes.Structure):
 
 class LaunchConfig(ctypes.Structure):
     _fields_ = [("block", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
-                ("active_cap", ctypes.c_uint32), ("batch_spp", ctypes.c_uint32), ("regen_min", ctypes.c_uint32)]
+                ("active_cap", ctypes.c_uint32), ("batch_spp", ctypes.c_uint32), ("regen_min", ctypes.c_uint32),
+                ("pipeline", ctypes.c_uint32), ("pool_paths", ctypes.c_uint32)]
+
+
+PIPELINE_MEGAKERNEL = 0
+PIPELINE_WAVEFRONT = 1
 
 
 class Job(ctypes.Structure):

@@ -106,7 +106,11 @@ typedef struct {
     uint32_t active_cap;   /* active-set capacity (entries per path) */
     uint32_t batch_spp;    /* samples per pixel per launch (0 = all) */
     uint32_t regen_min;    /* idle lanes a wavefront waits for before claiming new paths (0 = 1) */
+    uint32_t pipeline;     /* VP_PIPELINE_MEGAKERNEL (0) or VP_PIPELINE_WAVEFRONT (1) */
+    uint32_t pool_paths;   /* wavefront: path slots in flight (0 = 1 Mi) */
 } vp_launch_config;
+#define VP_PIPELINE_MEGAKERNEL 0u
+#define VP_PIPELINE_WAVEFRONT 1u
 
 /* ---- context ---------------------------------------------------------- */
 vp_status vp_context_create(int device, vp_context** out);

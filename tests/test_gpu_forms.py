@@ -34,6 +34,11 @@ def test_batched_jobs_equal_individual_renders(ctx):
     batched2, _ = ctx.render_jobs(jobs, launch=volprim.LaunchConfig(64, 2, 0, 2, 8))
     for a, b in zip(alone, batched2):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    # the wavefront pipeline over the same job table
+    wf = volprim.LaunchConfig(256, 0, 0, 0, 1, volprim.PIPELINE_WAVEFRONT, 4096)
+    batched3, _ = ctx.render_jobs(jobs, launch=wf)
+    for a, b in zip(alone, batched3):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     for o in scs + envs:
         o.close()
 
@@ -76,6 +81,7 @@ def test_autotune_picks_a_config_and_keeps_results(ctx):
     before = ctx.render(*jobs[1])[0]
     best = ctx.autotune(jobs)
     assert best.block in (64, 128, 256) and best.active_cap >= 16 and best.regen_min >= 1
+    assert best.pipeline in (volprim.PIPELINE_MEGAKERNEL, volprim.PIPELINE_WAVEFRONT)
     after = ctx.render(*jobs[1])[0]
     assert np.array_equal(before.view(np.uint32), after.view(np.uint32))
     for o in scs + envs:

@@ -178,6 +178,31 @@ def test_render_bitexact(ctx, case):
     assert st["active_overflow"] == k["active_overflow"]
 
 
+@pytest.mark.parametrize("case", sorted(RENDER_CASES))
+def test_render_wavefront_bitexact(ctx, case):
+    # the wavefront-queue pipeline (pool of 4096 path slots, refilled every
+    # iteration) writes the same per-sample radiance as the megakernel
+    c = RENDER_CASES[case]
+    env = scenes.constant_env() if c["env"] == "const" else scenes.sky_env(64, 32, sun_radius_deg=4.0, sun_scale=50.0)
+    cam = volprim.camera_perspective(**scenes.default_camera_args(c["w"], c["h"]))
+    print(f"[case] wavefront render {case} {c['w']}x{c['h']} spp={c['spp']}", flush=True)
+    rows = c["rows"]()
+    sc = ctx.scene(rows)
+    ev = ctx.env(env)
+    _, _, rec, _ = sc.export()
+    p = volprim.default_params(width=c["w"], height=c["h"], spp=c["spp"], **c["kw"])
+    launch = volprim.LaunchConfig(256, 0, 0, 0, 1, volprim.PIPELINE_WAVEFRONT, 4096)
+    imgs, st = ctx.render_jobs([(sc, ev, cam, p)], launch=launch)
+    sc.close()
+    ev.close()
+    ref, _, k = vpo.render(rec, env, cam.pod(), p.width, p.height, p.spp, seed=p.seed, max_depth=p.max_depth,
+                           rr_depth=p.rr_depth, rr_max_survival=p.rr_max_survival, firefly_clamp=p.firefly_clamp,
+                           filter_stddev=p.filter_stddev, phase_g=p.phase_g, use_ris=bool(p.use_ris),
+                           ris_candidates=p.ris_candidates, active_cap=p.active_cap)
+    assert_bitwise(imgs[0], ref, f"wavefront render {case}")
+    assert st["active_overflow"] == k["active_overflow"]
+
+
 def test_render_orthographic_bitexact(ctx):
     rows = scenes.random_scene(30, 9)
     cam = volprim.camera_orthographic(20, 20, np.array([0, 0, -5], np.float32), np.zeros(3, np.float32),
