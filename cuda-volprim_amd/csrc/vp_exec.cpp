@@ -56,7 +56,7 @@ void vp_free_wf(vp_context* ctx) {
 }
 
 // Path pool of P slots with active-set capacity cap (one allocation, carved).
-static vp_status vp_ensure_wf(vp_context* ctx, uint32_t P, uint32_t cap) {
+vp_status vp_ensure_wf(vp_context* ctx, uint32_t P, uint32_t cap) {
     if (ctx->wf_mem && ctx->wf.P == P && ctx->wf.cap == cap) return VP_OK;
     if (static_cast<uint64_t>(P) * cap * 2 >= (uint64_t(1) << 32))
         return fail(VP_ERR_INVALID, "wavefront pool: pool_paths x active_cap too large");
@@ -71,7 +71,8 @@ static vp_status vp_ensure_wf(vp_context* ctx, uint32_t P, uint32_t cap) {
     const size_t o_ob = carve(16 * p), o_dj = carve(16 * p), o_bt = carve(16 * p), o_lr = carve(16 * p);
     const size_t o_rng = carve(16 * p), o_misc = carve(16 * p);
     const size_t o_pw = carve(48 * p), o_pv = carve(48 * p), o_po = carve(48 * p), o_pd = carve(48 * p);
-    const size_t o_live = carve(4 * p), o_sh = carve(12 * p), o_qc = carve(16), o_fin = carve(8);
+    const size_t o_live = carve(4 * p), o_sh = carve(12 * p), o_free = carve(4 * p), o_qc = carve(32);
+    const size_t o_cb = carve(16), o_fin = carve(8);
     const size_t o_slab = carve(static_cast<size_t>(2) * cap * p * 4);
     if (hipMalloc(&ctx->wf_mem, off) != hipSuccess) {
         ctx->wf_mem = nullptr;
@@ -91,7 +92,9 @@ static vp_status vp_ensure_wf(vp_context* ctx, uint32_t P, uint32_t cap) {
     w.pd = reinterpret_cast<float4*>(b + o_pd);
     w.live = reinterpret_cast<uint32_t*>(b + o_live);
     w.shadow = reinterpret_cast<uint32_t*>(b + o_sh);
+    w.free_q = reinterpret_cast<uint32_t*>(b + o_free);
     w.qc = reinterpret_cast<unsigned int*>(b + o_qc);
+    w.claim_base = reinterpret_cast<unsigned long long*>(b + o_cb);
     w.finished = reinterpret_cast<unsigned long long*>(b + o_fin);
     w.slab = reinterpret_cast<uint32_t*>(b + o_slab);
     w.P = P;
@@ -107,15 +110,18 @@ vp_status vp_ensure_jobs(vp_context* ctx, uint32_t n) {
     vp_drop_graph(ctx);
     if (ctx->jobs_dev) (void)hipFree(ctx->jobs_dev);
     if (ctx->job_begin_dev) (void)hipFree(ctx->job_begin_dev);
+    if (ctx->job_done_dev) (void)hipFree(ctx->job_done_dev);
     if (ctx->jobs_pinned) (void)hipHostFree(ctx->jobs_pinned);
     if (ctx->begin_pinned) (void)hipHostFree(ctx->begin_pinned);
     ctx->jobs_dev = nullptr;
     ctx->job_begin_dev = nullptr;
+    ctx->job_done_dev = nullptr;
     ctx->jobs_pinned = nullptr;
     ctx->begin_pinned = nullptr;
     ctx->jobs_cap = ctx->pinned_cap = 0;
     VP_HIP_TRY(hipMalloc(&ctx->jobs_dev, sizeof(vp::JobDesc) * n));
     VP_HIP_TRY(hipMalloc(&ctx->job_begin_dev, sizeof(uint64_t) * n));
+    VP_HIP_TRY(hipMalloc(&ctx->job_done_dev, sizeof(unsigned long long) * n));
     VP_HIP_TRY(hipHostMalloc(&ctx->jobs_pinned, sizeof(vp::JobDesc) * n, hipHostMallocDefault));
     VP_HIP_TRY(hipHostMalloc(&ctx->begin_pinned, sizeof(uint64_t) * n, hipHostMallocDefault));
     ctx->jobs_cap = ctx->pinned_cap = n;
@@ -146,10 +152,12 @@ RenderLaunch vp_resolve_launch(vp_context* ctx, const vp_launch_config* cfg, uin
     L.pool = c.pool_paths ? std::max<uint32_t>(4096u, c.pool_paths / 64u * 64u) : (1u << 20);
     if (L.pipeline == VP_PIPELINE_WAVEFRONT) {
         const uint32_t cus = static_cast<uint32_t>(ctx->prop.multiProcessorCount);
-        int oe = 1, os = 1, osh = 1;
+        int oe = 1, os = 1, osh = 1, org = 1;
         vp_wf_occupancy(0, &oe);
         vp_wf_occupancy(1, &os);
         vp_wf_occupancy(2, &osh);
+        vp_wf_occupancy(3, &org);
+        L.wf.grid_regen = cus * static_cast<uint32_t>(org);
         L.wf.grid_extend = cus * static_cast<uint32_t>(oe);
         L.wf.grid_scatter = cus * static_cast<uint32_t>(os);
         L.wf.grid_shadow = cus * static_cast<uint32_t>(osh);
@@ -157,12 +165,43 @@ RenderLaunch vp_resolve_launch(vp_context* ctx, const vp_launch_config* cfg, uin
     return L;
 }
 
+// Wavefront pipeline resources for launch L: the path pool and the captured
+// two-iteration graph (recaptured when the shape, stream or job table moves).
+vp_status vp_wf_prepare(vp_context* ctx, const RenderLaunch& L, hipStream_t s) {
+    vp_status st = vp_ensure_wf(ctx, L.pool, L.active_cap);
+    if (st != VP_OK) return st;
+    ctx->wf.job_done = ctx->job_done_dev;
+    const uint64_t key[6] = {L.pool, L.active_cap, (uint64_t(L.wf.grid_regen) << 32) | L.wf.grid_extend,
+                             (uint64_t(L.wf.grid_scatter) << 32) | L.wf.grid_shadow,
+                             reinterpret_cast<uint64_t>(ctx->job_done_dev), reinterpret_cast<uint64_t>(s)};
+    if (ctx->wf_graph && std::equal(key, key + 6, ctx->wf_graph_key)) return VP_OK;
+    if (ctx->wf_graph) (void)hipGraphExecDestroy(ctx->wf_graph);
+    ctx->wf_graph = nullptr;
+    VP_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const hipError_t e = vp_wf_launch_iterations(ctx, ctx->wf, L.wf, s);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(s, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(VP_ERR_HIP,
+                    std::string("wavefront graph capture failed: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    const hipError_t e3 = hipGraphInstantiate(&ctx->wf_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e3 != hipSuccess) {
+        ctx->wf_graph = nullptr;
+        return fail(VP_ERR_HIP, std::string("wavefront graph instantiation failed: ") + hipGetErrorString(e3));
+    }
+    std::copy(key, key + 6, ctx->wf_graph_key);
+    return VP_OK;
+}
+
 // Wavefront step: stage the tables, reset the pool, replay the captured
 // two-iteration graph until every path of the step has retired (checked one
 // replay behind, so the device never waits on the host), fold the slices.
 static vp_status vp_run_step_wf(vp_context* ctx, const RenderLaunch& L, uint64_t total_paths, uint32_t max_jobs,
                                 uint32_t max_npix, hipStream_t s) {
-    vp_status st = vp_ensure_wf(ctx, L.pool, L.active_cap);
+    vp_status st = vp_wf_prepare(ctx, L, s);
     if (st != VP_OK) return st;
     VP_HIP_TRY(hipMemcpyAsync(ctx->jobs_dev, ctx->jobs_pinned, sizeof(vp::JobDesc) * max_jobs, hipMemcpyHostToDevice,
                               s));
@@ -170,29 +209,8 @@ static vp_status vp_run_step_wf(vp_context* ctx, const RenderLaunch& L, uint64_t
                               hipMemcpyHostToDevice, s));
     VP_HIP_TRY(hipMemcpyAsync(ctx->launch_desc, ctx->desc_pinned, sizeof(uint64_t) * 2, hipMemcpyHostToDevice, s));
     VP_HIP_TRY(vp_launch_fill_u64(ctx->path_counter, 1u, 0ull, s));
+    VP_HIP_TRY(vp_launch_fill_u64(ctx->job_done_dev, max_jobs, 0ull, s));
     VP_HIP_TRY(vp_wf_launch_init(ctx->wf, s));
-    const uint64_t key[6] = {L.pool, L.active_cap, L.wf.grid_extend, L.wf.grid_scatter, L.wf.grid_shadow,
-                             reinterpret_cast<uint64_t>(s)};
-    if (!ctx->wf_graph || !std::equal(key, key + 6, ctx->wf_graph_key)) {
-        if (ctx->wf_graph) (void)hipGraphExecDestroy(ctx->wf_graph);
-        ctx->wf_graph = nullptr;
-        VP_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        const hipError_t e = vp_wf_launch_iterations(ctx, ctx->wf, L.wf, s);
-        hipGraph_t g = nullptr;
-        const hipError_t e2 = hipStreamEndCapture(s, &g);
-        if (e != hipSuccess || e2 != hipSuccess) {
-            if (g) (void)hipGraphDestroy(g);
-            return fail(VP_ERR_HIP, std::string("wavefront graph capture failed: ") +
-                                        hipGetErrorString(e != hipSuccess ? e : e2));
-        }
-        const hipError_t e3 = hipGraphInstantiate(&ctx->wf_graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e3 != hipSuccess) {
-            ctx->wf_graph = nullptr;
-            return fail(VP_ERR_HIP, std::string("wavefront graph instantiation failed: ") + hipGetErrorString(e3));
-        }
-        std::copy(key, key + 6, ctx->wf_graph_key);
-    }
     // Every replay advances each live path by two bounces or retires it, and
     // refills free slots, so the loop ends; the bound below only guards
     // against a broken invariant.

@@ -74,13 +74,16 @@ struct WfPool {
     float4* pd;    // [3][P] shadow-ray direction.xyz, transmittance
     uint32_t* live;      // [P] live-slot queue
     uint32_t* shadow;    // [3P] shadow-ray queue (slot << 2 | term)
-    unsigned int* qc;    // [4] live / shadow counts, two parity sets
-    unsigned long long* finished;  // paths retired (or padding) so far
+    uint32_t* free_q;    // [P] free-slot queue
+    unsigned int* qc;    // [2][4] live / shadow / free / claimed counts per parity
+    unsigned long long* claim_base;  // [2] first path number claimed per parity
+    unsigned long long* finished;    // paths retired (or padding) so far
+    unsigned long long* job_done;    // per job-table entry: paths retired so far
     uint32_t* slab;      // [2][cap][P] active sets
     uint32_t P, cap;
 };
 struct WfLaunch {
-    uint32_t grid_extend, grid_scatter, grid_shadow;
+    uint32_t grid_regen, grid_extend, grid_scatter, grid_shadow;
 };
 
 // Counter slots of the render kernel.
@@ -123,6 +126,7 @@ struct vp_context {
     vp::JobDesc* jobs_dev = nullptr;
     uint32_t jobs_cap = 0;
     uint64_t* job_begin_dev = nullptr;
+    unsigned long long* job_done_dev = nullptr;  // per job-table entry (wavefront pipeline)
     uint64_t* launch_desc = nullptr;  // [0] job slices, [1] total paths
     // pinned staging of the job table, the slice begins and the launch descriptor
     vp::JobDesc* jobs_pinned = nullptr;
@@ -239,4 +243,8 @@ hipError_t vp_wf_launch_init(const vp::WfPool& w, hipStream_t s);
 hipError_t vp_wf_launch_iterations(vp_context* ctx, const vp::WfPool& w, const vp::WfLaunch& L, hipStream_t s);
 int vp_wf_occupancy(int which, int* blocks_per_cu);
 hipError_t vp_launch_welford(vp_context* ctx, uint32_t max_jobs, uint32_t max_npix, hipStream_t s);
+// Welford fold of every sample slot of job-table entry j (whole job at once).
+hipError_t vp_launch_welford_job(vp_context* ctx, uint32_t j, uint32_t npix, hipStream_t s);
 void vp_free_wf(vp_context* ctx);
+vp_status vp_ensure_wf(vp_context* ctx, uint32_t P, uint32_t cap);
+vp_status vp_wf_prepare(vp_context* ctx, const RenderLaunch& L, hipStream_t s);

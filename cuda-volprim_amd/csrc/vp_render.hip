@@ -325,21 +325,34 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
 // Fold every job slice of the launch into its Welford mean, in sample order
 // (raygen.cuh:214-238): n_inv = 1 / (samples before + 1).  blockIdx.y picks
 // the job; blocks beyond the launch's job count or the job's pixels exit.
+// Pixel i of job J: fold the slice's sample slots into the Welford mean in
+// sample order (raygen.cuh:214-238): n_inv = 1 / (samples before + 1).
+__device__ __forceinline__ void welford_fold(const JobDesc& J, uint32_t npix, uint32_t i) {
+    const float4 m = J.s_begin == 0 ? make_float4(0.0f, 0.0f, 0.0f, 1.0f) : J.mean[i];
+    f3 mu = f3{m.x, m.y, m.z};
+    for (uint32_t s = 0; s < J.s_count; ++s) {
+        const float4 v = J.samples[static_cast<size_t>(s) * npix + i];
+        const float ninv = rcpf(static_cast<float>(J.s_begin + s + 1));
+        mu = mu + (f3{v.x, v.y, v.z} - mu) * ninv;
+    }
+    J.mean[i] = make_float4(mu.x, mu.y, mu.z, 1.0f);
+}
+
 __global__ void k_welford_all(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ launch_desc) {
     const uint32_t j = blockIdx.y;
     if (j >= static_cast<uint32_t>(launch_desc[0])) return;
     const JobDesc& J = jobs[j];
     const uint32_t npix = J.width * J.height;
-    float4* mean = J.mean;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x) {
-        const float4 m = J.s_begin == 0 ? make_float4(0.0f, 0.0f, 0.0f, 1.0f) : mean[i];
-        f3 mu = f3{m.x, m.y, m.z};
-        for (uint32_t s = 0; s < J.s_count; ++s) {
-            const float4 v = J.samples[static_cast<size_t>(s) * npix + i];
-            const float ninv = rcpf(static_cast<float>(J.s_begin + s + 1));
-            mu = mu + (f3{v.x, v.y, v.z} - mu) * ninv;
-        }
-        mean[i] = make_float4(mu.x, mu.y, mu.z, 1.0f);
+        welford_fold(J, npix, i);
+    }
+}
+
+__global__ void k_welford_job(const JobDesc* __restrict__ jobs, uint32_t j) {
+    const JobDesc& J = jobs[j];
+    const uint32_t npix = J.width * J.height;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x) {
+        welford_fold(J, npix, i);
     }
 }
 
@@ -506,6 +519,12 @@ hipError_t vp_launch_step(vp_context* ctx, const RenderLaunch& L, uint32_t max_j
 hipError_t vp_launch_welford(vp_context* ctx, uint32_t max_jobs, uint32_t max_npix, hipStream_t s) {
     const uint32_t gx = std::max<uint32_t>(1u, std::min<uint32_t>(cdiv(max_npix, 256), 512u));
     hipLaunchKernelGGL(k_welford_all, dim3(gx, max_jobs), dim3(256), 0, s, ctx->jobs_dev, ctx->launch_desc);
+    return hipGetLastError();
+}
+
+hipError_t vp_launch_welford_job(vp_context* ctx, uint32_t j, uint32_t npix, hipStream_t s) {
+    const uint32_t gx = std::max<uint32_t>(1u, std::min<uint32_t>(cdiv(npix, 256), 1024u));
+    hipLaunchKernelGGL(k_welford_job, dim3(gx), dim3(256), 0, s, ctx->jobs_dev, j);
     return hipGetLastError();
 }
 

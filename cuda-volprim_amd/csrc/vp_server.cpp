@@ -25,6 +25,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -40,13 +41,19 @@ vp_status fail(vp_status code, const std::string& msg) {
     return code;
 }
 
+// Job-table entries one server epoch can hold before it drains and restarts.
+constexpr uint32_t kServerTable = 4096;
+
 struct Request {
     uint64_t ticket = 0;
     vp_job job{};
-    JobRun run;
+    vp::JobDesc desc{};
     float4* mean = nullptr;
-    float4* slice = nullptr;
+    float4* samples = nullptr;
     size_t bytes = 0;
+    uint32_t entry = 0;     // job-table entry in the current epoch
+    bool completing = false;
+    hipEvent_t ev = nullptr;  // recorded after the request's Welford fold
     clk::time_point t_submit;
     double latency_ms = 0.0;
     bool done = false;
@@ -72,99 +79,185 @@ struct vp_server {
     uint64_t completed = 0, samples = 0, steps = 0;
     double busy_ms = 0.0;
     std::string error;
+    // epoch state, server thread only
+    uint32_t njobs = 0;
+    uint64_t total = 0;
+    unsigned long long* done_pinned = nullptr;  // [2][kServerTable] per-entry retired paths
+    uint64_t* desc_pinned = nullptr;            // [2][2] launch descriptors
+    uint32_t njobs_copied[2] = {0, 0};
+    hipEvent_t rep_ev[2] = {nullptr, nullptr};
+    hipStream_t copy_stream = nullptr;
 };
 
 namespace {
 
-void free_request_device(Request* r) {
-    if (r->mean) (void)hipFree(r->mean);
-    if (r->slice) (void)hipFree(r->slice);
-    r->mean = r->slice = nullptr;
+void release_request(vp_server* S, Request* r, hipStream_t s) {
+    if (r->mean) (void)hipFreeAsync(r->mean, s);
+    if (r->samples) (void)hipFreeAsync(r->samples, s);
+    r->mean = r->samples = nullptr;
+    if (r->ev) (void)hipEventDestroy(r->ev);
+    r->ev = nullptr;
+    (void)S;
 }
 
+// Finish request r (under S->mu): record its latency and wake pollers.
+void finish_request(vp_server* S, Request* r, vp_status st) {
+    r->status = st;
+    r->latency_ms = std::chrono::duration<double, std::milli>(clk::now() - r->t_submit).count();
+    r->done = true;
+    S->bytes_inflight -= r->bytes;
+    S->latencies.push_back(r->latency_ms);
+    ++S->completed;
+    S->samples += static_cast<uint64_t>(r->job.params.width) * r->job.params.height * r->job.params.spp;
+    S->inflight.erase(std::find(S->inflight.begin(), S->inflight.end(), r));
+    S->cv_done.notify_all();
+}
+
+// Continuous batching over the wavefront pipeline.  Each admitted request
+// becomes one job-table entry (all its samples) appended between two graph
+// replays; the pipeline's free path slots pick its paths up at the next
+// iteration, next to whatever else is in flight.  A request is complete
+// when its entry's retired-path count reaches its path count; its samples
+// are then folded and copied out while the pipeline keeps running.
 void server_loop(vp_server* S) {
     vp_context* ctx = S->ctx;
     (void)hipSetDevice(ctx->device);
     hipStream_t s = ctx->stream;
-    std::vector<Request*> batch;
+    uint64_t k = 0;
+    bool fresh = true;  // the pool must be reset before the next replay
+    clk::time_point busy_since{};
+    std::vector<Request*> admit;
     while (true) {
+        admit.clear();
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            // admission: FIFO, by device bytes in flight and free slots
-            while (!S->pending.empty() && S->inflight.size() < S->cfg.max_inflight) {
-                Request* r = S->pending.front();
-                if (!S->inflight.empty() && S->bytes_inflight + r->bytes > S->cfg.mem_budget_bytes) break;
-                const size_t npix = static_cast<size_t>(r->job.params.width) * r->job.params.height;
-                if (hipMalloc(&r->mean, npix * sizeof(float4)) != hipSuccess ||
-                    hipMalloc(&r->slice, static_cast<size_t>(r->run.slice_spp) * npix * sizeof(float4)) != hipSuccess) {
-                    free_request_device(r);
-                    if (!S->inflight.empty()) break;  // wait for memory held by running requests
-                    r->status = VP_ERR_OOM;
-                    r->done = true;
-                    S->pending.pop_front();
-                    S->cv_done.notify_all();
+            if (S->inflight.empty() && (S->pending.empty() || S->njobs >= kServerTable)) {
+                // nothing on the GPU: start a new epoch
+                if (!fresh) S->busy_ms += std::chrono::duration<double, std::milli>(clk::now() - busy_since).count();
+                S->njobs = 0;
+                S->total = 0;
+                fresh = true;
+                k = 0;
+                if (S->pending.empty()) {
+                    if (S->stop) return;
+                    S->cv.wait(lk);
                     continue;
                 }
-                r->run.mean = r->mean;
+            }
+            // admission: FIFO, by device bytes in flight and free slots
+            while (!S->pending.empty() && S->inflight.size() < S->cfg.max_inflight && S->njobs < kServerTable) {
+                Request* r = S->pending.front();
+                if (!S->inflight.empty() && S->bytes_inflight + r->bytes > S->cfg.mem_budget_bytes) break;
                 S->pending.pop_front();
                 S->inflight.push_back(r);
                 S->bytes_inflight += r->bytes;
+                r->entry = S->njobs++;
+                admit.push_back(r);
             }
-            if (S->inflight.empty()) {
-                if (S->stop) return;
-                S->cv.wait(lk);
-                continue;
-            }
-            batch = S->inflight;
         }
-        // stage one slice of every admitted request
-        uint64_t total = 0;
-        uint32_t k = 0;
-        for (Request* r : batch) {
-            JobRun& run = r->run;
-            vp::JobDesc J = run.desc;
-            J.s_begin = run.done;
-            J.s_count = std::min(run.slice_spp, run.desc.spp - run.done);
-            J.path_begin = total;
-            J.path_count = static_cast<uint64_t>(J.npad) * J.s_count;
+        vp_status st = VP_OK;
+        if (fresh) {
+            // the previous epoch's staging copies must have run before its
+            // pinned entries are rewritten
+            if (hipStreamSynchronize(s) != hipSuccess) st = VP_ERR_HIP;
+            busy_since = clk::now();
+            if (st == VP_OK) st = vp_wf_prepare(ctx, S->L, s);
+            if (st == VP_OK && (vp_launch_fill_u64(ctx->path_counter, 1u, 0ull, s) != hipSuccess ||
+                                vp_wf_launch_init(ctx->wf, s) != hipSuccess))
+                st = VP_ERR_HIP;
+            fresh = false;
+        }
+        // install the admitted requests
+        for (Request* r : admit) {
+            const size_t npix = static_cast<size_t>(r->job.params.width) * r->job.params.height;
+            vp::JobDesc J = r->desc;
+            J.s_begin = 0;
+            J.s_count = J.spp;
+            J.path_begin = S->total;
+            J.path_count = static_cast<uint64_t>(J.npad) * J.spp;
+            if (st == VP_OK && (hipMallocAsync(reinterpret_cast<void**>(&r->mean), npix * sizeof(float4), s) != hipSuccess ||
+                                hipMallocAsync(reinterpret_cast<void**>(&r->samples),
+                                               npix * J.spp * sizeof(float4), s) != hipSuccess))
+                st = VP_ERR_OOM;
             J.mean = r->mean;
-            J.samples = r->slice;
-            ctx->jobs_pinned[k] = J;
-            ctx->begin_pinned[k] = J.path_begin;
-            total += J.path_count;
-            ++k;
+            J.samples = r->samples;
+            r->desc = J;
+            ctx->jobs_pinned[r->entry] = J;
+            ctx->begin_pinned[r->entry] = J.path_begin;
+            if (st == VP_OK &&
+                (hipMemcpyAsync(ctx->jobs_dev + r->entry, ctx->jobs_pinned + r->entry, sizeof(vp::JobDesc),
+                                hipMemcpyHostToDevice, s) != hipSuccess ||
+                 hipMemcpyAsync(ctx->job_begin_dev + r->entry, ctx->begin_pinned + r->entry, sizeof(uint64_t),
+                                hipMemcpyHostToDevice, s) != hipSuccess ||
+                 vp_launch_fill_u64(ctx->job_done_dev + r->entry, 1u, 0ull, s) != hipSuccess))
+                st = VP_ERR_HIP;
+            S->total += J.path_count;
         }
-        const auto t0 = clk::now();
-        vp_status st = vp_run_step(ctx, S->L, k, total, S->cfg.max_inflight, S->max_npix, true, s);
-        if (st == VP_OK && hipStreamSynchronize(s) != hipSuccess) st = VP_ERR_HIP;
-        const double ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
-        std::lock_guard<std::mutex> lk(S->mu);
-        S->busy_ms += ms;
+        if (!admit.empty() && st == VP_OK) {
+            uint64_t* d = S->desc_pinned + 2 * (k & 1);
+            d[0] = S->njobs;
+            d[1] = S->total;
+            if (hipMemcpyAsync(ctx->launch_desc, d, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, s) != hipSuccess)
+                st = VP_ERR_HIP;
+        }
+        // one replay (two iterations), then the per-entry progress snapshot
+        if (st == VP_OK && hipGraphLaunch(ctx->wf_graph, s) != hipSuccess) st = VP_ERR_HIP;
+        S->njobs_copied[k & 1] = S->njobs;
+        if (st == VP_OK &&
+            (hipMemcpyAsync(S->done_pinned + static_cast<size_t>(k & 1) * kServerTable, ctx->job_done_dev,
+                            sizeof(unsigned long long) * S->njobs, hipMemcpyDeviceToHost, s) != hipSuccess ||
+             hipEventRecord(S->rep_ev[k & 1], s) != hipSuccess))
+            st = VP_ERR_HIP;
+        if (st != VP_OK) {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->error = "server step failed";
+            while (!S->inflight.empty()) {
+                Request* r = S->inflight.back();
+                release_request(S, r, s);
+                finish_request(S, r, st);
+            }
+            continue;
+        }
         ++S->steps;
-        std::vector<Request*> still;
-        for (uint32_t i = 0; i < batch.size(); ++i) {
-            Request* r = batch[i];
-            r->run.done += ctx->jobs_pinned[i].s_count;
-            if (st != VP_OK || r->run.done >= r->run.desc.spp) {
-                const size_t npix = static_cast<size_t>(r->job.params.width) * r->job.params.height;
-                if (st == VP_OK &&
-                    hipMemcpy(r->job.out_rgba, r->mean, npix * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess)
-                    st = VP_ERR_HIP;
-                r->status = st;
-                r->latency_ms = std::chrono::duration<double, std::milli>(clk::now() - r->t_submit).count();
-                r->done = true;
-                free_request_device(r);
-                S->bytes_inflight -= r->bytes;
-                S->latencies.push_back(r->latency_ms);
-                ++S->completed;
-                S->samples += static_cast<uint64_t>(npix) * r->run.desc.spp;
-            } else {
-                still.push_back(r);
+        if (k > 0) {
+            // progress as of the previous replay (the current one keeps running)
+            const uint64_t pk = k - 1;
+            (void)hipEventSynchronize(S->rep_ev[pk & 1]);
+            const unsigned long long* dn = S->done_pinned + static_cast<size_t>(pk & 1) * kServerTable;
+            const uint32_t seen = S->njobs_copied[pk & 1];
+            std::vector<Request*> snapshot;
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                snapshot = S->inflight;
+            }
+            for (Request* r : snapshot) {
+                if (!r->completing && r->entry < seen && dn[r->entry] >= r->desc.path_count) {
+                    const uint32_t npix = r->desc.width * r->desc.height;
+                    vp_status rs = VP_OK;
+                    if (vp_launch_welford_job(ctx, r->entry, npix, s) != hipSuccess ||
+                        hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess ||
+                        hipEventRecord(r->ev, s) != hipSuccess)
+                        rs = VP_ERR_HIP;
+                    r->completing = true;
+                    if (rs != VP_OK) {
+                        release_request(S, r, s);
+                        std::lock_guard<std::mutex> lk(S->mu);
+                        finish_request(S, r, rs);
+                    }
+                } else if (r->completing && hipEventQuery(r->ev) == hipSuccess) {
+                    const size_t bytes = static_cast<size_t>(r->desc.width) * r->desc.height * sizeof(float4);
+                    vp_status rs = VP_OK;
+                    if (hipMemcpyAsync(r->job.out_rgba, r->mean, bytes, hipMemcpyDeviceToHost, S->copy_stream) !=
+                            hipSuccess ||
+                        hipStreamSynchronize(S->copy_stream) != hipSuccess)
+                        rs = VP_ERR_HIP;
+                    release_request(S, r, s);
+                    std::lock_guard<std::mutex> lk(S->mu);
+                    finish_request(S, r, rs);
+                }
             }
         }
-        S->inflight.swap(still);
-        if (st != VP_OK) S->error = "step failed";
-        S->cv_done.notify_all();
+        ++k;
     }
 }
 
@@ -183,30 +276,39 @@ extern "C" {
 
 vp_status vp_server_create(vp_context* ctx, const vp_server_config* cfg, vp_server** out) {
     if (!ctx || !out) return fail(VP_ERR_INVALID, "null argument");
-    auto* S = new vp_server();
+    auto S = std::make_unique<vp_server>();
     S->ctx = ctx;
     if (cfg) S->cfg = *cfg;
     if (S->cfg.mem_budget_bytes == 0) S->cfg.mem_budget_bytes = size_t(8) << 30;
     if (S->cfg.slice_spp == 0) S->cfg.slice_spp = 4;
     if (S->cfg.max_inflight == 0) S->cfg.max_inflight = 64;
     S->max_npix = 1024u * 1024u;
-    if (hipSetDevice(ctx->device) != hipSuccess) {
-        delete S;
-        return fail(VP_ERR_HIP, "hipSetDevice failed");
-    }
-    const uint32_t cap = ctx->have_tuned && ctx->tuned.active_cap ? ctx->tuned.active_cap : 128u;
-    S->L = vp_resolve_launch(ctx, nullptr, cap, 0, 0);
-    vp_status st = S->L.pipeline == VP_PIPELINE_MEGAKERNEL
-                       ? vp_ensure_slab(ctx, 2 * static_cast<size_t>(cap) * S->L.grid * S->L.block)
-                       : VP_OK;
-    if (st == VP_OK) st = vp_ensure_jobs(ctx, S->cfg.max_inflight);
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(VP_ERR_HIP, "hipSetDevice failed");
+    // the tuned (or default) shape, on the wavefront pipeline
+    vp_launch_config c{};
+    if (ctx->have_tuned) c = ctx->tuned;
+    c.pipeline = VP_PIPELINE_WAVEFRONT;
+    const uint32_t cap = c.active_cap ? c.active_cap : 128u;
+    S->L = vp_resolve_launch(ctx, &c, cap, 0, 0);
+    vp_status st = vp_ensure_jobs(ctx, kServerTable);
+    if (st == VP_OK) st = vp_wf_prepare(ctx, S->L, ctx->stream);
     if (st == VP_OK) st = vp_reset_counters(ctx, ctx->stream);
-    if (st != VP_OK) {
-        delete S;
-        return st;
+    if (st != VP_OK) return st;
+    if (hipHostMalloc(&S->done_pinned, 2 * kServerTable * sizeof(unsigned long long), hipHostMallocDefault) !=
+            hipSuccess ||
+        hipHostMalloc(&S->desc_pinned, 4 * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&S->rep_ev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&S->rep_ev[1], hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreateWithFlags(&S->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+        if (S->done_pinned) (void)hipHostFree(S->done_pinned);
+        if (S->desc_pinned) (void)hipHostFree(S->desc_pinned);
+        for (auto& e : S->rep_ev)
+            if (e) (void)hipEventDestroy(e);
+        if (S->copy_stream) (void)hipStreamDestroy(S->copy_stream);
+        return fail(VP_ERR_HIP, "server resources");
     }
-    S->th = std::thread(server_loop, S);
-    *out = S;
+    S->th = std::thread(server_loop, S.get());
+    *out = S.release();
     return VP_OK;
 }
 
@@ -218,12 +320,14 @@ vp_status vp_server_submit(vp_server* S, const vp_job* job, uint64_t* ticket) {
     if (job->params.width * job->params.height > S->max_npix)
         return fail(VP_ERR_INVALID, "request larger than the server's 1024x1024 pixel limit");
     if (job->params.measure_caps) return fail(VP_ERR_INVALID, "measure_caps is not available through the server");
+    if (job->params.active_cap > S->L.active_cap)
+        return fail(VP_ERR_INVALID, "request asks for an active-set capacity above the server's (" +
+                                        std::to_string(S->L.active_cap) + "): autotune with it first");
     auto* r = new Request();
     r->job = *job;
-    r->run.desc = vp_make_job(job->scene, job->env, &job->camera, &job->params);
-    r->run.slice_spp = std::min(S->cfg.slice_spp, job->params.spp);
+    r->desc = vp_make_job(job->scene, job->env, &job->camera, &job->params);
     const size_t npix = static_cast<size_t>(job->params.width) * job->params.height;
-    r->bytes = npix * sizeof(float4) * (1 + r->run.slice_spp);
+    r->bytes = npix * sizeof(float4) * (1 + static_cast<size_t>(job->params.spp));
     r->t_submit = clk::now();
     {
         std::lock_guard<std::mutex> lk(S->mu);
@@ -285,12 +389,16 @@ vp_status vp_server_destroy(vp_server* S) {
         S->pending.clear();
     }
     S->cv.notify_all();
+    // the loop runs until its in-flight requests are done, then exits
     if (S->th.joinable()) S->th.join();
-    for (auto& kv : S->by_ticket) {
-        free_request_device(kv.second);
-        delete kv.second;
-    }
-    vp_drop_graph(S->ctx);
+    (void)hipSetDevice(S->ctx->device);
+    (void)hipStreamSynchronize(S->ctx->stream);
+    for (auto& kv : S->by_ticket) delete kv.second;
+    if (S->done_pinned) (void)hipHostFree(S->done_pinned);
+    if (S->desc_pinned) (void)hipHostFree(S->desc_pinned);
+    for (auto& e : S->rep_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (S->copy_stream) (void)hipStreamDestroy(S->copy_stream);
     delete S;
     return VP_OK;
 }
@@ -301,7 +409,7 @@ vp_status vp_autotune(vp_context* ctx, vp_job* jobs, uint32_t njobs, vp_launch_c
     hipStream_t s = ctx->stream;
     // reduced copy of the mix: every view downscaled by an integer factor k
     // to at most kTuneRes pixels a side (same field of view, k-times wider
-    // pixels) at one sample per pixel
+    // pixels) at one sample per pixel and depth at most 16
     constexpr uint32_t kTuneRes = 64;
     std::vector<vp_render_params> params(njobs);
     std::vector<vp_camera> cams(njobs);
@@ -311,6 +419,8 @@ vp_status vp_autotune(vp_context* ctx, vp_job* jobs, uint32_t njobs, vp_launch_c
         params[i] = jobs[i].params;
         params[i].spp = 1;
         params[i].measure_caps = 0;
+        // long paths only lengthen each timing run's tail
+        params[i].max_depth = std::min<uint32_t>(params[i].max_depth, 16u);
         const uint32_t side = std::max(params[i].width, params[i].height);
         const uint32_t k = (side + kTuneRes - 1) / kTuneRes;
         params[i].width = (params[i].width + k - 1) / k;

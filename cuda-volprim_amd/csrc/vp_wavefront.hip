@@ -1,14 +1,16 @@
 // vp_wavefront.hip — the wavefront-queue variant of the renderer.
 //
 // A pool of P path slots lives in HBM (structure of arrays).  One iteration
-// advances every live path by one bounce with four kernels, each over a
+// advances every live path by one bounce with six kernels, each over a
 // compacted queue so its wavefronts do one kind of work:
-//   k_wf_shade_regen  per slot: add the previous bounce's shadow-ray terms to
+//   k_wf_shade        per slot: add the previous bounce's shadow-ray terms to
 //                     L in the estimator's order, retire finished paths to
-//                     their sample slots, claim new paths for free slots
-//                     (one atomic per wavefront, ballot + popcount ranks),
-//                     trace new paths' origin sets, and append live slots to
-//                     the live queue (ballot prefix compaction);
+//                     their sample slots (counted per job), queue free and
+//                     live slots (ballot + popcount prefix compaction, one
+//                     atomic per wavefront);
+//   k_wf_claim        one thread: hands out min(free slots, unclaimed paths)
+//                     path numbers, so jobs can be added between iterations;
+//   k_wf_regen        per claimed path: setup and origin set, queued live;
 //   k_wf_extend       per live path: argmin free flight (origin set + one
 //                     pruned BVH walk);
 //   k_wf_scatter      per live path: bounce-0 analytic direct ray, active-set
@@ -63,30 +65,44 @@ __device__ __forceinline__ void wave_count(unsigned long long* counter, bool fla
 
 #define VP_WF_LB __launch_bounds__(256)
 
+// Per-key counts of the flagged lanes of a wavefront, one atomic per distinct
+// key (lanes usually share one job).  Wave-uniform control flow.
+__device__ __forceinline__ void wave_add_per_key(unsigned long long* arr, bool flag, uint32_t key) {
+    unsigned long long pending = __ballot(flag);
+    while (pending) {
+        const uint32_t leader = __ffsll(static_cast<long long>(pending)) - 1;
+        const uint32_t k = __shfl(key, leader);
+        const unsigned long long same = __ballot(flag && key == k) & pending;
+        if (__lane_id() == leader) atomicAdd(&arr[k], static_cast<unsigned long long>(__popcll(same)));
+        pending &= ~same;
+    }
+}
+
+// queue counters of parity par: [0] live, [1] shadow, [2] free, [3] claimed
+__device__ __forceinline__ unsigned int* qcs(const WfPool& w, uint32_t par) { return w.qc + 4 * par; }
+
 __global__ void VP_WF_LB k_wf_init(WfPool w) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < w.P) w.misc[i] = make_uint4(0u, 0u, 0u, f_pack(0u, kSlotFree, 0u));
-    if (i < 4) w.qc[i] = 0u;
+    if (i < 8) w.qc[i] = 0u;
+    if (i < 2) w.claim_base[i] = 0ull;
     if (i == 0) *w.finished = 0ull;
 }
 
-__global__ void VP_WF_LB k_wf_shade_regen(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begin,
-                                          const uint64_t* __restrict__ launch_desc,
-                                          unsigned long long* __restrict__ path_counter, WfPool w, uint32_t par,
-                                          unsigned long long* __restrict__ counters) {
-    const uint32_t njobs = static_cast<uint32_t>(launch_desc[0]);
-    const uint64_t total_paths = launch_desc[1];
+// Stage 1 of an iteration, one thread per slot: add the finished shadow-ray
+// terms, retire finished paths, queue free and live slots.
+__global__ void VP_WF_LB k_wf_shade(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par) {
     const uint32_t P = w.P;
-    if (blockIdx.x == 0 && threadIdx.x < 2) w.qc[2 * (par ^ 1u) + threadIdx.x] = 0u;
-    // one slot per thread; the grid covers the pool
+    if (blockIdx.x == 0 && threadIdx.x < 4) qcs(w, par ^ 1u)[threadIdx.x] = 0u;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = i < P;
     uint4 m = in ? w.misc[i] : make_uint4(0u, 0u, 0u, f_pack(0u, kSlotFree, 0u));
-    uint32_t cur = f_cur(m.w), state = f_state(m.w);
+    uint32_t state = f_state(m.w);
     const uint32_t npend = f_npend(m.w);
     bool retired = false;
+    uint32_t job = 0;
     if (in && state != kSlotFree) {
-        float4 lr = w.lr[i];
+        const float4 lr = w.lr[i];
         f3 L = f3{lr.x, lr.y, lr.z};
         for (uint32_t k = 0; k < npend; ++k) {
             const float T = w.pd[k * P + i].w;
@@ -94,78 +110,103 @@ __global__ void VP_WF_LB k_wf_shade_regen(const JobDesc* __restrict__ jobs, cons
             L = L + ((((f3{a.x, a.y, a.z} * T) * f3{b.x, b.y, b.z}) * a.w) * b.w);
         }
         if (state == kSlotDone) {
-            const JobDesc& J = jobs[__float_as_uint(w.dj[i].w)];
+            job = __float_as_uint(w.dj[i].w);
+            const JobDesc& J = jobs[job];
             J.samples[m.z] = finalize_radiance(J, L);
             state = kSlotFree;
             retired = true;
         } else {
             w.lr[i] = make_float4(L.x, L.y, L.z, lr.w);
         }
+        m.w = f_pack(f_cur(m.w), state, 0u);
+        w.misc[i].w = m.w;
     }
     wave_count(w.finished, retired);
+    wave_add_per_key(w.job_done, retired, job);
+    const bool is_free = in && state == kSlotFree;
+    const uint32_t qf = wave_append(&qcs(w, par)[2], is_free);
+    if (is_free) w.free_q[qf] = i;
+    const bool live = in && state == kSlotAlive;
+    const uint32_t ql = wave_append(&qcs(w, par)[0], live);
+    if (live) w.live[ql] = i;
+}
 
-    // claim new paths for free slots
-    const bool want = in && state == kSlotFree;
-    const unsigned long long wmask = __ballot(want);
-    bool padded = false;
+// Stage 2, one thread: hand out min(free slots, unclaimed paths) path
+// numbers, so the counter never passes the installed total (jobs may be
+// added between iterations).
+__global__ void k_wf_claim(const uint64_t* __restrict__ launch_desc, unsigned long long* __restrict__ path_counter,
+                           WfPool w, uint32_t par) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const uint64_t total = launch_desc[1];
+    const unsigned long long c = *path_counter;
+    const uint64_t avail = total > c ? total - c : 0ull;
+    const uint32_t nfree = qcs(w, par)[2];
+    const uint32_t n = static_cast<uint32_t>(avail < nfree ? avail : nfree);
+    w.claim_base[par] = c;
+    *path_counter = c + n;
+    qcs(w, par)[3] = n;
+}
+
+// Stage 3: start the claimed paths in the first free slots, trace their
+// origin sets, queue them as live.
+__global__ void VP_WF_LB k_wf_regen(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begin,
+                                    const uint64_t* __restrict__ launch_desc, WfPool w, uint32_t par,
+                                    unsigned long long* __restrict__ counters) {
+    const uint32_t njobs = static_cast<uint32_t>(launch_desc[0]);
+    const uint32_t n = qcs(w, par)[3];
+    const uint64_t base = w.claim_base[par];
+    const uint32_t P = w.P;
     uint32_t overflow = 0;
-    if (wmask) {
-        const uint32_t lane = __lane_id();
-        const uint32_t leader = __ffsll(static_cast<long long>(wmask)) - 1;
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(path_counter, static_cast<unsigned long long>(__popcll(wmask)));
-        base = __shfl(base, leader);
-        if (want) {
-            const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-            const uint64_t p = base + __popcll(wmask & lt);
-            if (p < total_paths) {
-                const uint32_t j = find_job(job_begin, njobs, p);
-                const JobDesc& J = jobs[j];
-                f3 o, d;
-                Pcg32 rng;
-                uint64_t key;
-                uint32_t slot;
-                if (path_begin(J, p - J.path_begin, o, d, rng, key, slot)) {
-                    // origin-inside set (sampling.cuh:322-341) by point query, into buffer 0
-                    uint32_t na = 0, visits = 0;
-                    uint32_t* act = w.slab + i;
-                    const SceneView sc = J.scene;
-                    walk_point(sc, o, visits, [&](uint32_t prim) {
-                        if (point_inside(load_prim(sc.prims, prim), o)) {
-                            if (na < w.cap) act[static_cast<size_t>(na) * P] = prim, ++na;
-                            else ++overflow;
-                        }
-                    });
-                    w.ob[i] = make_float4(o.x, o.y, o.z, __uint_as_float(0u));
-                    w.dj[i] = make_float4(d.x, d.y, d.z, __uint_as_float(j));
-                    w.bt[i] = make_float4(1.0f, 1.0f, 1.0f, kTMax);
-                    w.lr[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(na));
-                    w.rng[i] = make_uint4(static_cast<uint32_t>(rng.state), static_cast<uint32_t>(rng.state >> 32),
-                                          static_cast<uint32_t>(rng.inc), static_cast<uint32_t>(rng.inc >> 32));
-                    m = make_uint4(static_cast<uint32_t>(key), static_cast<uint32_t>(key >> 32), slot, 0u);
-                    cur = 0;
-                    state = kSlotAlive;
-                } else {
-                    padded = true;  // a partial tile's padding path: done at once
-                }
+    const uint32_t span = gridDim.x * blockDim.x;
+    for (uint32_t first = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); first < n; first += span) {
+        const uint32_t idx = first + __lane_id();
+        bool live = false, padded = false;
+        uint32_t i = 0, j = 0;
+        if (idx < n) {
+            i = w.free_q[idx];
+            const uint64_t p = base + idx;
+            j = find_job(job_begin, njobs, p);
+            const JobDesc& J = jobs[j];
+            f3 o, d;
+            Pcg32 rng;
+            uint64_t key;
+            uint32_t slot;
+            if (path_begin(J, p - J.path_begin, o, d, rng, key, slot)) {
+                // origin-inside set (sampling.cuh:322-341) by point query, into buffer 0
+                uint32_t na = 0, visits = 0;
+                uint32_t* act = w.slab + i;
+                const SceneView sc = J.scene;
+                walk_point(sc, o, visits, [&](uint32_t prim) {
+                    if (point_inside(load_prim(sc.prims, prim), o)) {
+                        if (na < w.cap) act[static_cast<size_t>(na) * P] = prim, ++na;
+                        else ++overflow;
+                    }
+                });
+                w.ob[i] = make_float4(o.x, o.y, o.z, __uint_as_float(0u));
+                w.dj[i] = make_float4(d.x, d.y, d.z, __uint_as_float(j));
+                w.bt[i] = make_float4(1.0f, 1.0f, 1.0f, kTMax);
+                w.lr[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(na));
+                w.rng[i] = make_uint4(static_cast<uint32_t>(rng.state), static_cast<uint32_t>(rng.state >> 32),
+                                      static_cast<uint32_t>(rng.inc), static_cast<uint32_t>(rng.inc >> 32));
+                w.misc[i] = make_uint4(static_cast<uint32_t>(key), static_cast<uint32_t>(key >> 32), slot,
+                                       f_pack(0u, kSlotAlive, 0u));
+                live = true;
+            } else {
+                padded = true;  // a partial tile's padding path: done at once, the slot stays free
             }
         }
+        wave_count(w.finished, padded);
+        wave_add_per_key(w.job_done, padded, j);
+        const uint32_t q = wave_append(&qcs(w, par)[0], live);
+        if (live) w.live[q] = i;
     }
-    wave_count(w.finished, padded);
     if (overflow) atomicAdd(&counters[kCntActiveOverflow], static_cast<unsigned long long>(overflow));
-    if (in) {
-        m.w = f_pack(cur, state, 0u);
-        w.misc[i] = m;
-    }
-    const bool live = in && state == kSlotAlive;
-    const uint32_t q = wave_append(&w.qc[2 * par], live);
-    if (live) w.live[q] = i;
 }
 
 __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par) {
     extern __shared__ uint32_t s_leafq[];
     const LeafQueue lq{s_leafq + threadIdx.x, blockDim.x};
-    const uint32_t n = w.qc[2 * par];
+    const uint32_t n = qcs(w, par)[0];
     const uint32_t P = w.P;
     for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
         const uint32_t i = w.live[idx];
@@ -228,7 +269,7 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
                                       unsigned long long* __restrict__ counters) {
     extern __shared__ uint32_t s_leafq[];
     const LeafQueue lq{s_leafq + threadIdx.x, blockDim.x};
-    const uint32_t n = w.qc[2 * par];
+    const uint32_t n = qcs(w, par)[0];
     const uint32_t P = w.P;
     const size_t plane = static_cast<size_t>(w.cap) * P;
     uint32_t overflow = 0;
@@ -392,7 +433,7 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
         // shadow-queue appends, one atomic per wavefront and term index
         for (uint32_t k = 0; k < 3; ++k) {
             const bool has = active && npend > k;
-            const uint32_t q = wave_append(&w.qc[2 * par + 1], has);
+            const uint32_t q = wave_append(&qcs(w, par)[1], has);
             if (has) w.shadow[q] = (i << 2) | k;
         }
     }
@@ -402,7 +443,7 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
 __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par) {
     extern __shared__ uint32_t s_leafq[];
     const LeafQueue lq{s_leafq + threadIdx.x, blockDim.x};
-    const uint32_t n = w.qc[2 * par + 1];
+    const uint32_t n = qcs(w, par)[1];
     const uint32_t P = w.P;
     Ev<false> ev;
     for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
@@ -435,8 +476,10 @@ hipError_t vp_wf_launch_init(const WfPool& w, hipStream_t s) {
 hipError_t vp_wf_launch_iterations(vp_context* ctx, const WfPool& w, const WfLaunch& L, hipStream_t s) {
     const size_t lds = sizeof(uint32_t) * kLeafQ * 256;
     for (uint32_t par = 0; par < 2; ++par) {
-        hipLaunchKernelGGL(k_wf_shade_regen, dim3(cdiv(w.P, 256)), dim3(256), 0, s, ctx->jobs_dev,
-                           ctx->job_begin_dev, ctx->launch_desc, ctx->path_counter, w, par, ctx->counters);
+        hipLaunchKernelGGL(k_wf_shade, dim3(cdiv(w.P, 256)), dim3(256), 0, s, ctx->jobs_dev, w, par);
+        hipLaunchKernelGGL(k_wf_claim, dim3(1), dim3(64), 0, s, ctx->launch_desc, ctx->path_counter, w, par);
+        hipLaunchKernelGGL(k_wf_regen, dim3(L.grid_regen), dim3(256), 0, s, ctx->jobs_dev, ctx->job_begin_dev,
+                           ctx->launch_desc, w, par, ctx->counters);
         hipLaunchKernelGGL(k_wf_extend, dim3(L.grid_extend), dim3(256), lds, s, ctx->jobs_dev, w, par);
         hipLaunchKernelGGL(k_wf_scatter, dim3(L.grid_scatter), dim3(256), lds, s, ctx->jobs_dev, w, par,
                            ctx->counters);
@@ -446,10 +489,11 @@ hipError_t vp_wf_launch_iterations(vp_context* ctx, const WfPool& w, const WfLau
 }
 
 int vp_wf_occupancy(int which, int* blocks_per_cu) {
-    const size_t lds = sizeof(uint32_t) * kLeafQ * 256;
+    const size_t lds = which == 3 ? 0 : sizeof(uint32_t) * kLeafQ * 256;
     const void* f = which == 0   ? reinterpret_cast<const void*>(k_wf_extend)
                     : which == 1 ? reinterpret_cast<const void*>(k_wf_scatter)
-                                 : reinterpret_cast<const void*>(k_wf_shadow);
+                    : which == 2 ? reinterpret_cast<const void*>(k_wf_shadow)
+                                 : reinterpret_cast<const void*>(k_wf_regen);
     int nb = 0;
     const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 256, lds);
     *blocks_per_cu = nb > 0 ? nb : 1;

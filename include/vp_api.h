@@ -163,18 +163,26 @@ vp_status vp_render_jobs(vp_context* ctx, vp_job* jobs, uint32_t njobs, const vp
 vp_status vp_autotune(vp_context* ctx, vp_job* jobs, uint32_t njobs, vp_launch_config* best);
 
 /* ---- serving form ------------------------------------------------------ */
+/* Continuous batching on the wavefront pipeline: a server thread keeps the
+ * pipeline running while requests are in flight; each admitted request joins
+ * the job table between two iterations and its paths fill free path slots
+ * next to the other requests'.  Scenes and environments stay resident in
+ * their handles.  While a server exists its context belongs to it: other
+ * render calls on that context are allowed only while the server is idle. */
 typedef struct {
-    uint64_t mem_budget_bytes; /* admission limit on per-path state + image bytes in flight */
-    uint32_t slice_spp;        /* samples per pixel advanced per request per step */
+    uint64_t mem_budget_bytes; /* admission limit: sample slots + image bytes of the requests in flight */
+    uint32_t slice_spp;        /* reserved (requests enter the pipeline with all their samples) */
     uint32_t max_inflight;     /* requests on the GPU at once */
 } vp_server_config;
 typedef struct {
     uint64_t completed;
     uint64_t samples;
-    double p50_ms, p99_ms, mean_ms;
-    double busy_ms;
-    uint64_t steps;
+    double p50_ms, p99_ms, mean_ms; /* submit-to-result latency */
+    double busy_ms;                 /* time with requests in flight */
+    uint64_t steps;                 /* graph replays (two pipeline iterations each) */
 } vp_server_stats;
+/* The launch shape is the context's autotuned one (else the defaults), on the
+ * wavefront pipeline; its active-set capacity is the largest a request may ask for. */
 vp_status vp_server_create(vp_context* ctx, const vp_server_config* cfg, vp_server** out);
 /* Queue a request; *ticket identifies it.  The output buffer must stay valid
  * until vp_server_poll reports completion. */

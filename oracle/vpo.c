@@ -351,6 +351,149 @@ void vpo_pcg_raw(uint64_t state, uint64_t inc, long n, uint32_t* out) {
 uint64_t vpo_hash64(uint64_t x) { return hash64(x); }
 
 /* ------------------------------------------------------------------------ */
+/* Optional CPU bounding-volume hierarchy for the estimator (use_bvh).  Every  */
+/* query result is consumed order-independently (minimum, fixed-point sum) or */
+/* after sorting by primitive index, so renders are bit-identical to the      */
+/* brute-force loops; it only makes the CPU baseline fair at large N.         */
+/* ------------------------------------------------------------------------ */
+typedef struct { float lo[3], hi[3]; int32_t left, right; } cnode_t; /* leaf: left = -1 - prim */
+typedef struct { cnode_t* nodes; long nn; } cbvh_t;
+
+/* Conservative box of the 3-sigma ellipsoid: half extent 3 sqrt(sum_j (R_ij s_j)^2)
+ * along world axis i, widened by a relative 2^-8 and an absolute 2^-18 (|c| + h). */
+static void prim_box(const prim_t* p, float lo[3], float hi[3]) {
+    double qw = p->qw, qx = -(double)p->qx, qy = -(double)p->qy, qz = -(double)p->qz;
+    double R[3][3] = {
+        {1.0 - 2.0 * (qy * qy + qz * qz), 2.0 * (qx * qy - qw * qz), 2.0 * (qx * qz + qw * qy)},
+        {2.0 * (qx * qy + qw * qz), 1.0 - 2.0 * (qx * qx + qz * qz), 2.0 * (qy * qz - qw * qx)},
+        {2.0 * (qx * qz - qw * qy), 2.0 * (qy * qz + qw * qx), 1.0 - 2.0 * (qx * qx + qy * qy)},
+    };
+    double sv[3] = {1.0 / (double)p->rsx, 1.0 / (double)p->rsy, 1.0 / (double)p->rsz};
+    double cv[3] = {p->cx, p->cy, p->cz};
+    for (int a = 0; a < 3; ++a) {
+        double h2 = 0.0;
+        for (int j = 0; j < 3; ++j) h2 += (R[a][j] * sv[j]) * (R[a][j] * sv[j]);
+        double h = 3.0 * sqrt(h2);
+        h = h * (1.0 + 0x1p-8) + 0x1p-18 * (fabs(cv[a]) + h);
+        lo[a] = (float)(cv[a] - h);
+        hi[a] = (float)(cv[a] + h);
+        if ((double)lo[a] > cv[a] - h) lo[a] = nextafterf(lo[a], -3.0e38f);
+        if ((double)hi[a] < cv[a] + h) hi[a] = nextafterf(hi[a], 3.0e38f);
+    }
+}
+
+static int32_t cbvh_build_rec(cbvh_t* B, const uint32_t* perm, long lo, long hi, const float* boxes) {
+    int32_t id = (int32_t)B->nn++;
+    cnode_t* nd = &B->nodes[id];
+    if (hi - lo == 1) {
+        uint32_t p = perm[lo];
+        for (int a = 0; a < 3; ++a) { nd->lo[a] = boxes[6 * p + a]; nd->hi[a] = boxes[6 * p + 3 + a]; }
+        nd->left = -1 - (int32_t)p;
+        nd->right = -1;
+        return id;
+    }
+    long mid = lo + (hi - lo) / 2;
+    int32_t l = cbvh_build_rec(B, perm, lo, mid, boxes);
+    int32_t r = cbvh_build_rec(B, perm, mid, hi, boxes);
+    nd = &B->nodes[id];
+    for (int a = 0; a < 3; ++a) {
+        nd->lo[a] = vpo_min(B->nodes[l].lo[a], B->nodes[r].lo[a]);
+        nd->hi[a] = vpo_max(B->nodes[l].hi[a], B->nodes[r].hi[a]);
+    }
+    nd->left = l;
+    nd->right = r;
+    return id;
+}
+
+/* Primitives in Morton order of their centres, split in halves. */
+static void cbvh_build(cbvh_t* B, const prim_t* prims, long n) {
+    B->nodes = NULL;
+    B->nn = 0;
+    if (n <= 0) return;
+    float* boxes = (float*)malloc(sizeof(float) * 6 * (size_t)n);
+    float* centers = (float*)malloc(sizeof(float) * 3 * (size_t)n);
+    uint32_t* codes = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
+    uint32_t* perm = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
+    for (long i = 0; i < n; ++i) {
+        prim_box(prims + i, boxes + 6 * i, boxes + 6 * i + 3);
+        centers[3 * i] = prims[i].cx;
+        centers[3 * i + 1] = prims[i].cy;
+        centers[3 * i + 2] = prims[i].cz;
+    }
+    vpo_morton_codes(centers, n, codes);
+    vpo_sort_order(codes, n, perm);
+    B->nodes = (cnode_t*)malloc(sizeof(cnode_t) * (size_t)(2 * n - 1));
+    cbvh_build_rec(B, perm, 0, n, boxes);
+    free(boxes);
+    free(centers);
+    free(codes);
+    free(perm);
+}
+
+#define CBVH_STACK 128
+typedef struct { const cbvh_t* B; v3 o, inv, x; int sp; int32_t st[CBVH_STACK]; } cbvh_it;
+
+static inline float safe_rcp(float d) { return 1.0f / (fabsf(d) < 1e-20f ? copysignf(1e-20f, d) : d); }
+
+static void cbvh_ray_begin(cbvh_it* it, const cbvh_t* B, v3 o, v3 d) {
+    it->B = B;
+    it->o = o;
+    it->inv = V(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
+    it->sp = 0;
+    if (B->nn) it->st[it->sp++] = 0;
+}
+/* Next leaf whose box meets the ray over [0, tmax], or -1. */
+static long cbvh_ray_next(cbvh_it* it, float tmax) {
+    while (it->sp > 0) {
+        const cnode_t* nd = &it->B->nodes[it->st[--it->sp]];
+        float tx0 = (nd->lo[0] - it->o.x) * it->inv.x, tx1 = (nd->hi[0] - it->o.x) * it->inv.x;
+        float ty0 = (nd->lo[1] - it->o.y) * it->inv.y, ty1 = (nd->hi[1] - it->o.y) * it->inv.y;
+        float tz0 = (nd->lo[2] - it->o.z) * it->inv.z, tz1 = (nd->hi[2] - it->o.z) * it->inv.z;
+        float tn = vpo_max(vpo_max(vpo_min(tx0, tx1), vpo_min(ty0, ty1)), vpo_max(vpo_min(tz0, tz1), 0.0f));
+        float tf = vpo_min(vpo_min(vpo_max(tx0, tx1), vpo_max(ty0, ty1)), vpo_min(vpo_max(tz0, tz1), tmax));
+        if (!(tn <= tf)) continue;
+        if (nd->left < 0) return (long)(-1 - nd->left);
+        it->st[it->sp++] = nd->right;
+        it->st[it->sp++] = nd->left;
+    }
+    return -1;
+}
+static void cbvh_point_begin(cbvh_it* it, const cbvh_t* B, v3 x) {
+    it->B = B;
+    it->x = x;
+    it->sp = 0;
+    if (B->nn) it->st[it->sp++] = 0;
+}
+/* Next leaf whose box contains x, or -1. */
+static long cbvh_point_next(cbvh_it* it) {
+    while (it->sp > 0) {
+        const cnode_t* nd = &it->B->nodes[it->st[--it->sp]];
+        v3 x = it->x;
+        if (!(x.x >= nd->lo[0] && x.x <= nd->hi[0] && x.y >= nd->lo[1] && x.y <= nd->hi[1] && x.z >= nd->lo[2] &&
+              x.z <= nd->hi[2]))
+            continue;
+        if (nd->left < 0) return (long)(-1 - nd->left);
+        it->st[it->sp++] = nd->right;
+        it->st[it->sp++] = nd->left;
+    }
+    return -1;
+}
+
+static int cmp_u32(const void* a, const void* b) {
+    uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+/* Sorted indices of the primitives whose box contains x; returns the count. */
+static long cbvh_point_sorted(const cbvh_t* B, v3 x, uint32_t* out) {
+    cbvh_it it;
+    cbvh_point_begin(&it, B, x);
+    long c = 0, p;
+    while ((p = cbvh_point_next(&it)) >= 0) out[c++] = (uint32_t)p;
+    qsort(out, (size_t)c, sizeof(uint32_t), cmp_u32);
+    return c;
+}
+
+/* ------------------------------------------------------------------------ */
 /* shadow-ray transmittance (sampling.cuh:548-583, anyhit.cuh:53-66)          */
 /* ------------------------------------------------------------------------ */
 typedef struct {
@@ -368,7 +511,8 @@ static inline int tau_add(uint64_t* acc, float od) {
     return *acc >= TAU_LIMIT_FIXED;
 }
 
-static float transmittance(const prim_t* prims, long n, v3 o, v3 d, const uint32_t* set, int nset, counters_t* k) {
+static float transmittance(const prim_t* prims, long n, v3 o, v3 d, const uint32_t* set, int nset, counters_t* k,
+                           const cbvh_t* B) {
     uint64_t acc = 0;
     for (int s = 0; s < nset; ++s) {
         const prim_t* pr = prims + set[s];
@@ -378,7 +522,10 @@ static float transmittance(const prim_t* prims, long n, v3 o, v3 d, const uint32
             if (tau_add(&acc, optical_depth(pr, o, d, 0.0f, te))) return 0.0f;
         }
     }
-    for (long j = 0; j < n; ++j) {
+    /* entered primitives: every index (brute force) or the BVH's leaves along the ray */
+    cbvh_it it;
+    if (B) cbvh_ray_begin(&it, B, o, d);
+    for (long j = B ? cbvh_ray_next(&it, INF_F) : 0; B ? j >= 0 : j < n; j = B ? cbvh_ray_next(&it, INF_F) : j + 1) {
         const prim_t* pr = prims + j;
         float wl2;
         float t = entry_t(pr, o, d, &wl2);
@@ -404,7 +551,7 @@ void vpo_transmittance_rays(const float* prims16, long n, const float* rays, lon
         int ns = 0;
         for (long i = 0; i < n; ++i)
             if (point_inside(prims + i, o)) set[ns++] = (uint32_t)i;
-        T[r] = transmittance(prims, n, o, d, set, ns, NULL);
+        T[r] = transmittance(prims, n, o, d, set, ns, NULL, NULL);
     }
     free(set);
 }
@@ -760,6 +907,7 @@ typedef struct {
     uint32_t x0, y0, x1, y1;         /* pixel crop [x0,x1) x [y0,y1) */
     uint32_t sample_begin, sample_end; /* samples [begin, end) of spp */
     uint32_t stride;                   /* render every stride-th row and column of the crop (0 = 1) */
+    uint32_t use_bvh;                  /* queries through a CPU BVH instead of loops over all primitives */
 } vpo_params;
 
 typedef struct {
@@ -768,6 +916,7 @@ typedef struct {
     env_t env;
     const float* cam;
     const vpo_params* p;
+    const cbvh_t* bvh; /* NULL: brute force */
     hg_t hg;
     float* out_mean;
     float* out_samples;
@@ -787,10 +936,15 @@ static inline void set_push(uint32_t* s, int* ns, int cap, uint32_t v, counters_
  * accepted if inside the exit, and over the origin set of inv_cdf, accepted
  * if inside the exit (sampling.cuh:310-313,350-415).  INF = escape. */
 static float argmin_scatter(const prim_t* prims, long n, v3 o, v3 d, const uint32_t* act, int na, uint64_t key,
-                            uint32_t bounce, uint32_t* hits_out, counters_t* k) {
+                            uint32_t bounce, uint32_t* hits_out, counters_t* k, const cbvh_t* B, int prune) {
     float ts_min = INF_F;
     uint32_t hits = 0;
-    for (long j = 0; j < n; ++j) {
+    /* every index (brute force) or the BVH's leaves along the ray; with
+     * pruning, boxes beyond the running minimum are skipped */
+    cbvh_it it;
+    if (B) cbvh_ray_begin(&it, B, o, d);
+#define ARG_NEXT(j) (B ? cbvh_ray_next(&it, prune ? ts_min : INF_F) : (j) + 1)
+    for (long j = B ? ARG_NEXT(-1) : 0; B ? j >= 0 : j < n; j = ARG_NEXT(j)) {
         const prim_t* pr = prims + j;
         float wl2;
         float tj = entry_t(pr, o, d, &wl2);
@@ -806,6 +960,7 @@ static float argmin_scatter(const prim_t* prims, long n, v3 o, v3 d, const uint3
             if (ts <= te) ts_min = ts;
         }
     }
+#undef ARG_NEXT
     for (int s = 0; s < na; ++s) {
         const prim_t* pr = prims + act[s];
         k->n_arg_active++;
@@ -832,15 +987,17 @@ void vpo_first_scatter(const float* prims16, long n, const float* ray6, const ui
     memset(&k, 0, sizeof(k));
     for (long i = 0; i < m; ++i) {
         uint32_t hits;
-        out[i] = argmin_scatter(prims, n, o, d, act, na, keys[i], 0u, &hits, &k);
+        out[i] = argmin_scatter(prims, n, o, d, act, na, keys[i], 0u, &hits, &k, NULL, 0);
     }
     free(act);
 }
 
-static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, uint32_t* buf, counters_t* k) {
+static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, uint32_t* buf, uint32_t* cand,
+                       counters_t* k) {
     const vpo_params* P = J->p;
     const prim_t* prims = J->prims;
     const long n = J->n;
+    const cbvh_t* B = J->bvh;
     const int cap = (int)P->active_cap;
     uint32_t* act = buf;
     uint32_t* nxt = buf + cap;
@@ -872,24 +1029,33 @@ static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, 
     /* bounce-0 origin-inside set (sampling.cuh:322-341) */
     int na = 0;
     uint32_t overlap = 0;
-    for (long i = 0; i < n; ++i)
-        if (point_inside(prims + i, o)) {
-            ++overlap;
-            set_push(act, &na, cap, (uint32_t)i, k);
-        }
+    if (B) {
+        long c = cbvh_point_sorted(B, o, cand);
+        for (long q = 0; q < c; ++q)
+            if (point_inside(prims + cand[q], o)) {
+                ++overlap;
+                set_push(act, &na, cap, cand[q], k);
+            }
+    } else {
+        for (long i = 0; i < n; ++i)
+            if (point_inside(prims + i, o)) {
+                ++overlap;
+                set_push(act, &na, cap, (uint32_t)i, k);
+            }
+    }
     if (P->measure && overlap > k->max_overlap) k->max_overlap = overlap;
 
     for (uint32_t bounce = 0; bounce < P->max_depth; ++bounce) {
         /* argmin over entered primitives and the origin set (sampling.cuh:350-415) */
         uint32_t hits = 0;
         k->n_trace++;
-        float ts_min = argmin_scatter(prims, n, o, d, act, na, key, bounce, &hits, k);
+        float ts_min = argmin_scatter(prims, n, o, d, act, na, key, bounce, &hits, k, B, !P->measure);
         if (P->measure && hits > k->max_hits) k->max_hits = hits;
 
         /* analytic direct term at bounce 0 (raygen.cuh:148-160) */
         if (bounce == 0) {
             k->n_shadow_rays++;
-            float T = transmittance(prims, n, o, d, act, na, k);
+            float T = transmittance(prims, n, o, d, act, na, k, B);
             v3 env = env_lookup(&J->env, d);
             L = vadd(L, vmul(vscale(beta, T), env));
         }
@@ -897,9 +1063,25 @@ static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, 
 
         /* rebuild the active set at the scatter point (sampling.cuh:422-458) */
         int nn = 0;
+        v3 x = vadd(o, vscale(d, ts_min));
         {
+            /* ascending indices: all of them (brute force) or the union of the
+             * old set and the primitives whose box contains x */
             int ia = 0;
-            for (long j = 0; j < n; ++j) {
+            long nc = B ? cbvh_point_sorted(B, x, cand) : 0, ic = 0, prevj = -1;
+            for (long j = 0;; ++j) {
+                if (B) {
+                    while (ia < na && (long)act[ia] <= prevj) ++ia;
+                    while (ic < nc && (long)cand[ic] <= prevj) ++ic;
+                    uint32_t na_next = ia < na ? act[ia] : 0xffffffffu;
+                    uint32_t nc_next = ic < nc ? cand[ic] : 0xffffffffu;
+                    uint32_t nx = na_next < nc_next ? na_next : nc_next;
+                    if (nx == 0xffffffffu) break;
+                    j = nx;
+                } else if (j >= n) {
+                    break;
+                }
+                prevj = j;
                 int in_act = 0;
                 while (ia < na && act[ia] < (uint32_t)j) ++ia;
                 if (ia < na && act[ia] == (uint32_t)j) in_act = 1;
@@ -922,10 +1104,14 @@ static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, 
         }
         { uint32_t* t = act; act = nxt; nxt = t; na = nn; }
 
-        v3 x = vadd(o, vscale(d, ts_min));
         if (P->measure) {
             uint32_t ov = 0;
-            for (long i = 0; i < n; ++i) ov += (uint32_t)point_inside(prims + i, x);
+            if (B) {
+                long c = cbvh_point_sorted(B, x, cand);
+                for (long q = 0; q < c; ++q) ov += (uint32_t)point_inside(prims + cand[q], x);
+            } else {
+                for (long i = 0; i < n; ++i) ov += (uint32_t)point_inside(prims + i, x);
+            }
             if (ov > k->max_overlap) k->max_overlap = ov;
         }
         float phase_pdf;
@@ -964,7 +1150,7 @@ static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, 
                 float lumy = vdot(yenv, V(0.2126f, 0.7152f, 0.0722f));
                 if (lumy > 0.0f) {
                     k->n_shadow_rays++;
-                    float T = transmittance(prims, n, x, ydir, act, na, k);
+                    float T = transmittance(prims, n, x, ydir, act, na, k, B);
                     float Wr = wsum * vpo_rcp((float)P->ris_candidates);
                     L = vadd(L, vscale(vscale(vmul(base, vscale(yenv, vpo_rcp(lumy))), T), Wr));
                 }
@@ -977,7 +1163,7 @@ static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, 
             float wa = sa > 0.0f ? apdf / sa : 0.0f;
             v3 enva = env_lookup(&J->env, awo);
             k->n_shadow_rays++;
-            float Ta = transmittance(prims, n, x, awo, act, na, k);
+            float Ta = transmittance(prims, n, x, awo, act, na, k, B);
             L = vadd(L, vscale(vscale(vmul(base, enva), Ta), wa));
 
             float bpdf;
@@ -987,7 +1173,7 @@ static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, 
             float wb = sb > 0.0f ? bpdf / sb : 0.0f;
             v3 envb = env_lookup(&J->env, bwo);
             k->n_shadow_rays++;
-            float Tb = transmittance(prims, n, x, bwo, act, na, k);
+            float Tb = transmittance(prims, n, x, bwo, act, na, k, B);
             L = vadd(L, vscale(vscale(vscale(vscale(vmul(base, envb), Tb), wb), ph_b), vpo_rcp(bpdf)));
         }
         beta = vmul(beta, albedo);
@@ -1032,6 +1218,7 @@ static void* worker(void* arg) {
     job_t* J = (job_t*)arg;
     const vpo_params* P = J->p;
     uint32_t* buf = (uint32_t*)malloc(sizeof(uint32_t) * 2 * (P->active_cap ? P->active_cap : 1));
+    uint32_t* cand = J->bvh ? (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(J->n > 0 ? J->n : 1)) : NULL;
     counters_t k;
     memset(&k, 0, sizeof(k));
     for (;;) {
@@ -1045,7 +1232,7 @@ static void* worker(void* arg) {
             /* Welford mean over samples in order (raygen.cuh:214-219) */
             v3 mean = V(0, 0, 0);
             for (uint32_t s = P->sample_begin; s < P->sample_end; ++s) {
-                v3 Ls = trace_sample(J, x, y, s, buf, &k);
+                v3 Ls = trace_sample(J, x, y, s, buf, cand, &k);
                 if (J->out_samples) {
                     size_t idx = (((size_t)(s - P->sample_begin) * (P->y1 - P->y0) + (y - P->y0)) * (P->x1 - P->x0) + (x - P->x0)) * 4;
                     J->out_samples[idx] = Ls.x;
@@ -1069,6 +1256,7 @@ static void* worker(void* arg) {
     counters_add(&J->total, &k);
     pthread_mutex_unlock(&J->mu);
     free(buf);
+    free(cand);
     return NULL;
 }
 
@@ -1098,6 +1286,11 @@ int vpo_render(const float* prims16, long n, const float* env_rgba, const float*
     J.out_mean = out_mean;
     J.out_samples = out_samples;
     J.next_row = p->y0;
+    cbvh_t bvh = {NULL, 0};
+    if (p->use_bvh) {
+        cbvh_build(&bvh, J.prims, n);
+        J.bvh = &bvh;
+    }
     pthread_mutex_init(&J.mu, NULL);
     if (threads < 1) threads = 1;
     pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)threads);
@@ -1105,6 +1298,7 @@ int vpo_render(const float* prims16, long n, const float* env_rgba, const float*
     for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
     free(th);
     pthread_mutex_destroy(&J.mu);
+    free(bvh.nodes);
     if (counters) memcpy(counters, &J.total, sizeof(counters_t));
     return 0;
 }

@@ -432,3 +432,24 @@ def test_argmin_free_flight_distribution():
     for c in chords:
         inside_any |= (ts >= c[0] - 1e-5) & (ts <= c[1] + 1e-5)
     assert inside_any[~esc].all()
+
+
+@pytest.mark.parametrize("name", ["scatter", "ris", "small_cap", "measure", "dense"])
+def test_bvh_queries_match_brute_force(name):
+    # the CPU BVH (used for the timed CPU baseline) changes no image bit and
+    # no overflow count: every query result is consumed order-independently
+    rows = {"scatter": lambda: scenes.random_scene(200, 3), "ris": lambda: scenes.random_scene(120, 4),
+            "small_cap": lambda: scenes.random_scene(80, 8, smax=0.5), "measure": lambda: scenes.random_scene(90, 5),
+            "dense": lambda: scenes.c1_scattering(n=1024, seed=2)}[name]()
+    kw = {"ris": dict(use_ris=True), "small_cap": dict(active_cap=2), "measure": dict(measure=True),
+          "dense": dict(phase_g=0.0, max_depth=64)}.get(name, {})
+    rec = vpo.make_prims(rows)[vpo.scene_order(rows)]
+    env = scenes.sky_env(64, 32, sun_radius_deg=4.0, sun_scale=50.0)
+    cam = vpo.camera_perspective(20, 20, np.array([0, 0, -3.5], np.float32), np.zeros(3, np.float32),
+                                 np.array([0, 1, 0], np.float32), 40.0)
+    a, _, ka = vpo.render(rec, env, cam, 20, 20, 2, **kw)
+    b, _, kb = vpo.render(rec, env, cam, 20, 20, 2, bvh=True, **kw)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert ka["active_overflow"] == kb["active_overflow"]
+    if name == "measure":
+        assert ka["max_overlap"] == kb["max_overlap"] and ka["max_hits"] == kb["max_hits"]
