@@ -186,8 +186,9 @@ def run_mix(args):
         "plain": {"seconds": round(t_plain, 3), "msamples_s": round(samples / t_plain / 1e6, 4),
                   "p50_ms": ms(np.percentile(plain_lat, 50)), "p99_ms": ms(np.percentile(plain_lat, 99))},
         "batched": {"seconds": round(t_batch, 3), "msamples_s": round(samples / t_batch / 1e6, 4),
-                    "launch": {"block": tuned.block, "blocks_per_cu": tuned.blocks_per_cu,
-                               "active_cap": tuned.active_cap, "regen_min": tuned.regen_min},
+                    "launch": {"pipeline": "wavefront" if tuned.pipeline else "megakernel", "block": tuned.block,
+                               "blocks_per_cu": tuned.blocks_per_cu, "active_cap": tuned.active_cap,
+                               "regen_min": tuned.regen_min, "pool_paths": tuned.pool_paths},
                     "autotune_s": round(t_tune, 2), "launches": bst["launches"],
                     "speedup_vs_plain": round(t_plain / t_batch, 3)},
         "served_saturated": {"seconds": round(t_sat, 3), "msamples_s": round(samples / t_sat / 1e6, 4),

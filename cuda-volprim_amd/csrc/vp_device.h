@@ -19,8 +19,24 @@ __device__ __forceinline__ f3 safe_inv(f3 d) {
     return f3{1.0f / x, 1.0f / y, 1.0f / z};
 }
 
+// Scene arrays are reached through pointers stored in device memory (the job
+// table), which the compiler can only treat as generic; they always point to
+// global memory, and saying so turns flat loads into global loads (which do
+// not wait on the LDS counter).
+// (The host pass of the compiler only parses these device functions; the
+// qualifier is applied in the device pass.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define VP_GLOBAL_AS __attribute__((address_space(1)))
+#else
+#define VP_GLOBAL_AS
+#endif
+template <typename T>
+__device__ __forceinline__ const VP_GLOBAL_AS T* as_global(const T* p) {
+    return (const VP_GLOBAL_AS T*)(p);
+}
+
 __device__ __forceinline__ PrimRecord load_prim(const PrimRecord* prims, uint32_t i) {
-    const float4* p = reinterpret_cast<const float4*>(prims + i);
+    const auto* p = as_global(reinterpret_cast<const float4*>(prims + i));
     const float4 a = p[0], b = p[1], c = p[2], d = p[3];
     PrimRecord r;
     r.cx = a.x; r.cy = a.y; r.cz = a.z; r.qw = a.w;
@@ -38,8 +54,8 @@ __device__ __forceinline__ void walk_ray(const SceneView& s, f3 o, f3 inv, const
     uint32_t i = 0;
     const uint32_t nn = s.n_nodes;
     while (i < nn) {
-        const float4 a = s.nodes[2 * i];
-        const float4 b = s.nodes[2 * i + 1];
+        const float4 a = as_global(s.nodes)[2 * i];
+        const float4 b = as_global(s.nodes)[2 * i + 1];
         ++visits;
         const float tx0 = (a.x - o.x) * inv.x, tx1 = (b.x - o.x) * inv.x;
         const float ty0 = (a.y - o.y) * inv.y, ty1 = (b.y - o.y) * inv.y;
@@ -68,8 +84,8 @@ __device__ __forceinline__ void walk_point(const SceneView& s, f3 x, uint32_t& v
     uint32_t i = 0;
     const uint32_t nn = s.n_nodes;
     while (i < nn) {
-        const float4 a = s.nodes[2 * i];
-        const float4 b = s.nodes[2 * i + 1];
+        const float4 a = as_global(s.nodes)[2 * i];
+        const float4 b = as_global(s.nodes)[2 * i + 1];
         ++visits;
         const bool in = x.x >= a.x && x.x <= b.x && x.y >= a.y && x.y <= b.y && x.z >= a.z && x.z <= b.z;
         const uint32_t skip = __float_as_uint(a.w);
@@ -112,8 +128,8 @@ __device__ __forceinline__ void walk_ray_q(const SceneView& s, f3 o, f3 inv, con
     while (true) {
         uint32_t cnt = 0;
         while (i < nn && cnt < kLeafQ) {
-            const float4 a = s.nodes[2 * i];
-            const float4 b = s.nodes[2 * i + 1];
+            const float4 a = as_global(s.nodes)[2 * i];
+            const float4 b = as_global(s.nodes)[2 * i + 1];
             ++visits;
             const float tx0 = (a.x - o.x) * inv.x, tx1 = (b.x - o.x) * inv.x;
             const float ty0 = (a.y - o.y) * inv.y, ty1 = (b.y - o.y) * inv.y;
@@ -147,8 +163,8 @@ __device__ __forceinline__ void walk_point_q(const SceneView& s, f3 x, uint32_t&
     while (true) {
         uint32_t cnt = 0;
         while (i < nn && cnt < kLeafQ) {
-            const float4 a = s.nodes[2 * i];
-            const float4 b = s.nodes[2 * i + 1];
+            const float4 a = as_global(s.nodes)[2 * i];
+            const float4 b = as_global(s.nodes)[2 * i + 1];
             ++visits;
             const bool in = x.x >= a.x && x.x <= b.x && x.y >= a.y && x.y <= b.y && x.z >= a.z && x.z <= b.z;
             const uint32_t skip = __float_as_uint(a.w);
@@ -241,6 +257,7 @@ __device__ float transmittance(const SceneView& s, f3 o, f3 d, const uint32_t* a
         return true;
     });
     ev.add(kCntNodeVisits, visits);
+    ev.add(kCntVisitsShadow, visits);
     ev.add(kCntLeafTests, leaves);
     ev.add(kCntShHit, sh);
     if (full) return 0.0f;

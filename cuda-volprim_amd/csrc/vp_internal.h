@@ -106,7 +106,9 @@ enum CounterSlot : int {
     kCntLeafTests = 15,    // entry tests at BVH leaves (all traversals)
     kCntNodeVisits = 16,   // BVH nodes loaded (all traversals)
     kCntContain = 17,      // containment tests (point queries)
-    kNumCounters = 18,
+    kCntVisitsShadow = 18, // BVH nodes loaded by shadow-ray walks (part of kCntNodeVisits)
+    kCntVisitsPoint = 19,  // BVH nodes loaded by point queries (part of kCntNodeVisits)
+    kNumCounters = 20,
 };
 
 }  // namespace vp

@@ -95,13 +95,17 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                         walk_point(sc, o, visits, [&](uint32_t prim) {
                             ++tests;
                             if (point_inside(load_prim(sc.prims, prim), o)) {
+                                // counters updated without a branch keep them in registers
+                                const bool fits = na < cap;
+                                if (fits) my[na * stride] = prim;
+                                na += fits ? 1u : 0u;
+                                overflow += fits ? 0u : 1u;
                                 ++ov;
-                                if (na < cap) my[na * stride] = prim, ++na;
-                                else ++overflow;
                             }
                         });
                         st.na = na;
                         ev.add(kCntNodeVisits, visits);
+                        ev.add(kCntVisitsPoint, visits);
                         ev.add(kCntContain, tests);
                         ev.add(kCntPaths);
                         if (measure) atomicMax(&counters[kCntMaxOverlap], static_cast<unsigned long long>(ov));
@@ -185,8 +189,10 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
             const f3 x = o + ts * d;
             uint32_t nn = 0, ia = 0, visits = 0, rb_a = 0, rb_h = 0;
             auto push = [&](uint32_t v) {
-                if (nn < cap) nxt[nn * stride] = v, ++nn;
-                else ++overflow;
+                const bool fits = nn < cap;
+                if (fits) nxt[nn * stride] = v;
+                nn += fits ? 1u : 0u;
+                overflow += fits ? 0u : 1u;
             };
             walk_point_q(sc, x, visits, lq, [&](uint32_t c) {
                 while (ia < na) {
@@ -218,6 +224,7 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                 ++ia;
             }
             ev.add(kCntNodeVisits, visits);
+            ev.add(kCntVisitsPoint, visits);
             ev.add(kCntRbActive, rb_a);
             ev.add(kCntRbHit, rb_h);
             st.cur ^= 1u;

@@ -468,38 +468,25 @@ vp_status vp_autotune(vp_context* ctx, vp_job* jobs, uint32_t njobs, vp_launch_c
         // below the capacities the jobs ask for
         for (uint32_t i = 0; i < njobs; ++i) cap = std::max(cap, jobs[i].params.active_cap);
     }
-    vp_launch_config best_cfg{};
-    double best_rate = -1.0;
-    const uint32_t blocks[] = {64, 128, 256};
-    const uint32_t regens[] = {1, 8, 32};
-    for (uint32_t b : blocks) {
+    // candidate shapes: megakernel workgroup size x regeneration threshold
+    // at the occupancy limit, one at half occupancy, and the wavefront
+    // pipeline at two pool sizes
+    std::vector<vp_launch_config> cands;
+    for (uint32_t b : {128u, 256u}) {
         int occ = 1;
         vp_render_occupancy(b, cap, 0, &occ);
-        const uint32_t bpcs[] = {static_cast<uint32_t>(occ), std::max<uint32_t>(1u, static_cast<uint32_t>(occ) / 2u)};
-        for (uint32_t bi = 0; bi < 2; ++bi) {
-            if (bi == 1 && bpcs[1] == bpcs[0]) continue;
-            for (uint32_t rg : regens) {
-                vp_launch_config c{b, bpcs[bi], cap, 0, rg, VP_PIPELINE_MEGAKERNEL, 0};
-                std::vector<JobRun> runs;
-                make_runs(runs);
-                double ms = 0.0;
-                uint64_t paths = 0;
-                const auto t0 = clk::now();
-                vp_status st = vp_run_jobs(ctx, runs, &c, cap, 0, 0, s, &ms, &paths);
-                if (st != VP_OK) return st;
-                const double wall = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
-                const double rate = static_cast<double>(paths) / std::max(wall, 1e-3);
-                if (rate > best_rate) {
-                    best_rate = rate;
-                    best_cfg = c;
-                }
-            }
-        }
+        for (uint32_t rg : {1u, 8u})
+            cands.push_back(vp_launch_config{b, static_cast<uint32_t>(occ), cap, 0, rg, VP_PIPELINE_MEGAKERNEL, 0});
+        if (b == 256u && occ > 1)
+            cands.push_back(
+                vp_launch_config{b, static_cast<uint32_t>(occ) / 2u, cap, 0, 8u, VP_PIPELINE_MEGAKERNEL, 0});
     }
-    // the wavefront pipeline at three pool sizes
-    for (uint32_t pool : {1u << 19, 1u << 20, 1u << 21}) {
-        if (static_cast<uint64_t>(pool) * cap * 2 >= (uint64_t(1) << 32)) continue;
-        vp_launch_config c{256u, 0u, cap, 0u, 1u, VP_PIPELINE_WAVEFRONT, pool};
+    for (uint32_t pool : {1u << 20, 1u << 21})
+        if (static_cast<uint64_t>(pool) * cap * 2 < (uint64_t(1) << 32))
+            cands.push_back(vp_launch_config{256u, 0u, cap, 0u, 1u, VP_PIPELINE_WAVEFRONT, pool});
+    vp_launch_config best_cfg = cands.front();
+    double best_rate = -1.0;
+    for (const vp_launch_config& c : cands) {
         std::vector<JobRun> runs;
         make_runs(runs);
         uint64_t paths = 0;

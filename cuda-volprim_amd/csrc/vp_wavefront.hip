@@ -178,8 +178,10 @@ __global__ void VP_WF_LB k_wf_regen(const JobDesc* __restrict__ jobs, const uint
                 const SceneView sc = J.scene;
                 walk_point(sc, o, visits, [&](uint32_t prim) {
                     if (point_inside(load_prim(sc.prims, prim), o)) {
-                        if (na < w.cap) act[static_cast<size_t>(na) * P] = prim, ++na;
-                        else ++overflow;
+                        const bool fits = na < w.cap;
+                        if (fits) act[static_cast<size_t>(na) * P] = prim;
+                        na += fits ? 1u : 0u;
+                        overflow += fits ? 0u : 1u;
                     }
                 });
                 w.ob[i] = make_float4(o.x, o.y, o.z, __uint_as_float(0u));
@@ -315,8 +317,10 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
                 const f3 x = o + ts * d;
                 uint32_t cnt = 0, ia = 0, visits = 0;
                 auto push = [&](uint32_t v) {
-                    if (cnt < w.cap) nxt[static_cast<size_t>(cnt) * P] = v, ++cnt;
-                    else ++overflow;
+                    const bool fits = cnt < w.cap;
+                    if (fits) nxt[static_cast<size_t>(cnt) * P] = v;
+                    cnt += fits ? 1u : 0u;
+                    overflow += fits ? 0u : 1u;
                 };
                 walk_point_q(sc, x, visits, lq, [&](uint32_t c) {
                     while (ia < na) {

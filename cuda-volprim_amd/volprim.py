@@ -101,7 +101,7 @@ EXPORTED = [
 ]
 EVENT_NAMES = ["active_overflow", "max_hits", "max_overlap", "paths", "traces", "scatters", "shadow_rays",
                "arg_hit", "arg_active", "arg_accept", "rb_active", "rb_hit", "alb", "sh_active", "sh_hit",
-               "leaf_tests", "node_visits", "contain"]
+               "leaf_tests", "node_visits", "contain", "visits_shadow", "visits_point"]
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
