@@ -51,15 +51,19 @@ __device__ __forceinline__ PrimRecord load_prim(const PrimRecord* prims, uint32_
 template <typename Leaf>
 __device__ __forceinline__ void walk_ray(const SceneView& s, f3 o, f3 inv, const float& tmax, uint32_t& visits,
                                          Leaf&& leaf) {
+    // slab distances as fma(bound, 1/d, -o/d): which boxes a walk enters
+    // only decides what is visited (leaf boxes are conservative), never a
+    // result
+    const f3 oi = o * inv;
     uint32_t i = 0;
     const uint32_t nn = s.n_nodes;
     while (i < nn) {
         const float4 a = as_global(s.nodes)[2 * i];
         const float4 b = as_global(s.nodes)[2 * i + 1];
         ++visits;
-        const float tx0 = (a.x - o.x) * inv.x, tx1 = (b.x - o.x) * inv.x;
-        const float ty0 = (a.y - o.y) * inv.y, ty1 = (b.y - o.y) * inv.y;
-        const float tz0 = (a.z - o.z) * inv.z, tz1 = (b.z - o.z) * inv.z;
+        const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(b.x, inv.x, -oi.x);
+        const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(b.y, inv.y, -oi.y);
+        const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(b.z, inv.z, -oi.z);
         const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
         const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
         const uint32_t skip = __float_as_uint(a.w);
@@ -123,6 +127,7 @@ struct LeafQueue {
 template <typename Leaf>
 __device__ __forceinline__ void walk_ray_q(const SceneView& s, f3 o, f3 inv, const float& tmax, uint32_t& visits,
                                            const LeafQueue& q, Leaf&& leaf) {
+    const f3 oi = o * inv;
     uint32_t i = 0;
     const uint32_t nn = s.n_nodes;
     while (true) {
@@ -131,23 +136,16 @@ __device__ __forceinline__ void walk_ray_q(const SceneView& s, f3 o, f3 inv, con
             const float4 a = as_global(s.nodes)[2 * i];
             const float4 b = as_global(s.nodes)[2 * i + 1];
             ++visits;
-            const float tx0 = (a.x - o.x) * inv.x, tx1 = (b.x - o.x) * inv.x;
-            const float ty0 = (a.y - o.y) * inv.y, ty1 = (b.y - o.y) * inv.y;
-            const float tz0 = (a.z - o.z) * inv.z, tz1 = (b.z - o.z) * inv.z;
+            const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(b.x, inv.x, -oi.x);
+            const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(b.y, inv.y, -oi.y);
+            const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(b.z, inv.z, -oi.z);
             const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
             const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
             const uint32_t skip = __float_as_uint(a.w);
             const uint32_t prim = __float_as_uint(b.w);
-            if (tnear <= tfar) {
-                if (prim != kInnerNode) {
-                    q.at(cnt++) = prim;
-                    i = skip;
-                } else {
-                    i = i + 1;
-                }
-            } else {
-                i = skip;
-            }
+            const bool hit = tnear <= tfar, leaf_node = prim != kInnerNode;
+            if (hit && leaf_node) q.at(cnt++) = prim;
+            i = (hit && !leaf_node) ? i + 1 : skip;
         }
         for (uint32_t k = 0; k < cnt; ++k)
             if (!leaf(q.at(k))) return;
@@ -169,16 +167,9 @@ __device__ __forceinline__ void walk_point_q(const SceneView& s, f3 x, uint32_t&
             const bool in = x.x >= a.x && x.x <= b.x && x.y >= a.y && x.y <= b.y && x.z >= a.z && x.z <= b.z;
             const uint32_t skip = __float_as_uint(a.w);
             const uint32_t prim = __float_as_uint(b.w);
-            if (in) {
-                if (prim != kInnerNode) {
-                    q.at(cnt++) = prim;
-                    i = skip;
-                } else {
-                    i = i + 1;
-                }
-            } else {
-                i = skip;
-            }
+            const bool leaf_node = prim != kInnerNode;
+            if (in && leaf_node) q.at(cnt++) = prim;
+            i = (in && !leaf_node) ? i + 1 : skip;
         }
         for (uint32_t k = 0; k < cnt; ++k) leaf(q.at(k));
         if (i >= nn) return;

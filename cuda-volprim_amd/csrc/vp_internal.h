@@ -75,7 +75,7 @@ struct WfPool {
     uint32_t* live;      // [P] live-slot queue
     uint32_t* shadow;    // [3P] shadow-ray queue (slot << 2 | term)
     uint32_t* free_q;    // [P] free-slot queue
-    unsigned int* qc;    // [2][4] live / shadow / free / claimed counts per parity
+    unsigned int* qc;    // [2][8] per parity: live, shadow, free, claimed, extend and shadow fetch heads
     unsigned long long* claim_base;  // [2] first path number claimed per parity
     unsigned long long* finished;    // paths retired (or padding) so far
     unsigned long long* job_done;    // per job-table entry: paths retired so far

@@ -78,13 +78,15 @@ __device__ __forceinline__ void wave_add_per_key(unsigned long long* arr, bool f
     }
 }
 
-// queue counters of parity par: [0] live, [1] shadow, [2] free, [3] claimed
-__device__ __forceinline__ unsigned int* qcs(const WfPool& w, uint32_t par) { return w.qc + 4 * par; }
+// queue counters of parity par: [0] live, [1] shadow, [2] free, [3] claimed,
+// [4] live paths taken by k_wf_extend, [5] shadow rays taken by k_wf_shadow
+constexpr uint32_t kQc = 8;
+__device__ __forceinline__ unsigned int* qcs(const WfPool& w, uint32_t par) { return w.qc + kQc * par; }
 
 __global__ void VP_WF_LB k_wf_init(WfPool w) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < w.P) w.misc[i] = make_uint4(0u, 0u, 0u, f_pack(0u, kSlotFree, 0u));
-    if (i < 8) w.qc[i] = 0u;
+    if (i < 2 * kQc) w.qc[i] = 0u;
     if (i < 2) w.claim_base[i] = 0ull;
     if (i == 0) *w.finished = 0ull;
 }
@@ -93,7 +95,7 @@ __global__ void VP_WF_LB k_wf_init(WfPool w) {
 // terms, retire finished paths, queue free and live slots.
 __global__ void VP_WF_LB k_wf_shade(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par) {
     const uint32_t P = w.P;
-    if (blockIdx.x == 0 && threadIdx.x < 4) qcs(w, par ^ 1u)[threadIdx.x] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < kQc) qcs(w, par ^ 1u)[threadIdx.x] = 0u;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = i < P;
     uint4 m = in ? w.misc[i] : make_uint4(0u, 0u, 0u, f_pack(0u, kSlotFree, 0u));
@@ -205,54 +207,108 @@ __global__ void VP_WF_LB k_wf_regen(const JobDesc* __restrict__ jobs, const uint
     if (overflow) atomicAdd(&counters[kCntActiveOverflow], static_cast<unsigned long long>(overflow));
 }
 
+// Work fetch for the walk kernels: every idle lane of a wavefront takes the
+// next queue entry (one atomic per wavefront), so lanes whose walk ended
+// start another one instead of idling until the wavefront's longest walk is
+// done.  Returns the entry for this lane or n (none).
+__device__ __forceinline__ uint32_t wave_fetch(unsigned int* head, bool want, uint32_t n) {
+    const unsigned long long m = __ballot(want);
+    if (m == 0) return n;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = __ffsll(static_cast<long long>(m)) - 1;
+    uint32_t base = n;
+    if (lane == leader) {
+        // the head only grows; once it has passed n nobody adds to it again,
+        // so it stays far from wrapping around
+        if (__hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < n)
+            base = atomicAdd(head, static_cast<unsigned int>(__popcll(m)));
+    }
+    base = __shfl(base, leader);
+    if (base >= n) return n;
+    const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t q = base + static_cast<uint32_t>(__popcll(m & lt));
+    return want && q < n ? q : n;
+}
+
+// Argmin free flight (sampling.cuh:350-415) for every live path: origin-set
+// candidates first, then one pruned walk over entered primitives with the
+// leaf queue; lanes refill from the live queue as their walks end.
 __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par) {
     extern __shared__ uint32_t s_leafq[];
     const LeafQueue lq{s_leafq + threadIdx.x, blockDim.x};
-    const uint32_t n = qcs(w, par)[0];
+    unsigned int* qc = qcs(w, par);
+    const uint32_t n = qc[0];
     const uint32_t P = w.P;
-    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
-        const uint32_t i = w.live[idx];
-        const float4 ob = w.ob[i], dj = w.dj[i];
-        const f3 o = f3{ob.x, ob.y, ob.z}, d = f3{dj.x, dj.y, dj.z};
-        const uint32_t b = __float_as_uint(ob.w);
-        const JobDesc& J = jobs[__float_as_uint(dj.w)];
-        const SceneView sc = J.scene;
-        const uint4 m = w.misc[i];
-        const uint64_t key = static_cast<uint64_t>(m.x) | (static_cast<uint64_t>(m.y) << 32);
-        const uint32_t na = __float_as_uint(w.lr[i].w);
-        const uint32_t* act = w.slab + static_cast<size_t>(f_cur(m.w)) * w.cap * P + i;
-        // argmin over the origin set (sampling.cuh:356-370)
-        float ts = kTMax;
-        for (uint32_t k = 0; k < na; ++k) {
-            const uint32_t pi = act[static_cast<size_t>(k) * P];
-            const PrimRecord pr = load_prim(sc.prims, pi);
-            const float te = exit_from_inside(pr, o, d);
-            const float tau = free_flight_tau(prim_uniform(key, b, 0u, pi));
-            const float t = inv_cdf(pr, o, d, tau);
-            if (t >= 0.0f && t < ts && t <= te) ts = t;
+    bool have = false;
+    uint32_t i = 0, b = 0, node = 0, nn = 0;
+    uint64_t key = 0;
+    f3 o = mk(0.0f), d = mk(0.0f), inv = mk(0.0f), oi = mk(0.0f);
+    float ts = kTMax;
+    SceneView sc{};
+    while (true) {
+        const uint32_t idx = wave_fetch(&qc[4], !have, n);
+        if (!have && idx < n) {
+            i = w.live[idx];
+            const float4 ob = w.ob[i], dj = w.dj[i];
+            o = f3{ob.x, ob.y, ob.z};
+            d = f3{dj.x, dj.y, dj.z};
+            b = __float_as_uint(ob.w);
+            sc = jobs[__float_as_uint(dj.w)].scene;
+            const uint4 m = w.misc[i];
+            key = static_cast<uint64_t>(m.x) | (static_cast<uint64_t>(m.y) << 32);
+            const uint32_t na = __float_as_uint(w.lr[i].w);
+            const uint32_t* act = w.slab + static_cast<size_t>(f_cur(m.w)) * w.cap * P + i;
+            // argmin over the origin set (sampling.cuh:356-370)
+            ts = kTMax;
+            for (uint32_t k = 0; k < na; ++k) {
+                const uint32_t pi = act[static_cast<size_t>(k) * P];
+                const PrimRecord pr = load_prim(sc.prims, pi);
+                const float te = exit_from_inside(pr, o, d);
+                const float tau = free_flight_tau(prim_uniform(key, b, 0u, pi));
+                const float t = inv_cdf(pr, o, d, tau);
+                if (t >= 0.0f && t < ts && t <= te) ts = t;
+            }
+            inv = safe_inv(d);
+            oi = o * inv;
+            node = 0;
+            nn = sc.n_nodes;
+            have = true;
         }
-        // argmin over entered primitives (sampling.cuh:372-412) in one pruned walk
-        const f3 inv = safe_inv(d);
-        float tmax = ts;
-        uint32_t visits = 0;
-        walk_ray_q(sc, o, inv, tmax, visits, lq, [&](uint32_t prim) -> bool {
+        if (__ballot(have) == 0) break;
+        // node steps until the lane's leaf queue is full or its walk ends
+        uint32_t cnt = 0;
+        while (have && node < nn && cnt < kLeafQ) {
+            const float4 a = as_global(sc.nodes)[2 * node];
+            const float4 c = as_global(sc.nodes)[2 * node + 1];
+            const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(c.x, inv.x, -oi.x);
+            const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(c.y, inv.y, -oi.y);
+            const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(c.z, inv.z, -oi.z);
+            const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+            const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), ts));
+            const uint32_t skip = __float_as_uint(a.w), prim = __float_as_uint(c.w);
+            const bool hit = tnear <= tfar, leaf_node = prim != kInnerNode;
+            if (hit && leaf_node) lq.at(cnt++) = prim;
+            node = (hit && !leaf_node) ? node + 1 : skip;
+        }
+        // entered primitives (sampling.cuh:372-412)
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const uint32_t prim = lq.at(k);
             const PrimRecord pr = load_prim(sc.prims, prim);
             float wl2;
             const float tj = entry_t(pr, o, d, &wl2);
-            if (tj < 0.0f) return true;
+            if (tj < 0.0f) continue;
             const float tau = free_flight_tau(prim_uniform(key, b, 1u, prim));
             float t = inv_cdf_span(pr, o, d, tj, tau);
             if (t >= 0.0f && t < tj) t = tj;
             if (t >= 0.0f && t < ts) {
                 const float te = exit_from_entry(pr, o, d, tj, wl2);
-                if (t <= te) {
-                    ts = t;
-                    tmax = ts;
-                }
+                if (t <= te) ts = t;
             }
-            return true;
-        });
-        w.bt[i].w = ts;
+        }
+        if (have && node >= nn) {
+            w.bt[i].w = ts;
+            have = false;
+        }
     }
 }
 
@@ -444,23 +500,73 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
     if (overflow) atomicAdd(&counters[kCntActiveOverflow], static_cast<unsigned long long>(overflow));
 }
 
+// Fixed-point transmittance (sampling.cuh:548-583, anyhit.cuh:53-66) of every
+// queued shadow ray: origin-set chords first, then one walk over entered
+// primitives; lanes refill from the shadow queue as their rays end.
 __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par) {
     extern __shared__ uint32_t s_leafq[];
     const LeafQueue lq{s_leafq + threadIdx.x, blockDim.x};
-    const uint32_t n = qcs(w, par)[1];
+    unsigned int* qc = qcs(w, par);
+    const uint32_t n = qc[1];
     const uint32_t P = w.P;
-    Ev<false> ev;
-    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
-        const uint32_t e = w.shadow[idx];
-        const uint32_t i = e >> 2, k = e & 3u;
-        const float4 po = w.po[k * P + i];
-        const float4 pd = w.pd[k * P + i];
-        const uint32_t sc_bits = __float_as_uint(po.w);
-        const uint32_t set = sc_bits & 1u, count = sc_bits >> 1;
-        const JobDesc& J = jobs[__float_as_uint(w.dj[i].w)];
-        const uint32_t* act = w.slab + static_cast<size_t>(set) * w.cap * P + i;
-        const float T = transmittance(J.scene, f3{po.x, po.y, po.z}, f3{pd.x, pd.y, pd.z}, act, count, P, lq, ev);
-        w.pd[k * P + i].w = T;
+    bool have = false, full = false;
+    uint32_t e = 0, node = 0, nn = 0;
+    uint64_t acc = 0;
+    f3 o = mk(0.0f), d = mk(0.0f), inv = mk(0.0f), oi = mk(0.0f);
+    SceneView sc{};
+    while (true) {
+        const uint32_t idx = wave_fetch(&qc[5], !have, n);
+        if (!have && idx < n) {
+            const uint32_t ent = w.shadow[idx];
+            const uint32_t i = ent >> 2, k = ent & 3u;
+            e = k * P + i;
+            const float4 po = w.po[e], pd = w.pd[e];
+            o = f3{po.x, po.y, po.z};
+            d = f3{pd.x, pd.y, pd.z};
+            const uint32_t bits = __float_as_uint(po.w);
+            const uint32_t set = bits & 1u, count = bits >> 1;
+            sc = jobs[__float_as_uint(w.dj[i].w)].scene;
+            const uint32_t* act = w.slab + static_cast<size_t>(set) * w.cap * P + i;
+            acc = 0;
+            full = false;
+            for (uint32_t c = 0; c < count && !full; ++c) {
+                const PrimRecord pr = load_prim(sc.prims, act[static_cast<size_t>(c) * P]);
+                const float te = exit_from_inside(pr, o, d);
+                if (te > 0.0f && te < kTMax) full = tau_add(acc, optical_depth(pr, o, d, 0.0f, te));
+            }
+            inv = safe_inv(d);
+            oi = o * inv;
+            node = 0;
+            nn = sc.n_nodes;
+            have = true;
+        }
+        if (__ballot(have) == 0) break;
+        uint32_t cnt = 0;
+        while (have && !full && node < nn && cnt < kLeafQ) {
+            const float4 a = as_global(sc.nodes)[2 * node];
+            const float4 c = as_global(sc.nodes)[2 * node + 1];
+            const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(c.x, inv.x, -oi.x);
+            const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(c.y, inv.y, -oi.y);
+            const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(c.z, inv.z, -oi.z);
+            const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+            const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), kTMax));
+            const uint32_t skip = __float_as_uint(a.w), prim = __float_as_uint(c.w);
+            const bool hit = tnear <= tfar, leaf_node = prim != kInnerNode;
+            if (hit && leaf_node) lq.at(cnt++) = prim;
+            node = (hit && !leaf_node) ? node + 1 : skip;
+        }
+        for (uint32_t k = 0; k < cnt && !full; ++k) {
+            const PrimRecord pr = load_prim(sc.prims, lq.at(k));
+            float wl2;
+            const float tj = entry_t(pr, o, d, &wl2);
+            if (tj < 0.0f) continue;
+            const float te = exit_from_entry(pr, o, d, tj, wl2);
+            if (te > tj && te < kTMax) full = tau_add(acc, optical_depth(pr, o, d, tj, te));
+        }
+        if (have && (full || node >= nn)) {
+            w.pd[e].w = full ? 0.0f : tau_to_T(acc);
+            have = false;
+        }
     }
 }
 
