@@ -225,6 +225,8 @@ def main():
     ap.add_argument("--pipeline", choices=["mega", "wave"], default="mega",
                     help="persistent megakernel or wavefront-queue pipeline")
     ap.add_argument("--pool", type=int, default=0, help="wavefront path slots (0 = library default)")
+    ap.add_argument("--bvh", choices=["lbvh", "sah"], default="lbvh",
+                    help="GPU LBVH (default) or the host SAH tree in the same layout")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB elementwise copy kernel first (HBM counter calibration)")
     ap.add_argument("--mix", action="store_true", help="run the job mix (plain vs batched vs served) instead")
@@ -250,7 +252,7 @@ def main():
     rows, env, cam_args, pp = workload(scenes)
     ctx = volprim.Context(0)
     dev_name, n_cu, clk = ctx.device_info()
-    scene = ctx.scene(rows)
+    scene = ctx.scene(rows, bvh=args.bvh)
     envh = ctx.env(env)
     cam = volprim.camera_perspective(**cam_args)
     params = volprim.default_params(**pp)
@@ -349,6 +351,7 @@ def main():
         "device": dev_name,
         "kernel_ms_per_step": round(kernel_ms / args.steps, 3),
         "pipeline": args.pipeline,
+        "bvh": args.bvh,
         "events_per_step": events,
         "quality": quality,
         "image_mean": float(img[..., 0:3].mean()),

@@ -183,7 +183,13 @@ vp_status vp_device_info(vp_context* ctx, char* name, size_t name_len, uint32_t*
 }
 
 vp_status vp_scene_create(vp_context* ctx, const vp_primitive* prims, uint32_t n, vp_scene** out) {
+    return vp_scene_create_ex(ctx, prims, n, 0u, out);
+}
+
+vp_status vp_scene_create_ex(vp_context* ctx, const vp_primitive* prims, uint32_t n, uint32_t flags,
+                             vp_scene** out) {
     if (!ctx || !prims || !out) return fail(VP_ERR_INVALID, "null argument");
+    if (flags & ~VP_SCENE_BVH_SAH) return fail(VP_ERR_INVALID, "unknown scene flags");
     if (n == 0) return fail(VP_ERR_INVALID, "scene has no primitives");
     if (n > (1u << 30)) return fail(VP_ERR_INVALID, "scene has more than 2^30 primitives");
     // the validation of loadPrimitivesFromPLY (ply.cpp:146-172)
@@ -203,7 +209,8 @@ vp_status vp_scene_create(vp_context* ctx, const vp_primitive* prims, uint32_t n
     auto* sc = new vp_scene();
     sc->ctx = ctx;
     sc->n = n;
-    const vp_status st = vp_build_scene_device(ctx, prims, n, sc);
+    vp_status st = vp_build_scene_device(ctx, prims, n, sc);
+    if (st == VP_OK && (flags & VP_SCENE_BVH_SAH)) st = vp_build_scene_sah(ctx, prims, n, sc);
     if (st != VP_OK) {
         vp_scene_destroy(sc);
         return st;

@@ -26,11 +26,7 @@ __device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
     return v;
 }
 
-// Records, centres and conservative leaf boxes.  The 3-sigma ellipsoid with
-// forward rotation R and scale s has half extent 3 sqrt(sum_j (R_ij s_j)^2)
-// along world axis i; the box is widened by a relative 2^-10 and an absolute
-// 2^-20 (|c| + h) so every point the FP32 entry/containment tests accept lies
-// inside it.
+// Records, centres and conservative leaf boxes (conservative_box).
 __global__ void k_setup(const vp_primitive* in, uint32_t n, PrimRecord* rec, float* centers, float* box) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -43,23 +39,7 @@ __global__ void k_setup(const vp_primitive* in, uint32_t n, PrimRecord* rec, flo
     centers[3 * i + 0] = c.x;
     centers[3 * i + 1] = c.y;
     centers[3 * i + 2] = c.z;
-    // forward rotation = conjugate of the stored one
-    const double qw = r.qw, qx = -(double)r.qx, qy = -(double)r.qy, qz = -(double)r.qz;
-    const double R[3][3] = {
-        {1.0 - 2.0 * (qy * qy + qz * qz), 2.0 * (qx * qy - qw * qz), 2.0 * (qx * qz + qw * qy)},
-        {2.0 * (qx * qy + qw * qz), 1.0 - 2.0 * (qx * qx + qz * qz), 2.0 * (qy * qz - qw * qx)},
-        {2.0 * (qx * qz - qw * qy), 2.0 * (qy * qz + qw * qx), 1.0 - 2.0 * (qx * qx + qy * qy)},
-    };
-    const double sv[3] = {p.scale[0], p.scale[1], p.scale[2]};
-    const double cv[3] = {c.x, c.y, c.z};
-    for (int a = 0; a < 3; ++a) {
-        double h2 = 0.0;
-        for (int j = 0; j < 3; ++j) h2 += (R[a][j] * sv[j]) * (R[a][j] * sv[j]);
-        double h = 3.0 * sqrt(h2);
-        h = h * (1.0 + 0x1p-10) + 0x1p-20 * (fabs(cv[a]) + h);
-        box[6 * i + a] = (float)(cv[a] - h);
-        box[6 * i + 3 + a] = (float)(cv[a] + h);
-    }
+    conservative_box(r, p.scale, box + 6 * i, box + 6 * i + 3);
 }
 
 // Centre bounds: per-block partial min/max, then one block folds them.

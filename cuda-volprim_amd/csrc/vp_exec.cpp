@@ -158,6 +158,7 @@ RenderLaunch vp_resolve_launch(vp_context* ctx, const vp_launch_config* cfg, uin
         vp_wf_occupancy(2, &osh);
         vp_wf_occupancy(3, &org);
         L.wf.grid_regen = cus * static_cast<uint32_t>(org);
+        L.wf.fetch_min = c.regen_min ? L.regen_min : 32u;
         L.wf.grid_extend = cus * static_cast<uint32_t>(oe);
         L.wf.grid_scatter = cus * static_cast<uint32_t>(os);
         L.wf.grid_shadow = cus * static_cast<uint32_t>(osh);
@@ -171,7 +172,8 @@ vp_status vp_wf_prepare(vp_context* ctx, const RenderLaunch& L, hipStream_t s) {
     vp_status st = vp_ensure_wf(ctx, L.pool, L.active_cap);
     if (st != VP_OK) return st;
     ctx->wf.job_done = ctx->job_done_dev;
-    const uint64_t key[6] = {L.pool, L.active_cap, (uint64_t(L.wf.grid_regen) << 32) | L.wf.grid_extend,
+    const uint64_t key[6] = {(uint64_t(L.wf.fetch_min) << 32) | L.pool, L.active_cap,
+                             (uint64_t(L.wf.grid_regen) << 32) | L.wf.grid_extend,
                              (uint64_t(L.wf.grid_scatter) << 32) | L.wf.grid_shadow,
                              reinterpret_cast<uint64_t>(ctx->job_done_dev), reinterpret_cast<uint64_t>(s)};
     if (ctx->wf_graph && std::equal(key, key + 6, ctx->wf_graph_key)) return VP_OK;

@@ -25,6 +25,31 @@ namespace vp {
 // ---------------------------------------------------------------------------
 constexpr uint32_t kInnerNode = 0xffffffffu;
 
+// Conservative leaf box of record r (input scale s, centre c): the 3-sigma
+// ellipsoid with forward rotation R and scale s has half extent
+// 3 sqrt(sum_j (R_ij s_j)^2) along world axis i; the box is widened by a
+// relative 2^-10 and an absolute 2^-20 (|c| + h) so every point the FP32
+// entry/containment tests accept lies inside it.  Evaluated in double.
+VP_HD void conservative_box(const PrimRecord& r, const float s[3], float lo[3], float hi[3]) {
+    // forward rotation = conjugate of the stored one
+    const double qw = r.qw, qx = -(double)r.qx, qy = -(double)r.qy, qz = -(double)r.qz;
+    const double R[3][3] = {
+        {1.0 - 2.0 * (qy * qy + qz * qz), 2.0 * (qx * qy - qw * qz), 2.0 * (qx * qz + qw * qy)},
+        {2.0 * (qx * qy + qw * qz), 1.0 - 2.0 * (qx * qx + qz * qz), 2.0 * (qy * qz - qw * qx)},
+        {2.0 * (qx * qz - qw * qy), 2.0 * (qy * qz + qw * qx), 1.0 - 2.0 * (qx * qx + qy * qy)},
+    };
+    const double sv[3] = {s[0], s[1], s[2]};
+    const double cv[3] = {r.cx, r.cy, r.cz};
+    for (int a = 0; a < 3; ++a) {
+        double h2 = 0.0;
+        for (int j = 0; j < 3; ++j) h2 += (R[a][j] * sv[j]) * (R[a][j] * sv[j]);
+        double h = 3.0 * sqrt(h2);
+        h = h * (1.0 + 0x1p-10) + 0x1p-20 * (fabs(cv[a]) + h);
+        lo[a] = (float)(cv[a] - h);
+        hi[a] = (float)(cv[a] + h);
+    }
+}
+
 struct SceneView {
     const PrimRecord* prims;
     const float4* nodes;
@@ -79,11 +104,12 @@ struct WfPool {
     unsigned long long* claim_base;  // [2] first path number claimed per parity
     unsigned long long* finished;    // paths retired (or padding) so far
     unsigned long long* job_done;    // per job-table entry: paths retired so far
-    uint32_t* slab;      // [2][cap][P] active sets
+    uint32_t* slab;      // [P][2][cap] active sets, each slot's two buffers contiguous
     uint32_t P, cap;
 };
 struct WfLaunch {
     uint32_t grid_regen, grid_extend, grid_scatter, grid_shadow;
+    uint32_t fetch_min;  // idle lanes a wavefront waits for before taking new walks
 };
 
 // Counter slots of the render kernel.
@@ -191,6 +217,8 @@ void vp_set_error(const std::string& msg);
 
 // launchers (vp_bvh.hip)
 vp_status vp_build_scene_device(vp_context* ctx, const vp_primitive* host_prims, uint32_t n, vp_scene* scene);
+// host SAH tree replacing the device LBVH's nodes, records and order (vp_bvh_sah.cpp)
+vp_status vp_build_scene_sah(vp_context* ctx, const vp_primitive* host_prims, uint32_t n, vp_scene* scene);
 
 // launchers (vp_render.hip)
 struct RenderLaunch {

@@ -47,14 +47,17 @@ template <bool kCount>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VP_RENDER_WAVES_PER_EU)))
 k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begin,
          const uint64_t* __restrict__ launch_desc, unsigned long long* __restrict__ path_counter,
-         uint32_t* __restrict__ slab, uint32_t cap, uint32_t stride, uint32_t regen_min,
+         uint32_t* __restrict__ slab, uint32_t cap, uint32_t regen_min,
          unsigned long long* __restrict__ counters, int measure) {
     extern __shared__ uint32_t s_leafq[];  // [kLeafQ][blockDim.x]
     const uint32_t njobs = static_cast<uint32_t>(launch_desc[0]);
     const uint64_t total_paths = launch_desc[1];
     const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
-    uint32_t* my = slab + gl;  // entry k of buffer b at my[(b * cap + k) * stride]
+    // the lane's two active-set buffers, contiguous (lanes read and append
+    // to their own sets at different times, so each set stays in few lines)
+    constexpr uint32_t stride = 1;
+    uint32_t* my = slab + static_cast<size_t>(gl) * 2 * cap;  // entry k of buffer b at my[b * cap + k]
     const LeafQueue lq{s_leafq + threadIdx.x, blockDim.x};
     PathState st;
     bool alive = false;
@@ -510,15 +513,14 @@ using namespace vp;
 hipError_t vp_launch_step(vp_context* ctx, const RenderLaunch& L, uint32_t max_jobs, uint32_t max_npix,
                           hipStream_t s) {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctx->path_counter, 1u, 0ull);
-    const uint32_t stride = L.grid * L.block;
     const size_t lds = sizeof(uint32_t) * kLeafQ * L.block;
     if (L.count_events)
         hipLaunchKernelGGL(k_render<true>, dim3(L.grid), dim3(L.block), lds, s, ctx->jobs_dev, ctx->job_begin_dev,
-                           ctx->launch_desc, ctx->path_counter, ctx->slab, L.active_cap, stride, L.regen_min,
+                           ctx->launch_desc, ctx->path_counter, ctx->slab, L.active_cap, L.regen_min,
                            ctx->counters, L.measure);
     else
         hipLaunchKernelGGL(k_render<false>, dim3(L.grid), dim3(L.block), lds, s, ctx->jobs_dev, ctx->job_begin_dev,
-                           ctx->launch_desc, ctx->path_counter, ctx->slab, L.active_cap, stride, L.regen_min,
+                           ctx->launch_desc, ctx->path_counter, ctx->slab, L.active_cap, L.regen_min,
                            ctx->counters, L.measure);
     return vp_launch_welford(ctx, max_jobs, max_npix, s);
 }

@@ -176,12 +176,12 @@ __global__ void VP_WF_LB k_wf_regen(const JobDesc* __restrict__ jobs, const uint
             if (path_begin(J, p - J.path_begin, o, d, rng, key, slot)) {
                 // origin-inside set (sampling.cuh:322-341) by point query, into buffer 0
                 uint32_t na = 0, visits = 0;
-                uint32_t* act = w.slab + i;
+                uint32_t* act = w.slab + static_cast<size_t>(i) * 2 * w.cap;
                 const SceneView sc = J.scene;
                 walk_point(sc, o, visits, [&](uint32_t prim) {
                     if (point_inside(load_prim(sc.prims, prim), o)) {
                         const bool fits = na < w.cap;
-                        if (fits) act[static_cast<size_t>(na) * P] = prim;
+                        if (fits) act[na] = prim;
                         na += fits ? 1u : 0u;
                         overflow += fits ? 0u : 1u;
                     }
@@ -211,9 +211,11 @@ __global__ void VP_WF_LB k_wf_regen(const JobDesc* __restrict__ jobs, const uint
 // next queue entry (one atomic per wavefront), so lanes whose walk ended
 // start another one instead of idling until the wavefront's longest walk is
 // done.  Returns the entry for this lane or n (none).
-__device__ __forceinline__ uint32_t wave_fetch(unsigned int* head, bool want, uint32_t n) {
+__device__ __forceinline__ uint32_t wave_fetch(unsigned int* head, bool want, uint32_t n, uint32_t fetch_min) {
     const unsigned long long m = __ballot(want);
     if (m == 0) return n;
+    // fetch once fetch_min lanes are idle, or when none is busy
+    if (static_cast<uint32_t>(__popcll(m)) < fetch_min && __ballot(!want) != 0) return n;
     const uint32_t lane = __lane_id();
     const uint32_t leader = __ffsll(static_cast<long long>(m)) - 1;
     uint32_t base = n;
@@ -233,7 +235,7 @@ __device__ __forceinline__ uint32_t wave_fetch(unsigned int* head, bool want, ui
 // Argmin free flight (sampling.cuh:350-415) for every live path: origin-set
 // candidates first, then one pruned walk over entered primitives with the
 // leaf queue; lanes refill from the live queue as their walks end.
-__global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par) {
+__global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par, uint32_t fetch_min) {
     extern __shared__ uint32_t s_leafq[];
     const LeafQueue lq{s_leafq + threadIdx.x, blockDim.x};
     unsigned int* qc = qcs(w, par);
@@ -246,7 +248,7 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
     float ts = kTMax;
     SceneView sc{};
     while (true) {
-        const uint32_t idx = wave_fetch(&qc[4], !have, n);
+        const uint32_t idx = wave_fetch(&qc[4], !have, n, fetch_min);
         if (!have && idx < n) {
             i = w.live[idx];
             const float4 ob = w.ob[i], dj = w.dj[i];
@@ -257,11 +259,11 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
             const uint4 m = w.misc[i];
             key = static_cast<uint64_t>(m.x) | (static_cast<uint64_t>(m.y) << 32);
             const uint32_t na = __float_as_uint(w.lr[i].w);
-            const uint32_t* act = w.slab + static_cast<size_t>(f_cur(m.w)) * w.cap * P + i;
+            const uint32_t* act = w.slab + (static_cast<size_t>(i) * 2 + f_cur(m.w)) * w.cap;
             // argmin over the origin set (sampling.cuh:356-370)
             ts = kTMax;
             for (uint32_t k = 0; k < na; ++k) {
-                const uint32_t pi = act[static_cast<size_t>(k) * P];
+                const uint32_t pi = act[k];
                 const PrimRecord pr = load_prim(sc.prims, pi);
                 const float te = exit_from_inside(pr, o, d);
                 const float tau = free_flight_tau(prim_uniform(key, b, 0u, pi));
@@ -329,7 +331,6 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
     const LeafQueue lq{s_leafq + threadIdx.x, blockDim.x};
     const uint32_t n = qcs(w, par)[0];
     const uint32_t P = w.P;
-    const size_t plane = static_cast<size_t>(w.cap) * P;
     uint32_t overflow = 0;
     // every lane of a wavefront runs the same trip count so the queue
     // appends below stay wave-uniform
@@ -368,26 +369,26 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
                 finish = true;
             } else {
                 // rebuild the active set at the scatter point (sampling.cuh:422-458)
-                const uint32_t* act = w.slab + cur * plane + i;
-                uint32_t* nxt = w.slab + (cur ^ 1u) * plane + i;
+                const uint32_t* act = w.slab + (static_cast<size_t>(i) * 2 + cur) * w.cap;
+                uint32_t* nxt = w.slab + (static_cast<size_t>(i) * 2 + (cur ^ 1u)) * w.cap;
                 const f3 x = o + ts * d;
                 uint32_t cnt = 0, ia = 0, visits = 0;
                 auto push = [&](uint32_t v) {
                     const bool fits = cnt < w.cap;
-                    if (fits) nxt[static_cast<size_t>(cnt) * P] = v;
+                    if (fits) nxt[cnt] = v;
                     cnt += fits ? 1u : 0u;
                     overflow += fits ? 0u : 1u;
                 };
                 walk_point_q(sc, x, visits, lq, [&](uint32_t c) {
                     while (ia < na) {
-                        const uint32_t a = act[static_cast<size_t>(ia) * P];
+                        const uint32_t a = act[ia];
                         if (a >= c) break;
                         if (ts <= exit_from_inside(load_prim(sc.prims, a), o, d)) push(a);
                         ++ia;
                     }
                     const PrimRecord pr = load_prim(sc.prims, c);
                     bool keep = false;
-                    if (ia < na && act[static_cast<size_t>(ia) * P] == c) {
+                    if (ia < na && act[ia] == c) {
                         keep = ts <= exit_from_inside(pr, o, d);
                         ++ia;
                     }
@@ -399,7 +400,7 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
                     if (keep) push(c);
                 });
                 while (ia < na) {
-                    const uint32_t a = act[static_cast<size_t>(ia) * P];
+                    const uint32_t a = act[ia];
                     if (ts <= exit_from_inside(load_prim(sc.prims, a), o, d)) push(a);
                     ++ia;
                 }
@@ -414,7 +415,7 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
                 f3 aa = mk(0.0f);
                 float aw = 0.0f;
                 for (uint32_t k = 0; k < nn; ++k) {
-                    const PrimRecord pr = load_prim(sc.prims, nact[static_cast<size_t>(k) * P]);
+                    const PrimRecord pr = load_prim(sc.prims, nact[k]);
                     const float wgt = pr.mass * prim_pdf(pr, x);
                     aa = aa + f3{pr.ar, pr.ag, pr.ab} * wgt;
                     aw += wgt;
@@ -503,7 +504,7 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
 // Fixed-point transmittance (sampling.cuh:548-583, anyhit.cuh:53-66) of every
 // queued shadow ray: origin-set chords first, then one walk over entered
 // primitives; lanes refill from the shadow queue as their rays end.
-__global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par) {
+__global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par, uint32_t fetch_min) {
     extern __shared__ uint32_t s_leafq[];
     const LeafQueue lq{s_leafq + threadIdx.x, blockDim.x};
     unsigned int* qc = qcs(w, par);
@@ -515,7 +516,7 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
     f3 o = mk(0.0f), d = mk(0.0f), inv = mk(0.0f), oi = mk(0.0f);
     SceneView sc{};
     while (true) {
-        const uint32_t idx = wave_fetch(&qc[5], !have, n);
+        const uint32_t idx = wave_fetch(&qc[5], !have, n, fetch_min);
         if (!have && idx < n) {
             const uint32_t ent = w.shadow[idx];
             const uint32_t i = ent >> 2, k = ent & 3u;
@@ -526,11 +527,11 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
             const uint32_t bits = __float_as_uint(po.w);
             const uint32_t set = bits & 1u, count = bits >> 1;
             sc = jobs[__float_as_uint(w.dj[i].w)].scene;
-            const uint32_t* act = w.slab + static_cast<size_t>(set) * w.cap * P + i;
+            const uint32_t* act = w.slab + (static_cast<size_t>(i) * 2 + set) * w.cap;
             acc = 0;
             full = false;
             for (uint32_t c = 0; c < count && !full; ++c) {
-                const PrimRecord pr = load_prim(sc.prims, act[static_cast<size_t>(c) * P]);
+                const PrimRecord pr = load_prim(sc.prims, act[c]);
                 const float te = exit_from_inside(pr, o, d);
                 if (te > 0.0f && te < kTMax) full = tau_add(acc, optical_depth(pr, o, d, 0.0f, te));
             }
@@ -590,10 +591,10 @@ hipError_t vp_wf_launch_iterations(vp_context* ctx, const WfPool& w, const WfLau
         hipLaunchKernelGGL(k_wf_claim, dim3(1), dim3(64), 0, s, ctx->launch_desc, ctx->path_counter, w, par);
         hipLaunchKernelGGL(k_wf_regen, dim3(L.grid_regen), dim3(256), 0, s, ctx->jobs_dev, ctx->job_begin_dev,
                            ctx->launch_desc, w, par, ctx->counters);
-        hipLaunchKernelGGL(k_wf_extend, dim3(L.grid_extend), dim3(256), lds, s, ctx->jobs_dev, w, par);
+        hipLaunchKernelGGL(k_wf_extend, dim3(L.grid_extend), dim3(256), lds, s, ctx->jobs_dev, w, par, L.fetch_min);
         hipLaunchKernelGGL(k_wf_scatter, dim3(L.grid_scatter), dim3(256), lds, s, ctx->jobs_dev, w, par,
                            ctx->counters);
-        hipLaunchKernelGGL(k_wf_shadow, dim3(L.grid_shadow), dim3(256), lds, s, ctx->jobs_dev, w, par);
+        hipLaunchKernelGGL(k_wf_shadow, dim3(L.grid_shadow), dim3(256), lds, s, ctx->jobs_dev, w, par, L.fetch_min);
     }
     return hipGetLastError();
 }

@@ -98,7 +98,9 @@ EXPORTED = [
     "vp_render_device", "vp_normalize_accumulator", "vp_render_jobs", "vp_autotune", "vp_server_create",
     "vp_server_submit", "vp_server_poll", "vp_server_drain", "vp_server_stats_get", "vp_server_destroy",
     "vp_transmittance_rays", "vp_collect_overlaps", "vp_render_count_events", "vp_math_eval", "vp_prim_eval",
+    "vp_scene_create_ex",
 ]
+SCENE_BVH_SAH = 1
 EVENT_NAMES = ["active_overflow", "max_hits", "max_overlap", "paths", "traces", "scatters", "shadow_rays",
                "arg_hit", "arg_active", "arg_accept", "rb_active", "rb_hit", "alb", "sh_active", "sh_hit",
                "leaf_tests", "node_visits", "contain", "visits_shadow", "visits_point"]
@@ -122,6 +124,7 @@ def load():
     lib.vp_device_info.argtypes = [_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint32),
                                    ctypes.POINTER(ctypes.c_double)]
     lib.vp_scene_create.argtypes = [_vp, ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(_vp)]
+    lib.vp_scene_create_ex.argtypes = [_vp, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_vp)]
     lib.vp_scene_destroy.argtypes = [_vp]
     lib.vp_scene_export.argtypes = [_vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_uint32)]
     lib.vp_env_create.argtypes = [_vp, _f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_vp)]
@@ -200,14 +203,15 @@ def camera_orthographic(w, h, eye, target, up, ortho_h):
 
 
 class Scene:
-    def __init__(self, ctx, rows14):
+    def __init__(self, ctx, rows14, bvh="lbvh"):
         self.ctx = ctx
         self.rows = f32(rows14).reshape(-1, 14)
         self.n = self.rows.shape[0]
         prims = primitives_from_rows(self.rows)
         h = _vp()
-        _check(load().vp_scene_create(ctx.handle, ctypes.cast(prims, _vp), self.n, ctypes.byref(h)),
-               "vp_scene_create")
+        flags = {"lbvh": 0, "sah": SCENE_BVH_SAH}[bvh]
+        _check(load().vp_scene_create_ex(ctx.handle, ctypes.cast(prims, _vp), self.n, flags, ctypes.byref(h)),
+               "vp_scene_create_ex")
         self.handle = h
 
     def export(self):
@@ -276,8 +280,9 @@ class Context:
         _check(load().vp_device_info(self.handle, buf, 256, ctypes.byref(cus), ctypes.byref(mhz)), "vp_device_info")
         return buf.value.decode(), cus.value, mhz.value
 
-    def scene(self, rows14):
-        return Scene(self, rows14)
+    def scene(self, rows14, bvh="lbvh"):
+        """bvh: "lbvh" (GPU build, default) or "sah" (host SAH tree, same layout)."""
+        return Scene(self, rows14, bvh)
 
     def env(self, rgba):
         return Env(self, rgba)

@@ -120,6 +120,13 @@ vp_status vp_device_info(vp_context* ctx, char* name, size_t name_len, uint32_t*
 
 /* ---- scene / environment / camera -------------------------------------- */
 vp_status vp_scene_create(vp_context* ctx, const vp_primitive* prims, uint32_t n, vp_scene** out);
+/* vp_scene_create with build flags (no counterpart: the reference's OptiX
+ * builder has one mode).  VP_SCENE_BVH_SAH replaces the GPU LBVH's tree with
+ * a binned surface-area-heuristic tree built on the host, in the same
+ * layout; renders of the two differ only through the stored primitive order. */
+#define VP_SCENE_BVH_SAH 1u
+vp_status vp_scene_create_ex(vp_context* ctx, const vp_primitive* prims, uint32_t n, uint32_t flags,
+                             vp_scene** out);
 vp_status vp_scene_destroy(vp_scene* scene);
 /* Order the scene stores primitives in (order[k] = input index of the k-th),
  * their 30-bit Morton codes (input order) and the 64-byte device records

@@ -241,3 +241,55 @@ def test_gpu_transmittance_vs_fp64_golden(ctx, case):
     sc.close()
     mask = Tg > 1e-30
     assert (np.abs(T - Tg) / Tg)[mask].max() < 1e-5
+
+
+# ---- the optional host SAH tree: same layout, same results for its stored order ----
+@pytest.mark.parametrize("n,seed", [(1, 0), (2, 1), (300, 3), (4096, 4)])
+def test_sah_scene_is_a_permutation_with_exact_records(ctx, n, seed):
+    rows = scenes.c1_scattering() if n == 4096 else scenes.random_scene(n, seed)
+    print(f"[case] SAH scene build n={n}", flush=True)
+    sc = ctx.scene(rows, bvh="sah")
+    order, morton, rec, depth = sc.export()
+    sc.close()
+    assert np.array_equal(np.sort(order), np.arange(n, dtype=np.uint32))
+    assert np.array_equal(morton, vpo.morton_codes(rows[:, 0:3]))
+    assert_bitwise(rec, vpo.make_prims(rows)[order], "primitive records")
+
+
+def test_sah_overlaps_and_transmittance_bitexact(ctx):
+    rng = np.random.default_rng(77)
+    rows = scenes.random_scene(400, 12)
+    sc = ctx.scene(rows, bvh="sah")
+    _, _, rec, _ = sc.export()
+    rays = _random_rays(rng, 3000, 1.5)
+    print("[case] SAH overlaps + transmittance n=400", flush=True)
+    cg, tg, ig = ctx.collect_overlaps(sc, rays, 512)
+    T = ctx.transmittance_rays(sc, rays)
+    sc.close()
+    co, to, io = vpo.overlaps(rec, rays, 512)
+    assert np.array_equal(cg, co)
+    for r in range(rays.shape[0]):
+        k = int(cg[r])
+        assert np.array_equal(ig[r, :k], io[r, :k]) and np.array_equal(tg[r, :k], to[r, :k])
+    assert_bitwise(T, vpo.transmittance_rays(rec, rays), "transmittance")
+
+
+@pytest.mark.parametrize("case", ["mis_sky", "small_cap", "ris_sky"])
+def test_render_sah_bitexact(ctx, case):
+    c = RENDER_CASES[case]
+    env = scenes.constant_env() if c["env"] == "const" else scenes.sky_env(64, 32, sun_radius_deg=4.0, sun_scale=50.0)
+    cam = volprim.camera_perspective(**scenes.default_camera_args(c["w"], c["h"]))
+    print(f"[case] SAH render {case}", flush=True)
+    sc = ctx.scene(c["rows"](), bvh="sah")
+    ev = ctx.env(env)
+    _, _, rec, _ = sc.export()
+    p = volprim.default_params(width=c["w"], height=c["h"], spp=c["spp"], **c["kw"])
+    img, st = ctx.render(sc, ev, cam, p)
+    sc.close()
+    ev.close()
+    ref, _, k = vpo.render(rec, env, cam.pod(), p.width, p.height, p.spp, seed=p.seed, max_depth=p.max_depth,
+                           rr_depth=p.rr_depth, rr_max_survival=p.rr_max_survival, firefly_clamp=p.firefly_clamp,
+                           filter_stddev=p.filter_stddev, phase_g=p.phase_g, use_ris=bool(p.use_ris),
+                           ris_candidates=p.ris_candidates, active_cap=p.active_cap)
+    assert_bitwise(img, ref, f"SAH render {case}")
+    assert st["active_overflow"] == k["active_overflow"]
