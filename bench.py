@@ -90,6 +90,52 @@ def cpu_baseline(rows, env, cam_pod, params, threads, budget_s=20.0, repeats=5):
                       f"repeat, 1 warm-up + {repeats} timed repeats (median); CPU BVH (binary, Morton-ordered) queries"}
 
 
+def measure_traffic(args):
+    """HBM bytes of one k_render launch from two rocprofv3 --pmc runs of this
+    script (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by
+    what the same counter reads for an elementwise kernel that reads and
+    writes 256 MiB.  Returns (dict, note); dict is None when a run fails."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None, "rocprofv3 not found"
+    child = [sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu", "--no-count",
+             "--no-quality", "--no-traffic", "--copy-calibration", "--bvh", args.bvh]
+    if args.block or args.bpc or args.regen:
+        child += ["--block", str(args.block), "--bpc", str(args.bpc), "--regen", str(args.regen)]
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="vp_pmc_", dir=os.environ.get("TMPDIR", "/tmp"))
+        try:
+            subprocess.run([prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--"] + child,
+                           stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=180)
+        except subprocess.TimeoutExpired:
+            return None, f"rocprofv3 --pmc {counter} timed out"
+        path = os.path.join(d, "run_counter_collection.csv")
+        if not os.path.exists(path):
+            return None, f"rocprofv3 --pmc {counter} wrote no counters"
+        per = {}
+        names = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            per[r["Dispatch_Id"]] = per.get(r["Dispatch_Id"], 0.0) + float(r["Counter_Value"])
+            names[r["Dispatch_Id"]] = r["Kernel_Name"]
+        shutil.rmtree(d, ignore_errors=True)
+        render = [v for k, v in per.items() if "k_render" in names[k]]
+        calib = [v for k, v in per.items() if "elementwise" in names[k]]
+        if not render or not calib:
+            return None, f"{counter}: kernels not found in the counter dump"
+        scale = (256 * 1024) / calib[-1]
+        out[counter] = (render[-1], calib[-1], scale)
+    f, w = out["FETCH_SIZE"], out["WRITE_SIZE"]
+    total = (f[0] * f[2] + w[0] * w[2]) * 1024.0
+    return {"bytes": total, "fetch_kib": f[0], "write_kib": w[0], "fetch_scale": f[2], "write_scale": w[2]}, None
+
+
 MIX_ACTIVE_CAP = {"scattering": 128, "cloud": 384, "torus": 384}
 
 
@@ -217,6 +263,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
     ap.add_argument("--no-quality", action="store_true", help="skip the image-error estimate")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs for HBM traffic")
     ap.add_argument("--rmse-ref-spp", type=int, default=0, help="also measure relative RMSE against a render "
                                                                    "with this many spp (untimed)")
     ap.add_argument("--block", type=int, default=0, help="threads per workgroup (0 = library default)")
@@ -296,8 +343,18 @@ def main():
         avg_launch_s = kernel_ms / max(launches, 1) / 1e3
         flops_per_launch = flops * args.steps / max(launches, 1)
         achieved = flops_per_launch / avg_launch_s / 1e12
+        traffic, traffic_note = None, "not measured (--no-traffic)"
+        if not args.no_traffic and args.pipeline == "mega":
+            tr, traffic_note = measure_traffic(args)
+            if tr is not None:
+                traffic = int(tr["bytes"])
+                traffic_note = (f"rocprofv3 --pmc: FETCH_SIZE {tr['fetch_kib']:.0f} KiB x {tr['fetch_scale']:.3f}, "
+                                f"WRITE_SIZE {tr['write_kib']:.0f} KiB x {tr['write_scale']:.3f} (scales from a "
+                                f"256 MiB read + 256 MiB write elementwise kernel)")
+        elif args.pipeline != "mega":
+            traffic_note = "measured for the megakernel only"
         roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                    "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic, "traffic_note": traffic_note,
                     "peak_nonpacked": PEAK_FP32_NONPACKED_TFLOPS,
                     "kernel": "k_render" if args.pipeline == "mega" else "k_wf_* (one render step)",
                     "avg_launch_ms": round(avg_launch_s * 1e3, 3), "flops_per_launch": flops_per_launch}
