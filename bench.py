@@ -4,7 +4,8 @@ MI355X, on the scattering workload (4096 anisotropic Gaussians, 512x512,
 phase, 2048x1024 procedural sky environment with a 0.5 degree sun).
 
 A step is one full render of that image (512*512*64 = 16.8 M camera paths)
-with the scene, BVH and environment already resident in HBM.  Prints one
+with the scene, BVH (the SAH tree unless --bvh lbvh) and environment already
+resident in HBM.  Prints one
 JSON line with the throughput, the roofline of the render kernel (FP32 VALU:
 algorithmic FLOPs from the event-counting kernel variant weighted as
 documented in DESIGN.md, over the kernel time measured with HIP events on
@@ -272,8 +273,8 @@ def main():
     ap.add_argument("--pipeline", choices=["mega", "wave"], default="mega",
                     help="persistent megakernel or wavefront-queue pipeline")
     ap.add_argument("--pool", type=int, default=0, help="wavefront path slots (0 = library default)")
-    ap.add_argument("--bvh", choices=["lbvh", "sah"], default="lbvh",
-                    help="GPU LBVH (default) or the host SAH tree in the same layout")
+    ap.add_argument("--bvh", choices=["lbvh", "sah"], default="sah",
+                    help="host SAH tree (default, fewer node visits) or the GPU LBVH, same layout")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB elementwise copy kernel first (HBM counter calibration)")
     ap.add_argument("--mix", action="store_true", help="run the job mix (plain vs batched vs served) instead")
