@@ -207,6 +207,55 @@ __global__ void VP_WF_LB k_wf_regen(const JobDesc* __restrict__ jobs, const uint
     if (overflow) atomicAdd(&counters[kCntActiveOverflow], static_cast<unsigned long long>(overflow));
 }
 
+// Node steps of one lane's walk over [0, tmax] until its leaf queue is full
+// or the walk ends.  With VP_WF_PREFETCH the loads of both possible next
+// nodes (node + 1 and the skip target) are issued before the current node is
+// tested, so their latency overlaps the test.
+#ifndef VP_WF_PREFETCH
+#define VP_WF_PREFETCH 0
+#endif
+__device__ __forceinline__ void walk_steps(const SceneView& sc, bool active, uint32_t& node, uint32_t nn, f3 inv,
+                                           f3 oi, float tmax, const LeafQueue& lq, uint32_t& cnt) {
+#if VP_WF_PREFETCH
+    if (!(active && node < nn)) return;
+    float4 a = as_global(sc.nodes)[2 * node];
+    float4 c = as_global(sc.nodes)[2 * node + 1];
+    while (node < nn && cnt < kLeafQ) {
+        const uint32_t skip = __float_as_uint(a.w), prim = __float_as_uint(c.w);
+        // both candidates, clamped to a valid node when the walk would end
+        const uint32_t n1 = node + 1 < nn ? node + 1 : node;
+        const uint32_t n2 = skip < nn ? skip : node;
+        const float4 a1 = as_global(sc.nodes)[2 * n1], c1 = as_global(sc.nodes)[2 * n1 + 1];
+        const float4 a2 = as_global(sc.nodes)[2 * n2], c2 = as_global(sc.nodes)[2 * n2 + 1];
+        const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(c.x, inv.x, -oi.x);
+        const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(c.y, inv.y, -oi.y);
+        const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(c.z, inv.z, -oi.z);
+        const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+        const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+        const bool hit = tnear <= tfar, leaf_node = prim != kInnerNode;
+        if (hit && leaf_node) lq.at(cnt++) = prim;
+        const bool down = hit && !leaf_node;
+        node = down ? node + 1 : skip;
+        a = down ? a1 : a2;
+        c = down ? c1 : c2;
+    }
+#else
+    while (active && node < nn && cnt < kLeafQ) {
+        const float4 a = as_global(sc.nodes)[2 * node];
+        const float4 c = as_global(sc.nodes)[2 * node + 1];
+        const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(c.x, inv.x, -oi.x);
+        const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(c.y, inv.y, -oi.y);
+        const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(c.z, inv.z, -oi.z);
+        const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+        const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+        const uint32_t skip = __float_as_uint(a.w), prim = __float_as_uint(c.w);
+        const bool hit = tnear <= tfar, leaf_node = prim != kInnerNode;
+        if (hit && leaf_node) lq.at(cnt++) = prim;
+        node = (hit && !leaf_node) ? node + 1 : skip;
+    }
+#endif
+}
+
 // Work fetch for the walk kernels: every idle lane of a wavefront takes the
 // next queue entry (one atomic per wavefront), so lanes whose walk ended
 // start another one instead of idling until the wavefront's longest walk is
@@ -279,19 +328,7 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
         if (__ballot(have) == 0) break;
         // node steps until the lane's leaf queue is full or its walk ends
         uint32_t cnt = 0;
-        while (have && node < nn && cnt < kLeafQ) {
-            const float4 a = as_global(sc.nodes)[2 * node];
-            const float4 c = as_global(sc.nodes)[2 * node + 1];
-            const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(c.x, inv.x, -oi.x);
-            const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(c.y, inv.y, -oi.y);
-            const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(c.z, inv.z, -oi.z);
-            const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-            const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), ts));
-            const uint32_t skip = __float_as_uint(a.w), prim = __float_as_uint(c.w);
-            const bool hit = tnear <= tfar, leaf_node = prim != kInnerNode;
-            if (hit && leaf_node) lq.at(cnt++) = prim;
-            node = (hit && !leaf_node) ? node + 1 : skip;
-        }
+        walk_steps(sc, have, node, nn, inv, oi, ts, lq, cnt);
         // entered primitives (sampling.cuh:372-412)
         for (uint32_t k = 0; k < cnt; ++k) {
             const uint32_t prim = lq.at(k);
@@ -543,19 +580,7 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
         }
         if (__ballot(have) == 0) break;
         uint32_t cnt = 0;
-        while (have && !full && node < nn && cnt < kLeafQ) {
-            const float4 a = as_global(sc.nodes)[2 * node];
-            const float4 c = as_global(sc.nodes)[2 * node + 1];
-            const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(c.x, inv.x, -oi.x);
-            const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(c.y, inv.y, -oi.y);
-            const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(c.z, inv.z, -oi.z);
-            const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-            const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), kTMax));
-            const uint32_t skip = __float_as_uint(a.w), prim = __float_as_uint(c.w);
-            const bool hit = tnear <= tfar, leaf_node = prim != kInnerNode;
-            if (hit && leaf_node) lq.at(cnt++) = prim;
-            node = (hit && !leaf_node) ? node + 1 : skip;
-        }
+        walk_steps(sc, have && !full, node, nn, inv, oi, kTMax, lq, cnt);
         for (uint32_t k = 0; k < cnt && !full; ++k) {
             const PrimRecord pr = load_prim(sc.prims, lq.at(k));
             float wl2;
