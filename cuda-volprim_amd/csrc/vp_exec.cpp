@@ -158,7 +158,7 @@ RenderLaunch vp_resolve_launch(vp_context* ctx, const vp_launch_config* cfg, uin
         vp_wf_occupancy(2, &osh);
         vp_wf_occupancy(3, &org);
         L.wf.grid_regen = cus * static_cast<uint32_t>(org);
-        L.wf.fetch_min = c.regen_min ? L.regen_min : 32u;
+        L.wf.fetch_min = c.regen_min ? L.regen_min : 48u;
         L.wf.grid_extend = cus * static_cast<uint32_t>(oe);
         L.wf.grid_scatter = cus * static_cast<uint32_t>(os);
         L.wf.grid_shadow = cus * static_cast<uint32_t>(osh);

@@ -208,38 +208,9 @@ __global__ void VP_WF_LB k_wf_regen(const JobDesc* __restrict__ jobs, const uint
 }
 
 // Node steps of one lane's walk over [0, tmax] until its leaf queue is full
-// or the walk ends.  With VP_WF_PREFETCH the loads of both possible next
-// nodes (node + 1 and the skip target) are issued before the current node is
-// tested, so their latency overlaps the test.
-#ifndef VP_WF_PREFETCH
-#define VP_WF_PREFETCH 0
-#endif
+// or the walk ends.
 __device__ __forceinline__ void walk_steps(const SceneView& sc, bool active, uint32_t& node, uint32_t nn, f3 inv,
                                            f3 oi, float tmax, const LeafQueue& lq, uint32_t& cnt) {
-#if VP_WF_PREFETCH
-    if (!(active && node < nn)) return;
-    float4 a = as_global(sc.nodes)[2 * node];
-    float4 c = as_global(sc.nodes)[2 * node + 1];
-    while (node < nn && cnt < kLeafQ) {
-        const uint32_t skip = __float_as_uint(a.w), prim = __float_as_uint(c.w);
-        // both candidates, clamped to a valid node when the walk would end
-        const uint32_t n1 = node + 1 < nn ? node + 1 : node;
-        const uint32_t n2 = skip < nn ? skip : node;
-        const float4 a1 = as_global(sc.nodes)[2 * n1], c1 = as_global(sc.nodes)[2 * n1 + 1];
-        const float4 a2 = as_global(sc.nodes)[2 * n2], c2 = as_global(sc.nodes)[2 * n2 + 1];
-        const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(c.x, inv.x, -oi.x);
-        const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(c.y, inv.y, -oi.y);
-        const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(c.z, inv.z, -oi.z);
-        const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-        const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-        const bool hit = tnear <= tfar, leaf_node = prim != kInnerNode;
-        if (hit && leaf_node) lq.at(cnt++) = prim;
-        const bool down = hit && !leaf_node;
-        node = down ? node + 1 : skip;
-        a = down ? a1 : a2;
-        c = down ? c1 : c2;
-    }
-#else
     while (active && node < nn && cnt < kLeafQ) {
         const float4 a = as_global(sc.nodes)[2 * node];
         const float4 c = as_global(sc.nodes)[2 * node + 1];
@@ -253,7 +224,6 @@ __device__ __forceinline__ void walk_steps(const SceneView& sc, bool active, uin
         if (hit && leaf_node) lq.at(cnt++) = prim;
         node = (hit && !leaf_node) ? node + 1 : skip;
     }
-#endif
 }
 
 // Work fetch for the walk kernels: every idle lane of a wavefront takes the
