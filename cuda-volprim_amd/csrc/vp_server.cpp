@@ -475,7 +475,7 @@ vp_status vp_autotune(vp_context* ctx, vp_job* jobs, uint32_t njobs, vp_launch_c
     for (uint32_t b : {128u, 256u}) {
         int occ = 1;
         vp_render_occupancy(b, cap, 0, &occ);
-        for (uint32_t rg : {1u, 8u})
+        for (uint32_t rg : {1u, 8u, 16u})
             cands.push_back(vp_launch_config{b, static_cast<uint32_t>(occ), cap, 0, rg, VP_PIPELINE_MEGAKERNEL, 0});
         if (b == 256u && occ > 1)
             cands.push_back(
@@ -483,7 +483,7 @@ vp_status vp_autotune(vp_context* ctx, vp_job* jobs, uint32_t njobs, vp_launch_c
     }
     for (uint32_t pool : {1u << 20, 1u << 21})
         if (static_cast<uint64_t>(pool) * cap * 2 < (uint64_t(1) << 32))
-            cands.push_back(vp_launch_config{256u, 0u, cap, 0u, 1u, VP_PIPELINE_WAVEFRONT, pool});
+            cands.push_back(vp_launch_config{256u, 0u, cap, 0u, 48u, VP_PIPELINE_WAVEFRONT, pool});
     vp_launch_config best_cfg = cands.front();
     double best_rate = -1.0;
     for (const vp_launch_config& c : cands) {
