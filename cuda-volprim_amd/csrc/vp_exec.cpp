@@ -58,7 +58,7 @@ void vp_free_wf(vp_context* ctx) {
 // Path pool of P slots with active-set capacity cap (one allocation, carved).
 vp_status vp_ensure_wf(vp_context* ctx, uint32_t P, uint32_t cap) {
     if (ctx->wf_mem && ctx->wf.P == P && ctx->wf.cap == cap) return VP_OK;
-    if (static_cast<uint64_t>(P) * cap * 2 >= (uint64_t(1) << 32))
+    if (static_cast<uint64_t>(P) * cap * vp::kSlabBuffers >= (uint64_t(1) << 32))
         return fail(VP_ERR_INVALID, "wavefront pool: pool_paths x active_cap too large");
     vp_free_wf(ctx);
     const size_t p = P;
@@ -73,7 +73,8 @@ vp_status vp_ensure_wf(vp_context* ctx, uint32_t P, uint32_t cap) {
     const size_t o_pw = carve(48 * p), o_pv = carve(48 * p), o_po = carve(48 * p), o_pd = carve(48 * p);
     const size_t o_live = carve(4 * p), o_sh = carve(12 * p), o_free = carve(4 * p), o_qc = carve(64);
     const size_t o_cb = carve(16), o_fin = carve(8);
-    const size_t o_slab = carve(static_cast<size_t>(2) * cap * p * 4);
+    const size_t o_slab = carve(static_cast<size_t>(vp::kSlabBuffers) * cap * p * 4);
+    const size_t o_ncand = carve(4 * p);
     if (hipMalloc(&ctx->wf_mem, off) != hipSuccess) {
         ctx->wf_mem = nullptr;
         return fail(VP_ERR_OOM, "cannot allocate the wavefront path pool (" + std::to_string(off) + " bytes)");
@@ -97,6 +98,7 @@ vp_status vp_ensure_wf(vp_context* ctx, uint32_t P, uint32_t cap) {
     w.claim_base = reinterpret_cast<unsigned long long*>(b + o_cb);
     w.finished = reinterpret_cast<unsigned long long*>(b + o_fin);
     w.slab = reinterpret_cast<uint32_t*>(b + o_slab);
+    w.ncand = reinterpret_cast<uint32_t*>(b + o_ncand);
     w.P = P;
     w.cap = cap;
     VP_HIP_TRY(hipHostMalloc(&ctx->wf_done_pinned, 2 * sizeof(unsigned long long), hipHostMallocDefault));
@@ -284,7 +286,8 @@ vp_status vp_run_jobs(vp_context* ctx, std::vector<JobRun>& runs, const vp_launc
     const RenderLaunch L = vp_resolve_launch(ctx, cfg, active_cap, count, measure);
     const size_t lanes = static_cast<size_t>(L.grid) * L.block;
     vp_status st = VP_OK;
-    if (L.pipeline == VP_PIPELINE_MEGAKERNEL) st = vp_ensure_slab(ctx, 2 * static_cast<size_t>(active_cap) * lanes);
+    if (L.pipeline == VP_PIPELINE_MEGAKERNEL)
+        st = vp_ensure_slab(ctx, vp::kSlabBuffers * static_cast<size_t>(active_cap) * lanes);
     if (st != VP_OK) return st;
     st = vp_ensure_jobs(ctx, static_cast<uint32_t>(runs.size()));
     if (st != VP_OK) return st;

@@ -85,6 +85,11 @@ struct JobDesc {
     float4* mean;     // width * height Welford mean of the job
 };
 
+// Per-lane (megakernel) or per-slot (wavefront) slab: two active-set
+// buffers and one buffer of rebuild candidates, each of the active-set
+// capacity.
+constexpr uint32_t kSlabBuffers = 3;
+
 // Path pool of the wavefront pipeline (structure of arrays over P slots).
 struct WfPool {
     float4* ob;    // origin.xyz, bounce
@@ -104,7 +109,8 @@ struct WfPool {
     unsigned long long* claim_base;  // [2] first path number claimed per parity
     unsigned long long* finished;    // paths retired (or padding) so far
     unsigned long long* job_done;    // per job-table entry: paths retired so far
-    uint32_t* slab;      // [P][2][cap] active sets, each slot's two buffers contiguous
+    uint32_t* slab;      // [P][3][cap]: each slot's two active sets and its rebuild candidates
+    uint32_t* ncand;     // [P] rebuild candidates recorded by the last extend walk
     uint32_t P, cap;
 };
 struct WfLaunch {

@@ -57,7 +57,9 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
     // the lane's two active-set buffers, contiguous (lanes read and append
     // to their own sets at different times, so each set stays in few lines)
     constexpr uint32_t stride = 1;
-    uint32_t* my = slab + static_cast<size_t>(gl) * 2 * cap;  // entry k of buffer b at my[b * cap + k]
+    // buffers 0 and 1: active sets (ping-pong); buffer 2: primitives the
+    // argmin walk entered before the running minimum (rebuild candidates)
+    uint32_t* my = slab + static_cast<size_t>(gl) * kSlabBuffers * cap;  // entry k of buffer b at my[b * cap + k]
     const LeafQueue lq{s_leafq + threadIdx.x, blockDim.x};
     PathState st;
     bool alive = false;
@@ -145,7 +147,12 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
         }
         ev.add(kCntArgActive, na);
         ev.add(kCntTrace);
-        // argmin over entered primitives (sampling.cuh:372-412) in one walk
+        // argmin over entered primitives (sampling.cuh:372-412) in one walk;
+        // primitives entered before the running minimum are kept as rebuild
+        // candidates (every primitive entered before the final minimum is
+        // among them, because the walk only prunes beyond the running one)
+        uint32_t* cnd = my + 2 * cap * stride;
+        uint32_t nc = 0;
         {
             const f3 inv = safe_inv(d);
             float tmax = measure ? kTMax : ts;
@@ -157,6 +164,10 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                 const float tj = entry_t(pr, o, d, &wl2);
                 if (tj < 0.0f) return true;
                 ++hits;
+                if (tj <= ts) {
+                    if (nc < cap) cnd[nc * stride] = prim;
+                    ++nc;
+                }
                 const float tau = free_flight_tau(prim_uniform(st.key, b, 1u, prim));
                 float t = inv_cdf_span(pr, o, d, tj, tau);
                 if (t >= 0.0f && t < tj) t = tj;
@@ -197,14 +208,9 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                 nn += fits ? 1u : 0u;
                 overflow += fits ? 0u : 1u;
             };
-            walk_point_q(sc, x, visits, lq, [&](uint32_t c) {
-                while (ia < na) {
-                    const uint32_t a = act[ia * stride];
-                    if (a >= c) break;
-                    ++rb_a;
-                    if (ts <= exit_from_inside(load_prim(sc.prims, a), o, d)) push(a);
-                    ++ia;
-                }
+            // a candidate: old member kept while the vertex is inside it, else
+            // an entered primitive kept if the vertex lies on its chord
+            auto visit = [&](uint32_t c) {
                 const PrimRecord pr = load_prim(sc.prims, c);
                 bool keep = false;
                 if (ia < na && act[ia * stride] == c) {
@@ -219,7 +225,39 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                     keep = tj >= 0.0f && tj <= ts && ts <= exit_from_entry(pr, o, d, tj, wl2);
                 }
                 if (keep) push(c);
-            });
+            };
+            auto drain_below = [&](uint32_t c) {
+                while (ia < na) {
+                    const uint32_t a = act[ia * stride];
+                    if (a >= c) break;
+                    ++rb_a;
+                    if (ts <= exit_from_inside(load_prim(sc.prims, a), o, d)) push(a);
+                    ++ia;
+                }
+            };
+            if (nc <= cap && !measure) {
+                // candidates from the argmin walk, in ascending index order
+                for (uint32_t i = 1; i < nc; ++i) {
+                    const uint32_t v = cnd[i * stride];
+                    uint32_t j = i;
+                    while (j > 0 && cnd[(j - 1) * stride] > v) {
+                        cnd[j * stride] = cnd[(j - 1) * stride];
+                        --j;
+                    }
+                    cnd[j * stride] = v;
+                }
+                for (uint32_t i = 0; i < nc; ++i) {
+                    const uint32_t c = cnd[i * stride];
+                    drain_below(c);
+                    visit(c);
+                }
+            } else {
+                // more candidates than fit: one BVH point query at the vertex
+                walk_point_q(sc, x, visits, lq, [&](uint32_t c) {
+                    drain_below(c);
+                    visit(c);
+                });
+            }
             while (ia < na) {
                 const uint32_t a = act[ia * stride];
                 ++rb_a;

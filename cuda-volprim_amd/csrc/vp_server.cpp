@@ -482,7 +482,7 @@ vp_status vp_autotune(vp_context* ctx, vp_job* jobs, uint32_t njobs, vp_launch_c
                 vp_launch_config{b, static_cast<uint32_t>(occ) / 2u, cap, 0, 8u, VP_PIPELINE_MEGAKERNEL, 0});
     }
     for (uint32_t pool : {1u << 20, 1u << 21})
-        if (static_cast<uint64_t>(pool) * cap * 2 < (uint64_t(1) << 32))
+        if (static_cast<uint64_t>(pool) * cap * vp::kSlabBuffers < (uint64_t(1) << 32))
             cands.push_back(vp_launch_config{256u, 0u, cap, 0u, 48u, VP_PIPELINE_WAVEFRONT, pool});
     vp_launch_config best_cfg = cands.front();
     double best_rate = -1.0;

@@ -176,7 +176,7 @@ __global__ void VP_WF_LB k_wf_regen(const JobDesc* __restrict__ jobs, const uint
             if (path_begin(J, p - J.path_begin, o, d, rng, key, slot)) {
                 // origin-inside set (sampling.cuh:322-341) by point query, into buffer 0
                 uint32_t na = 0, visits = 0;
-                uint32_t* act = w.slab + static_cast<size_t>(i) * 2 * w.cap;
+                uint32_t* act = w.slab + static_cast<size_t>(i) * kSlabBuffers * w.cap;
                 const SceneView sc = J.scene;
                 walk_point(sc, o, visits, [&](uint32_t prim) {
                     if (point_inside(load_prim(sc.prims, prim), o)) {
@@ -261,7 +261,8 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
     const uint32_t n = qc[0];
     const uint32_t P = w.P;
     bool have = false;
-    uint32_t i = 0, b = 0, node = 0, nn = 0;
+    uint32_t i = 0, b = 0, node = 0, nn = 0, nc = 0;
+    uint32_t* cnd = nullptr;
     uint64_t key = 0;
     f3 o = mk(0.0f), d = mk(0.0f), inv = mk(0.0f), oi = mk(0.0f);
     float ts = kTMax;
@@ -278,7 +279,9 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
             const uint4 m = w.misc[i];
             key = static_cast<uint64_t>(m.x) | (static_cast<uint64_t>(m.y) << 32);
             const uint32_t na = __float_as_uint(w.lr[i].w);
-            const uint32_t* act = w.slab + (static_cast<size_t>(i) * 2 + f_cur(m.w)) * w.cap;
+            const uint32_t* act = w.slab + (static_cast<size_t>(i) * kSlabBuffers + f_cur(m.w)) * w.cap;
+            cnd = w.slab + (static_cast<size_t>(i) * kSlabBuffers + 2) * w.cap;
+            nc = 0;
             // argmin over the origin set (sampling.cuh:356-370)
             ts = kTMax;
             for (uint32_t k = 0; k < na; ++k) {
@@ -306,6 +309,11 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
             float wl2;
             const float tj = entry_t(pr, o, d, &wl2);
             if (tj < 0.0f) continue;
+            // entered before the running minimum: a rebuild candidate
+            if (tj <= ts) {
+                if (nc < w.cap) cnd[nc] = prim;
+                ++nc;
+            }
             const float tau = free_flight_tau(prim_uniform(key, b, 1u, prim));
             float t = inv_cdf_span(pr, o, d, tj, tau);
             if (t >= 0.0f && t < tj) t = tj;
@@ -316,6 +324,7 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
         }
         if (have && node >= nn) {
             w.bt[i].w = ts;
+            w.ncand[i] = nc;
             have = false;
         }
     }
@@ -376,8 +385,10 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
                 finish = true;
             } else {
                 // rebuild the active set at the scatter point (sampling.cuh:422-458)
-                const uint32_t* act = w.slab + (static_cast<size_t>(i) * 2 + cur) * w.cap;
-                uint32_t* nxt = w.slab + (static_cast<size_t>(i) * 2 + (cur ^ 1u)) * w.cap;
+                const uint32_t* act = w.slab + (static_cast<size_t>(i) * kSlabBuffers + cur) * w.cap;
+                uint32_t* nxt = w.slab + (static_cast<size_t>(i) * kSlabBuffers + (cur ^ 1u)) * w.cap;
+                uint32_t* cnd = w.slab + (static_cast<size_t>(i) * kSlabBuffers + 2) * w.cap;
+                const uint32_t nc = w.ncand[i];
                 const f3 x = o + ts * d;
                 uint32_t cnt = 0, ia = 0, visits = 0;
                 auto push = [&](uint32_t v) {
@@ -386,13 +397,15 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
                     cnt += fits ? 1u : 0u;
                     overflow += fits ? 0u : 1u;
                 };
-                walk_point_q(sc, x, visits, lq, [&](uint32_t c) {
+                auto drain_below = [&](uint32_t c) {
                     while (ia < na) {
                         const uint32_t a = act[ia];
                         if (a >= c) break;
                         if (ts <= exit_from_inside(load_prim(sc.prims, a), o, d)) push(a);
                         ++ia;
                     }
+                };
+                auto visit = [&](uint32_t c) {
                     const PrimRecord pr = load_prim(sc.prims, c);
                     bool keep = false;
                     if (ia < na && act[ia] == c) {
@@ -405,7 +418,29 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
                         keep = tj >= 0.0f && tj <= ts && ts <= exit_from_entry(pr, o, d, tj, wl2);
                     }
                     if (keep) push(c);
-                });
+                };
+                if (nc <= w.cap) {
+                    // the candidates the extend walk recorded, in ascending index order
+                    for (uint32_t q = 1; q < nc; ++q) {
+                        const uint32_t v = cnd[q];
+                        uint32_t j = q;
+                        while (j > 0 && cnd[j - 1] > v) {
+                            cnd[j] = cnd[j - 1];
+                            --j;
+                        }
+                        cnd[j] = v;
+                    }
+                    for (uint32_t q = 0; q < nc; ++q) {
+                        drain_below(cnd[q]);
+                        visit(cnd[q]);
+                    }
+                } else {
+                    // more candidates than fit: one BVH point query at the vertex
+                    walk_point_q(sc, x, visits, lq, [&](uint32_t c) {
+                        drain_below(c);
+                        visit(c);
+                    });
+                }
                 while (ia < na) {
                     const uint32_t a = act[ia];
                     if (ts <= exit_from_inside(load_prim(sc.prims, a), o, d)) push(a);
@@ -534,7 +569,7 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
             const uint32_t bits = __float_as_uint(po.w);
             const uint32_t set = bits & 1u, count = bits >> 1;
             sc = jobs[__float_as_uint(w.dj[i].w)].scene;
-            const uint32_t* act = w.slab + (static_cast<size_t>(i) * 2 + set) * w.cap;
+            const uint32_t* act = w.slab + (static_cast<size_t>(i) * kSlabBuffers + set) * w.cap;
             acc = 0;
             full = false;
             for (uint32_t c = 0; c < count && !full; ++c) {
