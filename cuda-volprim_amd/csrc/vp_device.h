@@ -118,6 +118,33 @@ __device__ __forceinline__ void walk_point(const SceneView& s, f3 x, uint32_t& v
 #endif
 constexpr uint32_t kLeafQ = VP_LEAF_Q;
 
+// Node phase of the walks: a lane steps while its leaf queue has room and its
+// walk has nodes left.  With VP_WALK_MIN > 0 the loop runs for the whole
+// wavefront, which leaves it early once fewer than VP_WALK_MIN lanes are
+// stepping and some lane's queue is full, so full lanes do not wait for the
+// last few; the stepping lanes resume after the leaf phase.  A node phase
+// that starts with every queue empty can only end early after some lane has
+// taken kLeafQ steps, so walks always progress.  Every query consumes leaves
+// order-independently, so this changes only the schedule.  On the scattering
+// workload (4096 Gaussians, 512x512, 64 spp) 4 and 8 measure alike, about
+// 1.5 % faster than 0; 16 and 32 are slower.
+#ifndef VP_WALK_MIN
+#define VP_WALK_MIN 8
+#endif
+
+// true while this lane (VP_WALK_MIN == 0) or the wavefront stays in the node
+// phase
+__device__ __forceinline__ bool node_phase(bool step, uint32_t cnt) {
+#if VP_WALK_MIN > 0
+    const unsigned long long sm = __ballot(step);
+    if (sm == 0) return false;
+    return __popcll(sm) >= VP_WALK_MIN || __ballot(cnt == kLeafQ) == 0;
+#else
+    (void)cnt;
+    return step;
+#endif
+}
+
 struct LeafQueue {
     uint32_t* col;  // &lds[0][lane]
     uint32_t stride;  // blockDim.x
@@ -132,7 +159,10 @@ __device__ __forceinline__ void walk_ray_q(const SceneView& s, f3 o, f3 inv, con
     const uint32_t nn = s.n_nodes;
     while (true) {
         uint32_t cnt = 0;
-        while (i < nn && cnt < kLeafQ) {
+        while (true) {
+            const bool step = i < nn && cnt < kLeafQ;
+            if (!node_phase(step, cnt)) break;
+            if (!step) continue;
             const float4 a = as_global(s.nodes)[2 * i];
             const float4 b = as_global(s.nodes)[2 * i + 1];
             ++visits;

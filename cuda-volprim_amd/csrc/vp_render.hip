@@ -135,6 +135,16 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
         const uint32_t na = st.na;
         bool finish = false;
 
+        // At bounce 0 the analytic direct term needs the transmittance of the
+        // camera ray itself, so the timed kernel integrates it during the
+        // argmin loops below (the walk then runs over the whole ray instead
+        // of stopping at the running minimum): the same chords, summed in the
+        // same fixed point as transmittance() does.  The counting build
+        // keeps the separate shadow walk, so its counts are per-ray work.
+        const bool direct = !kCount && b == 0;
+        uint64_t dacc = 0;
+        bool dfull = false;
+
         // argmin over the origin set (sampling.cuh:356-370)
         float ts = kTMax;
         for (uint32_t k = 0; k < na; ++k) {
@@ -144,6 +154,8 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
             const float tau = free_flight_tau(prim_uniform(st.key, b, 0u, i));
             const float t = inv_cdf(pr, o, d, tau);
             if (t >= 0.0f && t < ts && t <= te) ts = t;
+            if (direct && !dfull && te > 0.0f && te < kTMax)
+                dfull = tau_add(dacc, optical_depth(pr, o, d, 0.0f, te));
         }
         ev.add(kCntArgActive, na);
         ev.add(kCntTrace);
@@ -155,7 +167,8 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
         uint32_t nc = 0;
         {
             const f3 inv = safe_inv(d);
-            float tmax = measure ? kTMax : ts;
+            const bool prune = !measure && !direct;
+            float tmax = prune ? ts : kTMax;
             uint32_t hits = 0, visits = 0, leaves = 0, accept = 0;
             walk_ray_q(sc, o, inv, tmax, visits, lq, [&](uint32_t prim) -> bool {
                 const PrimRecord pr = load_prim(sc.prims, prim);
@@ -164,6 +177,13 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                 const float tj = entry_t(pr, o, d, &wl2);
                 if (tj < 0.0f) return true;
                 ++hits;
+                if (direct) {
+                    const float te = exit_from_entry(pr, o, d, tj, wl2);
+                    if (!dfull && te > tj && te < kTMax) dfull = tau_add(dacc, optical_depth(pr, o, d, tj, te));
+                    // beyond the running minimum nothing below can change
+                    // (t >= tj >= ts)
+                    if (tj > ts) return true;
+                }
                 if (tj <= ts) {
                     if (nc < cap) cnd[nc * stride] = prim;
                     ++nc;
@@ -176,7 +196,7 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                     const float te = exit_from_entry(pr, o, d, tj, wl2);
                     if (t <= te) {
                         ts = t;
-                        if (!measure) tmax = ts;
+                        if (prune) tmax = ts;
                     }
                 }
                 return true;
@@ -192,7 +212,9 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
         // analytic direct term at bounce 0 (raygen.cuh:148-160)
         if (b == 0) {
             ev.add(kCntShadowRays);
-            const float T = transmittance(sc, o, d, act, na, stride, lq, ev);
+            float T;
+            if constexpr (kCount) T = transmittance(sc, o, d, act, na, stride, lq, ev);
+            else T = dfull ? 0.0f : tau_to_T(dacc);
             st.L = st.L + (st.beta * T) * env_lookup(env, d);
         }
 

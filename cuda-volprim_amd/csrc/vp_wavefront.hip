@@ -211,7 +211,10 @@ __global__ void VP_WF_LB k_wf_regen(const JobDesc* __restrict__ jobs, const uint
 // or the walk ends.
 __device__ __forceinline__ void walk_steps(const SceneView& sc, bool active, uint32_t& node, uint32_t nn, f3 inv,
                                            f3 oi, float tmax, const LeafQueue& lq, uint32_t& cnt) {
-    while (active && node < nn && cnt < kLeafQ) {
+    while (true) {
+        const bool step = active && node < nn && cnt < kLeafQ;
+        if (!node_phase(step, cnt)) break;
+        if (!step) continue;
         const float4 a = as_global(sc.nodes)[2 * node];
         const float4 c = as_global(sc.nodes)[2 * node + 1];
         const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(c.x, inv.x, -oi.x);
