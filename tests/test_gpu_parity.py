@@ -293,3 +293,33 @@ def test_render_sah_bitexact(ctx, case):
                            ris_candidates=p.ris_candidates, active_cap=p.active_cap)
     assert_bitwise(img, ref, f"SAH render {case}")
     assert st["active_overflow"] == k["active_overflow"]
+
+
+def test_full_size_scattering_workload(ctx):
+    # the measured workload at its full size: 4096 Gaussians (seed 1,
+    # sigma multiplier 10, albedo 0.9, isotropic phase), 2048x1024 sky + sun,
+    # 512x512 at 64 spp, depth 64, RR from 12, SAH tree.  The megakernel and
+    # the wavefront pipeline must agree on every pixel; every 32nd row and
+    # column is checked against the oracle (256 pixels x 64 samples).
+    import os
+    rows = scenes.c1_scattering(n=4096, seed=1, sigma_multiplier=10.0, albedo=0.9)
+    env = scenes.sky_env(2048, 1024)
+    cam = volprim.camera_perspective(**scenes.default_camera_args(512, 512, dist=3.5, fov=40.0))
+    p = volprim.default_params(width=512, height=512, spp=64, seed=0, max_depth=64, rr_depth=12, phase_g=0.0)
+    print("[case] full-size scattering 4096 Gaussians 512x512x64 megakernel + wavefront", flush=True)
+    sc = ctx.scene(rows, bvh="sah")
+    ev = ctx.env(env)
+    _, _, rec, _ = sc.export()
+    img, st = ctx.render(sc, ev, cam, p)
+    wf = volprim.LaunchConfig(256, 0, 0, 0, 0, volprim.PIPELINE_WAVEFRONT, 1 << 20)
+    imgs_wf, st_wf = ctx.render_jobs([(sc, ev, cam, p)], launch=wf)
+    sc.close()
+    ev.close()
+    assert st["active_overflow"] == 0 and st_wf["active_overflow"] == 0
+    assert_bitwise(imgs_wf[0], img, "wavefront vs megakernel, full image")
+    s = 32
+    print(f"[case] oracle on every {s}th row and column", flush=True)
+    ref, _, k = vpo.render(rec, env, cam.pod(), 512, 512, 64, seed=0, max_depth=64, rr_depth=12, phase_g=0.0,
+                           stride=s, threads=min(os.cpu_count() or 1, 16), bvh=True)
+    assert_bitwise(img[::s, ::s], ref[::s, ::s], "full-size render vs oracle (strided pixels)")
+    assert np.isfinite(img).all() and img[..., :3].mean() > 0.0
