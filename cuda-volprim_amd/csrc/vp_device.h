@@ -285,6 +285,86 @@ __device__ float transmittance(const SceneView& s, f3 o, f3 d, const uint32_t* a
     return tau_to_T(acc);
 }
 
+// The two shadow rays of one scatter vertex (MIS), each lane walking its ray
+// a and then its ray b without waiting for the wavefront in between: a lane
+// whose ray a ends starts ray b at once, so the wavefront's walk lasts as
+// long as its longest a-plus-b pair instead of its longest a plus its longest
+// b.  Both rays leave o through the same active set, whose chords are summed
+// in one loop over the set.  Ta and Tb equal two transmittance() calls bit
+// for bit (the same chords in the same fixed-point sums, and a ray that
+// reaches tau >= 88 gives 0).
+__device__ __forceinline__ void transmittance_two(const SceneView& s, f3 o, f3 da, f3 db, const uint32_t* act,
+                                                  uint32_t na, uint32_t stride, const LeafQueue& q, float& Ta,
+                                                  float& Tb) {
+    uint64_t acc_a = 0, acc_b = 0;
+    bool full_a = false, full_b = false;
+    for (uint32_t k = 0; k < na && !(full_a && full_b); ++k) {
+        const PrimRecord pr = load_prim(s.prims, act[k * stride]);
+        if (!full_a) {
+            const float te = exit_from_inside(pr, o, da);
+            if (te > 0.0f && te < kTMax) full_a = tau_add(acc_a, optical_depth(pr, o, da, 0.0f, te));
+        }
+        if (!full_b) {
+            const float te = exit_from_inside(pr, o, db);
+            if (te > 0.0f && te < kTMax) full_b = tau_add(acc_b, optical_depth(pr, o, db, 0.0f, te));
+        }
+    }
+    Ta = 0.0f;
+    Tb = 0.0f;
+    // current ray r: 0 = a, 1 = b, 2 = both done; a saturated ray is done
+    uint32_t r = full_a ? (full_b ? 2u : 1u) : 0u;
+    f3 d = r == 0 ? da : db;
+    uint64_t acc = r == 0 ? acc_a : acc_b;
+    bool full = false;
+    f3 inv = safe_inv(d), oi = o * inv;
+    uint32_t i = 0;
+    const uint32_t nn = s.n_nodes;
+    while (__ballot(r < 2u) != 0) {
+        uint32_t cnt = 0;
+        while (true) {
+            const bool step = r < 2u && i < nn && cnt < kLeafQ;
+            if (!node_phase(step, cnt)) break;
+            if (!step) continue;
+            const float4 a = as_global(s.nodes)[2 * i];
+            const float4 b = as_global(s.nodes)[2 * i + 1];
+            const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(b.x, inv.x, -oi.x);
+            const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(b.y, inv.y, -oi.y);
+            const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(b.z, inv.z, -oi.z);
+            const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+            const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), kTMax));
+            const uint32_t skip = __float_as_uint(a.w);
+            const uint32_t prim = __float_as_uint(b.w);
+            const bool hit = tnear <= tfar, leaf_node = prim != kInnerNode;
+            if (hit && leaf_node) q.at(cnt++) = prim;
+            i = (hit && !leaf_node) ? i + 1 : skip;
+        }
+        for (uint32_t k = 0; k < cnt && !full; ++k) {
+            const PrimRecord pr = load_prim(s.prims, q.at(k));
+            float wl2;
+            const float tj = entry_t(pr, o, d, &wl2);
+            if (tj < 0.0f) continue;
+            const float te = exit_from_entry(pr, o, d, tj, wl2);
+            if (te > tj && te < kTMax) full = tau_add(acc, optical_depth(pr, o, d, tj, te));
+        }
+        if (r < 2u && (full || i >= nn)) {
+            const float T = full ? 0.0f : tau_to_T(acc);
+            if (r == 0u) {
+                Ta = T;
+                r = full_b ? 2u : 1u;
+                d = db;
+                acc = acc_b;
+                full = false;
+                inv = safe_inv(d);
+                oi = o * inv;
+                i = 0;
+            } else {
+                Tb = T;
+                r = 2u;
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ EnvView env_view(const EnvDev& e) {
     EnvView v;
     v.tex = e.tex;

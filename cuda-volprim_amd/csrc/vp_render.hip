@@ -351,9 +351,6 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                 const float sa = apdf + pb_a;
                 const float wa = sa > 0.0f ? apdf / sa : 0.0f;
                 const f3 enva = env_lookup(env, awo);
-                ev.add(kCntShadowRays);
-                const float Ta = transmittance(sc, x, awo, act, nn, stride, lq, ev);
-                st.L = st.L + ((base * enva) * Ta) * wa;
 
                 float bpdf;
                 const f3 bwo = env_sample(env, st.rng, &bpdf);
@@ -361,8 +358,17 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                 const float sb = bpdf + ph_b;
                 const float wb = sb > 0.0f ? bpdf / sb : 0.0f;
                 const f3 envb = env_lookup(env, bwo);
-                ev.add(kCntShadowRays);
-                const float Tb = transmittance(sc, x, bwo, act, nn, stride, lq, ev);
+                ev.add(kCntShadowRays, 2);
+                // the counting build walks the two rays with separate calls,
+                // so its counts are per-ray work
+                float Ta, Tb;
+                if constexpr (kCount) {
+                    Ta = transmittance(sc, x, awo, act, nn, stride, lq, ev);
+                    Tb = transmittance(sc, x, bwo, act, nn, stride, lq, ev);
+                } else {
+                    transmittance_two(sc, x, awo, bwo, act, nn, stride, lq, Ta, Tb);
+                }
+                st.L = st.L + ((base * enva) * Ta) * wa;
                 st.L = st.L + (((((base * envb) * Tb) * wb) * ph_b) * rcpf(bpdf));
             }
             st.beta = st.beta * albedo;
