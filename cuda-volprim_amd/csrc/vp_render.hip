@@ -49,7 +49,7 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
          const uint64_t* __restrict__ launch_desc, unsigned long long* __restrict__ path_counter,
          uint32_t* __restrict__ slab, uint32_t cap, uint32_t regen_min,
          unsigned long long* __restrict__ counters, int measure) {
-    extern __shared__ uint32_t s_leafq[];  // [kLeafQ][blockDim.x]
+    extern __shared__ uint32_t s_leafq[];  // [blockDim.x][kLeafRow]
     const uint32_t njobs = static_cast<uint32_t>(launch_desc[0]);
     const uint64_t total_paths = launch_desc[1];
     const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
@@ -60,7 +60,7 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
     // buffers 0 and 1: active sets (ping-pong); buffer 2: primitives the
     // argmin walk entered before the running minimum (rebuild candidates)
     uint32_t* my = slab + static_cast<size_t>(gl) * kSlabBuffers * cap;  // entry k of buffer b at my[b * cap + k]
-    const LeafQueue lq{s_leafq + threadIdx.x, blockDim.x};
+    const LeafQueue lq = leaf_queue(s_leafq);
     PathState st;
     bool alive = false;
     bool exhausted = false;
@@ -579,7 +579,7 @@ using namespace vp;
 hipError_t vp_launch_step(vp_context* ctx, const RenderLaunch& L, uint32_t max_jobs, uint32_t max_npix,
                           hipStream_t s) {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctx->path_counter, 1u, 0ull);
-    const size_t lds = sizeof(uint32_t) * kLeafQ * L.block;
+    const size_t lds = leaf_queue_bytes(L.block);
     if (L.count_events)
         hipLaunchKernelGGL(k_render<true>, dim3(L.grid), dim3(L.block), lds, s, ctx->jobs_dev, ctx->job_begin_dev,
                            ctx->launch_desc, ctx->path_counter, ctx->slab, L.active_cap, L.regen_min,
@@ -611,7 +611,7 @@ hipError_t vp_launch_fill_u64(unsigned long long* p, uint32_t n, unsigned long l
 int vp_render_occupancy(uint32_t block, uint32_t, int count_events, int* blocks_per_cu) {
     int nb = 0;
     hipError_t e;
-    const size_t lds = sizeof(uint32_t) * kLeafQ * block;
+    const size_t lds = leaf_queue_bytes(block);
     if (count_events)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(k_render<true>), block, lds);
     else

@@ -223,17 +223,7 @@ __device__ __forceinline__ void walk_steps(const SceneView& sc, bool active, uin
         const bool step = active && node < nn && cnt < kLeafQ;
         if (!node_phase(step, cnt)) break;
         if (!step) continue;
-        const float4 a = as_global(sc.nodes)[2 * node];
-        const float4 c = as_global(sc.nodes)[2 * node + 1];
-        const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(c.x, inv.x, -oi.x);
-        const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(c.y, inv.y, -oi.y);
-        const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(c.z, inv.z, -oi.z);
-        const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-        const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-        const uint32_t skip = __float_as_uint(a.w), prim = __float_as_uint(c.w);
-        const bool hit = tnear <= tfar, leaf_node = prim != kInnerNode;
-        if (hit && leaf_node) lq.at(cnt++) = prim;
-        node = (hit && !leaf_node) ? node + 1 : skip;
+        ray_node_step(sc, inv, oi, tmax, node, cnt, lq);
     }
 }
 
@@ -267,7 +257,7 @@ __device__ __forceinline__ uint32_t wave_fetch(unsigned int* head, bool want, ui
 // leaf queue; lanes refill from the live queue as their walks end.
 __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par, uint32_t fetch_min) {
     extern __shared__ uint32_t s_leafq[];
-    const LeafQueue lq{s_leafq + threadIdx.x, blockDim.x};
+    const LeafQueue lq = leaf_queue(s_leafq);
     unsigned int* qc = qcs(w, par);
     const uint32_t n = qc[0];
     const uint32_t P = w.P;
@@ -355,7 +345,7 @@ __device__ __forceinline__ void put_pending(const WfPool& w, uint32_t i, uint32_
 __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par,
                                       unsigned long long* __restrict__ counters) {
     extern __shared__ uint32_t s_leafq[];
-    const LeafQueue lq{s_leafq + threadIdx.x, blockDim.x};
+    const LeafQueue lq = leaf_queue(s_leafq);
     const uint32_t n = qcs(w, par)[0];
     const uint32_t P = w.P;
     uint32_t overflow = 0;
@@ -559,7 +549,7 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
 // primitives; lanes refill from the shadow queue as their rays end.
 __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par, uint32_t fetch_min) {
     extern __shared__ uint32_t s_leafq[];
-    const LeafQueue lq{s_leafq + threadIdx.x, blockDim.x};
+    const LeafQueue lq = leaf_queue(s_leafq);
     unsigned int* qc = qcs(w, par);
     const uint32_t n = qc[1];
     const uint32_t P = w.P;
@@ -626,7 +616,7 @@ hipError_t vp_wf_launch_init(const WfPool& w, hipStream_t s) {
 
 // Two iterations (parities 0 and 1) of the pipeline.
 hipError_t vp_wf_launch_iterations(vp_context* ctx, const WfPool& w, const WfLaunch& L, hipStream_t s) {
-    const size_t lds = sizeof(uint32_t) * kLeafQ * 256;
+    const size_t lds = leaf_queue_bytes(256);
     for (uint32_t par = 0; par < 2; ++par) {
         hipLaunchKernelGGL(k_wf_shade, dim3(cdiv(w.P, 256)), dim3(256), 0, s, ctx->jobs_dev, w, par);
         hipLaunchKernelGGL(k_wf_claim, dim3(1), dim3(64), 0, s, ctx->launch_desc, ctx->path_counter, w, par);
@@ -641,7 +631,7 @@ hipError_t vp_wf_launch_iterations(vp_context* ctx, const WfPool& w, const WfLau
 }
 
 int vp_wf_occupancy(int which, int* blocks_per_cu) {
-    const size_t lds = which == 3 ? 0 : sizeof(uint32_t) * kLeafQ * 256;
+    const size_t lds = which == 3 ? 0 : leaf_queue_bytes(256);
     const void* f = which == 0   ? reinterpret_cast<const void*>(k_wf_extend)
                     : which == 1 ? reinterpret_cast<const void*>(k_wf_scatter)
                     : which == 2 ? reinterpret_cast<const void*>(k_wf_shadow)
