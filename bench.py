@@ -411,6 +411,10 @@ def main():
         "pipeline": args.pipeline,
         "bvh": args.bvh,
         "events_per_step": events,
+        # rays traced per second: camera and continuation rays (one argmin
+        # walk each) plus shadow rays, from the counted events of one render
+        "mrays_per_s": (round((events["traces"] + events["shadow_rays"]) * args.steps / dt / 1e6, 1)
+                        if events else None),
         "quality": quality,
         "image_mean": float(img[..., 0:3].mean()),
     }
