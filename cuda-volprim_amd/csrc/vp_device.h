@@ -302,6 +302,48 @@ __device__ float transmittance(const SceneView& s, f3 o, f3 d, const uint32_t* a
     return tau_to_T(acc);
 }
 
+// Wave-shared leaf queue.  The per-thread queue rows of a wavefront (64 rows
+// of kLeafRow words) can instead hold one queue for the whole wavefront: node
+// phases append crossed leaves with their owner lane, and in the leaf phase
+// every active lane takes the next entry, whichever lane's ray it belongs to.
+// Leaf work then runs at full width however unevenly the lanes' walks found
+// leaves.  Results are combined with LDS atomics that do not depend on order
+// (fixed-point sums).
+constexpr uint32_t kWaveQ = 256;
+struct WaveQueue {
+    uint32_t* prim;                // [kWaveQ]
+    uint8_t* owner;                // [kWaveQ]
+    float4* ro;                    // [64] ray origin of each lane
+    float4* rd;                    // [64] ray direction of each lane
+    unsigned long long* acc;       // [64] fixed-point optical depth of each lane's ray
+};
+static_assert(kWaveQ * 5 + 64 * (16 + 16 + 8) <= 64 * kLeafRow * 4, "wave queue exceeds the wave's queue rows");
+
+// the calling wavefront's queue, over the same LDS as its threads' queue rows
+__device__ __forceinline__ WaveQueue wave_queue(uint32_t* lds) {
+    uint8_t* base = reinterpret_cast<uint8_t*>(lds + (threadIdx.x >> 6) * 64 * kLeafRow);
+    WaveQueue q;
+    q.prim = reinterpret_cast<uint32_t*>(base);
+    q.owner = base + 4 * kWaveQ;
+    q.ro = reinterpret_cast<float4*>(base + 5 * kWaveQ);
+    q.rd = q.ro + 64;
+    q.acc = reinterpret_cast<unsigned long long*>(q.rd + 64);
+    return q;
+}
+
+// number of set bits of m below the calling lane
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
+// one chord in fixed point, as tau_add adds it; a chord of tau >= 88 alone
+// adds the saturation limit
+__device__ __forceinline__ unsigned long long tau_fixed(float od) {
+    const float c = od > 0.0f ? od : 0.0f;
+    return c >= kMaxTau ? kTauLimitFixed : static_cast<unsigned long long>(c * kTauScaleF);
+}
+
 // The two shadow rays of one scatter vertex (MIS), each lane walking its ray
 // a and then its ray b without waiting for the wavefront in between: a lane
 // whose ray a ends starts ray b at once, so the wavefront's walk lasts as
@@ -366,6 +408,112 @@ __device__ __forceinline__ void transmittance_two(const SceneView& s, f3 o, f3 d
             } else {
                 Tb = T;
                 r = 2u;
+            }
+        }
+    }
+}
+
+// transmittance_two with the wave-shared leaf queue: the same rays, chords
+// and fixed-point sums (a ray is saturated once its sum reaches the tau = 88
+// limit, whatever the order its chords arrive in), so Ta and Tb equal two
+// transmittance() calls bit for bit.  Every lane of the calling wavefront
+// that is active here takes part in the leaf phases.
+__device__ __forceinline__ void transmittance_two_shared(const SceneView& s, f3 o, f3 da, f3 db,
+                                                         const uint32_t* act, uint32_t na, uint32_t stride,
+                                                         const WaveQueue& wq, float& Ta, float& Tb) {
+    uint64_t acc_a = 0, acc_b = 0;
+    bool full_a = false, full_b = false;
+    for (uint32_t k = 0; k < na && !(full_a && full_b); ++k) {
+        const PrimRecord pr = load_prim(s.prims, act[k * stride]);
+        if (!full_a) {
+            const float te = exit_from_inside(pr, o, da);
+            if (te > 0.0f && te < kTMax) full_a = tau_add(acc_a, optical_depth(pr, o, da, 0.0f, te));
+        }
+        if (!full_b) {
+            const float te = exit_from_inside(pr, o, db);
+            if (te > 0.0f && te < kTMax) full_b = tau_add(acc_b, optical_depth(pr, o, db, 0.0f, te));
+        }
+    }
+    Ta = 0.0f;
+    Tb = 0.0f;
+    const uint32_t lane = __lane_id();
+    // current ray r: 0 = a, 1 = b, 2 = both done; a saturated ray is done
+    uint32_t r = full_a ? (full_b ? 2u : 1u) : 0u;
+    f3 d = r == 0u ? da : db;
+    wq.ro[lane] = make_float4(o.x, o.y, o.z, 0.0f);
+    wq.rd[lane] = make_float4(d.x, d.y, d.z, 0.0f);
+    wq.acc[lane] = r == 0u ? acc_a : acc_b;
+    f3 inv = safe_inv(d), oi = o * inv;
+    uint32_t i = 0;
+    const uint32_t nn = s.n_nodes;
+    while (__ballot(r < 2u) != 0) {
+        // node phase: every lane with a ray steps; crossed leaves are queued
+        // for the wavefront.  It ends when no lane steps, when the queue
+        // might not take one more step of every lane, or, once a quarter of
+        // the queue is filled, when fewer than VP_WALK_MIN lanes still step.
+        uint32_t count = 0;
+        while (true) {
+            const bool step = r < 2u && i < nn;
+            const unsigned long long sm = __builtin_amdgcn_ballot_w64(step);
+            if (sm == 0 || count > kWaveQ - 64u) break;
+            if (count >= kWaveQ / 4u && static_cast<uint32_t>(__popcll(sm)) < VP_WALK_MIN) break;
+            bool push = false;
+            uint32_t prim = 0;
+            if (step) {
+                const float4 a = as_global(s.nodes)[2 * i];
+                const float4 b = as_global(s.nodes)[2 * i + 1];
+                const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(b.x, inv.x, -oi.x);
+                const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(b.y, inv.y, -oi.y);
+                const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(b.z, inv.z, -oi.z);
+                const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+                const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), kTMax));
+                const uint32_t skip = __float_as_uint(a.w);
+                prim = __float_as_uint(b.w);
+                const bool hit = tnear <= tfar, leaf_node = prim != kInnerNode;
+                push = hit && leaf_node;
+                i = (hit && !leaf_node) ? i + 1 : skip;
+            }
+            const unsigned long long pm = __builtin_amdgcn_ballot_w64(push);
+            if (push) {
+                const uint32_t pos = count + lanes_below(pm);
+                wq.prim[pos] = prim;
+                wq.owner[pos] = static_cast<uint8_t>(lane);
+            }
+            count += static_cast<uint32_t>(__popcll(pm));
+        }
+        // leaf phase: the active lanes share the queued leaves
+        const unsigned long long am = __builtin_amdgcn_ballot_w64(true);
+        const uint32_t nact = static_cast<uint32_t>(__popcll(am));
+        for (uint32_t g = lanes_below(am); g < count; g += nact) {
+            const uint32_t own = wq.owner[g];
+            const float4 o4 = wq.ro[own], d4 = wq.rd[own];
+            const f3 ro = f3{o4.x, o4.y, o4.z}, rd = f3{d4.x, d4.y, d4.z};
+            const PrimRecord pr = load_prim(s.prims, wq.prim[g]);
+            float wl2;
+            const float tj = entry_t(pr, ro, rd, &wl2);
+            if (tj < 0.0f) continue;
+            const float te = exit_from_entry(pr, ro, rd, tj, wl2);
+            if (te > tj && te < kTMax) atomicAdd(&wq.acc[own], tau_fixed(optical_depth(pr, ro, rd, tj, te)));
+        }
+        // each lane looks at its own ray
+        if (r < 2u) {
+            const unsigned long long acc = wq.acc[lane];
+            const bool full = acc >= kTauLimitFixed;
+            if (full || i >= nn) {
+                const float T = full ? 0.0f : tau_to_T(acc);
+                if (r == 0u) {
+                    Ta = T;
+                    r = full_b ? 2u : 1u;
+                    d = db;
+                    wq.rd[lane] = make_float4(d.x, d.y, d.z, 0.0f);
+                    wq.acc[lane] = acc_b;
+                    inv = safe_inv(d);
+                    oi = o * inv;
+                    i = 0;
+                } else {
+                    Tb = T;
+                    r = 2u;
+                }
             }
         }
     }

@@ -49,7 +49,7 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
          const uint64_t* __restrict__ launch_desc, unsigned long long* __restrict__ path_counter,
          uint32_t* __restrict__ slab, uint32_t cap, uint32_t regen_min,
          unsigned long long* __restrict__ counters, int measure) {
-    extern __shared__ uint32_t s_leafq[];  // [blockDim.x][kLeafRow]
+    extern __shared__ __align__(16) uint32_t s_leafq[];  // [blockDim.x][kLeafRow]
     const uint32_t njobs = static_cast<uint32_t>(launch_desc[0]);
     const uint64_t total_paths = launch_desc[1];
     const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
@@ -61,6 +61,7 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
     // argmin walk entered before the running minimum (rebuild candidates)
     uint32_t* my = slab + static_cast<size_t>(gl) * kSlabBuffers * cap;  // entry k of buffer b at my[b * cap + k]
     const LeafQueue lq = leaf_queue(s_leafq);
+    const WaveQueue wq = wave_queue(s_leafq);  // the same LDS, used by the MIS shadow walks
     PathState st;
     bool alive = false;
     bool exhausted = false;
@@ -366,7 +367,7 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                     Ta = transmittance(sc, x, awo, act, nn, stride, lq, ev);
                     Tb = transmittance(sc, x, bwo, act, nn, stride, lq, ev);
                 } else {
-                    transmittance_two(sc, x, awo, bwo, act, nn, stride, lq, Ta, Tb);
+                    transmittance_two_shared(sc, x, awo, bwo, act, nn, stride, wq, Ta, Tb);
                 }
                 st.L = st.L + ((base * enva) * Ta) * wa;
                 st.L = st.L + (((((base * envb) * Tb) * wb) * ph_b) * rcpf(bpdf));
