@@ -120,6 +120,47 @@ __device__ __forceinline__ void walk_point(const SceneView& s, f3 x, uint32_t& v
 constexpr uint32_t kLeafQ = VP_LEAF_Q;
 constexpr uint32_t kLeafRow = kLeafQ + 1;
 
+// Wave-shared leaf queue.  The LDS of a wavefront holds either its threads'
+// queue rows or one queue for the whole wavefront (the two are used at
+// different times): node phases append crossed leaves with their owner lane,
+// and in the leaf phase every active lane takes the next entry, whichever
+// lane's ray it belongs to.  Leaf work then runs at full width however
+// unevenly the lanes' walks found leaves.  Per-ray results are combined with
+// LDS atomics whose outcome does not depend on order (fixed-point sums, and
+// minima).
+constexpr uint32_t kWaveQ = 256;
+struct WaveQueue {
+    uint32_t* prim;                // [kWaveQ]
+    uint8_t* owner;                // [kWaveQ]
+    float4* ro;                    // [64] ray origin of each lane
+    float4* rd;                    // [64] ray direction of each lane
+    unsigned long long* acc;       // [64] fixed-point optical depth of each lane's ray
+    unsigned long long* key;       // [64] path key of each lane (argmin walks)
+    uint32_t* ts;                  // [64] running minimum of each lane's argmin, as float bits
+    uint32_t* nc;                  // [64] candidates recorded for each lane's argmin
+    uint32_t* meta;                // [64] bounce | direct-term flag << 31 (argmin walks)
+};
+constexpr uint32_t kWaveQueueBytes = kWaveQ * 5 + 64 * (16 + 16 + 8 + 8 + 4 + 4 + 4);
+// LDS bytes per wavefront: the larger of the two uses, in 16-byte units
+constexpr uint32_t kWaveBytes =
+    ((64 * kLeafRow * 4 > kWaveQueueBytes ? 64 * kLeafRow * 4 : kWaveQueueBytes) + 15) / 16 * 16;
+
+// the calling wavefront's queue
+__device__ __forceinline__ WaveQueue wave_queue(uint32_t* lds) {
+    uint8_t* base = reinterpret_cast<uint8_t*>(lds) + (threadIdx.x >> 6) * kWaveBytes;
+    WaveQueue q;
+    q.prim = reinterpret_cast<uint32_t*>(base);
+    q.owner = base + 4 * kWaveQ;
+    q.ro = reinterpret_cast<float4*>(base + 5 * kWaveQ);
+    q.rd = q.ro + 64;
+    q.acc = reinterpret_cast<unsigned long long*>(q.rd + 64);
+    q.key = q.acc + 64;
+    q.ts = reinterpret_cast<uint32_t*>(q.key + 64);
+    q.nc = q.ts + 64;
+    q.meta = q.nc + 64;
+    return q;
+}
+
 // Node phase of the walks: a lane steps while its leaf queue has room and its
 // walk has nodes left.  With VP_WALK_MIN > 0 the loop runs for the whole
 // wavefront, which leaves it early once fewer than VP_WALK_MIN lanes are
@@ -153,8 +194,10 @@ struct LeafQueue {
 };
 
 // the calling thread's queue in a dynamic LDS array of leaf_queue_bytes(blockDim.x)
-__device__ __forceinline__ LeafQueue leaf_queue(uint32_t* lds) { return LeafQueue{lds + threadIdx.x * kLeafRow}; }
-inline size_t leaf_queue_bytes(uint32_t block) { return sizeof(uint32_t) * kLeafRow * block; }
+__device__ __forceinline__ LeafQueue leaf_queue(uint32_t* lds) {
+    return LeafQueue{lds + (threadIdx.x >> 6) * (kWaveBytes / 4) + (threadIdx.x & 63u) * kLeafRow};
+}
+inline size_t leaf_queue_bytes(uint32_t block) { return static_cast<size_t>(kWaveBytes) * ((block + 63) / 64); }
 
 // One node step of a ray walk over [0, tmax]: a crossed leaf is queued, a
 // crossed inner node is entered, anything else is skipped.  The queue slot
@@ -300,35 +343,6 @@ __device__ float transmittance(const SceneView& s, f3 o, f3 d, const uint32_t* a
     ev.add(kCntShHit, sh);
     if (full) return 0.0f;
     return tau_to_T(acc);
-}
-
-// Wave-shared leaf queue.  The per-thread queue rows of a wavefront (64 rows
-// of kLeafRow words) can instead hold one queue for the whole wavefront: node
-// phases append crossed leaves with their owner lane, and in the leaf phase
-// every active lane takes the next entry, whichever lane's ray it belongs to.
-// Leaf work then runs at full width however unevenly the lanes' walks found
-// leaves.  Results are combined with LDS atomics that do not depend on order
-// (fixed-point sums).
-constexpr uint32_t kWaveQ = 256;
-struct WaveQueue {
-    uint32_t* prim;                // [kWaveQ]
-    uint8_t* owner;                // [kWaveQ]
-    float4* ro;                    // [64] ray origin of each lane
-    float4* rd;                    // [64] ray direction of each lane
-    unsigned long long* acc;       // [64] fixed-point optical depth of each lane's ray
-};
-static_assert(kWaveQ * 5 + 64 * (16 + 16 + 8) <= 64 * kLeafRow * 4, "wave queue exceeds the wave's queue rows");
-
-// the calling wavefront's queue, over the same LDS as its threads' queue rows
-__device__ __forceinline__ WaveQueue wave_queue(uint32_t* lds) {
-    uint8_t* base = reinterpret_cast<uint8_t*>(lds + (threadIdx.x >> 6) * 64 * kLeafRow);
-    WaveQueue q;
-    q.prim = reinterpret_cast<uint32_t*>(base);
-    q.owner = base + 4 * kWaveQ;
-    q.ro = reinterpret_cast<float4*>(base + 5 * kWaveQ);
-    q.rd = q.ro + 64;
-    q.acc = reinterpret_cast<unsigned long long*>(q.rd + 64);
-    return q;
 }
 
 // number of set bits of m below the calling lane

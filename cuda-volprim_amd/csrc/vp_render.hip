@@ -39,6 +39,107 @@ struct PathState {
 #define VP_RENDER_WAVES_PER_EU 6
 #endif
 
+// Argmin free flight over the entered primitives (sampling.cuh:372-412) with
+// the wave-shared leaf queue: every active lane of the wavefront walks its
+// own ray, and the queued leaves of all rays are shared out among the lanes.
+// Each ray's running minimum is an LDS atomic minimum over float bits (all
+// candidates are >= 0), so the result does not depend on which lane handles
+// which leaf.  Candidates for the rebuild are recorded while t_entry is at
+// most the running minimum at the time, which covers every primitive entered
+// before the final one; their order is fixed by the sort in the rebuild.  At
+// bounce 0 (direct) the walk runs over the whole ray and also sums the direct
+// term's optical depth.  ts_origin is the minimum over the origin set; the
+// result keeps it unless a walk candidate is strictly smaller.
+__device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3 d, uint64_t key, uint32_t b,
+                                                   bool direct, float ts_origin, uint64_t dacc, bool dfull,
+                                                   uint32_t* __restrict__ slab, uint32_t cap, const WaveQueue& wq,
+                                                   float& ts, uint32_t& nc, float& Tdirect) {
+    const uint32_t lane = __lane_id();
+    wq.ro[lane] = make_float4(o.x, o.y, o.z, 0.0f);
+    wq.rd[lane] = make_float4(d.x, d.y, d.z, 0.0f);
+    wq.key[lane] = key;
+    wq.meta[lane] = b | (direct ? 0x80000000u : 0u);
+    wq.acc[lane] = dfull ? kTauLimitFixed : dacc;
+    wq.ts[lane] = ts_origin == 0.0f ? 0u : __float_as_uint(ts_origin);
+    wq.nc[lane] = 0u;
+    const f3 inv = safe_inv(d), oi = o * inv;
+    float tmax = direct ? kTMax : ts_origin;
+    uint32_t i = 0;
+    const uint32_t nn = sc.n_nodes;
+    // the first thread of this wavefront, for the owners' candidate buffers
+    const uint32_t gbase = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    while (__ballot(i < nn) != 0) {
+        uint32_t count = 0;
+        while (true) {
+            const bool step = i < nn;
+            const unsigned long long sm = __builtin_amdgcn_ballot_w64(step);
+            if (sm == 0 || count > kWaveQ - 64u) break;
+            if (count >= kWaveQ / 4u && static_cast<uint32_t>(__popcll(sm)) < VP_WALK_MIN) break;
+            bool push = false;
+            uint32_t prim = 0;
+            if (step) {
+                const float4 a = as_global(sc.nodes)[2 * i];
+                const float4 c = as_global(sc.nodes)[2 * i + 1];
+                const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(c.x, inv.x, -oi.x);
+                const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(c.y, inv.y, -oi.y);
+                const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(c.z, inv.z, -oi.z);
+                const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+                const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+                const uint32_t skip = __float_as_uint(a.w);
+                prim = __float_as_uint(c.w);
+                const bool hit = tnear <= tfar, leaf_node = prim != kInnerNode;
+                push = hit && leaf_node;
+                i = (hit && !leaf_node) ? i + 1 : skip;
+            }
+            const unsigned long long pm = __builtin_amdgcn_ballot_w64(push);
+            if (push) {
+                const uint32_t pos = count + lanes_below(pm);
+                wq.prim[pos] = prim;
+                wq.owner[pos] = static_cast<uint8_t>(lane);
+            }
+            count += static_cast<uint32_t>(__popcll(pm));
+        }
+        const unsigned long long am = __builtin_amdgcn_ballot_w64(true);
+        const uint32_t nact = static_cast<uint32_t>(__popcll(am));
+        for (uint32_t g = lanes_below(am); g < count; g += nact) {
+            const uint32_t own = wq.owner[g];
+            const uint32_t prim = wq.prim[g];
+            const float4 o4 = wq.ro[own], d4 = wq.rd[own];
+            const f3 ro = f3{o4.x, o4.y, o4.z}, rd = f3{d4.x, d4.y, d4.z};
+            const PrimRecord pr = load_prim(sc.prims, prim);
+            float wl2;
+            const float tj = entry_t(pr, ro, rd, &wl2);
+            if (tj < 0.0f) continue;
+            const uint32_t meta = wq.meta[own];
+            const float ts_cur = __uint_as_float(wq.ts[own]);
+            if (meta >> 31) {
+                const float te = exit_from_entry(pr, ro, rd, tj, wl2);
+                if (te > tj && te < kTMax) atomicAdd(&wq.acc[own], tau_fixed(optical_depth(pr, ro, rd, tj, te)));
+                // beyond the running minimum nothing below can change
+                // (t >= tj > ts)
+                if (tj > ts_cur) continue;
+            }
+            if (tj <= ts_cur) {
+                const uint32_t pos = atomicAdd(&wq.nc[own], 1u);
+                if (pos < cap) slab[(static_cast<size_t>(gbase + own) * kSlabBuffers + 2) * cap + pos] = prim;
+            }
+            const float tau = free_flight_tau(prim_uniform(wq.key[own], meta & 0x7fffffffu, 1u, prim));
+            float t = inv_cdf_span(pr, ro, rd, tj, tau);
+            if (t >= 0.0f && t < tj) t = tj;
+            if (t >= 0.0f && t < ts_cur) {
+                const float te = exit_from_entry(pr, ro, rd, tj, wl2);
+                if (t <= te) atomicMin(&wq.ts[own], __float_as_uint(t));
+            }
+        }
+        if (!direct) tmax = __uint_as_float(wq.ts[lane]);
+    }
+    const float wmin = __uint_as_float(wq.ts[lane]);
+    ts = wmin < ts_origin ? wmin : ts_origin;
+    nc = wq.nc[lane];
+    const unsigned long long acc = wq.acc[lane];
+    Tdirect = acc >= kTauLimitFixed ? 0.0f : tau_to_T(acc);
+}
+
 // launch_desc[0] = number of job slices, launch_desc[1] = total paths; read
 // from device memory so one captured graph serves every step of a job mix.
 // A wavefront claims new paths once at least regen_min of its lanes are idle
@@ -166,7 +267,10 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
         // among them, because the walk only prunes beyond the running one)
         uint32_t* cnd = my + 2 * cap * stride;
         uint32_t nc = 0;
-        {
+        float Tdirect = 0.0f;
+        if (!kCount && !measure) {
+            argmin_walk_shared(sc, o, d, st.key, b, direct, ts, dacc, dfull, slab, cap, wq, ts, nc, Tdirect);
+        } else {
             const f3 inv = safe_inv(d);
             const bool prune = !measure && !direct;
             float tmax = prune ? ts : kTMax;
@@ -215,7 +319,7 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
             ev.add(kCntShadowRays);
             float T;
             if constexpr (kCount) T = transmittance(sc, o, d, act, na, stride, lq, ev);
-            else T = dfull ? 0.0f : tau_to_T(dacc);
+            else T = measure ? (dfull ? 0.0f : tau_to_T(dacc)) : Tdirect;
             st.L = st.L + (st.beta * T) * env_lookup(env, d);
         }
 
