@@ -128,7 +128,21 @@ constexpr uint32_t kLeafRow = kLeafQ + 1;
 // unevenly the lanes' walks found leaves.  Per-ray results are combined with
 // LDS atomics whose outcome does not depend on order (fixed-point sums, and
 // minima).
-constexpr uint32_t kWaveQ = 256;
+#ifndef VP_WAVE_Q
+#define VP_WAVE_Q 256
+#endif
+constexpr uint32_t kWaveQ = VP_WAVE_Q;
+// A node phase of the shared-queue walks ends early, to share out the queued
+// leaves, once the queue holds kWaveQ / 4 entries and fewer than
+// VP_SHARED_STEP_MIN lanes still step.
+// Shadow leaf phases in two passes (entry tests, then the chords of the
+// entered leaves), each at full width: +2 % on the scattering workload.
+#ifndef VP_SHADOW_TWO_STAGE
+#define VP_SHADOW_TWO_STAGE 1
+#endif
+#ifndef VP_SHARED_STEP_MIN
+#define VP_SHARED_STEP_MIN VP_WALK_MIN
+#endif
 struct WaveQueue {
     uint32_t* prim;                // [kWaveQ]
     uint8_t* owner;                // [kWaveQ]
@@ -139,8 +153,9 @@ struct WaveQueue {
     uint32_t* ts;                  // [64] running minimum of each lane's argmin, as float bits
     uint32_t* nc;                  // [64] candidates recorded for each lane's argmin
     uint32_t* meta;                // [64] bounce | direct-term flag << 31 (argmin walks)
+    float* tj;                     // [kWaveQ] entry distances of the leaves that passed the entry test
 };
-constexpr uint32_t kWaveQueueBytes = kWaveQ * 5 + 64 * (16 + 16 + 8 + 8 + 4 + 4 + 4);
+constexpr uint32_t kWaveQueueBytes = kWaveQ * 9 + 64 * (16 + 16 + 8 + 8 + 4 + 4 + 4);
 // LDS bytes per wavefront: the larger of the two uses, in 16-byte units
 constexpr uint32_t kWaveBytes =
     ((64 * kLeafRow * 4 > kWaveQueueBytes ? 64 * kLeafRow * 4 : kWaveQueueBytes) + 15) / 16 * 16;
@@ -158,6 +173,7 @@ __device__ __forceinline__ WaveQueue wave_queue(uint32_t* lds) {
     q.ts = reinterpret_cast<uint32_t*>(q.key + 64);
     q.nc = q.ts + 64;
     q.meta = q.nc + 64;
+    q.tj = reinterpret_cast<float*>(q.meta + 64);
     return q;
 }
 
@@ -470,7 +486,7 @@ __device__ __forceinline__ void transmittance_two_shared(const SceneView& s, f3 
             const bool step = r < 2u && i < nn;
             const unsigned long long sm = __builtin_amdgcn_ballot_w64(step);
             if (sm == 0 || count > kWaveQ - 64u) break;
-            if (count >= kWaveQ / 4u && static_cast<uint32_t>(__popcll(sm)) < VP_WALK_MIN) break;
+            if (count >= kWaveQ / 4u && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
             bool push = false;
             uint32_t prim = 0;
             if (step) {
@@ -498,6 +514,44 @@ __device__ __forceinline__ void transmittance_two_shared(const SceneView& s, f3 
         // leaf phase: the active lanes share the queued leaves
         const unsigned long long am = __builtin_amdgcn_ballot_w64(true);
         const uint32_t nact = static_cast<uint32_t>(__popcll(am));
+#if VP_SHADOW_TWO_STAGE
+        // entry tests first; the leaves that are entered move down to the
+        // front of the queue (an entry is only overwritten after it has been
+        // read), then their chords are integrated at full width
+        uint32_t nhit = 0;
+        for (uint32_t base = 0; base < count; base += nact) {
+            const uint32_t g = base + lanes_below(am);
+            bool entered = false;
+            uint32_t own = 0, prim = 0;
+            float tj = -1.0f;
+            if (g < count) {
+                own = wq.owner[g];
+                prim = wq.prim[g];
+                const float4 o4 = wq.ro[own], d4 = wq.rd[own];
+                const PrimRecord pr = load_prim(s.prims, prim);
+                float wl2;
+                tj = entry_t(pr, f3{o4.x, o4.y, o4.z}, f3{d4.x, d4.y, d4.z}, &wl2);
+                entered = tj >= 0.0f;
+            }
+            const unsigned long long hm = __builtin_amdgcn_ballot_w64(entered);
+            if (entered) {
+                const uint32_t h = nhit + lanes_below(hm);
+                wq.prim[h] = prim;
+                wq.owner[h] = static_cast<uint8_t>(own);
+                wq.tj[h] = tj;
+            }
+            nhit += static_cast<uint32_t>(__popcll(hm));
+        }
+        for (uint32_t g = lanes_below(am); g < nhit; g += nact) {
+            const uint32_t own = wq.owner[g];
+            const float4 o4 = wq.ro[own], d4 = wq.rd[own];
+            const f3 ro = f3{o4.x, o4.y, o4.z}, rd = f3{d4.x, d4.y, d4.z};
+            const PrimRecord pr = load_prim(s.prims, wq.prim[g]);
+            const float tj = wq.tj[g];
+            const float te = exit_from_entry(pr, ro, rd, tj, length2(to_local_dir(pr, rd)));
+            if (te > tj && te < kTMax) atomicAdd(&wq.acc[own], tau_fixed(optical_depth(pr, ro, rd, tj, te)));
+        }
+#else
         for (uint32_t g = lanes_below(am); g < count; g += nact) {
             const uint32_t own = wq.owner[g];
             const float4 o4 = wq.ro[own], d4 = wq.rd[own];
@@ -509,6 +563,7 @@ __device__ __forceinline__ void transmittance_two_shared(const SceneView& s, f3 
             const float te = exit_from_entry(pr, ro, rd, tj, wl2);
             if (te > tj && te < kTMax) atomicAdd(&wq.acc[own], tau_fixed(optical_depth(pr, ro, rd, tj, te)));
         }
+#endif
         // each lane looks at its own ray
         if (r < 2u) {
             const unsigned long long acc = wq.acc[lane];

@@ -74,7 +74,7 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
             const bool step = i < nn;
             const unsigned long long sm = __builtin_amdgcn_ballot_w64(step);
             if (sm == 0 || count > kWaveQ - 64u) break;
-            if (count >= kWaveQ / 4u && static_cast<uint32_t>(__popcll(sm)) < VP_WALK_MIN) break;
+            if (count >= kWaveQ / 4u && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
             bool push = false;
             uint32_t prim = 0;
             if (step) {
