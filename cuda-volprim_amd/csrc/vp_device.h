@@ -149,7 +149,7 @@ struct WaveQueue {
     float4* ro;                    // [64] ray origin of each lane
     float4* rd;                    // [64] ray direction of each lane
     unsigned long long* acc;       // [64] fixed-point optical depth of each lane's ray
-    unsigned long long* key;       // [64] path key of each lane (argmin walks)
+    unsigned long long* key;       // [64] path key of each lane (argmin walks), or its ray's primitive array
     uint32_t* ts;                  // [64] running minimum of each lane's argmin, as float bits
     uint32_t* nc;                  // [64] candidates recorded for each lane's argmin
     uint32_t* meta;                // [64] bounce | direct-term flag << 31 (argmin walks)

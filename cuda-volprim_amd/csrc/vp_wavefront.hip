@@ -548,15 +548,17 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
 // queued shadow ray: origin-set chords first, then one walk over entered
 // primitives; lanes refill from the shadow queue as their rays end.
 __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par, uint32_t fetch_min) {
-    extern __shared__ uint32_t s_leafq[];
-    const LeafQueue lq = leaf_queue(s_leafq);
+    extern __shared__ __align__(16) uint32_t s_leafq[];
+    // the wave-shared leaf queue; its per-lane key slot holds the address of
+    // the primitive array of the lane's ray
+    const WaveQueue wq = wave_queue(s_leafq);
+    const uint32_t lane = __lane_id();
     unsigned int* qc = qcs(w, par);
     const uint32_t n = qc[1];
     const uint32_t P = w.P;
-    bool have = false, full = false;
+    bool have = false;
     uint32_t e = 0, node = 0, nn = 0;
-    uint64_t acc = 0;
-    f3 o = mk(0.0f), d = mk(0.0f), inv = mk(0.0f), oi = mk(0.0f);
+    f3 inv = mk(0.0f), oi = mk(0.0f);
     SceneView sc{};
     while (true) {
         const uint32_t idx = wave_fetch(&qc[5], !have, n, fetch_min);
@@ -565,39 +567,107 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
             const uint32_t i = ent >> 2, k = ent & 3u;
             e = k * P + i;
             const float4 po = w.po[e], pd = w.pd[e];
-            o = f3{po.x, po.y, po.z};
-            d = f3{pd.x, pd.y, pd.z};
+            const f3 o = f3{po.x, po.y, po.z};
+            const f3 d = f3{pd.x, pd.y, pd.z};
             const uint32_t bits = __float_as_uint(po.w);
             const uint32_t set = bits & 1u, count = bits >> 1;
             sc = jobs[__float_as_uint(w.dj[i].w)].scene;
             const uint32_t* act = w.slab + (static_cast<size_t>(i) * kSlabBuffers + set) * w.cap;
-            acc = 0;
-            full = false;
+            uint64_t acc = 0;
+            bool full = false;
             for (uint32_t c = 0; c < count && !full; ++c) {
                 const PrimRecord pr = load_prim(sc.prims, act[c]);
                 const float te = exit_from_inside(pr, o, d);
                 if (te > 0.0f && te < kTMax) full = tau_add(acc, optical_depth(pr, o, d, 0.0f, te));
             }
+            wq.ro[lane] = make_float4(o.x, o.y, o.z, 0.0f);
+            wq.rd[lane] = make_float4(d.x, d.y, d.z, 0.0f);
+            wq.acc[lane] = full ? kTauLimitFixed : acc;
+            wq.key[lane] = reinterpret_cast<unsigned long long>(sc.prims);
             inv = safe_inv(d);
             oi = o * inv;
-            node = 0;
+            node = full ? sc.n_nodes : 0u;
             nn = sc.n_nodes;
             have = true;
         }
         if (__ballot(have) == 0) break;
-        uint32_t cnt = 0;
-        walk_steps(sc, have && !full, node, nn, inv, oi, kTMax, lq, cnt);
-        for (uint32_t k = 0; k < cnt && !full; ++k) {
-            const PrimRecord pr = load_prim(sc.prims, lq.at(k));
-            float wl2;
-            const float tj = entry_t(pr, o, d, &wl2);
-            if (tj < 0.0f) continue;
-            const float te = exit_from_entry(pr, o, d, tj, wl2);
-            if (te > tj && te < kTMax) full = tau_add(acc, optical_depth(pr, o, d, tj, te));
+        // node phase: crossed leaves go to the wave queue
+        uint32_t count = 0;
+        while (true) {
+            const bool step = have && node < nn;
+            const unsigned long long sm = __builtin_amdgcn_ballot_w64(step);
+            if (sm == 0 || count > kWaveQ - 64u) break;
+            if (count >= kWaveQ / 4u && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
+            bool push = false;
+            uint32_t prim = 0;
+            if (step) {
+                const float4 a = as_global(sc.nodes)[2 * node];
+                const float4 c = as_global(sc.nodes)[2 * node + 1];
+                const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(c.x, inv.x, -oi.x);
+                const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(c.y, inv.y, -oi.y);
+                const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(c.z, inv.z, -oi.z);
+                const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+                const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), kTMax));
+                const uint32_t skip = __float_as_uint(a.w);
+                prim = __float_as_uint(c.w);
+                const bool hit = tnear <= tfar, leaf_node = prim != kInnerNode;
+                push = hit && leaf_node;
+                node = (hit && !leaf_node) ? node + 1 : skip;
+            }
+            const unsigned long long pm = __builtin_amdgcn_ballot_w64(push);
+            if (push) {
+                const uint32_t pos = count + lanes_below(pm);
+                wq.prim[pos] = prim;
+                wq.owner[pos] = static_cast<uint8_t>(lane);
+            }
+            count += static_cast<uint32_t>(__popcll(pm));
         }
-        if (have && (full || node >= nn)) {
-            w.pd[e].w = full ? 0.0f : tau_to_T(acc);
-            have = false;
+        // leaf phase, shared by all lanes: entry tests, then the chords of
+        // the entered leaves (fixed-point sums, order free)
+        const unsigned long long am = __builtin_amdgcn_ballot_w64(true);
+        const uint32_t nact = static_cast<uint32_t>(__popcll(am));
+        uint32_t nhit = 0;
+        for (uint32_t base = 0; base < count; base += nact) {
+            const uint32_t g = base + lanes_below(am);
+            bool entered = false;
+            uint32_t own = 0, prim = 0;
+            float tj = -1.0f;
+            if (g < count) {
+                own = wq.owner[g];
+                prim = wq.prim[g];
+                const float4 o4 = wq.ro[own], d4 = wq.rd[own];
+                const PrimRecord* prims = reinterpret_cast<const PrimRecord*>(wq.key[own]);
+                const PrimRecord pr = load_prim(prims, prim);
+                float wl2;
+                tj = entry_t(pr, f3{o4.x, o4.y, o4.z}, f3{d4.x, d4.y, d4.z}, &wl2);
+                entered = tj >= 0.0f;
+            }
+            const unsigned long long hm = __builtin_amdgcn_ballot_w64(entered);
+            if (entered) {
+                const uint32_t h = nhit + lanes_below(hm);
+                wq.prim[h] = prim;
+                wq.owner[h] = static_cast<uint8_t>(own);
+                wq.tj[h] = tj;
+            }
+            nhit += static_cast<uint32_t>(__popcll(hm));
+        }
+        for (uint32_t g = lanes_below(am); g < nhit; g += nact) {
+            const uint32_t own = wq.owner[g];
+            const float4 o4 = wq.ro[own], d4 = wq.rd[own];
+            const f3 ro = f3{o4.x, o4.y, o4.z}, rd = f3{d4.x, d4.y, d4.z};
+            const PrimRecord* prims = reinterpret_cast<const PrimRecord*>(wq.key[own]);
+            const PrimRecord pr = load_prim(prims, wq.prim[g]);
+            const float tj = wq.tj[g];
+            const float te = exit_from_entry(pr, ro, rd, tj, length2(to_local_dir(pr, rd)));
+            if (te > tj && te < kTMax) atomicAdd(&wq.acc[own], tau_fixed(optical_depth(pr, ro, rd, tj, te)));
+        }
+        if (have) {
+            const unsigned long long acc = wq.acc[lane];
+            const bool full = acc >= kTauLimitFixed;
+            if (full || node >= nn) {
+                w.pd[e].w = full ? 0.0f : tau_to_T(acc);
+                have = false;
+            }
         }
     }
 }
