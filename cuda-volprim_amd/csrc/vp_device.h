@@ -149,13 +149,15 @@ struct WaveQueue {
     float4* ro;                    // [64] ray origin of each lane
     float4* rd;                    // [64] ray direction of each lane
     unsigned long long* acc;       // [64] fixed-point optical depth of each lane's ray
-    unsigned long long* key;       // [64] path key of each lane (argmin walks), or its ray's primitive array
+    unsigned long long* key;       // [64] path key of each lane (argmin walks)
+    const PrimRecord** prims;      // [64] primitive array of each lane's scene (lanes of one
+                                   //      wavefront may run jobs over different scenes)
     uint32_t* ts;                  // [64] running minimum of each lane's argmin, as float bits
     uint32_t* nc;                  // [64] candidates recorded for each lane's argmin
     uint32_t* meta;                // [64] bounce | direct-term flag << 31 (argmin walks)
     float* tj;                     // [kWaveQ] entry distances of the leaves that passed the entry test
 };
-constexpr uint32_t kWaveQueueBytes = kWaveQ * 9 + 64 * (16 + 16 + 8 + 8 + 4 + 4 + 4);
+constexpr uint32_t kWaveQueueBytes = kWaveQ * 9 + 64 * (16 + 16 + 8 + 8 + 8 + 4 + 4 + 4);
 // LDS bytes per wavefront: the larger of the two uses, in 16-byte units
 constexpr uint32_t kWaveBytes =
     ((64 * kLeafRow * 4 > kWaveQueueBytes ? 64 * kLeafRow * 4 : kWaveQueueBytes) + 15) / 16 * 16;
@@ -170,7 +172,8 @@ __device__ __forceinline__ WaveQueue wave_queue(uint32_t* lds) {
     q.rd = q.ro + 64;
     q.acc = reinterpret_cast<unsigned long long*>(q.rd + 64);
     q.key = q.acc + 64;
-    q.ts = reinterpret_cast<uint32_t*>(q.key + 64);
+    q.prims = reinterpret_cast<const PrimRecord**>(q.key + 64);
+    q.ts = reinterpret_cast<uint32_t*>(q.prims + 64);
     q.nc = q.ts + 64;
     q.meta = q.nc + 64;
     q.tj = reinterpret_cast<float*>(q.meta + 64);
@@ -473,6 +476,7 @@ __device__ __forceinline__ void transmittance_two_shared(const SceneView& s, f3 
     wq.ro[lane] = make_float4(o.x, o.y, o.z, 0.0f);
     wq.rd[lane] = make_float4(d.x, d.y, d.z, 0.0f);
     wq.acc[lane] = r == 0u ? acc_a : acc_b;
+    wq.prims[lane] = s.prims;
     f3 inv = safe_inv(d), oi = o * inv;
     uint32_t i = 0;
     const uint32_t nn = s.n_nodes;
@@ -528,7 +532,7 @@ __device__ __forceinline__ void transmittance_two_shared(const SceneView& s, f3 
                 own = wq.owner[g];
                 prim = wq.prim[g];
                 const float4 o4 = wq.ro[own], d4 = wq.rd[own];
-                const PrimRecord pr = load_prim(s.prims, prim);
+                const PrimRecord pr = load_prim(wq.prims[own], prim);
                 float wl2;
                 tj = entry_t(pr, f3{o4.x, o4.y, o4.z}, f3{d4.x, d4.y, d4.z}, &wl2);
                 entered = tj >= 0.0f;
@@ -546,7 +550,7 @@ __device__ __forceinline__ void transmittance_two_shared(const SceneView& s, f3 
             const uint32_t own = wq.owner[g];
             const float4 o4 = wq.ro[own], d4 = wq.rd[own];
             const f3 ro = f3{o4.x, o4.y, o4.z}, rd = f3{d4.x, d4.y, d4.z};
-            const PrimRecord pr = load_prim(s.prims, wq.prim[g]);
+            const PrimRecord pr = load_prim(wq.prims[own], wq.prim[g]);
             const float tj = wq.tj[g];
             const float te = exit_from_entry(pr, ro, rd, tj, length2(to_local_dir(pr, rd)));
             if (te > tj && te < kTMax) atomicAdd(&wq.acc[own], tau_fixed(optical_depth(pr, ro, rd, tj, te)));
@@ -556,7 +560,7 @@ __device__ __forceinline__ void transmittance_two_shared(const SceneView& s, f3 
             const uint32_t own = wq.owner[g];
             const float4 o4 = wq.ro[own], d4 = wq.rd[own];
             const f3 ro = f3{o4.x, o4.y, o4.z}, rd = f3{d4.x, d4.y, d4.z};
-            const PrimRecord pr = load_prim(s.prims, wq.prim[g]);
+            const PrimRecord pr = load_prim(wq.prims[own], wq.prim[g]);
             float wl2;
             const float tj = entry_t(pr, ro, rd, &wl2);
             if (tj < 0.0f) continue;

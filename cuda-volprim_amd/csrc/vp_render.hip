@@ -58,6 +58,7 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
     wq.ro[lane] = make_float4(o.x, o.y, o.z, 0.0f);
     wq.rd[lane] = make_float4(d.x, d.y, d.z, 0.0f);
     wq.key[lane] = key;
+    wq.prims[lane] = sc.prims;
     wq.meta[lane] = b | (direct ? 0x80000000u : 0u);
     wq.acc[lane] = dfull ? kTauLimitFixed : dacc;
     wq.ts[lane] = ts_origin == 0.0f ? 0u : __float_as_uint(ts_origin);
@@ -106,7 +107,7 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
             const uint32_t prim = wq.prim[g];
             const float4 o4 = wq.ro[own], d4 = wq.rd[own];
             const f3 ro = f3{o4.x, o4.y, o4.z}, rd = f3{d4.x, d4.y, d4.z};
-            const PrimRecord pr = load_prim(sc.prims, prim);
+            const PrimRecord pr = load_prim(wq.prims[own], prim);
             float wl2;
             const float tj = entry_t(pr, ro, rd, &wl2);
             if (tj < 0.0f) continue;

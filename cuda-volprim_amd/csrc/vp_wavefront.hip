@@ -549,8 +549,6 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
 // primitives; lanes refill from the shadow queue as their rays end.
 __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par, uint32_t fetch_min) {
     extern __shared__ __align__(16) uint32_t s_leafq[];
-    // the wave-shared leaf queue; its per-lane key slot holds the address of
-    // the primitive array of the lane's ray
     const WaveQueue wq = wave_queue(s_leafq);
     const uint32_t lane = __lane_id();
     unsigned int* qc = qcs(w, par);
@@ -583,7 +581,7 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
             wq.ro[lane] = make_float4(o.x, o.y, o.z, 0.0f);
             wq.rd[lane] = make_float4(d.x, d.y, d.z, 0.0f);
             wq.acc[lane] = full ? kTauLimitFixed : acc;
-            wq.key[lane] = reinterpret_cast<unsigned long long>(sc.prims);
+            wq.prims[lane] = sc.prims;
             inv = safe_inv(d);
             oi = o * inv;
             node = full ? sc.n_nodes : 0u;
@@ -636,8 +634,7 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
                 own = wq.owner[g];
                 prim = wq.prim[g];
                 const float4 o4 = wq.ro[own], d4 = wq.rd[own];
-                const PrimRecord* prims = reinterpret_cast<const PrimRecord*>(wq.key[own]);
-                const PrimRecord pr = load_prim(prims, prim);
+                const PrimRecord pr = load_prim(wq.prims[own], prim);
                 float wl2;
                 tj = entry_t(pr, f3{o4.x, o4.y, o4.z}, f3{d4.x, d4.y, d4.z}, &wl2);
                 entered = tj >= 0.0f;
@@ -655,8 +652,7 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
             const uint32_t own = wq.owner[g];
             const float4 o4 = wq.ro[own], d4 = wq.rd[own];
             const f3 ro = f3{o4.x, o4.y, o4.z}, rd = f3{d4.x, d4.y, d4.z};
-            const PrimRecord* prims = reinterpret_cast<const PrimRecord*>(wq.key[own]);
-            const PrimRecord pr = load_prim(prims, wq.prim[g]);
+            const PrimRecord pr = load_prim(wq.prims[own], wq.prim[g]);
             const float tj = wq.tj[g];
             const float te = exit_from_entry(pr, ro, rd, tj, length2(to_local_dir(pr, rd)));
             if (te > tj && te < kTMax) atomicAdd(&wq.acc[own], tau_fixed(optical_depth(pr, ro, rd, tj, te)));

@@ -43,6 +43,30 @@ def test_batched_jobs_equal_individual_renders(ctx):
         o.close()
 
 
+def test_batched_jobs_mixing_scenes_within_wavefronts(ctx):
+    # 30 tiny jobs (48 paths each) over three scenes of different sizes, so
+    # most wavefronts run paths of two scenes at once and share leaf work
+    # between them
+    print("[case] 30 tiny jobs over 3 scenes, mixed within wavefronts", flush=True)
+    env = ctx.env(scenes.sky_env(64, 32, sun_radius_deg=4.0, sun_scale=50.0))
+    scs = [ctx.scene(scenes.random_scene(60, 41, albedo=0.9)), ctx.scene(scenes.random_scene(7, 42, smax=0.5)),
+           ctx.scene(scenes.c0_single_gaussian(albedo=0.9))]
+    jobs = []
+    for i in range(30):
+        cam = volprim.camera_perspective(**scenes.default_camera_args(4, 4, view=["negz", "posx", "diag"][i % 3]))
+        p = volprim.default_params(width=4, height=4, spp=3, seed=200 + i, phase_g=0.3 * (i % 2),
+                                   use_ris=int(i % 5 == 0), max_depth=24)
+        jobs.append((scs[i % 3], env, cam, p))
+    alone = [ctx.render(*j)[0] for j in jobs]
+    wf = volprim.LaunchConfig(256, 0, 0, 0, 1, volprim.PIPELINE_WAVEFRONT, 4096)
+    for launch in (None, wf):
+        batched, _ = ctx.render_jobs(jobs, launch=launch)
+        for k, (a, b) in enumerate(zip(alone, batched)):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), f"job {k} ({'wavefront' if launch else 'megakernel'})"
+    for o in scs + [env]:
+        o.close()
+
+
 def test_server_result_same_alone_and_in_mix(ctx):
     print("[case] server alone vs mix", flush=True)
     jobs, scs, envs = _mix(ctx)
