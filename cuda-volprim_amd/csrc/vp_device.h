@@ -129,7 +129,7 @@ constexpr uint32_t kLeafRow = kLeafQ + 1;
 // LDS atomics whose outcome does not depend on order (fixed-point sums, and
 // minima).
 #ifndef VP_WAVE_Q
-#define VP_WAVE_Q 256
+#define VP_WAVE_Q 224
 #endif
 constexpr uint32_t kWaveQ = VP_WAVE_Q;
 // A node phase of the shared-queue walks ends early, to share out the queued
@@ -143,21 +143,30 @@ constexpr uint32_t kWaveQ = VP_WAVE_Q;
 #ifndef VP_SHARED_STEP_MIN
 #define VP_SHARED_STEP_MIN VP_WALK_MIN
 #endif
+// Rays are numbered per wavefront: ray k of a one-ray-per-lane walk belongs
+// to lane k; the MIS shadow rays of lane l are rays 2l and 2l + 1.
 struct WaveQueue {
-    uint32_t* prim;                // [kWaveQ]
-    uint8_t* owner;                // [kWaveQ]
-    float4* ro;                    // [64] ray origin of each lane
-    float4* rd;                    // [64] ray direction of each lane
-    unsigned long long* acc;       // [64] fixed-point optical depth of each lane's ray
-    unsigned long long* key;       // [64] path key of each lane (argmin walks)
+    uint32_t* prim;                // [kWaveQ] queued leaf
+    float* tj;                     // [kWaveQ] entry distances of the leaves that passed the entry test
+    uint8_t* owner;                // [kWaveQ] the ray each leaf belongs to
+    float* ro;                     // [64 * 3] ray origin of each lane
     const PrimRecord** prims;      // [64] primitive array of each lane's scene (lanes of one
                                    //      wavefront may run jobs over different scenes)
+    float* rd;                     // [128 * 3] ray directions
+    unsigned long long* acc;       // [128] fixed-point optical depth of each ray
+    // MIS shadow-ray pool
+    const float4** nodes;          // [64] node array of each lane's scene
+    uint32_t* nn;                  // [64] its node count
+    uint8_t* list;                 // [128] the rays still to be walked
+    // argmin walks (one ray per lane); they lie over rd[64..128) and acc[64..128)
+    unsigned long long* key;       // [64] path key of each lane
     uint32_t* ts;                  // [64] running minimum of each lane's argmin, as float bits
     uint32_t* nc;                  // [64] candidates recorded for each lane's argmin
-    uint32_t* meta;                // [64] bounce | direct-term flag << 31 (argmin walks)
-    float* tj;                     // [kWaveQ] entry distances of the leaves that passed the entry test
+    uint32_t* meta;                // [64] bounce | direct-term flag << 31
 };
-constexpr uint32_t kWaveQueueBytes = kWaveQ * 9 + 64 * (16 + 16 + 8 + 8 + 8 + 4 + 4 + 4);
+constexpr uint32_t kWaveQueueBytes = kWaveQ * 9 + 64 * 12 + 64 * 8 + 128 * 12 + 128 * 8 + 64 * 8 + 64 * 4 + 128;
+static_assert(kWaveQ % 8 == 0, "queue arrays keep the per-lane arrays 8-byte aligned");
+static_assert(64 * (8 + 4) <= 64 * 12 && 64 * (4 + 4) <= 64 * 8, "argmin arrays fit over the upper rays");
 // LDS bytes per wavefront: the larger of the two uses, in 16-byte units
 constexpr uint32_t kWaveBytes =
     ((64 * kLeafRow * 4 > kWaveQueueBytes ? 64 * kLeafRow * 4 : kWaveQueueBytes) + 15) / 16 * 16;
@@ -167,17 +176,27 @@ __device__ __forceinline__ WaveQueue wave_queue(uint32_t* lds) {
     uint8_t* base = reinterpret_cast<uint8_t*>(lds) + (threadIdx.x >> 6) * kWaveBytes;
     WaveQueue q;
     q.prim = reinterpret_cast<uint32_t*>(base);
-    q.owner = base + 4 * kWaveQ;
-    q.ro = reinterpret_cast<float4*>(base + 5 * kWaveQ);
-    q.rd = q.ro + 64;
-    q.acc = reinterpret_cast<unsigned long long*>(q.rd + 64);
-    q.key = q.acc + 64;
-    q.prims = reinterpret_cast<const PrimRecord**>(q.key + 64);
-    q.ts = reinterpret_cast<uint32_t*>(q.prims + 64);
-    q.nc = q.ts + 64;
+    q.tj = reinterpret_cast<float*>(base + 4 * kWaveQ);
+    q.owner = base + 8 * kWaveQ;
+    q.ro = reinterpret_cast<float*>(base + 9 * kWaveQ);
+    q.prims = reinterpret_cast<const PrimRecord**>(q.ro + 64 * 3);
+    q.rd = reinterpret_cast<float*>(q.prims + 64);
+    q.acc = reinterpret_cast<unsigned long long*>(q.rd + 128 * 3);
+    q.nodes = reinterpret_cast<const float4**>(q.acc + 128);
+    q.nn = reinterpret_cast<uint32_t*>(q.nodes + 64);
+    q.list = reinterpret_cast<uint8_t*>(q.nn + 64);
+    q.key = reinterpret_cast<unsigned long long*>(q.rd + 64 * 3);
+    q.ts = reinterpret_cast<uint32_t*>(q.key + 64);
+    q.nc = reinterpret_cast<uint32_t*>(q.acc + 64);
     q.meta = q.nc + 64;
-    q.tj = reinterpret_cast<float*>(q.meta + 64);
     return q;
+}
+
+__device__ __forceinline__ f3 lds_load3(const float* p) { return f3{p[0], p[1], p[2]}; }
+__device__ __forceinline__ void lds_store3(float* p, f3 v) {
+    p[0] = v.x;
+    p[1] = v.y;
+    p[2] = v.z;
 }
 
 // Node phase of the walks: a lane steps while its leaf queue has room and its
@@ -446,14 +465,18 @@ __device__ __forceinline__ void transmittance_two(const SceneView& s, f3 o, f3 d
     }
 }
 
-// transmittance_two with the wave-shared leaf queue: the same rays, chords
-// and fixed-point sums (a ray is saturated once its sum reaches the tau = 88
-// limit, whatever the order its chords arrive in), so Ta and Tb equal two
-// transmittance() calls bit for bit.  Every lane of the calling wavefront
-// that is active here takes part in the leaf phases.
-__device__ __forceinline__ void transmittance_two_shared(const SceneView& s, f3 o, f3 da, f3 db,
-                                                         const uint32_t* act, uint32_t na, uint32_t stride,
-                                                         const WaveQueue& wq, float& Ta, float& Tb) {
+// The MIS shadow rays of the calling lanes, with the wave-shared leaf queue
+// and a wave-wide ray pool: every lane starts on one ray of the wavefront's
+// list (all rays a, then all rays b) and, when its ray is done, takes the
+// next one from the list, whichever lane the ray belongs to.  The node
+// phases then stay full however the walk lengths are spread, until the list
+// runs out.  Each ray's chords are summed in fixed point in LDS (a ray is
+// saturated once its sum reaches the tau = 88 limit, in whatever order the
+// chords arrive), so Ta and Tb equal two transmittance() calls bit for bit.
+// Every lane of the calling wavefront that is active here takes part.
+__device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 da, f3 db, const uint32_t* act,
+                                                   uint32_t na, uint32_t stride, const WaveQueue& wq, float& Ta,
+                                                   float& Tb) {
     uint64_t acc_a = 0, acc_b = 0;
     bool full_a = false, full_b = false;
     for (uint32_t k = 0; k < na && !(full_a && full_b); ++k) {
@@ -467,35 +490,60 @@ __device__ __forceinline__ void transmittance_two_shared(const SceneView& s, f3 
             if (te > 0.0f && te < kTMax) full_b = tau_add(acc_b, optical_depth(pr, o, db, 0.0f, te));
         }
     }
-    Ta = 0.0f;
-    Tb = 0.0f;
     const uint32_t lane = __lane_id();
-    // current ray r: 0 = a, 1 = b, 2 = both done; a saturated ray is done
-    uint32_t r = full_a ? (full_b ? 2u : 1u) : 0u;
-    f3 d = r == 0u ? da : db;
-    wq.ro[lane] = make_float4(o.x, o.y, o.z, 0.0f);
-    wq.rd[lane] = make_float4(d.x, d.y, d.z, 0.0f);
-    wq.acc[lane] = r == 0u ? acc_a : acc_b;
+    lds_store3(wq.ro + 3 * lane, o);
     wq.prims[lane] = s.prims;
-    f3 inv = safe_inv(d), oi = o * inv;
-    uint32_t i = 0;
-    const uint32_t nn = s.n_nodes;
-    while (__ballot(r < 2u) != 0) {
+    wq.nodes[lane] = s.nodes;
+    wq.nn[lane] = s.n_nodes;
+    lds_store3(wq.rd + 3 * (2 * lane), da);
+    lds_store3(wq.rd + 3 * (2 * lane + 1), db);
+    wq.acc[2 * lane] = full_a ? kTauLimitFixed : acc_a;
+    wq.acc[2 * lane + 1] = full_b ? kTauLimitFixed : acc_b;
+    // the rays still to walk: every ray a, then every ray b (a saturated ray
+    // is done already)
+    const unsigned long long am = __builtin_amdgcn_ballot_w64(true);
+    const unsigned long long pa = __builtin_amdgcn_ballot_w64(!full_a);
+    const unsigned long long pb = __builtin_amdgcn_ballot_w64(!full_b);
+    const uint32_t npa = static_cast<uint32_t>(__popcll(pa));
+    const uint32_t nr = npa + static_cast<uint32_t>(__popcll(pb));
+    if (!full_a) wq.list[lanes_below(pa)] = static_cast<uint8_t>(2 * lane);
+    if (!full_b) wq.list[npa + lanes_below(pb)] = static_cast<uint8_t>(2 * lane + 1);
+    const uint32_t nact = static_cast<uint32_t>(__popcll(am));
+    uint32_t next = nact < nr ? nact : nr;  // list entries handed out
+    constexpr uint32_t kNoRay = 0xffu;
+    // the ray this lane walks, and its walk state
+    const uint32_t rank = lanes_below(am);
+    uint32_t ray = rank < nr ? wq.list[rank] : kNoRay;
+    const float4* nodes = nullptr;
+    uint32_t i = 0, nn = 0;
+    f3 inv = mk(0.0f), oi = mk(0.0f);
+    auto start = [&]() {
+        const uint32_t own = ray >> 1;
+        const f3 ro = lds_load3(wq.ro + 3 * own), rd = lds_load3(wq.rd + 3 * ray);
+        inv = safe_inv(rd);
+        oi = ro * inv;
+        nodes = wq.nodes[own];
+        nn = wq.nn[own];
+        i = 0;
+    };
+    if (ray != kNoRay) start();
+    while (__builtin_amdgcn_ballot_w64(ray != kNoRay) != 0) {
         // node phase: every lane with a ray steps; crossed leaves are queued
         // for the wavefront.  It ends when no lane steps, when the queue
         // might not take one more step of every lane, or, once a quarter of
-        // the queue is filled, when fewer than VP_WALK_MIN lanes still step.
+        // the queue is filled, when fewer than VP_SHARED_STEP_MIN lanes
+        // still step.
         uint32_t count = 0;
         while (true) {
-            const bool step = r < 2u && i < nn;
+            const bool step = ray != kNoRay && i < nn;
             const unsigned long long sm = __builtin_amdgcn_ballot_w64(step);
             if (sm == 0 || count > kWaveQ - 64u) break;
             if (count >= kWaveQ / 4u && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
             bool push = false;
             uint32_t prim = 0;
             if (step) {
-                const float4 a = as_global(s.nodes)[2 * i];
-                const float4 b = as_global(s.nodes)[2 * i + 1];
+                const float4 a = as_global(nodes)[2 * i];
+                const float4 b = as_global(nodes)[2 * i + 1];
                 const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(b.x, inv.x, -oi.x);
                 const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(b.y, inv.y, -oi.y);
                 const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(b.z, inv.z, -oi.z);
@@ -511,30 +559,25 @@ __device__ __forceinline__ void transmittance_two_shared(const SceneView& s, f3 
             if (push) {
                 const uint32_t pos = count + lanes_below(pm);
                 wq.prim[pos] = prim;
-                wq.owner[pos] = static_cast<uint8_t>(lane);
+                wq.owner[pos] = static_cast<uint8_t>(ray);
             }
             count += static_cast<uint32_t>(__popcll(pm));
         }
-        // leaf phase: the active lanes share the queued leaves
-        const unsigned long long am = __builtin_amdgcn_ballot_w64(true);
-        const uint32_t nact = static_cast<uint32_t>(__popcll(am));
-#if VP_SHADOW_TWO_STAGE
-        // entry tests first; the leaves that are entered move down to the
-        // front of the queue (an entry is only overwritten after it has been
-        // read), then their chords are integrated at full width
+        // leaf phase, shared by all lanes: entry tests; the entered leaves
+        // move down to the front of the queue (an entry is only overwritten
+        // after it has been read); then their chords
         uint32_t nhit = 0;
         for (uint32_t base = 0; base < count; base += nact) {
-            const uint32_t g = base + lanes_below(am);
+            const uint32_t g = base + rank;
             bool entered = false;
             uint32_t own = 0, prim = 0;
             float tj = -1.0f;
             if (g < count) {
                 own = wq.owner[g];
                 prim = wq.prim[g];
-                const float4 o4 = wq.ro[own], d4 = wq.rd[own];
-                const PrimRecord pr = load_prim(wq.prims[own], prim);
+                const PrimRecord pr = load_prim(wq.prims[own >> 1], prim);
                 float wl2;
-                tj = entry_t(pr, f3{o4.x, o4.y, o4.z}, f3{d4.x, d4.y, d4.z}, &wl2);
+                tj = entry_t(pr, lds_load3(wq.ro + 3 * (own >> 1)), lds_load3(wq.rd + 3 * own), &wl2);
                 entered = tj >= 0.0f;
             }
             const unsigned long long hm = __builtin_amdgcn_ballot_w64(entered);
@@ -546,50 +589,28 @@ __device__ __forceinline__ void transmittance_two_shared(const SceneView& s, f3 
             }
             nhit += static_cast<uint32_t>(__popcll(hm));
         }
-        for (uint32_t g = lanes_below(am); g < nhit; g += nact) {
+        for (uint32_t g = rank; g < nhit; g += nact) {
             const uint32_t own = wq.owner[g];
-            const float4 o4 = wq.ro[own], d4 = wq.rd[own];
-            const f3 ro = f3{o4.x, o4.y, o4.z}, rd = f3{d4.x, d4.y, d4.z};
-            const PrimRecord pr = load_prim(wq.prims[own], wq.prim[g]);
+            const f3 ro = lds_load3(wq.ro + 3 * (own >> 1)), rd = lds_load3(wq.rd + 3 * own);
+            const PrimRecord pr = load_prim(wq.prims[own >> 1], wq.prim[g]);
             const float tj = wq.tj[g];
             const float te = exit_from_entry(pr, ro, rd, tj, length2(to_local_dir(pr, rd)));
             if (te > tj && te < kTMax) atomicAdd(&wq.acc[own], tau_fixed(optical_depth(pr, ro, rd, tj, te)));
         }
-#else
-        for (uint32_t g = lanes_below(am); g < count; g += nact) {
-            const uint32_t own = wq.owner[g];
-            const float4 o4 = wq.ro[own], d4 = wq.rd[own];
-            const f3 ro = f3{o4.x, o4.y, o4.z}, rd = f3{d4.x, d4.y, d4.z};
-            const PrimRecord pr = load_prim(wq.prims[own], wq.prim[g]);
-            float wl2;
-            const float tj = entry_t(pr, ro, rd, &wl2);
-            if (tj < 0.0f) continue;
-            const float te = exit_from_entry(pr, ro, rd, tj, wl2);
-            if (te > tj && te < kTMax) atomicAdd(&wq.acc[own], tau_fixed(optical_depth(pr, ro, rd, tj, te)));
+        // lanes whose ray is done (walked to the end, or saturated) take the
+        // next ray of the list
+        const bool fin = ray != kNoRay && (i >= nn || wq.acc[ray] >= kTauLimitFixed);
+        const unsigned long long fm = __builtin_amdgcn_ballot_w64(fin);
+        if (fin) {
+            const uint32_t k = next + lanes_below(fm);
+            ray = k < nr ? wq.list[k] : kNoRay;
+            if (ray != kNoRay) start();
         }
-#endif
-        // each lane looks at its own ray
-        if (r < 2u) {
-            const unsigned long long acc = wq.acc[lane];
-            const bool full = acc >= kTauLimitFixed;
-            if (full || i >= nn) {
-                const float T = full ? 0.0f : tau_to_T(acc);
-                if (r == 0u) {
-                    Ta = T;
-                    r = full_b ? 2u : 1u;
-                    d = db;
-                    wq.rd[lane] = make_float4(d.x, d.y, d.z, 0.0f);
-                    wq.acc[lane] = acc_b;
-                    inv = safe_inv(d);
-                    oi = o * inv;
-                    i = 0;
-                } else {
-                    Tb = T;
-                    r = 2u;
-                }
-            }
-        }
+        next += static_cast<uint32_t>(__popcll(fm));
     }
+    const unsigned long long ta = wq.acc[2 * lane], tb = wq.acc[2 * lane + 1];
+    Ta = ta >= kTauLimitFixed ? 0.0f : tau_to_T(ta);
+    Tb = tb >= kTauLimitFixed ? 0.0f : tau_to_T(tb);
 }
 
 __device__ __forceinline__ EnvView env_view(const EnvDev& e) {

@@ -55,8 +55,8 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
                                                    uint32_t* __restrict__ slab, uint32_t cap, const WaveQueue& wq,
                                                    float& ts, uint32_t& nc, float& Tdirect) {
     const uint32_t lane = __lane_id();
-    wq.ro[lane] = make_float4(o.x, o.y, o.z, 0.0f);
-    wq.rd[lane] = make_float4(d.x, d.y, d.z, 0.0f);
+    lds_store3(wq.ro + 3 * lane, o);
+    lds_store3(wq.rd + 3 * lane, d);
     wq.key[lane] = key;
     wq.prims[lane] = sc.prims;
     wq.meta[lane] = b | (direct ? 0x80000000u : 0u);
@@ -105,8 +105,7 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
         for (uint32_t g = lanes_below(am); g < count; g += nact) {
             const uint32_t own = wq.owner[g];
             const uint32_t prim = wq.prim[g];
-            const float4 o4 = wq.ro[own], d4 = wq.rd[own];
-            const f3 ro = f3{o4.x, o4.y, o4.z}, rd = f3{d4.x, d4.y, d4.z};
+            const f3 ro = lds_load3(wq.ro + 3 * own), rd = lds_load3(wq.rd + 3 * own);
             const PrimRecord pr = load_prim(wq.prims[own], prim);
             float wl2;
             const float tj = entry_t(pr, ro, rd, &wl2);
@@ -472,7 +471,7 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                     Ta = transmittance(sc, x, awo, act, nn, stride, lq, ev);
                     Tb = transmittance(sc, x, bwo, act, nn, stride, lq, ev);
                 } else {
-                    transmittance_two_shared(sc, x, awo, bwo, act, nn, stride, wq, Ta, Tb);
+                    transmittance_pool(sc, x, awo, bwo, act, nn, stride, wq, Ta, Tb);
                 }
                 st.L = st.L + ((base * enva) * Ta) * wa;
                 st.L = st.L + (((((base * envb) * Tb) * wb) * ph_b) * rcpf(bpdf));

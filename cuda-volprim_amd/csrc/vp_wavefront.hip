@@ -578,8 +578,8 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
                 const float te = exit_from_inside(pr, o, d);
                 if (te > 0.0f && te < kTMax) full = tau_add(acc, optical_depth(pr, o, d, 0.0f, te));
             }
-            wq.ro[lane] = make_float4(o.x, o.y, o.z, 0.0f);
-            wq.rd[lane] = make_float4(d.x, d.y, d.z, 0.0f);
+            lds_store3(wq.ro + 3 * lane, o);
+            lds_store3(wq.rd + 3 * lane, d);
             wq.acc[lane] = full ? kTauLimitFixed : acc;
             wq.prims[lane] = sc.prims;
             inv = safe_inv(d);
@@ -633,10 +633,9 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
             if (g < count) {
                 own = wq.owner[g];
                 prim = wq.prim[g];
-                const float4 o4 = wq.ro[own], d4 = wq.rd[own];
                 const PrimRecord pr = load_prim(wq.prims[own], prim);
                 float wl2;
-                tj = entry_t(pr, f3{o4.x, o4.y, o4.z}, f3{d4.x, d4.y, d4.z}, &wl2);
+                tj = entry_t(pr, lds_load3(wq.ro + 3 * own), lds_load3(wq.rd + 3 * own), &wl2);
                 entered = tj >= 0.0f;
             }
             const unsigned long long hm = __builtin_amdgcn_ballot_w64(entered);
@@ -650,8 +649,7 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
         }
         for (uint32_t g = lanes_below(am); g < nhit; g += nact) {
             const uint32_t own = wq.owner[g];
-            const float4 o4 = wq.ro[own], d4 = wq.rd[own];
-            const f3 ro = f3{o4.x, o4.y, o4.z}, rd = f3{d4.x, d4.y, d4.z};
+            const f3 ro = lds_load3(wq.ro + 3 * own), rd = lds_load3(wq.rd + 3 * own);
             const PrimRecord pr = load_prim(wq.prims[own], wq.prim[g]);
             const float tj = wq.tj[g];
             const float te = exit_from_entry(pr, ro, rd, tj, length2(to_local_dir(pr, rd)));
