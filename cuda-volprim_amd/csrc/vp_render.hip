@@ -1,20 +1,25 @@
 // vp_render.hip — the render path on gfx950.
 //
 // One persistent megakernel traces camera paths; each 64-lane wavefront
-// claims new paths with one atomic per wave (ballot + popcount rank) as soon
-// as any of its lanes finishes, so lanes stay busy across uneven path
+// claims new paths with one atomic per wave (ballot + popcount rank) once
+// enough of its lanes are idle, so lanes stay busy across uneven path
 // lengths.  Per bounce a lane
 //   1. takes the argmin of closed-form per-primitive free flights over the
 //      primitives containing its origin and, in one stackless walk of the
 //      BVH, over every primitive the ray enters (no hit buffer: each
 //      primitive's uniform is a counter hash, so the minimum can be taken in
-//      any order and the walk prunes boxes beyond the running minimum);
-//   2. rebuilds the active set at the scatter point with one BVH point query,
-//      applying the reference's per-primitive exit tests;
+//      any order and the walk prunes boxes beyond the running minimum); at
+//      bounce 0 the same walk integrates the analytic direct term;
+//   2. rebuilds the active set at the scatter point from the primitives the
+//      argmin walk entered, applying the reference's per-primitive exit
+//      tests;
 //   3. evaluates albedo, samples the HG phase, and runs NEE (balance-MIS of
 //      phase and environment sampling, or product RIS) with shadow rays whose
-//      optical depth is accumulated in fixed point during a single walk;
+//      optical depth is accumulated in fixed point;
 //   4. applies Russian roulette and the throughput guards.
+// The argmin and MIS shadow walks share one leaf queue per wavefront: every
+// lane walks a ray, and the leaves all rays cross are evaluated by whichever
+// lanes are free (see WaveQueue).
 // Each path writes its radiance to a per-sample slot; a second kernel folds
 // slots into the per-pixel Welford mean in sample order, so the image does
 // not depend on which lane ran which path or on what else shared the launch.
