@@ -156,7 +156,8 @@ struct WaveQueue {
     unsigned long long* acc;       // [128] fixed-point optical depth of each ray
     // MIS shadow-ray pool
     const float4** nodes;          // [64] node array of each lane's scene
-    uint32_t* nn;                  // [64] its node count
+    uint32_t* nn;                  // [64] its node count (or, in the wavefront argmin
+                                   //      kernel, the pool slot of each lane's path)
     uint8_t* list;                 // [128] the rays still to be walked
     // argmin walks (one ray per lane); they lie over rd[64..128) and acc[64..128)
     unsigned long long* key;       // [64] path key of each lane

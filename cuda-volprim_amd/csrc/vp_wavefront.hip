@@ -215,18 +215,6 @@ __global__ void VP_WF_LB k_wf_regen(const JobDesc* __restrict__ jobs, const uint
     if (overflow) atomicAdd(&counters[kCntActiveOverflow], static_cast<unsigned long long>(overflow));
 }
 
-// Node steps of one lane's walk over [0, tmax] until its leaf queue is full
-// or the walk ends.
-__device__ __forceinline__ void walk_steps(const SceneView& sc, bool active, uint32_t& node, uint32_t nn, f3 inv,
-                                           f3 oi, float tmax, const LeafQueue& lq, uint32_t& cnt) {
-    while (true) {
-        const bool step = active && node < nn && cnt < kLeafQ;
-        if (!node_phase(step, cnt)) break;
-        if (!step) continue;
-        ray_node_step(sc, inv, oi, tmax, node, cnt, lq);
-    }
-}
-
 // Work fetch for the walk kernels: every idle lane of a wavefront takes the
 // next queue entry (one atomic per wavefront), so lanes whose walk ended
 // start another one instead of idling until the wavefront's longest walk is
@@ -253,38 +241,39 @@ __device__ __forceinline__ uint32_t wave_fetch(unsigned int* head, bool want, ui
 }
 
 // Argmin free flight (sampling.cuh:350-415) for every live path: origin-set
-// candidates first, then one pruned walk over entered primitives with the
-// leaf queue; lanes refill from the live queue as their walks end.
+// candidates first, then one pruned walk over entered primitives; lanes
+// refill from the live queue as their walks end.  The leaves all lanes' walks
+// cross go to the wave-shared queue and are evaluated by whichever lanes are
+// free: each path's running minimum is an LDS atomic minimum over float bits
+// (every candidate is >= 0) and its rebuild candidates go through an LDS
+// counter, so the results do not depend on which lane handles which leaf.
 __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par, uint32_t fetch_min) {
-    extern __shared__ uint32_t s_leafq[];
-    const LeafQueue lq = leaf_queue(s_leafq);
+    extern __shared__ __align__(16) uint32_t s_leafq[];
+    const WaveQueue wq = wave_queue(s_leafq);
+    uint32_t* slot = wq.nn;  // the pool slot of each lane's path
+    const uint32_t lane = __lane_id();
     unsigned int* qc = qcs(w, par);
     const uint32_t n = qc[0];
-    const uint32_t P = w.P;
     bool have = false;
-    uint32_t i = 0, b = 0, node = 0, nn = 0, nc = 0;
-    uint32_t* cnd = nullptr;
-    uint64_t key = 0;
-    f3 o = mk(0.0f), d = mk(0.0f), inv = mk(0.0f), oi = mk(0.0f);
-    float ts = kTMax;
+    uint32_t i = 0, node = 0, nn = 0;
+    float ts_origin = kTMax, tmax = kTMax;
+    f3 inv = mk(0.0f), oi = mk(0.0f);
     SceneView sc{};
     while (true) {
         const uint32_t idx = wave_fetch(&qc[4], !have, n, fetch_min);
         if (!have && idx < n) {
             i = w.live[idx];
             const float4 ob = w.ob[i], dj = w.dj[i];
-            o = f3{ob.x, ob.y, ob.z};
-            d = f3{dj.x, dj.y, dj.z};
-            b = __float_as_uint(ob.w);
+            const f3 o = f3{ob.x, ob.y, ob.z};
+            const f3 d = f3{dj.x, dj.y, dj.z};
+            const uint32_t b = __float_as_uint(ob.w);
             sc = jobs[__float_as_uint(dj.w)].scene;
             const uint4 m = w.misc[i];
-            key = static_cast<uint64_t>(m.x) | (static_cast<uint64_t>(m.y) << 32);
+            const uint64_t key = static_cast<uint64_t>(m.x) | (static_cast<uint64_t>(m.y) << 32);
             const uint32_t na = __float_as_uint(w.lr[i].w);
             const uint32_t* act = w.slab + (static_cast<size_t>(i) * kSlabBuffers + f_cur(m.w)) * w.cap;
-            cnd = w.slab + (static_cast<size_t>(i) * kSlabBuffers + 2) * w.cap;
-            nc = 0;
             // argmin over the origin set (sampling.cuh:356-370)
-            ts = kTMax;
+            float ts = kTMax;
             for (uint32_t k = 0; k < na; ++k) {
                 const uint32_t pi = act[k];
                 const PrimRecord pr = load_prim(sc.prims, pi);
@@ -293,6 +282,16 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
                 const float t = inv_cdf(pr, o, d, tau);
                 if (t >= 0.0f && t < ts && t <= te) ts = t;
             }
+            lds_store3(wq.ro + 3 * lane, o);
+            lds_store3(wq.rd + 3 * lane, d);
+            wq.key[lane] = key;
+            wq.meta[lane] = b;
+            wq.prims[lane] = sc.prims;
+            wq.ts[lane] = ts == 0.0f ? 0u : __float_as_uint(ts);
+            wq.nc[lane] = 0u;
+            slot[lane] = i;
+            ts_origin = ts;
+            tmax = ts;
             inv = safe_inv(d);
             oi = o * inv;
             node = 0;
@@ -300,33 +299,70 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
             have = true;
         }
         if (__ballot(have) == 0) break;
-        // node steps until the lane's leaf queue is full or its walk ends
-        uint32_t cnt = 0;
-        walk_steps(sc, have, node, nn, inv, oi, ts, lq, cnt);
-        // entered primitives (sampling.cuh:372-412)
-        for (uint32_t k = 0; k < cnt; ++k) {
-            const uint32_t prim = lq.at(k);
-            const PrimRecord pr = load_prim(sc.prims, prim);
-            float wl2;
-            const float tj = entry_t(pr, o, d, &wl2);
-            if (tj < 0.0f) continue;
-            // entered before the running minimum: a rebuild candidate
-            if (tj <= ts) {
-                if (nc < w.cap) cnd[nc] = prim;
-                ++nc;
+        // node phase: crossed leaves go to the wave queue
+        uint32_t count = 0;
+        while (true) {
+            const bool step = have && node < nn;
+            const unsigned long long sm = __builtin_amdgcn_ballot_w64(step);
+            if (sm == 0 || count > kWaveQ - 64u) break;
+            if (count >= kWaveQ / 4u && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
+            bool push = false;
+            uint32_t prim = 0;
+            if (step) {
+                const float4 a = as_global(sc.nodes)[2 * node];
+                const float4 c = as_global(sc.nodes)[2 * node + 1];
+                const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(c.x, inv.x, -oi.x);
+                const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(c.y, inv.y, -oi.y);
+                const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(c.z, inv.z, -oi.z);
+                const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+                const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+                const uint32_t skip = __float_as_uint(a.w);
+                prim = __float_as_uint(c.w);
+                const bool hit = tnear <= tfar, leaf_node = prim != kInnerNode;
+                push = hit && leaf_node;
+                node = (hit && !leaf_node) ? node + 1 : skip;
             }
-            const float tau = free_flight_tau(prim_uniform(key, b, 1u, prim));
-            float t = inv_cdf_span(pr, o, d, tj, tau);
+            const unsigned long long pm = __builtin_amdgcn_ballot_w64(push);
+            if (push) {
+                const uint32_t pos = count + lanes_below(pm);
+                wq.prim[pos] = prim;
+                wq.owner[pos] = static_cast<uint8_t>(lane);
+            }
+            count += static_cast<uint32_t>(__popcll(pm));
+        }
+        // leaf phase, shared by all lanes (sampling.cuh:372-412)
+        const unsigned long long am = __builtin_amdgcn_ballot_w64(true);
+        const uint32_t nact = static_cast<uint32_t>(__popcll(am));
+        for (uint32_t g = lanes_below(am); g < count; g += nact) {
+            const uint32_t own = wq.owner[g];
+            const uint32_t prim = wq.prim[g];
+            const f3 ro = lds_load3(wq.ro + 3 * own), rd = lds_load3(wq.rd + 3 * own);
+            const PrimRecord pr = load_prim(wq.prims[own], prim);
+            float wl2;
+            const float tj = entry_t(pr, ro, rd, &wl2);
+            if (tj < 0.0f) continue;
+            const float ts_cur = __uint_as_float(wq.ts[own]);
+            // entered before the running minimum: a rebuild candidate
+            if (tj <= ts_cur) {
+                const uint32_t pos = atomicAdd(&wq.nc[own], 1u);
+                if (pos < w.cap) w.slab[(static_cast<size_t>(slot[own]) * kSlabBuffers + 2) * w.cap + pos] = prim;
+            }
+            const float tau = free_flight_tau(prim_uniform(wq.key[own], wq.meta[own], 1u, prim));
+            float t = inv_cdf_span(pr, ro, rd, tj, tau);
             if (t >= 0.0f && t < tj) t = tj;
-            if (t >= 0.0f && t < ts) {
-                const float te = exit_from_entry(pr, o, d, tj, wl2);
-                if (t <= te) ts = t;
+            if (t >= 0.0f && t < ts_cur) {
+                const float te = exit_from_entry(pr, ro, rd, tj, wl2);
+                if (t <= te) atomicMin(&wq.ts[own], __float_as_uint(t));
             }
         }
-        if (have && node >= nn) {
-            w.bt[i].w = ts;
-            w.ncand[i] = nc;
-            have = false;
+        if (have) {
+            const float wmin = __uint_as_float(wq.ts[lane]);
+            tmax = wmin;
+            if (node >= nn) {
+                w.bt[i].w = wmin < ts_origin ? wmin : ts_origin;
+                w.ncand[i] = wq.nc[lane];
+                have = false;
+            }
         }
     }
 }
