@@ -28,14 +28,6 @@
 //
 // Reference (kfernandez31/cuda-volprim): device/entry/raygen.cuh:21-243,
 // device/entry/path_stages.cuh:25-226, device/core/sampling.cuh:257-583.
-
-// Leaf-queue depth of the walk kernels here (the megakernel keeps 16): with
-// whole walks fetched per lane, deeper queues keep more lanes in the leaf
-// phase (24: +3 % on the scattering workload, though 8 walk-kernel blocks per
-// CU no longer fit in LDS).
-#ifndef VP_LEAF_Q
-#define VP_LEAF_Q 24
-#endif
 #include "vp_device.h"
 
 namespace vp {
