@@ -32,9 +32,11 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_FP32_NONPACKED_TFLOPS = 78.6
 
 # FLOP weight of each counted event: FMA = 2, add/mul = 1, each transcendental
-# or rcp/sqrt = 1, erf = 16, erfinv = 20 (derivations in DESIGN.md).
+# or rcp/sqrt = 1, erf = 16, erfinv = 20 (derivations in DESIGN.md).  A node
+# visit loads one 4-wide walk-tree record and slab-tests its 4 child boxes
+# (6 FMA each).
 FLOP_WEIGHTS = {
-    "node_visits": 12, "leaf_tests": 105, "contain": 41, "arg_active": 230, "arg_hit": 141,
+    "node_visits": 48, "leaf_tests": 105, "contain": 41, "arg_active": 230, "arg_hit": 141,
     "arg_accept": 122, "rb_active": 92, "rb_hit": 122, "sh_active": 238, "sh_hit": 268, "alb": 52,
     "scatters": 220, "paths": 35,
 }
@@ -274,7 +276,7 @@ def main():
                     help="persistent megakernel or wavefront-queue pipeline")
     ap.add_argument("--pool", type=int, default=0, help="wavefront path slots (0 = library default)")
     ap.add_argument("--bvh", choices=["lbvh", "sah"], default="sah",
-                    help="host SAH tree (default, fewer node visits) or the GPU LBVH, same layout")
+                    help="leaf order of the walk tree: host SAH tree (default) or the GPU LBVH's Morton order")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB elementwise copy kernel first (HBM counter calibration)")
     ap.add_argument("--mix", action="store_true", help="run the job mix (plain vs batched vs served) instead")

@@ -211,6 +211,7 @@ vp_status vp_scene_create_ex(vp_context* ctx, const vp_primitive* prims, uint32_
     sc->n = n;
     vp_status st = vp_build_scene_device(ctx, prims, n, sc);
     if (st == VP_OK && (flags & VP_SCENE_BVH_SAH)) st = vp_build_scene_sah(ctx, prims, n, sc);
+    if (st == VP_OK) st = vp_build_walk_tree(ctx, sc);
     if (st != VP_OK) {
         vp_scene_destroy(sc);
         return st;
@@ -224,6 +225,7 @@ vp_status vp_scene_destroy(vp_scene* sc) {
     if (sc->ctx && sc->ctx->stream) (void)hipStreamSynchronize(sc->ctx->stream);
     if (sc->prims) (void)hipFree(sc->prims);
     if (sc->nodes) (void)hipFree(sc->nodes);
+    if (sc->w4) (void)hipFree(sc->w4);
     if (sc->order) (void)hipFree(sc->order);
     if (sc->morton) (void)hipFree(sc->morton);
     delete sc;

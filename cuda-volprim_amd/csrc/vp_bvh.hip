@@ -324,6 +324,75 @@ __global__ void k_single_leaf(const float* box, float4* nodes) {
     nodes[1] = make_float4(box[3], box[4], box[5], __uint_as_float(0u));
 }
 
+// ---- walk tree (implicit 4-ary, see vp_internal.h) ----
+
+// Leaf boxes in stored order, taken from the rope tree's leaves.
+__global__ void k_w4_leaf_boxes(const float4* nodes, uint32_t nn, float* box) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const float4 a = nodes[2 * i], b = nodes[2 * i + 1];
+    const uint32_t prim = __float_as_uint(b.w);
+    if (prim == kInnerNode) return;
+    float* o = box + 6 * static_cast<size_t>(prim);
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = a.z;
+    o[3] = b.x;
+    o[4] = b.y;
+    o[5] = b.z;
+}
+
+__device__ __forceinline__ void w4_store(float4* w4, uint32_t idx, const float lo[3][4], const float hi[3][4]) {
+    float4* r = w4 + static_cast<size_t>(kW4RecordF4) * idx;
+    for (int a = 0; a < 3; ++a) {
+        r[a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+        r[3 + a] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+    }
+    r[6] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    r[7] = r[6];
+}
+
+// Last level: node k holds the leaves 4k .. 4k + 3.
+__global__ void k_w4_bottom(const float* box, uint32_t n, uint32_t levels, float4* w4) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (1u << (2u * (levels - 1u)))) return;
+    float lo[3][4], hi[3][4];
+    for (uint32_t c = 0; c < 4; ++c) {
+        const uint32_t j = 4u * k + c;
+        for (int a = 0; a < 3; ++a) {
+            lo[a][c] = j < n ? box[6 * static_cast<size_t>(j) + a] : INFINITY;
+            hi[a][c] = j < n ? box[6 * static_cast<size_t>(j) + 3 + a] : INFINITY;
+        }
+    }
+    w4_store(w4, w4_level_base(levels - 1u) + k, lo, hi);
+}
+
+// Level l above the last: the box of child c of node k is the union of the
+// children of node 4k + c of level l + 1 (empty children are skipped).
+__global__ void k_w4_level(uint32_t l, float4* w4) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (1u << (2u * l))) return;
+    float lo[3][4], hi[3][4];
+    for (uint32_t c = 0; c < 4; ++c) {
+        const float4* r = w4 + static_cast<size_t>(kW4RecordF4) * (w4_level_base(l + 1u) + 4u * k + c);
+        const float4 mnx = r[0];
+        const float4 full = make_float4(mnx.x < INFINITY ? 1.0f : 0.0f, mnx.y < INFINITY ? 1.0f : 0.0f,
+                                        mnx.z < INFINITY ? 1.0f : 0.0f, mnx.w < INFINITY ? 1.0f : 0.0f);
+        const bool any = full.x + full.y + full.z + full.w > 0.0f;
+        for (int a = 0; a < 3; ++a) {
+            const float4 mn = r[a], mx = r[3 + a];
+            lo[a][c] = fminf(fminf(mn.x, mn.y), fminf(mn.z, mn.w));
+            float h = -INFINITY;
+            if (full.x > 0.0f) h = fmaxf(h, mx.x);
+            if (full.y > 0.0f) h = fmaxf(h, mx.y);
+            if (full.z > 0.0f) h = fmaxf(h, mx.z);
+            if (full.w > 0.0f) h = fmaxf(h, mx.w);
+            hi[a][c] = any ? h : INFINITY;
+        }
+    }
+    w4_store(w4, w4_level_base(l) + k, lo, hi);
+}
+
 inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 struct DevBuf {
@@ -423,5 +492,33 @@ vp_status vp_build_scene_device(vp_context* ctx, const vp_primitive* host_prims,
     }
     VP_HIP_TRY(hipGetLastError());
     VP_HIP_TRY(hipStreamSynchronize(s));
+    return VP_OK;
+}
+
+vp_status vp_build_walk_tree(vp_context* ctx, vp_scene* sc) {
+    using namespace vp;
+    hipStream_t s = ctx->stream;
+    const uint32_t n = sc->n;
+    const uint32_t levels = w4_levels(n);
+    const size_t records = w4_records(levels);
+    if (sc->w4) {
+        VP_HIP_TRY(hipStreamSynchronize(s));
+        VP_HIP_TRY(hipFree(sc->w4));
+        sc->w4 = nullptr;
+    }
+    DevBuf d_box;
+    VP_HIP_TRY(hipMalloc(&d_box.p, sizeof(float) * 6 * static_cast<size_t>(n)));
+    VP_HIP_TRY(hipMalloc(&sc->w4, sizeof(float4) * kW4RecordF4 * records));
+    const uint32_t nn = 2 * n - 1;
+    hipLaunchKernelGGL(k_w4_leaf_boxes, dim3(cdiv(nn, 256u)), dim3(256), 0, s, (const float4*)sc->nodes, nn,
+                       (float*)d_box.p);
+    const uint32_t bottom = 1u << (2u * (levels - 1u));
+    hipLaunchKernelGGL(k_w4_bottom, dim3(cdiv(bottom, 256u)), dim3(256), 0, s, (const float*)d_box.p, n, levels,
+                       sc->w4);
+    for (uint32_t l = levels - 1u; l-- > 0;)
+        hipLaunchKernelGGL(k_w4_level, dim3(cdiv(1u << (2u * l), 256u)), dim3(256), 0, s, l, sc->w4);
+    VP_HIP_TRY(hipGetLastError());
+    VP_HIP_TRY(hipStreamSynchronize(s));
+    sc->w4_levels = levels;
     return VP_OK;
 }

@@ -46,64 +46,76 @@ __device__ __forceinline__ PrimRecord load_prim(const PrimRecord* prims, uint32_
     return r;
 }
 
-// Stackless walk of the rope layout over [0, tmax]; leaf(prim) returns false
-// to stop.  tmax may be lowered by the caller between leaves.
+// number of set bits of m below the calling lane
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
+// ---- walk-tree traversal (tree layout and Walk4 in vp_walk.h) ----
+
+// One box against the ray segment [0, tmax], slab distances as
+// fma(bound, 1/d, -o/d).  Which boxes a walk enters only decides what is
+// visited (leaf boxes are conservative and the test is monotone in the box),
+// never a result.
+__device__ __forceinline__ bool slab_hit(float lx, float ly, float lz, float hx, float hy, float hz, f3 inv, f3 oi,
+                                         float tmax) {
+    const float tx0 = fmaf_(lx, inv.x, -oi.x), tx1 = fmaf_(hx, inv.x, -oi.x);
+    const float ty0 = fmaf_(ly, inv.y, -oi.y), ty1 = fmaf_(hy, inv.y, -oi.y);
+    const float tz0 = fmaf_(lz, inv.z, -oi.z), tz1 = fmaf_(hz, inv.z, -oi.z);
+    const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+    const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+    return tnear <= tfar;
+}
+
+// children of node n that the ray crosses within [0, tmax], as a 4-bit mask
+__device__ __forceinline__ uint32_t w4_ray_mask(const float4* w4, uint32_t n, f3 inv, f3 oi, float tmax) {
+    const auto* r = as_global(w4) + static_cast<size_t>(kW4RecordF4) * n;
+    const float4 lx = r[0], ly = r[1], lz = r[2], hx = r[3], hy = r[4], hz = r[5];
+    return (slab_hit(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, inv, oi, tmax) ? 1u : 0u) |
+           (slab_hit(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, inv, oi, tmax) ? 2u : 0u) |
+           (slab_hit(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, inv, oi, tmax) ? 4u : 0u) |
+           (slab_hit(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, inv, oi, tmax) ? 8u : 0u);
+}
+
+// children of node n whose box contains x
+__device__ __forceinline__ uint32_t w4_point_mask(const float4* w4, uint32_t n, f3 x) {
+    const auto* r = as_global(w4) + static_cast<size_t>(kW4RecordF4) * n;
+    const float4 lx = r[0], ly = r[1], lz = r[2], hx = r[3], hy = r[4], hz = r[5];
+    auto in = [&](float l0, float l1, float l2, float h0, float h1, float h2) {
+        return x.x >= l0 && x.x <= h0 && x.y >= l1 && x.y <= h1 && x.z >= l2 && x.z <= h2;
+    };
+    return (in(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x) ? 1u : 0u) | (in(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y) ? 2u : 0u) |
+           (in(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z) ? 4u : 0u) | (in(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w) ? 8u : 0u);
+}
+
+// Walk over [0, tmax]; leaf(prim) returns false to stop.  tmax may be
+// lowered by the caller between leaves.
 template <typename Leaf>
 __device__ __forceinline__ void walk_ray(const SceneView& s, f3 o, f3 inv, const float& tmax, uint32_t& visits,
                                          Leaf&& leaf) {
-    // slab distances as fma(bound, 1/d, -o/d): which boxes a walk enters
-    // only decides what is visited (leaf boxes are conservative), never a
-    // result
     const f3 oi = o * inv;
-    uint32_t i = 0;
-    const uint32_t nn = s.n_nodes;
-    while (i < nn) {
-        const float4 a = as_global(s.nodes)[2 * i];
-        const float4 b = as_global(s.nodes)[2 * i + 1];
+    Walk4 w = walk4_begin();
+    while (w.n != kWalkEnd) {
         ++visits;
-        const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(b.x, inv.x, -oi.x);
-        const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(b.y, inv.y, -oi.y);
-        const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(b.z, inv.z, -oi.z);
-        const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-        const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-        const uint32_t skip = __float_as_uint(a.w);
-        if (tnear <= tfar) {
-            const uint32_t prim = __float_as_uint(b.w);
-            if (prim != kInnerNode) {
-                if (!leaf(prim)) return;
-                i = skip;
-            } else {
-                i = i + 1;
-            }
-        } else {
-            i = skip;
-        }
+        const uint32_t m = w4_ray_mask(s.w4, w.n, inv, oi, tmax);
+        uint32_t first;
+        for (uint32_t lm = walk4_advance(w, s.w4_levels, m, first); lm != 0u; lm &= lm - 1u)
+            if (!leaf(first + static_cast<uint32_t>(__builtin_ctz(lm)))) return;
     }
 }
 
-// Stackless walk visiting every leaf whose box contains x, in ascending
-// stored-primitive order.
+// Walk visiting every leaf whose box contains x, in ascending stored-primitive
+// order.
 template <typename Leaf>
 __device__ __forceinline__ void walk_point(const SceneView& s, f3 x, uint32_t& visits, Leaf&& leaf) {
-    uint32_t i = 0;
-    const uint32_t nn = s.n_nodes;
-    while (i < nn) {
-        const float4 a = as_global(s.nodes)[2 * i];
-        const float4 b = as_global(s.nodes)[2 * i + 1];
+    Walk4 w = walk4_begin();
+    while (w.n != kWalkEnd) {
         ++visits;
-        const bool in = x.x >= a.x && x.x <= b.x && x.y >= a.y && x.y <= b.y && x.z >= a.z && x.z <= b.z;
-        const uint32_t skip = __float_as_uint(a.w);
-        if (in) {
-            const uint32_t prim = __float_as_uint(b.w);
-            if (prim != kInnerNode) {
-                leaf(prim);
-                i = skip;
-            } else {
-                i = i + 1;
-            }
-        } else {
-            i = skip;
-        }
+        const uint32_t m = w4_point_mask(s.w4, w.n, x);
+        uint32_t first;
+        for (uint32_t lm = walk4_advance(w, s.w4_levels, m, first); lm != 0u; lm &= lm - 1u)
+            leaf(first + static_cast<uint32_t>(__builtin_ctz(lm)));
     }
 }
 
@@ -155,8 +167,8 @@ struct WaveQueue {
     float* rd;                     // [128 * 3] ray directions
     unsigned long long* acc;       // [128] fixed-point optical depth of each ray
     // MIS shadow-ray pool
-    const float4** nodes;          // [64] node array of each lane's scene
-    uint32_t* nn;                  // [64] its node count (or, in the wavefront argmin
+    const float4** nodes;          // [64] walk-tree records of each lane's scene
+    uint32_t* nn;                  // [64] its level count (or, in the wavefront argmin
                                    //      kernel, the pool slot of each lane's path)
     uint8_t* list;                 // [128] the rays still to be walked
     // argmin walks (one ray per lane); they lie over rd[64..128) and acc[64..128)
@@ -200,19 +212,21 @@ __device__ __forceinline__ void lds_store3(float* p, f3 v) {
     p[2] = v.z;
 }
 
-// Node phase of the walks: a lane steps while its leaf queue has room and its
-// walk has nodes left.  With VP_WALK_MIN > 0 the loop runs for the whole
-// wavefront, which leaves it early once fewer than VP_WALK_MIN lanes are
-// stepping and some lane's queue is full, so full lanes do not wait for the
-// last few; the stepping lanes resume after the leaf phase.  A node phase
-// that starts with every queue empty can only end early after some lane has
-// taken kLeafQ steps, so walks always progress.  Every query consumes leaves
-// order-independently, so this changes only the schedule.  On the scattering
-// workload (4096 Gaussians, 512x512, 64 spp) 4 and 8 measure alike, about
-// 1.5 % faster than 0; 16 and 32 are slower.
+// Node phase of the walks: a lane steps while its leaf queue has room for a
+// step (up to 4 leaves) and its walk has nodes left.  With VP_WALK_MIN > 0 the
+// loop runs for the whole wavefront, which leaves it early once fewer than
+// VP_WALK_MIN lanes are stepping and some lane's queue is full, so full lanes
+// do not wait for the last few; the stepping lanes resume after the leaf
+// phase.  A node phase that starts with every queue empty can only end early
+// after some lane has queued kLeafQ - 3 leaves, so walks always progress.
+// Every query consumes leaves order-independently, so this changes only the
+// schedule.
 #ifndef VP_WALK_MIN
 #define VP_WALK_MIN 8
 #endif
+
+// room in a lane's queue for one more node step
+__device__ __forceinline__ bool queue_room(uint32_t cnt) { return cnt + 4u <= kLeafQ; }
 
 // true while this lane (VP_WALK_MIN == 0) or the wavefront stays in the node
 // phase
@@ -220,7 +234,7 @@ __device__ __forceinline__ bool node_phase(bool step, uint32_t cnt) {
 #if VP_WALK_MIN > 0
     const unsigned long long sm = __builtin_amdgcn_ballot_w64(step);
     if (sm == 0) return false;
-    return __popcll(sm) >= VP_WALK_MIN || __builtin_amdgcn_ballot_w64(cnt == kLeafQ) == 0;
+    return __popcll(sm) >= VP_WALK_MIN || __builtin_amdgcn_ballot_w64(!queue_room(cnt)) == 0;
 #else
     (void)cnt;
     return step;
@@ -238,71 +252,87 @@ __device__ __forceinline__ LeafQueue leaf_queue(uint32_t* lds) {
 }
 inline size_t leaf_queue_bytes(uint32_t block) { return static_cast<size_t>(kWaveBytes) * ((block + 63) / 64); }
 
-// One node step of a ray walk over [0, tmax]: a crossed leaf is queued, a
-// crossed inner node is entered, anything else is skipped.  The queue slot
-// cnt (< kLeafQ) is written whether or not the node is a crossed leaf; cnt
-// only advances for one, so a later leaf overwrites the slot.  This keeps the
-// step free of branches (with the padded rows, 1 % faster on the scattering
-// workload than a conditional write into [slot][lane] columns).
-__device__ __forceinline__ void ray_node_step(const SceneView& s, f3 inv, f3 oi, float tmax, uint32_t& i,
-                                              uint32_t& cnt, const LeafQueue& q) {
-    const float4 a = as_global(s.nodes)[2 * i];
-    const float4 b = as_global(s.nodes)[2 * i + 1];
-    const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(b.x, inv.x, -oi.x);
-    const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(b.y, inv.y, -oi.y);
-    const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(b.z, inv.z, -oi.z);
-    const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-    const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-    const uint32_t skip = __float_as_uint(a.w);
-    const uint32_t prim = __float_as_uint(b.w);
-    const bool hit = tnear <= tfar, leaf_node = prim != kInnerNode;
-    q.at(cnt) = prim;
-    cnt += (hit && leaf_node) ? 1u : 0u;
-    i = (hit && !leaf_node) ? i + 1 : skip;
+// Appends the leaves of mask lm (over first .. first + 3) to a lane's queue at
+// cnt, which has room for four.  Slot cnt + (set bits of lm below k) is
+// written for every k, set or not, and a later set bit overwrites it, so the
+// step has no branches; the writes stay below cnt + 4.
+__device__ __forceinline__ void queue_leaves(const LeafQueue& q, uint32_t& cnt, uint32_t lm, uint32_t first) {
+    for (uint32_t k = 0; k < 4u; ++k) q.at(cnt + static_cast<uint32_t>(__popc(lm & ((1u << k) - 1u)))) = first + k;
+    cnt += static_cast<uint32_t>(__popc(lm));
+}
+
+// One node step of a ray walk over [0, tmax]: crossed leaves are queued,
+// the walk moves on.
+__device__ __forceinline__ void ray_node_step(const SceneView& s, f3 inv, f3 oi, float tmax, Walk4& w, uint32_t& cnt,
+                                              const LeafQueue& q) {
+    const uint32_t m = w4_ray_mask(s.w4, w.n, inv, oi, tmax);
+    uint32_t first;
+    const uint32_t lm = walk4_advance(w, s.w4_levels, m, first);
+    queue_leaves(q, cnt, lm, first);
 }
 
 template <typename Leaf>
 __device__ __forceinline__ void walk_ray_q(const SceneView& s, f3 o, f3 inv, const float& tmax, uint32_t& visits,
                                            const LeafQueue& q, Leaf&& leaf) {
     const f3 oi = o * inv;
-    uint32_t i = 0;
-    const uint32_t nn = s.n_nodes;
+    Walk4 w = walk4_begin();
     while (true) {
         uint32_t cnt = 0;
         while (true) {
-            const bool step = i < nn && cnt < kLeafQ;
+            const bool step = w.n != kWalkEnd && queue_room(cnt);
             if (!node_phase(step, cnt)) break;
             if (!step) continue;
             ++visits;
-            ray_node_step(s, inv, oi, tmax, i, cnt, q);
+            ray_node_step(s, inv, oi, tmax, w, cnt, q);
         }
         for (uint32_t k = 0; k < cnt; ++k)
             if (!leaf(q.at(k))) return;
-        if (i >= nn) return;
+        if (w.n == kWalkEnd) return;
     }
 }
 
 template <typename Leaf>
 __device__ __forceinline__ void walk_point_q(const SceneView& s, f3 x, uint32_t& visits, const LeafQueue& q,
                                              Leaf&& leaf) {
-    uint32_t i = 0;
-    const uint32_t nn = s.n_nodes;
+    Walk4 w = walk4_begin();
     while (true) {
         uint32_t cnt = 0;
-        while (i < nn && cnt < kLeafQ) {
-            const float4 a = as_global(s.nodes)[2 * i];
-            const float4 b = as_global(s.nodes)[2 * i + 1];
+        while (w.n != kWalkEnd && queue_room(cnt)) {
             ++visits;
-            const bool in = x.x >= a.x && x.x <= b.x && x.y >= a.y && x.y <= b.y && x.z >= a.z && x.z <= b.z;
-            const uint32_t skip = __float_as_uint(a.w);
-            const uint32_t prim = __float_as_uint(b.w);
-            const bool leaf_node = prim != kInnerNode;
-            if (in && leaf_node) q.at(cnt++) = prim;
-            i = (in && !leaf_node) ? i + 1 : skip;
+            const uint32_t m = w4_point_mask(s.w4, w.n, x);
+            uint32_t first;
+            const uint32_t lm = walk4_advance(w, s.w4_levels, m, first);
+            if (lm != 0u) queue_leaves(q, cnt, lm, first);
         }
         for (uint32_t k = 0; k < cnt; ++k) leaf(q.at(k));
-        if (i >= nn) return;
+        if (w.n == kWalkEnd) return;
     }
+}
+
+// Appends the crossed leaves of the calling lanes (lm: a mask over the stored
+// primitives first .. first + 3) to the wave queue at count, in lane order,
+// as far as they fit in kWaveQ.  Returns false for a lane whose leaves did not
+// fit: its node step is to be repeated after the leaf phase.  The lanes that
+// fit form a prefix in lane order, so the queue has no gaps; count advances by
+// the leaves written.  Every lane of the wavefront that is active here must
+// call it.
+__device__ __forceinline__ bool wave_push(const WaveQueue& wq, uint32_t& count, uint32_t lm, uint32_t first,
+                                          uint32_t owner) {
+    const uint32_t np = static_cast<uint32_t>(__popc(lm));
+    const unsigned long long b0 = __builtin_amdgcn_ballot_w64(np > 0u), b1 = __builtin_amdgcn_ballot_w64(np > 1u);
+    const unsigned long long b2 = __builtin_amdgcn_ballot_w64(np > 2u), b3 = __builtin_amdgcn_ballot_w64(np > 3u);
+    const uint32_t pos = count + lanes_below(b0) + lanes_below(b1) + lanes_below(b2) + lanes_below(b3);
+    const bool fits = np == 0u || pos + np <= kWaveQ;
+    const unsigned long long f = __builtin_amdgcn_ballot_w64(fits);
+    if (fits && np != 0u) {
+        uint32_t p = pos;
+        for (uint32_t b = lm; b != 0u; b &= b - 1u, ++p) {
+            wq.prim[p] = first + static_cast<uint32_t>(__builtin_ctz(b));
+            wq.owner[p] = static_cast<uint8_t>(owner);
+        }
+    }
+    count += static_cast<uint32_t>(__popcll(b0 & f) + __popcll(b1 & f) + __popcll(b2 & f) + __popcll(b3 & f));
+    return fits;
 }
 
 // Add one chord to the fixed-point optical depth; true once the total
@@ -384,86 +414,11 @@ __device__ float transmittance(const SceneView& s, f3 o, f3 d, const uint32_t* a
     return tau_to_T(acc);
 }
 
-// number of set bits of m below the calling lane
-__device__ __forceinline__ uint32_t lanes_below(unsigned long long m) {
-    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
-                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
-}
-
 // one chord in fixed point, as tau_add adds it; a chord of tau >= 88 alone
 // adds the saturation limit
 __device__ __forceinline__ unsigned long long tau_fixed(float od) {
     const float c = od > 0.0f ? od : 0.0f;
     return c >= kMaxTau ? kTauLimitFixed : static_cast<unsigned long long>(c * kTauScaleF);
-}
-
-// The two shadow rays of one scatter vertex (MIS), each lane walking its ray
-// a and then its ray b without waiting for the wavefront in between: a lane
-// whose ray a ends starts ray b at once, so the wavefront's walk lasts as
-// long as its longest a-plus-b pair instead of its longest a plus its longest
-// b.  Both rays leave o through the same active set, whose chords are summed
-// in one loop over the set.  Ta and Tb equal two transmittance() calls bit
-// for bit (the same chords in the same fixed-point sums, and a ray that
-// reaches tau >= 88 gives 0).
-__device__ __forceinline__ void transmittance_two(const SceneView& s, f3 o, f3 da, f3 db, const uint32_t* act,
-                                                  uint32_t na, uint32_t stride, const LeafQueue& q, float& Ta,
-                                                  float& Tb) {
-    uint64_t acc_a = 0, acc_b = 0;
-    bool full_a = false, full_b = false;
-    for (uint32_t k = 0; k < na && !(full_a && full_b); ++k) {
-        const PrimRecord pr = load_prim(s.prims, act[k * stride]);
-        if (!full_a) {
-            const float te = exit_from_inside(pr, o, da);
-            if (te > 0.0f && te < kTMax) full_a = tau_add(acc_a, optical_depth(pr, o, da, 0.0f, te));
-        }
-        if (!full_b) {
-            const float te = exit_from_inside(pr, o, db);
-            if (te > 0.0f && te < kTMax) full_b = tau_add(acc_b, optical_depth(pr, o, db, 0.0f, te));
-        }
-    }
-    Ta = 0.0f;
-    Tb = 0.0f;
-    // current ray r: 0 = a, 1 = b, 2 = both done; a saturated ray is done
-    uint32_t r = full_a ? (full_b ? 2u : 1u) : 0u;
-    f3 d = r == 0 ? da : db;
-    uint64_t acc = r == 0 ? acc_a : acc_b;
-    bool full = false;
-    f3 inv = safe_inv(d), oi = o * inv;
-    uint32_t i = 0;
-    const uint32_t nn = s.n_nodes;
-    while (__ballot(r < 2u) != 0) {
-        uint32_t cnt = 0;
-        while (true) {
-            const bool step = r < 2u && i < nn && cnt < kLeafQ;
-            if (!node_phase(step, cnt)) break;
-            if (!step) continue;
-            ray_node_step(s, inv, oi, kTMax, i, cnt, q);
-        }
-        for (uint32_t k = 0; k < cnt && !full; ++k) {
-            const PrimRecord pr = load_prim(s.prims, q.at(k));
-            float wl2;
-            const float tj = entry_t(pr, o, d, &wl2);
-            if (tj < 0.0f) continue;
-            const float te = exit_from_entry(pr, o, d, tj, wl2);
-            if (te > tj && te < kTMax) full = tau_add(acc, optical_depth(pr, o, d, tj, te));
-        }
-        if (r < 2u && (full || i >= nn)) {
-            const float T = full ? 0.0f : tau_to_T(acc);
-            if (r == 0u) {
-                Ta = T;
-                r = full_b ? 2u : 1u;
-                d = db;
-                acc = acc_b;
-                full = false;
-                inv = safe_inv(d);
-                oi = o * inv;
-                i = 0;
-            } else {
-                Tb = T;
-                r = 2u;
-            }
-        }
-    }
 }
 
 // The MIS shadow rays of the calling lanes, with the wave-shared leaf queue
@@ -494,8 +449,8 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
     const uint32_t lane = __lane_id();
     lds_store3(wq.ro + 3 * lane, o);
     wq.prims[lane] = s.prims;
-    wq.nodes[lane] = s.nodes;
-    wq.nn[lane] = s.n_nodes;
+    wq.nodes[lane] = s.w4;
+    wq.nn[lane] = s.w4_levels;
     lds_store3(wq.rd + 3 * (2 * lane), da);
     lds_store3(wq.rd + 3 * (2 * lane + 1), db);
     wq.acc[2 * lane] = full_a ? kTauLimitFixed : acc_a;
@@ -515,54 +470,37 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
     // the ray this lane walks, and its walk state
     const uint32_t rank = lanes_below(am);
     uint32_t ray = rank < nr ? wq.list[rank] : kNoRay;
-    const float4* nodes = nullptr;
-    uint32_t i = 0, nn = 0;
+    const float4* w4 = nullptr;
+    uint32_t levels = 0;
+    Walk4 w = walk4_begin();
     f3 inv = mk(0.0f), oi = mk(0.0f);
     auto start = [&]() {
         const uint32_t own = ray >> 1;
         const f3 ro = lds_load3(wq.ro + 3 * own), rd = lds_load3(wq.rd + 3 * ray);
         inv = safe_inv(rd);
         oi = ro * inv;
-        nodes = wq.nodes[own];
-        nn = wq.nn[own];
-        i = 0;
+        w4 = wq.nodes[own];
+        levels = wq.nn[own];
+        w = walk4_begin();
     };
     if (ray != kNoRay) start();
     while (__builtin_amdgcn_ballot_w64(ray != kNoRay) != 0) {
         // node phase: every lane with a ray steps; crossed leaves are queued
         // for the wavefront.  It ends when no lane steps, when the queue
-        // might not take one more step of every lane, or, once a quarter of
-        // the queue is filled, when fewer than VP_SHARED_STEP_MIN lanes
-        // still step.
+        // could not take some lane's leaves, or, once a quarter of the queue
+        // is filled, when fewer than VP_SHARED_STEP_MIN lanes still step.
         uint32_t count = 0;
         while (true) {
-            const bool step = ray != kNoRay && i < nn;
+            const bool step = ray != kNoRay && w.n != kWalkEnd;
             const unsigned long long sm = __builtin_amdgcn_ballot_w64(step);
-            if (sm == 0 || count > kWaveQ - 64u) break;
+            if (sm == 0) break;
             if (count >= kWaveQ / 4u && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
-            bool push = false;
-            uint32_t prim = 0;
-            if (step) {
-                const float4 a = as_global(nodes)[2 * i];
-                const float4 b = as_global(nodes)[2 * i + 1];
-                const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(b.x, inv.x, -oi.x);
-                const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(b.y, inv.y, -oi.y);
-                const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(b.z, inv.z, -oi.z);
-                const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-                const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), kTMax));
-                const uint32_t skip = __float_as_uint(a.w);
-                prim = __float_as_uint(b.w);
-                const bool hit = tnear <= tfar, leaf_node = prim != kInnerNode;
-                push = hit && leaf_node;
-                i = (hit && !leaf_node) ? i + 1 : skip;
-            }
-            const unsigned long long pm = __builtin_amdgcn_ballot_w64(push);
-            if (push) {
-                const uint32_t pos = count + lanes_below(pm);
-                wq.prim[pos] = prim;
-                wq.owner[pos] = static_cast<uint8_t>(ray);
-            }
-            count += static_cast<uint32_t>(__popcll(pm));
+            uint32_t lm = 0, first = 0;
+            Walk4 nw = w;
+            if (step) lm = walk4_advance(nw, levels, w4_ray_mask(w4, w.n, inv, oi, kTMax), first);
+            const bool fits = wave_push(wq, count, lm, first, ray);
+            if (step && fits) w = nw;
+            if (__builtin_amdgcn_ballot_w64(!fits) != 0) break;
         }
         // leaf phase, shared by all lanes: entry tests; the entered leaves
         // move down to the front of the queue (an entry is only overwritten
@@ -600,7 +538,7 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
         }
         // lanes whose ray is done (walked to the end, or saturated) take the
         // next ray of the list
-        const bool fin = ray != kNoRay && (i >= nn || wq.acc[ray] >= kTauLimitFixed);
+        const bool fin = ray != kNoRay && (w.n == kWalkEnd || wq.acc[ray] >= kTauLimitFixed);
         const unsigned long long fm = __builtin_amdgcn_ballot_w64(fin);
         if (fin) {
             const uint32_t k = next + lanes_below(fm);

@@ -11,6 +11,7 @@
 
 #include "../../include/vp_api.h"
 #include "vp_core.h"
+#include "vp_walk.h"
 
 namespace vp {
 
@@ -24,6 +25,9 @@ namespace vp {
 // stored-primitive order, so a walk meets primitives in ascending index.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kInnerNode = 0xffffffffu;
+
+// The walk tree the render kernels traverse is defined in vp_walk.h; its leaf
+// boxes are the rope tree's, so a walk of either meets the same leaves.
 
 // Conservative leaf box of record r (input scale s, centre c): the 3-sigma
 // ellipsoid with forward rotation R and scale s has half extent
@@ -55,6 +59,8 @@ struct SceneView {
     const float4* nodes;
     uint32_t n_prims;
     uint32_t n_nodes;
+    const float4* w4;    // walk-tree records (kW4RecordF4 float4 each)
+    uint32_t w4_levels;  // its level count L
 };
 
 struct EnvDev {
@@ -193,7 +199,9 @@ struct vp_scene {
     float4* nodes = nullptr;           // 2 * (2n - 1)
     uint32_t* order = nullptr;         // order[k] = input index
     uint32_t* morton = nullptr;        // codes in input order
-    vp::SceneView view() const { return vp::SceneView{prims, nodes, n, n ? 2 * n - 1 : 0}; }
+    float4* w4 = nullptr;              // walk tree, kW4RecordF4 * w4_records(w4_levels)
+    uint32_t w4_levels = 0;
+    vp::SceneView view() const { return vp::SceneView{prims, nodes, n, n ? 2 * n - 1 : 0, w4, w4_levels}; }
 };
 
 struct vp_env {
@@ -225,6 +233,8 @@ void vp_set_error(const std::string& msg);
 vp_status vp_build_scene_device(vp_context* ctx, const vp_primitive* host_prims, uint32_t n, vp_scene* scene);
 // host SAH tree replacing the device LBVH's nodes, records and order (vp_bvh_sah.cpp)
 vp_status vp_build_scene_sah(vp_context* ctx, const vp_primitive* host_prims, uint32_t n, vp_scene* scene);
+// walk tree over the scene's stored order, from its rope tree's leaf boxes (vp_bvh.hip)
+vp_status vp_build_walk_tree(vp_context* ctx, vp_scene* scene);
 
 // launchers (vp_render.hip)
 struct RenderLaunch {

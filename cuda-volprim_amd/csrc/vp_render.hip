@@ -70,40 +70,22 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
     wq.nc[lane] = 0u;
     const f3 inv = safe_inv(d), oi = o * inv;
     float tmax = direct ? kTMax : ts_origin;
-    uint32_t i = 0;
-    const uint32_t nn = sc.n_nodes;
+    Walk4 w = walk4_begin();
     // the first thread of this wavefront, for the owners' candidate buffers
     const uint32_t gbase = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-    while (__ballot(i < nn) != 0) {
+    while (__ballot(w.n != kWalkEnd) != 0) {
         uint32_t count = 0;
         while (true) {
-            const bool step = i < nn;
+            const bool step = w.n != kWalkEnd;
             const unsigned long long sm = __builtin_amdgcn_ballot_w64(step);
-            if (sm == 0 || count > kWaveQ - 64u) break;
+            if (sm == 0) break;
             if (count >= kWaveQ / 4u && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
-            bool push = false;
-            uint32_t prim = 0;
-            if (step) {
-                const float4 a = as_global(sc.nodes)[2 * i];
-                const float4 c = as_global(sc.nodes)[2 * i + 1];
-                const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(c.x, inv.x, -oi.x);
-                const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(c.y, inv.y, -oi.y);
-                const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(c.z, inv.z, -oi.z);
-                const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-                const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-                const uint32_t skip = __float_as_uint(a.w);
-                prim = __float_as_uint(c.w);
-                const bool hit = tnear <= tfar, leaf_node = prim != kInnerNode;
-                push = hit && leaf_node;
-                i = (hit && !leaf_node) ? i + 1 : skip;
-            }
-            const unsigned long long pm = __builtin_amdgcn_ballot_w64(push);
-            if (push) {
-                const uint32_t pos = count + lanes_below(pm);
-                wq.prim[pos] = prim;
-                wq.owner[pos] = static_cast<uint8_t>(lane);
-            }
-            count += static_cast<uint32_t>(__popcll(pm));
+            uint32_t lm = 0, first = 0;
+            Walk4 nw = w;
+            if (step) lm = walk4_advance(nw, sc.w4_levels, w4_ray_mask(sc.w4, w.n, inv, oi, tmax), first);
+            const bool fits = wave_push(wq, count, lm, first, lane);
+            if (step && fits) w = nw;
+            if (__builtin_amdgcn_ballot_w64(!fits) != 0) break;
         }
         const unsigned long long am = __builtin_amdgcn_ballot_w64(true);
         const uint32_t nact = static_cast<uint32_t>(__popcll(am));

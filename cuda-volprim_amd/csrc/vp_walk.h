@@ -1,0 +1,82 @@
+// vp_walk.h — index arithmetic of the walk tree, shared by the device walks
+// (vp_device.h), the tree build (vp_bvh.hip) and a host check of the
+// traversal (tests/walk_tree_check.cpp).
+//
+// The walk tree is an implicit complete 4-ary tree over the stored
+// primitives.  With L levels (the smallest L >= 1 with 4^L >= N), node k of
+// level l has heap index base(l) + k, base(l) = (4^l - 1) / 3, and its
+// children are nodes 4k .. 4k + 3 of level l + 1; below the last level
+// (l = L - 1) the four children of node k are the leaves 4k .. 4k + 3, i.e.
+// stored primitives (slots >= N are empty).  A node is one 128-byte record
+// holding the boxes of its four children, one float4 per bound and axis:
+//   r[0] = min.x of children 0..3, r[1] = min.y, r[2] = min.z,
+//   r[3] = max.x,                  r[4] = max.y, r[5] = max.z, r[6..7] unused.
+// An empty child has min = max = +inf on every axis, which no ray or point
+// test accepts.  Parents and siblings follow from the index, so walks need no
+// stack (see Walk4).
+#pragma once
+
+#include "vp_math.h"
+
+namespace vp {
+
+constexpr uint32_t kW4RecordF4 = 8;   // float4 per node record
+constexpr uint32_t kW4MaxLevels = 15; // 4^15 = 2^30 leaves
+
+VP_HD uint32_t w4_levels(uint32_t n) {
+    uint32_t l = 1;
+    while (l < kW4MaxLevels && (1ull << (2 * l)) < n) ++l;
+    return l;
+}
+// (4^l - 1) / 3 = 0b0101...01 with l ones
+VP_HD uint32_t w4_level_base(uint32_t l) { return 0x55555555u & ((1u << (2u * l)) - 1u); }
+// node records of a tree of `levels` levels
+VP_HD uint32_t w4_records(uint32_t levels) { return w4_level_base(levels); }
+
+constexpr uint32_t kWalkEnd = 0xffffffffu;
+
+// Position of a walk: the node to load next (kWalkEnd once the walk is
+// over), its level, and a trail of 4 bits per level above it: bit c of level
+// l is set while child c of the level-l node on the current path is still to
+// be visited.  Backtracking takes the deepest level with a pending child and
+// computes that child's index from the current node's (an ancestor k levels
+// up has the level-local index shifted right by 2k), so a walk keeps no stack
+// and loads every visited node exactly once.  Children are taken in
+// ascending order, so leaves are met in ascending stored index.
+struct Walk4 {
+    uint32_t n, lvl;
+    uint64_t trail;
+};
+VP_HD Walk4 walk4_begin() { return Walk4{0u, 0u, 0ull}; }
+
+// Moves the walk past node w.n, whose crossed children are m (4 bits).  On
+// the last level the crossed children are leaves: returns them as a mask
+// over the stored primitives first .. first + 3; elsewhere returns 0.
+VP_HD uint32_t walk4_advance(Walk4& w, uint32_t levels, uint32_t m, uint32_t& first) {
+    const bool last = w.lvl + 1u == levels;
+    const uint32_t local = w.n - w4_level_base(w.lvl);
+    first = 4u * local;
+    if (!last && m != 0u) {
+        // descend into the first crossed child; the others become pending
+        const uint32_t sh = 4u * w.lvl;
+        w.trail = (w.trail & ~(0xfull << sh)) | (static_cast<uint64_t>(m & (m - 1u)) << sh);
+        w.n = w4_level_base(w.lvl + 1u) + 4u * local + static_cast<uint32_t>(__builtin_ctz(m));
+        w.lvl += 1u;
+    } else {
+        // backtrack to the deepest pending child
+        const uint64_t t = w.trail & ((1ull << (4u * w.lvl)) - 1ull);
+        if (t == 0ull) {
+            w.n = kWalkEnd;
+        } else {
+            const uint32_t l = static_cast<uint32_t>(63 - __builtin_clzll(t)) >> 2;
+            const uint32_t c = static_cast<uint32_t>(__builtin_ctz(static_cast<uint32_t>(t >> (4u * l)) & 0xfu));
+            w.trail = t & ~(1ull << (4u * l + c));
+            const uint32_t anc = local >> (2u * (w.lvl - l));
+            w.n = w4_level_base(l + 1u) + 4u * anc + c;
+            w.lvl = l + 1u;
+        }
+    }
+    return last ? m : 0u;
+}
+
+}  // namespace vp

@@ -247,7 +247,8 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
     unsigned int* qc = qcs(w, par);
     const uint32_t n = qc[0];
     bool have = false;
-    uint32_t i = 0, node = 0, nn = 0;
+    uint32_t i = 0;
+    Walk4 w4w = walk4_begin();
     float ts_origin = kTMax, tmax = kTMax;
     f3 inv = mk(0.0f), oi = mk(0.0f);
     SceneView sc{};
@@ -286,41 +287,23 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
             tmax = ts;
             inv = safe_inv(d);
             oi = o * inv;
-            node = 0;
-            nn = sc.n_nodes;
+            w4w = walk4_begin();
             have = true;
         }
         if (__ballot(have) == 0) break;
         // node phase: crossed leaves go to the wave queue
         uint32_t count = 0;
         while (true) {
-            const bool step = have && node < nn;
+            const bool step = have && w4w.n != kWalkEnd;
             const unsigned long long sm = __builtin_amdgcn_ballot_w64(step);
-            if (sm == 0 || count > kWaveQ - 64u) break;
+            if (sm == 0) break;
             if (count >= kWaveQ / 4u && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
-            bool push = false;
-            uint32_t prim = 0;
-            if (step) {
-                const float4 a = as_global(sc.nodes)[2 * node];
-                const float4 c = as_global(sc.nodes)[2 * node + 1];
-                const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(c.x, inv.x, -oi.x);
-                const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(c.y, inv.y, -oi.y);
-                const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(c.z, inv.z, -oi.z);
-                const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-                const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-                const uint32_t skip = __float_as_uint(a.w);
-                prim = __float_as_uint(c.w);
-                const bool hit = tnear <= tfar, leaf_node = prim != kInnerNode;
-                push = hit && leaf_node;
-                node = (hit && !leaf_node) ? node + 1 : skip;
-            }
-            const unsigned long long pm = __builtin_amdgcn_ballot_w64(push);
-            if (push) {
-                const uint32_t pos = count + lanes_below(pm);
-                wq.prim[pos] = prim;
-                wq.owner[pos] = static_cast<uint8_t>(lane);
-            }
-            count += static_cast<uint32_t>(__popcll(pm));
+            uint32_t lm = 0, first = 0;
+            Walk4 nw = w4w;
+            if (step) lm = walk4_advance(nw, sc.w4_levels, w4_ray_mask(sc.w4, w4w.n, inv, oi, tmax), first);
+            const bool fits = wave_push(wq, count, lm, first, lane);
+            if (step && fits) w4w = nw;
+            if (__builtin_amdgcn_ballot_w64(!fits) != 0) break;
         }
         // leaf phase, shared by all lanes (sampling.cuh:372-412)
         const unsigned long long am = __builtin_amdgcn_ballot_w64(true);
@@ -350,7 +333,7 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
         if (have) {
             const float wmin = __uint_as_float(wq.ts[lane]);
             tmax = wmin;
-            if (node >= nn) {
+            if (w4w.n == kWalkEnd) {
                 w.bt[i].w = wmin < ts_origin ? wmin : ts_origin;
                 w.ncand[i] = wq.nc[lane];
                 have = false;
@@ -583,7 +566,8 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
     const uint32_t n = qc[1];
     const uint32_t P = w.P;
     bool have = false;
-    uint32_t e = 0, node = 0, nn = 0;
+    uint32_t e = 0;
+    Walk4 w4w = walk4_begin();
     f3 inv = mk(0.0f), oi = mk(0.0f);
     SceneView sc{};
     while (true) {
@@ -612,41 +596,24 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
             wq.prims[lane] = sc.prims;
             inv = safe_inv(d);
             oi = o * inv;
-            node = full ? sc.n_nodes : 0u;
-            nn = sc.n_nodes;
+            w4w = walk4_begin();
+            if (full) w4w.n = kWalkEnd;
             have = true;
         }
         if (__ballot(have) == 0) break;
         // node phase: crossed leaves go to the wave queue
         uint32_t count = 0;
         while (true) {
-            const bool step = have && node < nn;
+            const bool step = have && w4w.n != kWalkEnd;
             const unsigned long long sm = __builtin_amdgcn_ballot_w64(step);
-            if (sm == 0 || count > kWaveQ - 64u) break;
+            if (sm == 0) break;
             if (count >= kWaveQ / 4u && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
-            bool push = false;
-            uint32_t prim = 0;
-            if (step) {
-                const float4 a = as_global(sc.nodes)[2 * node];
-                const float4 c = as_global(sc.nodes)[2 * node + 1];
-                const float tx0 = fmaf_(a.x, inv.x, -oi.x), tx1 = fmaf_(c.x, inv.x, -oi.x);
-                const float ty0 = fmaf_(a.y, inv.y, -oi.y), ty1 = fmaf_(c.y, inv.y, -oi.y);
-                const float tz0 = fmaf_(a.z, inv.z, -oi.z), tz1 = fmaf_(c.z, inv.z, -oi.z);
-                const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-                const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), kTMax));
-                const uint32_t skip = __float_as_uint(a.w);
-                prim = __float_as_uint(c.w);
-                const bool hit = tnear <= tfar, leaf_node = prim != kInnerNode;
-                push = hit && leaf_node;
-                node = (hit && !leaf_node) ? node + 1 : skip;
-            }
-            const unsigned long long pm = __builtin_amdgcn_ballot_w64(push);
-            if (push) {
-                const uint32_t pos = count + lanes_below(pm);
-                wq.prim[pos] = prim;
-                wq.owner[pos] = static_cast<uint8_t>(lane);
-            }
-            count += static_cast<uint32_t>(__popcll(pm));
+            uint32_t lm = 0, first = 0;
+            Walk4 nw = w4w;
+            if (step) lm = walk4_advance(nw, sc.w4_levels, w4_ray_mask(sc.w4, w4w.n, inv, oi, kTMax), first);
+            const bool fits = wave_push(wq, count, lm, first, lane);
+            if (step && fits) w4w = nw;
+            if (__builtin_amdgcn_ballot_w64(!fits) != 0) break;
         }
         // leaf phase, shared by all lanes: entry tests, then the chords of
         // the entered leaves (fixed-point sums, order free)
@@ -686,7 +653,7 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
         if (have) {
             const unsigned long long acc = wq.acc[lane];
             const bool full = acc >= kTauLimitFixed;
-            if (full || node >= nn) {
+            if (full || w4w.n == kWalkEnd) {
                 w.pd[e].w = full ? 0.0f : tau_to_T(acc);
                 have = false;
             }

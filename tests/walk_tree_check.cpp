@@ -1,0 +1,193 @@
+// Host check of the walk-tree traversal (cuda-volprim_amd/csrc/vp_walk.h).
+//
+// Builds the implicit 4-ary tree over random leaf boxes with the rule of the
+// device build (a child box is the union of its children, empty children are
+// +inf boxes), walks it with walk4_advance for random rays and points, and
+// checks against brute force over the leaves:
+//   * the walk meets exactly the leaves whose own box passes the test, in
+//     ascending order (the device walks rely on both);
+//   * every node index it loads is inside the tree and no node is loaded
+//     twice;
+//   * a walk whose segment end is lowered while it runs (as the argmin walk
+//     lowers it) still meets every leaf that passes at the final end.
+// Prints "ok <walks>" and exits 0, or prints the first mismatch and exits 1.
+// Usage: walk_tree_check <seed> <walks per size>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <vector>
+
+#include "vp_walk.h"
+
+namespace {
+
+using vp::Walk4;
+
+struct Box {
+    float lo[3], hi[3];
+};
+
+bool slab(const Box& b, const float o[3], const float inv[3], float tmax) {
+    float tn = 0.0f, tf = tmax;
+    float t0[3], t1[3];
+    for (int a = 0; a < 3; ++a) {
+        const float oi = o[a] * inv[a];
+        t0[a] = std::fma(b.lo[a], inv[a], -oi);
+        t1[a] = std::fma(b.hi[a], inv[a], -oi);
+    }
+    // the device formula: max of the near planes and 0, min of the far planes and tmax
+    tn = std::fmax(std::fmax(std::fmin(t0[0], t1[0]), std::fmin(t0[1], t1[1])), std::fmax(std::fmin(t0[2], t1[2]), 0.0f));
+    tf = std::fmin(std::fmin(std::fmax(t0[0], t1[0]), std::fmax(t0[1], t1[1])), std::fmin(std::fmax(t0[2], t1[2]), tmax));
+    return tn <= tf;
+}
+
+bool inside(const Box& b, const float x[3]) {
+    for (int a = 0; a < 3; ++a)
+        if (!(x[a] >= b.lo[a] && x[a] <= b.hi[a])) return false;
+    return true;
+}
+
+struct Tree {
+    uint32_t levels = 0;
+    std::vector<Box> child;  // 4 per record
+};
+
+Tree build(const std::vector<Box>& leaves) {
+    const uint32_t n = static_cast<uint32_t>(leaves.size());
+    Tree t;
+    t.levels = vp::w4_levels(n);
+    t.child.resize(4 * static_cast<size_t>(vp::w4_records(t.levels)));
+    const Box empty{{INFINITY, INFINITY, INFINITY}, {INFINITY, INFINITY, INFINITY}};
+    const uint32_t L = t.levels;
+    for (uint32_t k = 0; k < (1u << (2 * (L - 1))); ++k)
+        for (uint32_t c = 0; c < 4; ++c) {
+            const uint32_t j = 4 * k + c;
+            t.child[4 * (vp::w4_level_base(L - 1) + k) + c] = j < n ? leaves[j] : empty;
+        }
+    for (uint32_t l = L - 1; l-- > 0;)
+        for (uint32_t k = 0; k < (1u << (2 * l)); ++k)
+            for (uint32_t c = 0; c < 4; ++c) {
+                const uint32_t r = vp::w4_level_base(l + 1) + 4 * k + c;
+                Box u{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}};
+                bool any = false;
+                for (uint32_t g = 0; g < 4; ++g) {
+                    const Box& b = t.child[4 * r + g];
+                    if (!(b.lo[0] < INFINITY)) continue;
+                    any = true;
+                    for (int a = 0; a < 3; ++a) {
+                        u.lo[a] = std::fmin(u.lo[a], b.lo[a]);
+                        u.hi[a] = std::fmax(u.hi[a], b.hi[a]);
+                    }
+                }
+                t.child[4 * (vp::w4_level_base(l) + k) + c] = any ? u : empty;
+            }
+    return t;
+}
+
+int fail(const char* what, uint32_t n, uint32_t walk) {
+    std::printf("FAIL %s: n=%u walk=%u\n", what, n, walk);
+    return 1;
+}
+
+// Walks the tree; mask(node) gives the crossed children of a node, and
+// after each node on_node() may lower the segment end.  Returns 0 and the
+// leaves met, or 1 on an index error.
+template <typename Mask>
+int walk(const Tree& t, Mask&& mask, std::vector<uint32_t>& met) {
+    const uint32_t records = vp::w4_records(t.levels);
+    std::vector<uint8_t> seen(records, 0);
+    Walk4 w = vp::walk4_begin();
+    while (w.n != vp::kWalkEnd) {
+        if (w.n >= records || w.lvl >= t.levels) return 1;
+        if (w.n < vp::w4_level_base(w.lvl) || w.n >= vp::w4_level_base(w.lvl + 1)) return 1;
+        if (seen[w.n]++) return 1;
+        const uint32_t m = mask(w.n);
+        uint32_t first = 0;
+        for (uint32_t lm = vp::walk4_advance(w, t.levels, m, first); lm; lm &= lm - 1)
+            met.push_back(first + static_cast<uint32_t>(__builtin_ctz(lm)));
+    }
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    const uint32_t seed = argc > 1 ? static_cast<uint32_t>(std::atoi(argv[1])) : 1u;
+    const uint32_t walks = argc > 2 ? static_cast<uint32_t>(std::atoi(argv[2])) : 200u;
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<float> U(-1.0f, 1.0f), H(0.01f, 0.25f), T(0.0f, 1.0f);
+    const uint32_t sizes[] = {1, 2, 3, 4, 5, 15, 16, 17, 63, 64, 65, 255, 256, 257, 1000, 4096, 5000, 20000};
+    uint64_t total = 0;
+    for (uint32_t n : sizes) {
+        std::vector<Box> leaves(n);
+        for (auto& b : leaves)
+            for (int a = 0; a < 3; ++a) {
+                const float c = U(rng), h = H(rng);
+                b.lo[a] = c - h;
+                b.hi[a] = c + h;
+            }
+        const Tree t = build(leaves);
+        for (uint32_t k = 0; k < walks; ++k) {
+            // ray: origin inside or outside the leaves' region, direction random
+            float o[3], d[3], inv[3];
+            const float spread = (k & 1) ? 1.2f : 3.0f;
+            float d2 = 0.0f;
+            for (int a = 0; a < 3; ++a) {
+                o[a] = spread * U(rng);
+                d[a] = U(rng);
+                if (k % 7 == 3 && a == 1) d[a] = 0.0f;  // axis-parallel rays
+                d2 += d[a] * d[a];
+            }
+            if (d2 == 0.0f) d[0] = d2 = 1.0f;
+            for (int a = 0; a < 3; ++a) {
+                d[a] /= std::sqrt(d2);
+                const float x = std::fabs(d[a]) < 1e-20f ? std::copysign(1e-20f, d[a]) : d[a];
+                inv[a] = 1.0f / x;
+            }
+            const float tmax = (k % 3 == 0) ? 1e20f : 4.0f * T(rng);
+            std::vector<uint32_t> met, want;
+            for (uint32_t j = 0; j < n; ++j)
+                if (slab(leaves[j], o, inv, tmax)) want.push_back(j);
+            if (walk(t, [&](uint32_t node) {
+                    uint32_t m = 0;
+                    for (uint32_t c = 0; c < 4; ++c) m |= slab(t.child[4 * node + c], o, inv, tmax) ? (1u << c) : 0u;
+                    return m;
+                }, met))
+                return fail("ray walk index", n, k);
+            if (met != want) return fail("ray walk leaves", n, k);
+            // the same ray with the end lowered as the walk goes (argmin style):
+            // every leaf passing at the final end must be met
+            float tcur = 1e20f;
+            std::vector<uint32_t> met2;
+            if (walk(t, [&](uint32_t node) {
+                    uint32_t m = 0;
+                    for (uint32_t c = 0; c < 4; ++c) m |= slab(t.child[4 * node + c], o, inv, tcur) ? (1u << c) : 0u;
+                    tcur = std::fmax(tmax, tcur * 0.7f);
+                    return m;
+                }, met2))
+                return fail("shrinking walk index", n, k);
+            for (uint32_t j : want) {
+                bool found = false;
+                for (uint32_t v : met2) found |= v == j;
+                if (!found) return fail("shrinking walk missed a leaf", n, k);
+            }
+            // point query
+            float x[3];
+            for (int a = 0; a < 3; ++a) x[a] = 1.1f * U(rng);
+            std::vector<uint32_t> pm, pw;
+            for (uint32_t j = 0; j < n; ++j)
+                if (inside(leaves[j], x)) pw.push_back(j);
+            if (walk(t, [&](uint32_t node) {
+                    uint32_t m = 0;
+                    for (uint32_t c = 0; c < 4; ++c) m |= inside(t.child[4 * node + c], x) ? (1u << c) : 0u;
+                    return m;
+                }, pm))
+                return fail("point walk index", n, k);
+            if (pm != pw) return fail("point walk leaves", n, k);
+            total += 3;
+        }
+    }
+    std::printf("ok %llu\n", static_cast<unsigned long long>(total));
+    return 0;
+}
