@@ -54,10 +54,10 @@ __device__ __forceinline__ uint32_t lanes_below(unsigned long long m) {
 
 // ---- walk-tree traversal (tree layout and Walk4 in vp_walk.h) ----
 
-// One box against the ray segment [0, tmax], slab distances as
-// fma(bound, 1/d, -o/d).  Which boxes a walk enters only decides what is
-// visited (leaf boxes are conservative and the test is monotone in the box),
-// never a result.
+// Children of node n that the ray crosses within [0, tmax], as a 4-bit mask.
+// Slab distances are fma(bound, 1/d, -o/d).  Which boxes a walk enters only
+// decides what is visited (leaf boxes are conservative and the test is
+// monotone in the box), never a result.
 __device__ __forceinline__ bool slab_hit(float lx, float ly, float lz, float hx, float hy, float hz, f3 inv, f3 oi,
                                          float tmax) {
     const float tx0 = fmaf_(lx, inv.x, -oi.x), tx1 = fmaf_(hx, inv.x, -oi.x);
@@ -67,8 +67,6 @@ __device__ __forceinline__ bool slab_hit(float lx, float ly, float lz, float hx,
     const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
     return tnear <= tfar;
 }
-
-// children of node n that the ray crosses within [0, tmax], as a 4-bit mask
 __device__ __forceinline__ uint32_t w4_ray_mask(const float4* w4, uint32_t n, f3 inv, f3 oi, float tmax) {
     const auto* r = as_global(w4) + static_cast<size_t>(kW4RecordF4) * n;
     const float4 lx = r[0], ly = r[1], lz = r[2], hx = r[3], hy = r[4], hz = r[5];
@@ -76,6 +74,37 @@ __device__ __forceinline__ uint32_t w4_ray_mask(const float4* w4, uint32_t n, f3
            (slab_hit(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, inv, oi, tmax) ? 2u : 0u) |
            (slab_hit(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, inv, oi, tmax) ? 4u : 0u) |
            (slab_hit(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, inv, oi, tmax) ? 8u : 0u);
+}
+
+// The same mask with the near and far bound of each axis picked by the sign
+// of the direction component, through the load address: since t is monotone
+// in the bound, the near distance is exactly min(t(min), t(max)) and needs
+// no min/max per axis.  A third fewer instructions per node, but six 64-bit
+// addresses: faster where registers are spare (the wavefront walk kernels,
+// +5 % on the scattering workload), slower in the megakernel, whose extra
+// spills cost more (-20 %).
+__device__ __forceinline__ uint32_t w4_ray_mask_nf(const float4* w4, uint32_t n, f3 inv, f3 oi, float tmax) {
+    // byte offsets of the near bounds (min.x at 0, min.y 16, min.z 32,
+    // max.x 48, max.y 64, max.z 80)
+    const uint32_t fx = (__float_as_uint(inv.x) >> 31) * 48u;
+    const uint32_t fy = (__float_as_uint(inv.y) >> 31) * 48u;
+    const uint32_t fz = (__float_as_uint(inv.z) >> 31) * 48u;
+    const auto* rec = reinterpret_cast<const VP_GLOBAL_AS char*>(as_global(w4) + static_cast<size_t>(kW4RecordF4) * n);
+    const float4 nx = *reinterpret_cast<const VP_GLOBAL_AS float4*>(rec + fx);
+    const float4 ny = *reinterpret_cast<const VP_GLOBAL_AS float4*>(rec + 16u + fy);
+    const float4 nz = *reinterpret_cast<const VP_GLOBAL_AS float4*>(rec + 32u + fz);
+    const float4 gx = *reinterpret_cast<const VP_GLOBAL_AS float4*>(rec + 48u - fx);
+    const float4 gy = *reinterpret_cast<const VP_GLOBAL_AS float4*>(rec + 64u - fy);
+    const float4 gz = *reinterpret_cast<const VP_GLOBAL_AS float4*>(rec + 80u - fz);
+    auto hit = [&](float ax, float ay, float az, float bx, float by, float bz) {
+        const float tn = fmaxf(fmaxf(fmaf_(ax, inv.x, -oi.x), fmaf_(ay, inv.y, -oi.y)),
+                               fmaxf(fmaf_(az, inv.z, -oi.z), 0.0f));
+        const float tf = fminf(fminf(fmaf_(bx, inv.x, -oi.x), fmaf_(by, inv.y, -oi.y)),
+                               fminf(fmaf_(bz, inv.z, -oi.z), tmax));
+        return tn <= tf;
+    };
+    return (hit(nx.x, ny.x, nz.x, gx.x, gy.x, gz.x) ? 1u : 0u) | (hit(nx.y, ny.y, nz.y, gx.y, gy.y, gz.y) ? 2u : 0u) |
+           (hit(nx.z, ny.z, nz.z, gx.z, gy.z, gz.z) ? 4u : 0u) | (hit(nx.w, ny.w, nz.w, gx.w, gy.w, gz.w) ? 8u : 0u);
 }
 
 // children of node n whose box contains x
@@ -319,7 +348,9 @@ __device__ __forceinline__ void walk_point_q(const SceneView& s, f3 x, uint32_t&
 __device__ __forceinline__ bool wave_push(const WaveQueue& wq, uint32_t& count, uint32_t lm, uint32_t first,
                                           uint32_t owner) {
     const uint32_t np = static_cast<uint32_t>(__popc(lm));
-    const unsigned long long b0 = __builtin_amdgcn_ballot_w64(np > 0u), b1 = __builtin_amdgcn_ballot_w64(np > 1u);
+    const unsigned long long b0 = __builtin_amdgcn_ballot_w64(np > 0u);
+    if (b0 == 0ull) return true;  // no leaves this step (inner nodes)
+    const unsigned long long b1 = __builtin_amdgcn_ballot_w64(np > 1u);
     const unsigned long long b2 = __builtin_amdgcn_ballot_w64(np > 2u), b3 = __builtin_amdgcn_ballot_w64(np > 3u);
     const uint32_t pos = count + lanes_below(b0) + lanes_below(b1) + lanes_below(b2) + lanes_below(b3);
     const bool fits = np == 0u || pos + np <= kWaveQ;

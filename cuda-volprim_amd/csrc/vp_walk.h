@@ -52,6 +52,9 @@ VP_HD Walk4 walk4_begin() { return Walk4{0u, 0u, 0ull}; }
 // Moves the walk past node w.n, whose crossed children are m (4 bits).  On
 // the last level the crossed children are leaves: returns them as a mask
 // over the stored primitives first .. first + 3; elsewhere returns 0.
+// (A branch-free form, descending as a backtrack with this node's children
+// added to the trail, measured 12 % slower on the scattering workload: the
+// lanes of a wavefront mostly take the same branch.)
 VP_HD uint32_t walk4_advance(Walk4& w, uint32_t levels, uint32_t m, uint32_t& first) {
     const bool last = w.lvl + 1u == levels;
     const uint32_t local = w.n - w4_level_base(w.lvl);
