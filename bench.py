@@ -37,7 +37,7 @@ PEAK_FP32_NONPACKED_TFLOPS = 78.6
 # (6 FMA each).
 FLOP_WEIGHTS = {
     "node_visits": 48, "leaf_tests": 105, "contain": 41, "arg_active": 230, "arg_hit": 141,
-    "arg_accept": 122, "rb_active": 92, "rb_hit": 122, "sh_active": 238, "sh_hit": 268, "alb": 52,
+    "arg_accept": 0, "rb_active": 1, "rb_hit": 122, "sh_active": 238, "sh_hit": 268, "alb": 52,
     "scatters": 220, "paths": 35,
 }
 

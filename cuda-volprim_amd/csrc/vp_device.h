@@ -81,8 +81,8 @@ __device__ __forceinline__ uint32_t w4_ray_mask(const float4* w4, uint32_t n, f3
 // in the bound, the near distance is exactly min(t(min), t(max)) and needs
 // no min/max per axis.  A third fewer instructions per node, but six 64-bit
 // addresses: faster where registers are spare (the wavefront walk kernels,
-// +5 % on the scattering workload), slower in the megakernel, whose extra
-// spills cost more (-20 %).
+// +5 % on the scattering workload; the megakernel at 4 waves per SIMD),
+// slower where they are not (the megakernel at 6 waves per SIMD: -20 %).
 __device__ __forceinline__ uint32_t w4_ray_mask_nf(const float4* w4, uint32_t n, f3 inv, f3 oi, float tmax) {
     // byte offsets of the near bounds (min.x at 0, min.y 16, min.z 32,
     // max.x 48, max.y 64, max.z 80)
@@ -105,6 +105,19 @@ __device__ __forceinline__ uint32_t w4_ray_mask_nf(const float4* w4, uint32_t n,
     };
     return (hit(nx.x, ny.x, nz.x, gx.x, gy.x, gz.x) ? 1u : 0u) | (hit(nx.y, ny.y, nz.y, gx.y, gy.y, gz.y) ? 2u : 0u) |
            (hit(nx.z, ny.z, nz.z, gx.z, gy.z, gz.z) ? 4u : 0u) | (hit(nx.w, ny.w, nz.w, gx.w, gy.w, gz.w) ? 8u : 0u);
+}
+
+// The node test of the megakernel's shared-queue walks: 1 selects the
+// near/far form (vp_render.hip sets it).
+#ifndef VP_MEGA_NEAR_FAR
+#define VP_MEGA_NEAR_FAR 0
+#endif
+__device__ __forceinline__ uint32_t mega_ray_mask(const float4* w4, uint32_t n, f3 inv, f3 oi, float tmax) {
+#if VP_MEGA_NEAR_FAR
+    return w4_ray_mask_nf(w4, n, inv, oi, tmax);
+#else
+    return w4_ray_mask(w4, n, inv, oi, tmax);
+#endif
 }
 
 // children of node n whose box contains x
@@ -528,7 +541,7 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
             if (count >= kWaveQ / 4u && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
             uint32_t lm = 0, first = 0;
             Walk4 nw = w;
-            if (step) lm = walk4_advance(nw, levels, w4_ray_mask(w4, w.n, inv, oi, kTMax), first);
+            if (step) lm = walk4_advance(nw, levels, mega_ray_mask(w4, w.n, inv, oi, kTMax), first);
             const bool fits = wave_push(wq, count, lm, first, ray);
             if (step && fits) w = nw;
             if (__builtin_amdgcn_ballot_w64(!fits) != 0) break;

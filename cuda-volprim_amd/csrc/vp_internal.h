@@ -91,10 +91,18 @@ struct JobDesc {
     float4* mean;     // width * height Welford mean of the job
 };
 
-// Per-lane (megakernel) or per-slot (wavefront) slab: two active-set
-// buffers and one buffer of rebuild candidates, each of the active-set
-// capacity.
+// Per-slot slab of the wavefront pipeline: two active-set buffers and one
+// buffer of rebuild candidates, each of the active-set capacity.
 constexpr uint32_t kSlabBuffers = 3;
+// Per-lane slab of the megakernel, buffers of the active-set capacity:
+//   0, 1  active sets (ping-pong)
+//   2     rebuild candidates (stored primitive index)
+//   3, 4  their entry and exit distances along the ray (float bits)
+//   5     exit distance of each member of the current active set along the
+//         current ray
+// The argmin computes the values of 3, 4 and 5 anyway, so the rebuild only
+// compares them with the scatter distance.
+constexpr uint32_t kMegaSlabBuffers = 6;
 
 // Path pool of the wavefront pipeline (structure of arrays over P slots).
 struct WfPool {
