@@ -465,6 +465,47 @@ __device__ __forceinline__ unsigned long long tau_fixed(float od) {
     return c >= kMaxTau ? kTauLimitFixed : static_cast<unsigned long long>(c * kTauScaleF);
 }
 
+// Work sharing between the walks of one wavefront, for queries whose result
+// does not depend on visiting order.  Lanes without a walk (idle) each take
+// the shallowest pending subtree of a running walk (can_give): the donor
+// clears that child from its trail and the thief starts a walk at the child
+// with an empty trail, which ends once the subtree is done, so every subtree
+// is still walked exactly once.  The handoff uses the first 128 words of
+// wq.tj, which are free between leaf phases.  Returns true for a lane that
+// received a subtree; its ray and walk are set.
+#ifndef VP_STEAL_IDLE
+#define VP_STEAL_IDLE 16
+#endif
+__device__ __forceinline__ bool wave_steal(const WaveQueue& wq, bool idle, bool can_give, uint32_t& ray, Walk4& w) {
+    const uint64_t pend = can_give ? walk4_pending(w) : 0ull;
+    const unsigned long long im = __builtin_amdgcn_ballot_w64(idle);
+    const unsigned long long dm = __builtin_amdgcn_ballot_w64(pend != 0ull);
+    const uint32_t ni = static_cast<uint32_t>(__popcll(im)), nd = static_cast<uint32_t>(__popcll(dm));
+    const uint32_t pairs = ni < nd ? ni : nd;
+    if (pairs == 0u) return false;
+    uint32_t* slot = reinterpret_cast<uint32_t*>(wq.tj);
+    if (pend != 0ull) {
+        const uint32_t r = lanes_below(dm);
+        if (r < pairs) {
+            Walk4 part;
+            walk4_split(w, part);
+            slot[r] = part.n;
+            slot[64 + r] = ray | (part.lvl << 8);
+        }
+    }
+    bool got = false;
+    if (idle) {
+        const uint32_t r = lanes_below(im);
+        if (r < pairs) {
+            const uint32_t meta = slot[64 + r];
+            ray = meta & 0xffu;
+            w = Walk4{slot[r], meta >> 8, 0ull};
+            got = true;
+        }
+    }
+    return got;
+}
+
 // The MIS shadow rays of the calling lanes, with the wave-shared leaf queue
 // and a wave-wide ray pool: every lane starts on one ray of the wavefront's
 // list (all rays a, then all rays b) and, when its ray is done, takes the
@@ -518,27 +559,37 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
     uint32_t levels = 0;
     Walk4 w = walk4_begin();
     f3 inv = mk(0.0f), oi = mk(0.0f);
-    auto start = [&]() {
-        const uint32_t own = ray >> 1;
-        const f3 ro = lds_load3(wq.ro + 3 * own), rd = lds_load3(wq.rd + 3 * ray);
+    // the walk setup of the lane's ray (any lane may run it: an idle lane
+    // reads ray 127's data and does not use it)
+    auto setup = [&]() {
+        const uint32_t r = ray & 127u, own = r >> 1;
+        const f3 ro = lds_load3(wq.ro + 3 * own), rd = lds_load3(wq.rd + 3 * r);
         inv = safe_inv(rd);
         oi = ro * inv;
         w4 = wq.nodes[own];
         levels = wq.nn[own];
-        w = walk4_begin();
     };
-    if (ray != kNoRay) start();
+    setup();
     while (__builtin_amdgcn_ballot_w64(ray != kNoRay) != 0) {
         // node phase: every lane with a ray steps; crossed leaves are queued
         // for the wavefront.  It ends when no lane steps, when the queue
         // could not take some lane's leaves, or, once a quarter of the queue
         // is filled, when fewer than VP_SHARED_STEP_MIN lanes still step.
         uint32_t count = 0;
+        bool stepped = false;
         while (true) {
             const bool step = ray != kNoRay && w.n != kWalkEnd;
             const unsigned long long sm = __builtin_amdgcn_ballot_w64(step);
             if (sm == 0) break;
-            if (count >= kWaveQ / 4u && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
+            const uint32_t ns = static_cast<uint32_t>(__popcll(sm));
+            if (count >= kWaveQ / 4u && ns < VP_SHARED_STEP_MIN) break;
+            // the list is used up and many lanes are idle: end the phase so
+            // the running walks can be shared out (after at least one step,
+            // so every phase progresses)
+            if (stepped && next >= nr && ns + VP_STEAL_IDLE <= nact &&
+                __builtin_amdgcn_ballot_w64(step && walk4_pending(w) != 0ull) != 0ull)
+                break;
+            stepped = true;
             uint32_t lm = 0, first = 0;
             Walk4 nw = w;
             if (step) lm = walk4_advance(nw, levels, mega_ray_mask(w4, w.n, inv, oi, kTMax), first);
@@ -582,14 +633,21 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
         }
         // lanes whose ray is done (walked to the end, or saturated) take the
         // next ray of the list
-        const bool fin = ray != kNoRay && (w.n == kWalkEnd || wq.acc[ray] >= kTauLimitFixed);
+        const bool fin = ray != kNoRay && (w.n == kWalkEnd || wq.acc[ray & 127u] >= kTauLimitFixed);
         const unsigned long long fm = __builtin_amdgcn_ballot_w64(fin);
-        if (fin) {
-            const uint32_t k = next + lanes_below(fm);
-            ray = k < nr ? wq.list[k] : kNoRay;
-            if (ray != kNoRay) start();
-        }
+        const uint32_t k = next + lanes_below(fm);
+        const uint32_t listed = k < nr ? wq.list[k < 128u ? k : 0u] : kNoRay;
+        ray = fin ? listed : ray;
+        if (fin) w = walk4_begin();
         next += static_cast<uint32_t>(__popcll(fm));
+        // once the list is used up, idle lanes take over subtrees of running
+        // walks.  Every lane then redoes its ray setup (the same values for a
+        // lane that kept its ray): the refill and the handoff stay free of
+        // branches around the setup, so every lane of the wavefront stays in
+        // the loop and takes part in its ballots and leaf passes.
+        const bool can_give = ray != kNoRay && w.n != kWalkEnd && wq.acc[ray & 127u] < kTauLimitFixed;
+        if (next >= nr) wave_steal(wq, ray == kNoRay, can_give, ray, w);
+        setup();
     }
     const unsigned long long ta = wq.acc[2 * lane], tb = wq.acc[2 * lane + 1];
     Ta = ta >= kTauLimitFixed ? 0.0f : tau_to_T(ta);

@@ -82,4 +82,24 @@ VP_HD uint32_t walk4_advance(Walk4& w, uint32_t levels, uint32_t m, uint32_t& fi
     return last ? m : 0u;
 }
 
+// Children still pending in walk w (trail bits of the levels above w.n).
+VP_HD uint64_t walk4_pending(const Walk4& w) {
+    return w.n == kWalkEnd ? 0ull : w.trail & ((1ull << (4u * w.lvl)) - 1ull);
+}
+
+// Splits off the shallowest pending subtree of walk w as a walk of its own
+// (part), which starts at that child with an empty trail and so ends once the
+// subtree is done; w no longer visits it.  Every subtree is still walked
+// exactly once, by one of the two.  Returns false when nothing is pending.
+VP_HD bool walk4_split(Walk4& w, Walk4& part) {
+    const uint64_t pend = walk4_pending(w);
+    if (pend == 0ull) return false;
+    const uint32_t bit = static_cast<uint32_t>(__builtin_ctzll(pend));
+    const uint32_t l = bit >> 2, c = bit & 3u;
+    w.trail &= ~(1ull << bit);
+    const uint32_t anc = (w.n - w4_level_base(w.lvl)) >> (2u * (w.lvl - l));
+    part = Walk4{w4_level_base(l + 1u) + 4u * anc + c, l + 1u, 0ull};
+    return true;
+}
+
 }  // namespace vp

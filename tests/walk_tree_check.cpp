@@ -9,9 +9,12 @@
 //   * every node index it loads is inside the tree and no node is loaded
 //     twice;
 //   * a walk whose segment end is lowered while it runs (as the argmin walk
-//     lowers it) still meets every leaf that passes at the final end.
+//     lowers it) still meets every leaf that passes at the final end;
+//   * a walk split among several walkers (walk4_split, as the wavefront
+//     shares out running walks) meets every leaf exactly once.
 // Prints "ok <walks>" and exits 0, or prints the first mismatch and exits 1.
 // Usage: walk_tree_check <seed> <walks per size>
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -171,6 +174,37 @@ int main(int argc, char** argv) {
                 bool found = false;
                 for (uint32_t v : met2) found |= v == j;
                 if (!found) return fail("shrinking walk missed a leaf", n, k);
+            }
+            // the same ray walked by several walkers that split pending
+            // subtrees off each other at random (the wavefront's work
+            // sharing): every leaf exactly once, none lost
+            {
+                std::vector<Walk4> ws{vp::walk4_begin()};
+                std::vector<uint32_t> got;
+                const uint32_t records = vp::w4_records(t.levels);
+                std::vector<uint8_t> seen(records, 0);
+                std::uniform_int_distribution<uint32_t> pick(0, 1 << 20);
+                while (!ws.empty()) {
+                    const uint32_t i = pick(rng) % ws.size();
+                    if (pick(rng) % 3 == 0 && ws.size() < 64) {
+                        Walk4 part;
+                        if (vp::walk4_split(ws[i], part)) ws.push_back(part);
+                        continue;
+                    }
+                    Walk4& w = ws[i];
+                    if (w.n == vp::kWalkEnd) {
+                        ws.erase(ws.begin() + i);
+                        continue;
+                    }
+                    if (w.n >= records || seen[w.n]++) return fail("split walk index", n, k);
+                    uint32_t m = 0;
+                    for (uint32_t c = 0; c < 4; ++c) m |= slab(t.child[4 * w.n + c], o, inv, tmax) ? (1u << c) : 0u;
+                    uint32_t first = 0;
+                    for (uint32_t lm = vp::walk4_advance(w, t.levels, m, first); lm; lm &= lm - 1)
+                        got.push_back(first + static_cast<uint32_t>(__builtin_ctz(lm)));
+                }
+                std::sort(got.begin(), got.end());
+                if (got != want) return fail("split walk leaves", n, k);
             }
             // point query
             float x[3];
