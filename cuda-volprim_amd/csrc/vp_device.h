@@ -99,6 +99,8 @@ __device__ __forceinline__ uint32_t w4_ray_mask_nf(const float4* w4, uint32_t n,
     auto hit = [&](float ax, float ay, float az, float bx, float by, float bz) {
         const float tn = fmaxf(fmaxf(fmaf_(ax, inv.x, -oi.x), fmaf_(ay, inv.y, -oi.y)),
                                fmaxf(fmaf_(az, inv.z, -oi.z), 0.0f));
+        // (the clamp to tmax also rejects the +inf boxes of empty children
+        // for a direction with all components positive)
         const float tf = fminf(fminf(fmaf_(bx, inv.x, -oi.x), fmaf_(by, inv.y, -oi.y)),
                                fminf(fmaf_(bz, inv.z, -oi.z), tmax));
         return tn <= tf;

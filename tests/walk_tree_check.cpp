@@ -11,7 +11,9 @@
 //   * a walk whose segment end is lowered while it runs (as the argmin walk
 //     lowers it) still meets every leaf that passes at the final end;
 //   * a walk split among several walkers (walk4_split, as the wavefront
-//     shares out running walks) meets every leaf exactly once.
+//     shares out running walks) meets every leaf exactly once;
+//   * the near/far form of the node test decides every child box (empty
+//     ones included) exactly as the min/max form.
 // Prints "ok <walks>" and exits 0, or prints the first mismatch and exits 1.
 // Usage: walk_tree_check <seed> <walks per size>
 #include <algorithm>
@@ -42,6 +44,23 @@ bool slab(const Box& b, const float o[3], const float inv[3], float tmax) {
     // the device formula: max of the near planes and 0, min of the far planes and tmax
     tn = std::fmax(std::fmax(std::fmin(t0[0], t1[0]), std::fmin(t0[1], t1[1])), std::fmax(std::fmin(t0[2], t1[2]), 0.0f));
     tf = std::fmin(std::fmin(std::fmax(t0[0], t1[0]), std::fmax(t0[1], t1[1])), std::fmin(std::fmax(t0[2], t1[2]), tmax));
+    return tn <= tf;
+}
+
+// The near/far form of the device walks (w4_ray_mask_nf): the sign of each
+// direction component picks the near bound, and the near and far distances
+// are taken without a per-axis min/max.  Must decide exactly as slab().
+bool slab_nf(const Box& b, const float o[3], const float inv[3], float tmax) {
+    float tn = 0.0f, tf = tmax;
+    float nr[3], fr[3];
+    for (int a = 0; a < 3; ++a) {
+        const float oi = o[a] * inv[a];
+        const bool neg = std::signbit(inv[a]);
+        nr[a] = std::fma(neg ? b.hi[a] : b.lo[a], inv[a], -oi);
+        fr[a] = std::fma(neg ? b.lo[a] : b.hi[a], inv[a], -oi);
+    }
+    tn = std::fmax(std::fmax(nr[0], nr[1]), std::fmax(nr[2], 0.0f));
+    tf = std::fmin(std::fmin(fr[0], fr[1]), std::fmin(fr[2], tmax));
     return tn <= tf;
 }
 
@@ -152,12 +171,18 @@ int main(int argc, char** argv) {
             std::vector<uint32_t> met, want;
             for (uint32_t j = 0; j < n; ++j)
                 if (slab(leaves[j], o, inv, tmax)) want.push_back(j);
+            bool nf_same = true;
             if (walk(t, [&](uint32_t node) {
                     uint32_t m = 0;
-                    for (uint32_t c = 0; c < 4; ++c) m |= slab(t.child[4 * node + c], o, inv, tmax) ? (1u << c) : 0u;
+                    for (uint32_t c = 0; c < 4; ++c) {
+                        const bool h = slab(t.child[4 * node + c], o, inv, tmax);
+                        nf_same &= h == slab_nf(t.child[4 * node + c], o, inv, tmax);
+                        m |= h ? (1u << c) : 0u;
+                    }
                     return m;
                 }, met))
                 return fail("ray walk index", n, k);
+            if (!nf_same) return fail("near/far node test decides differently", n, k);
             if (met != want) return fail("ray walk leaves", n, k);
             // the same ray with the end lowered as the walk goes (argmin style):
             // every leaf passing at the final end must be met
