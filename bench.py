@@ -77,7 +77,8 @@ def cpu_baseline(rows, env, cam_pod, params, threads, budget_s=20.0, repeats=5):
         t0 = time.perf_counter()
         vpo.render(rec, env, cam_pod, params["width"], params["height"], params["spp"], seed=params["seed"],
                    max_depth=params["max_depth"], rr_depth=params["rr_depth"], phase_g=params["phase_g"],
-                   stride=stride, samples=(0, spp_used), threads=threads, env_tables_=tabs, bvh=True)
+                   use_ris=bool(params.get("use_ris", 0)), stride=stride, samples=(0, spp_used), threads=threads,
+                   env_tables_=tabs, bvh=True)
         return time.perf_counter() - t0
 
     probe = run(1)
@@ -279,6 +280,8 @@ def main():
                     help="leaf order of the walk tree: host SAH tree (default) or the GPU LBVH's Morton order")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB elementwise copy kernel first (HBM counter calibration)")
+    ap.add_argument("--ris", action="store_true",
+                    help="NEE by resampled importance sampling (the reference's --ris) instead of MIS")
     ap.add_argument("--mix", action="store_true", help="run the job mix (plain vs batched vs served) instead")
     ap.add_argument("--mix-jobs", type=int, default=64)
     ap.add_argument("--mix-res-div", type=int, default=4)
@@ -300,6 +303,8 @@ def main():
     from volprim_amd import scenes, volprim
 
     rows, env, cam_args, pp = workload(scenes)
+    if args.ris:
+        pp = dict(pp, use_ris=1)
     ctx = volprim.Context(0)
     dev_name, n_cu, clk = ctx.device_info()
     scene = ctx.scene(rows, bvh=args.bvh)
@@ -404,7 +409,8 @@ def main():
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": "scattering: 4096 anisotropic Gaussians (seed 1), sigma multiplier 10, albedo 0.9, "
                                "isotropic phase, 2048x1024 sky + sun environment, 512x512, 64 spp, max depth 64, "
-                               "RR from depth 12, MIS NEE", "width": 512, "height": 512, "spp": 64,
+                               "RR from depth 12, " + ("RIS NEE (6 candidates)" if args.ris else "MIS NEE"),
+                   "width": 512, "height": 512, "spp": 64,
                    "primitives": 4096},
         "roofline": roofline,
         "cpu_baseline": cpu,

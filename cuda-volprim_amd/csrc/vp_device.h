@@ -520,11 +520,13 @@ __device__ __forceinline__ bool wave_steal(const WaveQueue& wq, bool idle, bool 
 // saturated once its sum reaches the tau = 88 limit, in whatever order the
 // chords arrive), so Ta and Tb equal two transmittance() calls bit for bit.
 // Every lane of the calling wavefront that is active here takes part.
+// With two = false only ray a is walked (the one shadow ray of RIS); Tb is
+// then 0.
 __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 da, f3 db, const uint32_t* act,
                                                    uint32_t na, uint32_t stride, const WaveQueue& wq, float& Ta,
-                                                   float& Tb) {
+                                                   float& Tb, bool two = true) {
     uint64_t acc_a = 0, acc_b = 0;
-    bool full_a = false, full_b = false;
+    bool full_a = false, full_b = !two;
     for (uint32_t k = 0; k < na && !(full_a && full_b); ++k) {
         const PrimRecord pr = load_prim(s.prims, act[k * stride]);
         if (!full_a) {

@@ -464,7 +464,9 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                     const float lumy = dot(yenv, luma);
                     if (lumy > 0.0f) {
                         ev.add(kCntShadowRays);
-                        const float T = transmittance(sc, x, ydir, act, nn, stride, lq, ev);
+                        float T, T_unused;
+                        if constexpr (kCount) T = transmittance(sc, x, ydir, act, nn, stride, lq, ev);
+                        else transmittance_pool(sc, x, ydir, ydir, act, nn, stride, wq, T, T_unused, false);
                         const float Wr = wsum * rcpf(static_cast<float>(J.ris_candidates));
                         st.L = st.L + ((base * (yenv * rcpf(lumy))) * T) * Wr;
                     }
