@@ -199,6 +199,10 @@ constexpr uint32_t kWaveQ = VP_WAVE_Q;
 #ifndef VP_SHARED_STEP_MIN
 #define VP_SHARED_STEP_MIN VP_WALK_MIN
 #endif
+// (the queue fill, as a fraction 1 / VP_SHARED_END_DIV, from which that rule applies)
+#ifndef VP_SHARED_END_DIV
+#define VP_SHARED_END_DIV 4u
+#endif
 // Rays are numbered per wavefront: ray k of a one-ray-per-lane walk belongs
 // to lane k; the MIS shadow rays of lane l are rays 2l and 2l + 1.
 struct WaveQueue {
@@ -589,7 +593,7 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
             const unsigned long long sm = __builtin_amdgcn_ballot_w64(step);
             if (sm == 0) break;
             const uint32_t ns = static_cast<uint32_t>(__popcll(sm));
-            if (count >= kWaveQ / 4u && ns < VP_SHARED_STEP_MIN) break;
+            if (count >= kWaveQ / VP_SHARED_END_DIV && ns < VP_SHARED_STEP_MIN) break;
             // the list is used up and many lanes are idle: end the phase so
             // the running walks can be shared out (after at least one step,
             // so every phase progresses)

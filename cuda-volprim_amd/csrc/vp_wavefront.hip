@@ -297,7 +297,7 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
             const bool step = have && w4w.n != kWalkEnd;
             const unsigned long long sm = __builtin_amdgcn_ballot_w64(step);
             if (sm == 0) break;
-            if (count >= kWaveQ / 4u && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
+            if (count >= kWaveQ / VP_SHARED_END_DIV && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
             uint32_t lm = 0, first = 0;
             Walk4 nw = w4w;
             if (step) lm = walk4_advance(nw, sc.w4_levels, w4_ray_mask_nf(sc.w4, w4w.n, inv, oi, tmax), first);
@@ -607,7 +607,7 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
             const bool step = have && w4w.n != kWalkEnd;
             const unsigned long long sm = __builtin_amdgcn_ballot_w64(step);
             if (sm == 0) break;
-            if (count >= kWaveQ / 4u && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
+            if (count >= kWaveQ / VP_SHARED_END_DIV && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
             uint32_t lm = 0, first = 0;
             Walk4 nw = w4w;
             if (step) lm = walk4_advance(nw, sc.w4_levels, w4_ray_mask_nf(sc.w4, w4w.n, inv, oi, kTMax), first);
