@@ -485,6 +485,12 @@ __device__ __forceinline__ unsigned long long tau_fixed(float od) {
 #ifndef VP_STEAL_IDLE
 #define VP_STEAL_IDLE 32
 #endif
+// ... and only after at least VP_STEAL_GAP node steps in the phase, so each
+// handoff (a leaf phase, the handoff and the lanes' ray setup) is paid for by
+// that many steps of the lanes it feeds.
+#ifndef VP_STEAL_GAP
+#define VP_STEAL_GAP 1u
+#endif
 __device__ __forceinline__ bool wave_steal(const WaveQueue& wq, bool idle, bool can_give, uint32_t& ray, Walk4& w) {
     const uint64_t pend = can_give ? walk4_pending(w) : 0ull;
     const unsigned long long im = __builtin_amdgcn_ballot_w64(idle);
@@ -586,8 +592,7 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
         // for the wavefront.  It ends when no lane steps, when the queue
         // could not take some lane's leaves, or, once a quarter of the queue
         // is filled, when fewer than VP_SHARED_STEP_MIN lanes still step.
-        uint32_t count = 0;
-        bool stepped = false;
+        uint32_t count = 0, nsteps = 0;
         while (true) {
             const bool step = ray != kNoRay && w.n != kWalkEnd;
             const unsigned long long sm = __builtin_amdgcn_ballot_w64(step);
@@ -597,10 +602,10 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
             // the list is used up and many lanes are idle: end the phase so
             // the running walks can be shared out (after at least one step,
             // so every phase progresses)
-            if (stepped && next >= nr && ns + VP_STEAL_IDLE <= nact &&
+            if (nsteps >= VP_STEAL_GAP && next >= nr && ns + VP_STEAL_IDLE <= nact &&
                 __builtin_amdgcn_ballot_w64(step && walk4_pending(w) != 0ull) != 0ull)
                 break;
-            stepped = true;
+            ++nsteps;
             uint32_t lm = 0, first = 0;
             Walk4 nw = w;
             if (step) lm = walk4_advance(nw, levels, mega_ray_mask(w4, w.n, inv, oi, kTMax), first);
