@@ -45,6 +45,7 @@ vp_status check_params_impl(const vp_render_params* p) {
     if (!(p->firefly_clamp >= 0.0f)) return fail(VP_ERR_INVALID, "firefly_clamp must be >= 0");
     if (!(p->filter_stddev >= 0.0f)) return fail(VP_ERR_INVALID, "filter_stddev must be >= 0");
     if (p->use_ris && p->ris_candidates == 0) return fail(VP_ERR_INVALID, "ris_candidates must be positive");
+    if (p->active_cap > vp::kMaxActiveCap) return fail(VP_ERR_INVALID, "active_cap must be at most 65535");
     return VP_OK;
 }
 
@@ -505,6 +506,7 @@ vp_status vp_render_jobs(vp_context* ctx, vp_job* jobs, uint32_t njobs, const vp
     // one capacity serves the whole launch: the largest any job or the
     // launch configuration asks for
     if (launch && launch->active_cap) cap = std::max(cap, launch->active_cap);
+    if (cap > vp::kMaxActiveCap) return fail(VP_ERR_INVALID, "active_cap must be at most 65535");
     vp_status st = vp_reset_counters(ctx, ctx->stream);
     if (st != VP_OK) return st;
     double kms = 0.0;

@@ -535,19 +535,6 @@ __device__ __forceinline__ bool wave_steal(const WaveQueue& wq, bool idle, bool 
 __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 da, f3 db, const uint32_t* act,
                                                    uint32_t na, uint32_t stride, const WaveQueue& wq, float& Ta,
                                                    float& Tb, bool two = true) {
-    uint64_t acc_a = 0, acc_b = 0;
-    bool full_a = false, full_b = !two;
-    for (uint32_t k = 0; k < na && !(full_a && full_b); ++k) {
-        const PrimRecord pr = load_prim(s.prims, act[k * stride]);
-        if (!full_a) {
-            const float te = exit_from_inside(pr, o, da);
-            if (te > 0.0f && te < kTMax) full_a = tau_add(acc_a, optical_depth(pr, o, da, 0.0f, te));
-        }
-        if (!full_b) {
-            const float te = exit_from_inside(pr, o, db);
-            if (te > 0.0f && te < kTMax) full_b = tau_add(acc_b, optical_depth(pr, o, db, 0.0f, te));
-        }
-    }
     const uint32_t lane = __lane_id();
     lds_store3(wq.ro + 3 * lane, o);
     wq.prims[lane] = s.prims;
@@ -555,8 +542,54 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
     wq.nn[lane] = s.w4_levels;
     lds_store3(wq.rd + 3 * (2 * lane), da);
     lds_store3(wq.rd + 3 * (2 * lane + 1), db);
-    wq.acc[2 * lane] = full_a ? kTauLimitFixed : acc_a;
-    wq.acc[2 * lane + 1] = full_b ? kTauLimitFixed : acc_b;
+    wq.acc[2 * lane] = 0ull;
+    wq.acc[2 * lane + 1] = two ? 0ull : kTauLimitFixed;
+    // Chords of the origin set (every member's optical depth from the origin
+    // to its exit), for all lanes' rays together: the wavefront's (member,
+    // lane) pairs are numbered through a prefix sum of the set sizes and
+    // shared out over the lanes, so the loop runs sum(na) / lanes times
+    // instead of max(na).  Each chord goes into its ray's fixed-point sum; a
+    // sum that passes the tau = 88 limit gives T = 0 however far it passes,
+    // as in transmittance().  The owners' set pointers and prefix sums sit in
+    // wq.tj and their lane ids in wq.list, both free until the walks start.
+    {
+        const unsigned long long m0 = __builtin_amdgcn_ballot_w64(true);
+        const uint32_t rk = lanes_below(m0), nl = static_cast<uint32_t>(__popcll(m0));
+        uint32_t pre = 0, total = 0;
+        for (uint32_t bit = 0; bit < 16u; ++bit) {
+            const unsigned long long bm = __builtin_amdgcn_ballot_w64(((na >> bit) & 1u) != 0u);
+            pre += lanes_below(bm) << bit;
+            total += static_cast<uint32_t>(__popcll(bm)) << bit;
+        }
+        const uint32_t** set_of = reinterpret_cast<const uint32_t**>(wq.tj);  // [64] by rank
+        uint32_t* pre_of = reinterpret_cast<uint32_t*>(wq.tj) + 2 * 64;     // [64] by rank
+        set_of[rk] = act;
+        pre_of[rk] = pre;
+        wq.list[rk] = static_cast<uint8_t>(lane);
+        for (uint32_t g = rk; g < total; g += nl) {
+            // the owner: the last rank whose prefix is at most g
+            uint32_t lo = 0, hi = nl - 1u;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1u) >> 1;
+                if (pre_of[mid] <= g) lo = mid;
+                else hi = mid - 1u;
+            }
+            const uint32_t ol = wq.list[lo];
+            const PrimRecord pr = load_prim(wq.prims[ol], set_of[lo][(g - pre_of[lo]) * stride]);
+            const f3 xo = lds_load3(wq.ro + 3 * ol);
+            const f3 dra = lds_load3(wq.rd + 3 * (2 * ol));
+            const float tea = exit_from_inside(pr, xo, dra);
+            if (tea > 0.0f && tea < kTMax) atomicAdd(&wq.acc[2 * ol], tau_fixed(optical_depth(pr, xo, dra, 0.0f, tea)));
+            if (two) {
+                const f3 drb = lds_load3(wq.rd + 3 * (2 * ol + 1));
+                const float teb = exit_from_inside(pr, xo, drb);
+                if (teb > 0.0f && teb < kTMax)
+                    atomicAdd(&wq.acc[2 * ol + 1], tau_fixed(optical_depth(pr, xo, drb, 0.0f, teb)));
+            }
+        }
+    }
+    const bool full_a = wq.acc[2 * lane] >= kTauLimitFixed;
+    const bool full_b = wq.acc[2 * lane + 1] >= kTauLimitFixed;
     // the rays still to walk: every ray a, then every ray b (a saturated ray
     // is done already)
     const unsigned long long am = __builtin_amdgcn_ballot_w64(true);

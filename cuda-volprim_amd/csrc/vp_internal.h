@@ -103,6 +103,8 @@ constexpr uint32_t kSlabBuffers = 3;
 // The argmin computes the values of 3, 4 and 5 anyway, so the rebuild only
 // compares them with the scatter distance.
 constexpr uint32_t kMegaSlabBuffers = 6;
+// Largest active-set capacity: the walks count set sizes in 16 bits.
+constexpr uint32_t kMaxActiveCap = 65535;
 
 // Path pool of the wavefront pipeline (structure of arrays over P slots).
 struct WfPool {

@@ -467,6 +467,7 @@ vp_status vp_autotune(vp_context* ctx, vp_job* jobs, uint32_t njobs, vp_launch_c
         // the reduced mix sees fewer vertices than the full one: never go
         // below the capacities the jobs ask for
         for (uint32_t i = 0; i < njobs; ++i) cap = std::max(cap, jobs[i].params.active_cap);
+        cap = std::min(cap, vp::kMaxActiveCap);  // beyond it sets overflow and are counted
     }
     // candidate shapes: megakernel workgroup size x regeneration threshold
     // at the occupancy limit, one at half occupancy, and the wavefront
