@@ -68,7 +68,7 @@ struct PathState {
 // term's optical depth.  ts_origin is the minimum over the origin set; the
 // result keeps it unless a walk candidate is strictly smaller.
 __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3 d, uint64_t key, uint32_t b,
-                                                   bool direct, float ts_origin, uint64_t dacc, bool dfull,
+                                                   bool direct, const uint32_t* act, uint32_t na, float* act_exit,
                                                    uint32_t* __restrict__ slab, uint32_t cap, const WaveQueue& wq,
                                                    float& ts, uint32_t& nc, float& Tdirect) {
     const uint32_t lane = __lane_id();
@@ -77,9 +77,57 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
     wq.key[lane] = key;
     wq.prims[lane] = sc.prims;
     wq.meta[lane] = b | (direct ? 0x80000000u : 0u);
-    wq.acc[lane] = dfull ? kTauLimitFixed : dacc;
-    wq.ts[lane] = ts_origin == 0.0f ? 0u : __float_as_uint(ts_origin);
+    wq.acc[lane] = 0ull;
+    wq.ts[lane] = __float_as_uint(kTMax);
     wq.nc[lane] = 0u;
+    // The origin set first (sampling.cuh:356-370), for all lanes' rays
+    // together: the wavefront's (member, lane) pairs are numbered through a
+    // prefix sum of the set sizes and shared out over the lanes.  Per member:
+    // its exit (kept in the owner's slab for the rebuild), its free flight
+    // into the owner's running minimum, and at bounce 0 its chord into the
+    // owner's direct-term optical depth; minima and fixed-point sums do not
+    // depend on which lane adds what.  The owners' pointers and prefix sums
+    // sit in wq.tj and their lane ids in wq.list, both free here.  (Active
+    // sets of the megakernel are contiguous.)
+    {
+        const unsigned long long m0 = __builtin_amdgcn_ballot_w64(true);
+        const uint32_t rk = lanes_below(m0), nl = static_cast<uint32_t>(__popcll(m0));
+        uint32_t pre = 0, total = 0;
+        for (uint32_t bit = 0; bit < 16u; ++bit) {
+            const unsigned long long bm = __builtin_amdgcn_ballot_w64(((na >> bit) & 1u) != 0u);
+            pre += lanes_below(bm) << bit;
+            total += static_cast<uint32_t>(__popcll(bm)) << bit;
+        }
+        const uint32_t** set_of = reinterpret_cast<const uint32_t**>(wq.tj);  // [64] by rank
+        float** exit_of = reinterpret_cast<float**>(wq.tj) + 64;               // [64] by rank
+        uint32_t* pre_of = reinterpret_cast<uint32_t*>(wq.tj) + 4 * 64;       // [64] by rank
+        set_of[rk] = act;
+        exit_of[rk] = act_exit;
+        pre_of[rk] = pre;
+        wq.list[rk] = static_cast<uint8_t>(lane);
+        for (uint32_t g = rk; g < total; g += nl) {
+            // the owner: the last rank whose prefix is at most g
+            uint32_t lo = 0, hi = nl - 1u;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1u) >> 1;
+                if (pre_of[mid] <= g) lo = mid;
+                else hi = mid - 1u;
+            }
+            const uint32_t ol = wq.list[lo], k = g - pre_of[lo];
+            const uint32_t i = set_of[lo][k];
+            const PrimRecord pr = load_prim(wq.prims[ol], i);
+            const f3 ro = lds_load3(wq.ro + 3 * ol), rd = lds_load3(wq.rd + 3 * ol);
+            const float te = exit_from_inside(pr, ro, rd);
+            exit_of[lo][k] = te;
+            const uint32_t mo = wq.meta[ol];
+            const float tau = free_flight_tau(prim_uniform(wq.key[ol], mo & 0x7fffffffu, 0u, i));
+            const float t = inv_cdf(pr, ro, rd, tau);
+            if (t >= 0.0f && t <= te) atomicMin(&wq.ts[ol], t == 0.0f ? 0u : __float_as_uint(t));
+            if ((mo >> 31) != 0u && te > 0.0f && te < kTMax)
+                atomicAdd(&wq.acc[ol], tau_fixed(optical_depth(pr, ro, rd, 0.0f, te)));
+        }
+    }
+    const float ts_origin = __uint_as_float(wq.ts[lane]);
     const f3 inv = safe_inv(d), oi = o * inv;
     float tmax = direct ? kTMax : ts_origin;
     Walk4 w = walk4_begin();
@@ -249,19 +297,22 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
         bool dfull = false;
 
         // argmin over the origin set (sampling.cuh:356-370); the members'
-        // exits are kept for the rebuild
+        // exits are kept for the rebuild.  The timed kernel does this for the
+        // whole wavefront inside argmin_walk_shared.
         float ts = kTMax;
         float* act_exit = reinterpret_cast<float*>(my + 5 * cap * stride);
-        for (uint32_t k = 0; k < na; ++k) {
-            const uint32_t i = act[k * stride];
-            const PrimRecord pr = load_prim(sc.prims, i);
-            const float te = exit_from_inside(pr, o, d);
-            act_exit[k * stride] = te;
-            const float tau = free_flight_tau(prim_uniform(st.key, b, 0u, i));
-            const float t = inv_cdf(pr, o, d, tau);
-            if (t >= 0.0f && t < ts && t <= te) ts = t;
-            if (direct && !dfull && te > 0.0f && te < kTMax)
-                dfull = tau_add(dacc, optical_depth(pr, o, d, 0.0f, te));
+        if (kCount || measure) {
+            for (uint32_t k = 0; k < na; ++k) {
+                const uint32_t i = act[k * stride];
+                const PrimRecord pr = load_prim(sc.prims, i);
+                const float te = exit_from_inside(pr, o, d);
+                act_exit[k * stride] = te;
+                const float tau = free_flight_tau(prim_uniform(st.key, b, 0u, i));
+                const float t = inv_cdf(pr, o, d, tau);
+                if (t >= 0.0f && t < ts && t <= te) ts = t;
+                if (direct && !dfull && te > 0.0f && te < kTMax)
+                    dfull = tau_add(dacc, optical_depth(pr, o, d, 0.0f, te));
+            }
         }
         ev.add(kCntArgActive, na);
         ev.add(kCntTrace);
@@ -273,7 +324,7 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
         uint32_t nc = 0;
         float Tdirect = 0.0f;
         if (!kCount && !measure) {
-            argmin_walk_shared(sc, o, d, st.key, b, direct, ts, dacc, dfull, slab, cap, wq, ts, nc, Tdirect);
+            argmin_walk_shared(sc, o, d, st.key, b, direct, act, na, act_exit, slab, cap, wq, ts, nc, Tdirect);
         } else {
             const f3 inv = safe_inv(d);
             const bool prune = !measure && !direct;
