@@ -149,14 +149,44 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
         }
         const unsigned long long am = __builtin_amdgcn_ballot_w64(true);
         const uint32_t nact = static_cast<uint32_t>(__popcll(am));
-        for (uint32_t g = lanes_below(am); g < count; g += nact) {
+        // leaf phase in two passes, each at full width: entry tests, keeping
+        // the entries that can matter (entered, and before the running
+        // minimum or on a direct ray) at the front of the queue; then their
+        // exits, direct-term chords, candidates and free flights.  The second
+        // pass reads the running minimum again; it can only have dropped,
+        // and a candidate it then skips lies beyond the final minimum, which
+        // the rebuild would not keep.
+        const uint32_t rk = lanes_below(am);
+        uint32_t nk = 0;
+        for (uint32_t base = 0; base < count; base += nact) {
+            const uint32_t g = base + rk;
+            bool keep = false;
+            uint32_t own = 0, prim = 0;
+            float tj = -1.0f;
+            if (g < count) {
+                own = wq.owner[g];
+                prim = wq.prim[g];
+                const PrimRecord pr = load_prim(wq.prims[own], prim);
+                float wl2;
+                tj = entry_t(pr, lds_load3(wq.ro + 3 * own), lds_load3(wq.rd + 3 * own), &wl2);
+                keep = tj >= 0.0f && (tj <= __uint_as_float(wq.ts[own]) || (wq.meta[own] >> 31) != 0u);
+            }
+            const unsigned long long km = __builtin_amdgcn_ballot_w64(keep);
+            if (keep) {
+                const uint32_t h = nk + lanes_below(km);
+                wq.prim[h] = prim;
+                wq.owner[h] = static_cast<uint8_t>(own);
+                wq.tj[h] = tj;
+            }
+            nk += static_cast<uint32_t>(__popcll(km));
+        }
+        for (uint32_t g = rk; g < nk; g += nact) {
             const uint32_t own = wq.owner[g];
             const uint32_t prim = wq.prim[g];
             const f3 ro = lds_load3(wq.ro + 3 * own), rd = lds_load3(wq.rd + 3 * own);
             const PrimRecord pr = load_prim(wq.prims[own], prim);
-            float wl2;
-            const float tj = entry_t(pr, ro, rd, &wl2);
-            if (tj < 0.0f) continue;
+            const float tj = wq.tj[g];
+            const float wl2 = length2(to_local_dir(pr, rd));
             const uint32_t meta = wq.meta[own];
             const float ts_cur = __uint_as_float(wq.ts[own]);
             // entered before the running minimum: a rebuild candidate.
