@@ -4,7 +4,7 @@ MI355X, on the scattering workload (4096 anisotropic Gaussians, 512x512,
 phase, 2048x1024 procedural sky environment with a 0.5 degree sun).
 
 A step is one full render of that image (512*512*64 = 16.8 M camera paths)
-with the scene, BVH (the SAH tree unless --bvh lbvh) and environment already
+with the scene, BVH (the median-split tree unless --bvh says otherwise) and environment already
 resident in HBM.  Prints one
 JSON line with the throughput, the roofline of the render kernel (FP32 VALU:
 algorithmic FLOPs from the event-counting kernel variant weighted as
@@ -158,7 +158,7 @@ def run_mix(args):
     jobs = scenes.mix_jobs(n_jobs=args.mix_jobs, seed=4, res_div=args.mix_res_div, spp_div=args.mix_spp_div)
     ctx = volprim.Context(0)
     dev_name = ctx.device_info()[0]
-    handles = {k: ctx.scene(g()) for k, g in gens.items() if any(j["scene"] == k for j in jobs)}
+    handles = {k: ctx.scene(g(), bvh=args.bvh) for k, g in gens.items() if any(j["scene"] == k for j in jobs)}
     envh = ctx.env(scenes.sky_env(2048, 1024))
     print(f"[mix] {len(jobs)} jobs, scenes and environment resident after {time.perf_counter() - t0:.1f} s",
           file=sys.stderr, flush=True)
@@ -276,8 +276,9 @@ def main():
     ap.add_argument("--pipeline", choices=["mega", "wave"], default="mega",
                     help="persistent megakernel or wavefront-queue pipeline")
     ap.add_argument("--pool", type=int, default=0, help="wavefront path slots (0 = library default)")
-    ap.add_argument("--bvh", choices=["lbvh", "sah"], default="sah",
-                    help="leaf order of the walk tree: host SAH tree (default) or the GPU LBVH's Morton order")
+    ap.add_argument("--bvh", choices=["lbvh", "sah", "median"], default="median",
+                    help="leaf order of the walk tree: host median splits on the walk tree's record boundaries "
+                         "(default), host SAH tree, or the GPU LBVH's Morton order")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB elementwise copy kernel first (HBM counter calibration)")
     ap.add_argument("--ris", action="store_true",

@@ -190,7 +190,9 @@ vp_status vp_scene_create(vp_context* ctx, const vp_primitive* prims, uint32_t n
 vp_status vp_scene_create_ex(vp_context* ctx, const vp_primitive* prims, uint32_t n, uint32_t flags,
                              vp_scene** out) {
     if (!ctx || !prims || !out) return fail(VP_ERR_INVALID, "null argument");
-    if (flags & ~VP_SCENE_BVH_SAH) return fail(VP_ERR_INVALID, "unknown scene flags");
+    if (flags & ~(VP_SCENE_BVH_SAH | VP_SCENE_BVH_MEDIAN)) return fail(VP_ERR_INVALID, "unknown scene flags");
+    if ((flags & VP_SCENE_BVH_SAH) && (flags & VP_SCENE_BVH_MEDIAN))
+        return fail(VP_ERR_INVALID, "VP_SCENE_BVH_SAH and VP_SCENE_BVH_MEDIAN exclude each other");
     if (n == 0) return fail(VP_ERR_INVALID, "scene has no primitives");
     if (n > (1u << 30)) return fail(VP_ERR_INVALID, "scene has more than 2^30 primitives");
     // the validation of loadPrimitivesFromPLY (ply.cpp:146-172)
@@ -211,7 +213,8 @@ vp_status vp_scene_create_ex(vp_context* ctx, const vp_primitive* prims, uint32_
     sc->ctx = ctx;
     sc->n = n;
     vp_status st = vp_build_scene_device(ctx, prims, n, sc);
-    if (st == VP_OK && (flags & VP_SCENE_BVH_SAH)) st = vp_build_scene_sah(ctx, prims, n, sc);
+    if (st == VP_OK && (flags & (VP_SCENE_BVH_SAH | VP_SCENE_BVH_MEDIAN)))
+        st = vp_build_scene_sah(ctx, prims, n, sc, (flags & VP_SCENE_BVH_MEDIAN) != 0u);
     if (st == VP_OK) st = vp_build_walk_tree(ctx, sc);
     if (st != VP_OK) {
         vp_scene_destroy(sc);

@@ -1,4 +1,5 @@
-// vp_bvh_sah.cpp — optional surface-area-heuristic BVH (VP_SCENE_BVH_SAH).
+// vp_bvh_sah.cpp — optional host-built trees: a surface-area-heuristic BVH
+// (VP_SCENE_BVH_SAH) and a balanced median-split tree (VP_SCENE_BVH_MEDIAN).
 //
 // The default scene build is the GPU LBVH of vp_bvh.hip.  Its splits follow
 // Morton-code bits, which leaves sibling boxes overlapping when primitive
@@ -142,9 +143,74 @@ int32_t build(Tree& T, std::vector<uint32_t>& idx, uint32_t a, uint32_t b, const
     return root;
 }
 
+// Balanced median splits over idx[0, n): a range of at most `cap` leaf slots
+// (cap a power of two) splits into cap/2 primitives and the rest, so every
+// split falls on the boundary of a 4-wide walk-tree record (whose groups
+// are 4^k consecutive leaves starting at a multiple of 4^k) and every
+// record's children hold disjoint, spatially split quarters of its
+// primitives.  The split axis is the one whose two halves have the smaller
+// summed box surface; within an axis the order is (centre, input index), a
+// strict order, so the tree does not depend on how ties are broken.
+int32_t build_median(Tree& T, std::vector<uint32_t>& idx, uint32_t n, const std::vector<Box>& boxes,
+                     const std::vector<float>& cen) {
+    struct Item {
+        uint32_t a, b, cap;
+        int32_t node;
+    };
+    uint32_t cap0 = 1;
+    while (cap0 < n) cap0 *= 4u;
+    const int32_t root = static_cast<int32_t>(T.nodes.size());
+    T.nodes.emplace_back();
+    std::vector<Item> stack{{0u, n, cap0, root}};
+    std::vector<uint32_t> trial[3];
+    while (!stack.empty()) {
+        Item it = stack.back();
+        stack.pop_back();
+        const uint32_t cnt = it.b - it.a;
+        while (it.cap > 1u && cnt <= it.cap / 2u) it.cap /= 2u;
+        if (cnt == 1) {
+            T.nodes[it.node].box = boxes[idx[it.a]];
+            T.nodes[it.node].prim = static_cast<int32_t>(idx[it.a]);
+            continue;
+        }
+        const uint32_t nl = it.cap / 2u;  // cnt > nl here
+        double best = 1e300;
+        int best_axis = 0;
+        Box nb;
+        for (uint32_t i = it.a; i < it.b; ++i) nb.grow(boxes[idx[i]]);
+        T.nodes[it.node].box = nb;
+        for (int ax = 0; ax < 3; ++ax) {
+            std::vector<uint32_t>& t = trial[ax];
+            t.assign(idx.begin() + it.a, idx.begin() + it.b);
+            std::nth_element(t.begin(), t.begin() + nl, t.end(), [&](uint32_t p, uint32_t q) {
+                const float cp = cen[3 * p + ax], cq = cen[3 * q + ax];
+                return cp < cq || (cp == cq && p < q);
+            });
+            Box l, r;
+            for (uint32_t i = 0; i < nl; ++i) l.grow(boxes[t[i]]);
+            for (uint32_t i = nl; i < cnt; ++i) r.grow(boxes[t[i]]);
+            const double cost = l.area() + r.area();
+            if (cost < best) {
+                best = cost;
+                best_axis = ax;
+            }
+        }
+        std::copy(trial[best_axis].begin(), trial[best_axis].end(), idx.begin() + it.a);
+        const int32_t l = static_cast<int32_t>(T.nodes.size());
+        T.nodes.emplace_back();
+        const int32_t r = static_cast<int32_t>(T.nodes.size());
+        T.nodes.emplace_back();
+        T.nodes[it.node].left = l;
+        T.nodes[it.node].right = r;
+        stack.push_back({it.a + nl, it.b, it.cap / 2u, r});
+        stack.push_back({it.a, it.a + nl, it.cap / 2u, l});
+    }
+    return root;
+}
+
 }  // namespace
 
-vp_status vp_build_scene_sah(vp_context* ctx, const vp_primitive* in, uint32_t n, vp_scene* sc) {
+vp_status vp_build_scene_sah(vp_context* ctx, const vp_primitive* in, uint32_t n, vp_scene* sc, bool median) {
     // records and conservative boxes exactly as the GPU setup kernel makes them
     std::vector<vp::PrimRecord> rec(n);
     std::vector<Box> boxes(n);
@@ -162,7 +228,10 @@ vp_status vp_build_scene_sah(vp_context* ctx, const vp_primitive* in, uint32_t n
     T.nodes.reserve(2 * static_cast<size_t>(n));
     std::vector<uint32_t> idx(n);
     for (uint32_t i = 0; i < n; ++i) idx[i] = i;
-    build(T, idx, 0, n, boxes, cen);
+    if (median)
+        build_median(T, idx, n, boxes, cen);
+    else
+        build(T, idx, 0, n, boxes, cen);
     // depth-first layout with skip links; leaves numbered in visiting order
     const size_t nn = 2 * static_cast<size_t>(n) - 1;
     std::vector<float4> nodes(2 * nn);

@@ -241,8 +241,11 @@ void vp_set_error(const std::string& msg);
 
 // launchers (vp_bvh.hip)
 vp_status vp_build_scene_device(vp_context* ctx, const vp_primitive* host_prims, uint32_t n, vp_scene* scene);
-// host SAH tree replacing the device LBVH's nodes, records and order (vp_bvh_sah.cpp)
-vp_status vp_build_scene_sah(vp_context* ctx, const vp_primitive* host_prims, uint32_t n, vp_scene* scene);
+// host tree replacing the device LBVH's nodes, records and order (vp_bvh_sah.cpp):
+// binned SAH, or (median) balanced median splits on the walk tree's record
+// boundaries
+vp_status vp_build_scene_sah(vp_context* ctx, const vp_primitive* host_prims, uint32_t n, vp_scene* scene,
+                             bool median);
 // walk tree over the scene's stored order, from its rope tree's leaf boxes (vp_bvh.hip)
 vp_status vp_build_walk_tree(vp_context* ctx, vp_scene* scene);
 

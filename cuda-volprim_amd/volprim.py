@@ -101,6 +101,7 @@ EXPORTED = [
     "vp_scene_create_ex",
 ]
 SCENE_BVH_SAH = 1
+SCENE_BVH_MEDIAN = 2
 EVENT_NAMES = ["active_overflow", "max_hits", "max_overlap", "paths", "traces", "scatters", "shadow_rays",
                "arg_hit", "arg_active", "arg_accept", "rb_active", "rb_hit", "alb", "sh_active", "sh_hit",
                "leaf_tests", "node_visits", "contain", "visits_shadow", "visits_point"]
@@ -209,7 +210,7 @@ class Scene:
         self.n = self.rows.shape[0]
         prims = primitives_from_rows(self.rows)
         h = _vp()
-        flags = {"lbvh": 0, "sah": SCENE_BVH_SAH}[bvh]
+        flags = {"lbvh": 0, "sah": SCENE_BVH_SAH, "median": SCENE_BVH_MEDIAN}[bvh]
         _check(load().vp_scene_create_ex(ctx.handle, ctypes.cast(prims, _vp), self.n, flags, ctypes.byref(h)),
                "vp_scene_create_ex")
         self.handle = h
@@ -281,7 +282,8 @@ class Context:
         return buf.value.decode(), cus.value, mhz.value
 
     def scene(self, rows14, bvh="lbvh"):
-        """bvh: "lbvh" (GPU build, default) or "sah" (host SAH tree, same layout)."""
+        """bvh: "lbvh" (GPU build, default), "sah" (host SAH tree, same layout) or
+        "median" (host tree of median splits on the walk tree's record boundaries)."""
         return Scene(self, rows14, bvh)
 
     def env(self, rgba):

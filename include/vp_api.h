@@ -123,8 +123,14 @@ vp_status vp_scene_create(vp_context* ctx, const vp_primitive* prims, uint32_t n
 /* vp_scene_create with build flags (no counterpart: the reference's OptiX
  * builder has one mode).  VP_SCENE_BVH_SAH replaces the GPU LBVH's tree with
  * a binned surface-area-heuristic tree built on the host, in the same
- * layout; renders of the two differ only through the stored primitive order. */
+ * layout; renders of the two differ only through the stored primitive order.
+ * VP_SCENE_BVH_MEDIAN instead builds, on the host, a balanced tree of median
+ * splits that fall on the 4-wide walk tree's record boundaries (each split
+ * on the axis whose halves have the smaller summed box surface), so each
+ * walk-tree record's children are spatial quarters of its primitives.  The
+ * two flags exclude each other. */
 #define VP_SCENE_BVH_SAH 1u
+#define VP_SCENE_BVH_MEDIAN 2u
 vp_status vp_scene_create_ex(vp_context* ctx, const vp_primitive* prims, uint32_t n, uint32_t flags,
                              vp_scene** out);
 vp_status vp_scene_destroy(vp_scene* scene);

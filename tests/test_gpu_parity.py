@@ -243,12 +243,13 @@ def test_gpu_transmittance_vs_fp64_golden(ctx, case):
     assert (np.abs(T - Tg) / Tg)[mask].max() < 1e-5
 
 
-# ---- the optional host SAH tree: same layout, same results for its stored order ----
-@pytest.mark.parametrize("n,seed", [(1, 0), (2, 1), (300, 3), (4096, 4)])
-def test_sah_scene_is_a_permutation_with_exact_records(ctx, n, seed):
+# ---- the optional host trees (SAH, median splits): same layout, same results for their stored order ----
+@pytest.mark.parametrize("bvh", ["sah", "median"])
+@pytest.mark.parametrize("n,seed", [(1, 0), (2, 1), (5, 2), (300, 3), (4096, 4)])
+def test_sah_scene_is_a_permutation_with_exact_records(ctx, bvh, n, seed):
     rows = scenes.c1_scattering() if n == 4096 else scenes.random_scene(n, seed)
-    print(f"[case] SAH scene build n={n}", flush=True)
-    sc = ctx.scene(rows, bvh="sah")
+    print(f"[case] {bvh} scene build n={n}", flush=True)
+    sc = ctx.scene(rows, bvh=bvh)
     order, morton, rec, depth = sc.export()
     sc.close()
     assert np.array_equal(np.sort(order), np.arange(n, dtype=np.uint32))
@@ -256,13 +257,81 @@ def test_sah_scene_is_a_permutation_with_exact_records(ctx, n, seed):
     assert_bitwise(rec, vpo.make_prims(rows)[order], "primitive records")
 
 
-def test_sah_overlaps_and_transmittance_bitexact(ctx):
+def _three_sigma_boxes(rows):
+    # axis-aligned 3-sigma boxes of the rows' Gaussians (half extent along
+    # world axis a: 3 sqrt(sum_j (R_aj s_j)^2)), in float64
+    c = rows[:, 0:3].astype(np.float64)
+    w, x, y, z = (rows[:, 3:7].astype(np.float64) / np.linalg.norm(rows[:, 3:7], axis=1, keepdims=True)).T
+    s = rows[:, 7:10].astype(np.float64)
+    R = np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], 1),
+                  np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], 1),
+                  np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], 1)], 1)
+    h = 3.0 * np.sqrt(((R * R) * (s * s)[:, None, :]).sum(2))
+    return c - h, c + h
+
+
+@pytest.mark.parametrize("n", [300, 1000])
+def test_median_order_splits_on_record_boundaries(ctx, n):
+    # every aligned group of 2^k stored leaves whose second half is not empty
+    # is split in two by centre along some axis, ties going by input index:
+    # the halves (and so the walk tree's 4-wide records, which group 4^k
+    # aligned leaves) are spatial splits
+    rows = scenes.random_scene(n, 21)
+    print(f"[case] median order structure n={n}", flush=True)
+    sc = ctx.scene(rows, bvh="median")
+    order, _, _, _ = sc.export()
+    sc.close()
+    cen = rows[:, 0:3].astype(np.float32)
+    size = 2
+    while size // 2 < n:
+        for start in range(0, n, size):
+            left = order[start:start + size // 2]
+            right = order[start + size // 2:start + size]
+            if len(right) == 0:
+                continue
+            ok = False
+            for ax in range(3):
+                lk = sorted(zip(cen[left, ax].tolist(), left.tolist()))
+                rk = sorted(zip(cen[right, ax].tolist(), right.tolist()))
+                ok |= lk[-1] < rk[0]
+            assert ok, (size, start)
+        size *= 2
+
+
+def test_median_order_has_smaller_walk_tree_surface_than_sah(ctx):
+    # the surface-area cost of the 4-wide walk tree (sum over records of the
+    # record box's surface relative to the root's) on the 4096-Gaussian
+    # scattering scene: the median order's must be below the SAH order's
+    rows = scenes.c1_scattering(n=4096, seed=1, sigma_multiplier=10.0, albedo=0.9)
+    print("[case] median vs SAH walk-tree surface, 4096 Gaussians", flush=True)
+
+    def cost(bvh):
+        sc = ctx.scene(rows, bvh=bvh)
+        order, _, _, _ = sc.export()
+        sc.close()
+        lo, hi = _three_sigma_boxes(rows)
+        lo, hi = lo[order], hi[order]
+        area = lambda a, b: 2.0 * ((b - a)[..., 0] * (b - a)[..., 1] + (b - a)[..., 1] * (b - a)[..., 2] +
+                                   (b - a)[..., 2] * (b - a)[..., 0])
+        total, root = 0.0, area(lo.min(0), hi.max(0))
+        while lo.shape[0] > 4:
+            lo, hi = lo.reshape(-1, 4, 3).min(1), hi.reshape(-1, 4, 3).max(1)
+            total += area(lo, hi).sum() / root
+        return total
+
+    c_sah, c_med = cost("sah"), cost("median")
+    print(f"walk-tree surface: sah {c_sah:.1f}, median {c_med:.1f}", flush=True)
+    assert c_med < 0.9 * c_sah
+
+
+@pytest.mark.parametrize("bvh", ["sah", "median"])
+def test_sah_overlaps_and_transmittance_bitexact(ctx, bvh):
     rng = np.random.default_rng(77)
     rows = scenes.random_scene(400, 12)
-    sc = ctx.scene(rows, bvh="sah")
+    sc = ctx.scene(rows, bvh=bvh)
     _, _, rec, _ = sc.export()
     rays = _random_rays(rng, 3000, 1.5)
-    print("[case] SAH overlaps + transmittance n=400", flush=True)
+    print(f"[case] {bvh} overlaps + transmittance n=400", flush=True)
     cg, tg, ig = ctx.collect_overlaps(sc, rays, 512)
     T = ctx.transmittance_rays(sc, rays)
     sc.close()
@@ -274,13 +343,14 @@ def test_sah_overlaps_and_transmittance_bitexact(ctx):
     assert_bitwise(T, vpo.transmittance_rays(rec, rays), "transmittance")
 
 
+@pytest.mark.parametrize("bvh", ["sah", "median"])
 @pytest.mark.parametrize("case", ["mis_sky", "small_cap", "ris_sky"])
-def test_render_sah_bitexact(ctx, case):
+def test_render_sah_bitexact(ctx, case, bvh):
     c = RENDER_CASES[case]
     env = scenes.constant_env() if c["env"] == "const" else scenes.sky_env(64, 32, sun_radius_deg=4.0, sun_scale=50.0)
     cam = volprim.camera_perspective(**scenes.default_camera_args(c["w"], c["h"]))
-    print(f"[case] SAH render {case}", flush=True)
-    sc = ctx.scene(c["rows"](), bvh="sah")
+    print(f"[case] {bvh} render {case}", flush=True)
+    sc = ctx.scene(c["rows"](), bvh=bvh)
     ev = ctx.env(env)
     _, _, rec, _ = sc.export()
     p = volprim.default_params(width=c["w"], height=c["h"], spp=c["spp"], **c["kw"])
@@ -291,14 +361,14 @@ def test_render_sah_bitexact(ctx, case):
                            rr_depth=p.rr_depth, rr_max_survival=p.rr_max_survival, firefly_clamp=p.firefly_clamp,
                            filter_stddev=p.filter_stddev, phase_g=p.phase_g, use_ris=bool(p.use_ris),
                            ris_candidates=p.ris_candidates, active_cap=p.active_cap)
-    assert_bitwise(img, ref, f"SAH render {case}")
+    assert_bitwise(img, ref, f"{bvh} render {case}")
     assert st["active_overflow"] == k["active_overflow"]
 
 
 def test_full_size_scattering_workload(ctx):
     # the measured workload at its full size: 4096 Gaussians (seed 1,
     # sigma multiplier 10, albedo 0.9, isotropic phase), 2048x1024 sky + sun,
-    # 512x512 at 64 spp, depth 64, RR from 12, SAH tree.  The megakernel and
+    # 512x512 at 64 spp, depth 64, RR from 12, median-split tree.  The megakernel and
     # the wavefront pipeline must agree on every pixel; every 32nd row and
     # column is checked against the oracle (256 pixels x 64 samples).
     import os
@@ -307,7 +377,7 @@ def test_full_size_scattering_workload(ctx):
     cam = volprim.camera_perspective(**scenes.default_camera_args(512, 512, dist=3.5, fov=40.0))
     p = volprim.default_params(width=512, height=512, spp=64, seed=0, max_depth=64, rr_depth=12, phase_g=0.0)
     print("[case] full-size scattering 4096 Gaussians 512x512x64 megakernel + wavefront", flush=True)
-    sc = ctx.scene(rows, bvh="sah")
+    sc = ctx.scene(rows, bvh="median")
     ev = ctx.env(env)
     _, _, rec, _ = sc.export()
     img, st = ctx.render(sc, ev, cam, p)
