@@ -76,7 +76,10 @@ Renderer::Renderer(const Config& config, std::vector<vp_primitive>&& primitives,
     if (config_.denoise)
         die("--denoise: the OptiX AI denoiser has no counterpart in this build (the AOVs it needs are not produced)");
     check(vp_context_create(0, &ctx_), "vp_context_create");
-    check(vp_scene_create(ctx_, primitives.data(), num_primitives_, &scene_), "vp_scene_create");
+    // the walk tree over the median-split leaf order: a host build (about 1 s for 10^6 Gaussians)
+    // that the render repays (fewer node visits per ray than the GPU LBVH's Morton order)
+    check(vp_scene_create_ex(ctx_, primitives.data(), num_primitives_, VP_SCENE_BVH_MEDIAN, &scene_),
+          "vp_scene_create_ex");
     std::vector<float> env;
     uint32_t ew = 0, eh = 0;
     std::string err;

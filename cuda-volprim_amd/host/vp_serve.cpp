@@ -325,7 +325,8 @@ class Service {
         vp_primitive* prims = nullptr;
         uint32_t n = 0;
         if (vp_io_load_ply(path.c_str(), sigma, nullptr, &prims, &n) != 0) return err = vp_io_last_error(), false;
-        const vp_status st = vp_scene_create(ctx_, prims, n, &out);
+        // median-split leaf order (host build, once per cached scene)
+        const vp_status st = vp_scene_create_ex(ctx_, prims, n, VP_SCENE_BVH_MEDIAN, &out);
         vp_io_free(prims);
         if (st != VP_OK) return err = vp_last_error(), false;
         scenes_[key] = out;

@@ -270,6 +270,20 @@ def _three_sigma_boxes(rows):
     return c - h, c + h
 
 
+@pytest.mark.parametrize("flags,msg", [(volprim.SCENE_BVH_SAH | volprim.SCENE_BVH_MEDIAN, "exclude each other"),
+                                       (4, "unknown scene flags")])
+def test_scene_flags_rejected(ctx, flags, msg):
+    import ctypes
+    rows = scenes.random_scene(8, 1)
+    prims = volprim.primitives_from_rows(volprim.f32(rows))
+    h = ctypes.c_void_p()
+    print(f"[case] scene flags {flags} rejected", flush=True)
+    rc = volprim.load().vp_scene_create_ex(ctx.handle, ctypes.cast(prims, ctypes.c_void_p), 8, flags,
+                                           ctypes.byref(h))
+    assert rc != volprim.VP_OK and not h.value
+    assert msg in volprim.load().vp_last_error().decode()
+
+
 @pytest.mark.parametrize("n", [300, 1000])
 def test_median_order_splits_on_record_boundaries(ctx, n):
     # every aligned group of 2^k stored leaves whose second half is not empty

@@ -51,7 +51,7 @@ def test_served_requests_match_direct_renders(tmp_path, ctx):
     assert all(resp[r["id"]]["status"] == "ok" for r in reqs), out
     assert out[-1]["stats"]["completed"] == 3
     # the same renders through the C ABI directly
-    sc = ctx.scene(vpio.load_ply(ply, 1.0))
+    sc = ctx.scene(vpio.load_ply(ply, 1.0), bvh="median")
     env = ctx.env(np.ones((32, 64, 4), np.float32))
     for r in reqs:
         c = r["camera"]
