@@ -1,18 +1,26 @@
-"""Benchmark: Msamples/s of the Gaussian-mixture volumetric path tracer on one
-MI355X, on the scattering workload (4096 anisotropic Gaussians, 512x512,
-64 spp, max depth 64, Russian roulette from depth 12, albedo 0.9, isotropic
-phase, 2048x1024 procedural sky environment with a 0.5 degree sun).
+"""Benchmark of the Gaussian-mixture volumetric path tracer on one MI355X.
 
-A step is one full render of that image (512*512*64 = 16.8 M camera paths)
-with the scene, BVH (the median-split tree unless --bvh says otherwise) and environment already
-resident in HBM.  Prints one
-JSON line with the throughput, the roofline of the render kernel (FP32 VALU:
-algorithmic FLOPs from the event-counting kernel variant weighted as
-documented in DESIGN.md, over the kernel time measured with HIP events on
-the library stream) and the CPU baseline (the oracle's C restatement of the
-estimator with its CPU BVH, on a bounded pixel sample, all host cores).
+Default: the served request stream.  Render requests drawn from the batched
+job mix (resolution uniform over {64,...,1024}^2, spp round(exp U[ln 4,
+ln 256]) divided by --spp-div, scene uniform over the 4096-Gaussian
+scattering scene, the 10^6-Gaussian cloud and the 2.5*10^5-Gaussian torus,
+camera on a sphere of radius 3.5) arrive as a Poisson stream at an offered
+load of the batched capacity; one step is one request (--steps requests are
+timed, --warmup smaller ones go first).  value = the served aggregate
+Msamples/s from the first submission to the last result, with scenes, BVHs
+and environment tables resident.  Beside it: plain one-launch-per-request
+rendering and one batched launch of the same requests (all three images are
+checked bit-identical), p50/p99 latency, the roofline of the batched
+k_render launch (FP32 VALU: algorithmic FLOPs from the event-counting kernel
+variant weighted as documented in DESIGN.md, over the launch's kernel time
+from HIP events on the library stream), a per-scene image-error estimate and
+the CPU baseline (the oracle's C restatement of the estimator with its CPU
+BVH, on a bounded pixel sample, all host cores).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+--single {scattering,cloud,torus}: one scene rendered whole per step
+(512x512x64 for scattering, 1024x1024x4 for cloud and torus), for profiling.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--spp-div D] [--no-cpu]
 """
 import argparse
 import importlib.util
@@ -53,12 +61,32 @@ def load_pkg():
     return mod
 
 
-def workload(scenes):
-    rows = scenes.c1_scattering(n=4096, seed=1, sigma_multiplier=10.0, albedo=0.9)
-    env = scenes.sky_env(2048, 1024)
-    cam_args = scenes.default_camera_args(512, 512, dist=3.5, fov=40.0)
-    params = dict(width=512, height=512, spp=64, seed=0, max_depth=64, rr_depth=12, phase_g=0.0)
-    return rows, env, cam_args, params
+# FLOP weights that differ from the per-event derivations of the reference
+# formulas, and why (the counted events are this build's own work):
+WEIGHTS_NOTE = ("arg_accept 0 (not 122) and rb_active 1 (not 92): the exit distance a candidate or an origin-set "
+                "member is compared with is computed once during the argmin walk and charged there, so the accept "
+                "and the rebuild's membership test are a compare each")
+
+
+def workload(scenes, name):
+    """--single workloads: rows, camera arguments, render parameters, description."""
+    if name == "scattering":
+        rows = scenes.c1_scattering(n=4096, seed=1, sigma_multiplier=10.0, albedo=0.9)
+        return (rows, scenes.default_camera_args(512, 512, dist=3.5, fov=40.0),
+                dict(width=512, height=512, spp=64, seed=0, max_depth=64, rr_depth=12, phase_g=0.0, active_cap=128),
+                "scattering: 4096 anisotropic Gaussians (seed 1), sigma multiplier 10, albedo 0.9, isotropic "
+                "phase, 2048x1024 sky + sun environment, 512x512, 64 spp, max depth 64, RR from depth 12")
+    if name == "cloud":
+        rows = scenes.c2_cloud(n=1_000_000, seed=2, sigma_multiplier=7.5, albedo=0.99)
+        return (rows, scenes.default_camera_args(1024, 1024, dist=3.5, fov=40.0),
+                dict(width=1024, height=1024, spp=4, seed=0, max_depth=64, rr_depth=12, phase_g=0.85, active_cap=384),
+                "cloud: 10^6 Gaussians (fBm noise, seed 2), sigma multiplier 7.5, albedo 0.99, HG g 0.85, "
+                "2048x1024 sky + sun environment, 1024x1024, 4 spp, max depth 64, RR from depth 12")
+    rows = scenes.torus_shell(n=250_000, seed=3, sigma_multiplier=10.0, albedo=0.95)
+    return (rows, scenes.default_camera_args(1024, 1024, dist=3.5, fov=40.0),
+            dict(width=1024, height=1024, spp=4, seed=0, max_depth=64, rr_depth=12, phase_g=0.0, active_cap=384),
+            "torus: 2.5*10^5 Gaussians on a torus shell (seed 3), sigma multiplier 10, albedo 0.95, isotropic "
+            "phase, 2048x1024 sky + sun environment, 1024x1024, 4 spp, max depth 64, RR from depth 12")
 
 
 def cpu_baseline(rows, env, cam_pod, params, threads, budget_s=20.0, repeats=5):
@@ -78,7 +106,7 @@ def cpu_baseline(rows, env, cam_pod, params, threads, budget_s=20.0, repeats=5):
         vpo.render(rec, env, cam_pod, params["width"], params["height"], params["spp"], seed=params["seed"],
                    max_depth=params["max_depth"], rr_depth=params["rr_depth"], phase_g=params["phase_g"],
                    use_ris=bool(params.get("use_ris", 0)), stride=stride, samples=(0, spp_used), threads=threads,
-                   env_tables_=tabs, bvh=True)
+                   env_tables_=tabs, bvh=True, active_cap=params.get("active_cap", 128))
         return time.perf_counter() - t0
 
     probe = run(1)
@@ -90,7 +118,8 @@ def cpu_baseline(rows, env, cam_pod, params, threads, budget_s=20.0, repeats=5):
     rates = sorted(n / t / 1e6 for t in times)
     return {"value": float(np.median(rates)), "unit": "Msamples/s", "cores": threads, "kind": "port",
             "spread": [rates[0], rates[-1]],
-            "sample": f"{npx} pixels (every {stride}th row and column of 512x512) x {spp_used} spp = {n} paths per "
+            "sample": f"{npx} pixels (every {stride}th row and column of {params['width']}x{params['height']}) x "
+                      f"{spp_used} spp = {n} paths per "
                       f"repeat, 1 warm-up + {repeats} timed repeats (median); CPU BVH (binary, Morton-ordered) queries"}
 
 
@@ -106,8 +135,8 @@ def measure_traffic(args):
     prof = shutil.which("rocprofv3")
     if prof is None:
         return None, "rocprofv3 not found"
-    child = [sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu", "--no-count",
-             "--no-quality", "--no-traffic", "--copy-calibration", "--bvh", args.bvh]
+    child = [sys.executable, os.path.abspath(__file__), "--single", args.single, "--steps", "1", "--warmup", "0",
+             "--no-cpu", "--no-count", "--no-quality", "--no-traffic", "--copy-calibration", "--bvh", args.bvh]
     if args.block or args.bpc or args.regen:
         child += ["--block", str(args.block), "--bpc", str(args.bpc), "--regen", str(args.regen)]
     out = {}
@@ -141,114 +170,213 @@ def measure_traffic(args):
 
 
 MIX_ACTIVE_CAP = {"scattering": 128, "cloud": 384, "torus": 384}
+MIX_SCENES = {"scattering": "4096 anisotropic Gaussians (seed 1, sigma multiplier 10, albedo 0.9, isotropic phase)",
+              "cloud": "10^6 Gaussians from 5-octave fBm noise in a 2x1x1 box (seed 2, sigma multiplier 7.5, "
+                       "albedo 0.99, HG g 0.85)",
+              "torus": "2.5*10^5 Gaussians on a torus shell R 1 r 0.3 (seed 3, sigma multiplier 10, albedo 0.95)"}
 
 
-def run_mix(args):
-    """The job mix three ways on resident scenes: plain (one vp_render per
-    request, in order), batched (every job in one vp_render_jobs call with
-    the autotuned launch), served (a Poisson request stream into vp_server).
-    All three must return bit-identical images."""
-    import threading
-    from volprim_amd import scenes, volprim
-
-    t0 = time.perf_counter()
+def mix_scenes(ctx, scenes, bvh, names):
     gens = {"scattering": lambda: scenes.c1_scattering(n=4096, seed=1, sigma_multiplier=10.0, albedo=0.9),
             "cloud": lambda: scenes.c2_cloud(n=1_000_000, seed=2, sigma_multiplier=7.5, albedo=0.99),
             "torus": lambda: scenes.torus_shell(n=250_000, seed=3, sigma_multiplier=10.0, albedo=0.95)}
-    jobs = scenes.mix_jobs(n_jobs=args.mix_jobs, seed=4, res_div=args.mix_res_div, spp_div=args.mix_spp_div)
-    ctx = volprim.Context(0)
-    dev_name = ctx.device_info()[0]
-    handles = {k: ctx.scene(g(), bvh=args.bvh) for k, g in gens.items() if any(j["scene"] == k for j in jobs)}
-    envh = ctx.env(scenes.sky_env(2048, 1024))
-    print(f"[mix] {len(jobs)} jobs, scenes and environment resident after {time.perf_counter() - t0:.1f} s",
-          file=sys.stderr, flush=True)
+    rows = {k: gens[k]() for k in names}
+    return {k: ctx.scene(r, bvh=bvh) for k, r in rows.items()}, rows
+
+
+def mix_requests(volprim, jobs, handles, envh):
     reqs = []
     for j in jobs:
         p = volprim.default_params(width=j["width"], height=j["height"], spp=j["spp"], seed=j["seed"], max_depth=64,
                                    rr_depth=12, phase_g=j["phase_g"], active_cap=MIX_ACTIVE_CAP[j["scene"]])
         reqs.append((handles[j["scene"]], envh, volprim.camera_perspective(**j["cam"]), p))
-    samples = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
+    return reqs
 
-    # plain: one request at a time, one render call each
+
+def serve(ctx, reqs, arrivals, mem_gib=16, max_inflight=64):
+    """Submit reqs at the given arrival times (seconds from the start) to a
+    vp_server; returns (seconds from the first submission to the last
+    result, per-request latencies in ms, server stats, images)."""
+    import threading
+    srv = ctx.server(mem_budget_bytes=mem_gib << 30, max_inflight=max_inflight)
+    tickets = [None] * len(reqs)
     t0 = time.perf_counter()
-    plain_imgs, plain_lat = [], []
+
+    def feeder():
+        for i, r in enumerate(reqs):
+            d = t0 + arrivals[i] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets[i] = srv.submit(*r)
+
+    th = threading.Thread(target=feeder)
+    th.start()
+    th.join()
+    srv.drain()
+    dt = time.perf_counter() - t0
+    lat = [srv.poll(t[0]) for t in tickets]
+    st = srv.stats()
+    srv.close()
+    return dt, lat, st, [t[1] for t in tickets]
+
+
+def rel_rmse_est(ctx, torch, req):
+    """Relative RMSE of one request's image from the per-pixel sample variance
+    of the same render (an unbiased estimate of its MSE against the converged
+    image), and the spp that would reach 1 %."""
+    sc, envh, cam, p = req
+    out = torch.empty((p.height, p.width, 4), dtype=torch.float32, device="cuda")
+    smp = torch.empty((p.spp, p.height, p.width, 4), dtype=torch.float32, device="cuda")
+    ctx.render_device(sc, envh, cam, p, out.data_ptr(), samples_ptr=smp.data_ptr())
+    torch.cuda.synchronize()
+    rgb = smp[..., :3].double()
+    var_mean = rgb.var(dim=0, unbiased=True) / p.spp if p.spp > 1 else torch.zeros_like(rgb[0])
+    rel = float(var_mean.mean().sqrt() / out[..., :3].double().mean())
+    return {"res": p.width, "spp": p.spp, "rel_rmse_est": round(rel, 5),
+            "spp_for_1pct_est": int(np.ceil(p.spp * (rel / 0.01) ** 2))}
+
+
+def run_stream(args):
+    """The default measurement: the served Poisson request stream of the mixed
+    workload, with plain one-launch-per-request rendering and the batched
+    one-launch form of the same requests beside it.  All three return
+    bit-identical images (checked), so they are compared at matched RMSE."""
+    import torch
+    from volprim_amd import scenes, volprim
+
+    t_setup = time.perf_counter()
+    ctx = volprim.Context(0)
+    dev_name = ctx.device_info()[0]
+    K, W = args.steps, args.warmup
+    jobs = scenes.mix_jobs(n_jobs=K, seed=args.mix_seed, res_div=args.res_div, spp_div=args.spp_div)
+    # warm-up requests: the same distribution at 1/8 resolution and 1 spp
+    # (first launches load code objects and grow the allocator pools)
+    wjobs = scenes.mix_jobs(n_jobs=W, seed=args.mix_seed + 100, res_div=8, spp_div=256) if W else []
+    names = sorted({j["scene"] for j in jobs + wjobs})
+    handles, rows = mix_scenes(ctx, scenes, args.bvh, names)
+    env_rgba = scenes.sky_env(2048, 1024)
+    envh = ctx.env(env_rgba)
+    reqs = mix_requests(volprim, jobs, handles, envh)
+    wreqs = mix_requests(volprim, wjobs, handles, envh)
+    samples = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
+    print(f"[stream] {K} requests ({samples} samples) over {names}, scenes resident after "
+          f"{time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
+
+    # warm-up: the server path and the plain path once each
+    if wreqs:
+        serve(ctx, wreqs, np.zeros(len(wreqs)))
+        for r in wreqs:
+            ctx.render(*r)
+    torch.cuda.synchronize()
+
+    # plain: one vp_render per request, in arrival order (the reference's
+    # one-launch-per-request renderer, scenes kept resident)
+    t0 = time.perf_counter()
+    plain_imgs, plain_ms = [], []
     for r in reqs:
         s = time.perf_counter()
         img, st = ctx.render(*r)
-        if st["active_overflow"] or st["hit_overflow"]:
+        if st["active_overflow"]:
             raise RuntimeError(f"active-set overflow in the plain run: {st}")
         plain_imgs.append(img)
-        plain_lat.append((time.perf_counter() - s) * 1e3)
+        plain_ms.append((time.perf_counter() - s) * 1e3)
     t_plain = time.perf_counter() - t0
-    print(f"[mix] plain {t_plain:.2f} s", file=sys.stderr, flush=True)
+    print(f"[stream] plain {t_plain:.2f} s", file=sys.stderr, flush=True)
 
-    # batched: all jobs in one call, launch shape from the autotuner
+    # batched: every request in one call, launch shape from the autotuner
     t0 = time.perf_counter()
     tuned = ctx.autotune(reqs)
     t_tune = time.perf_counter() - t0
     t0 = time.perf_counter()
     batch_imgs, bst = ctx.render_jobs(reqs, launch=tuned)
     t_batch = time.perf_counter() - t0
-    print(f"[mix] batched {t_batch:.2f} s (autotune {t_tune:.1f} s)", file=sys.stderr, flush=True)
+    print(f"[stream] batched {t_batch:.2f} s (autotune {t_tune:.1f} s, "
+          f"{'wavefront' if tuned.pipeline else 'megakernel'})", file=sys.stderr, flush=True)
 
-    # served: (1) every request submitted at once (capacity), (2) a Poisson
-    # stream at half the batched capacity (latency under load)
-    def serve(arrivals):
-        srv = ctx.server(mem_budget_bytes=args.mix_mem_gib << 30, max_inflight=64)
-        tickets = [None] * len(reqs)
-        t0 = time.perf_counter()
+    # served: a Poisson stream at the stated offered load of the batched
+    # capacity (load 1: requests arrive as fast as the batched form finishes
+    # them, so the GPU has work but the queue does not start full)
+    lam = args.load * K / t_batch
+    arrivals = np.cumsum(np.random.default_rng(args.mix_seed + 1).exponential(1.0 / lam, size=K))
+    arrivals -= arrivals[0]
+    t_served, lat, sst, served_imgs = serve(ctx, reqs, arrivals, mem_gib=args.mem_gib)
+    print(f"[stream] served {t_served:.2f} s, {sst['launches']} launches", file=sys.stderr, flush=True)
+    same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) and
+               np.array_equal(a.view(np.uint32), c.view(np.uint32))
+               for a, b, c in zip(plain_imgs, batch_imgs, served_imgs))
 
-        def feeder():
-            for i, r in enumerate(reqs):
-                d = t0 + arrivals[i] - time.perf_counter()
-                if d > 0:
-                    time.sleep(d)
-                tickets[i] = srv.submit(*r)
+    # roofline of the dominant kernel: k_render over the batched table.
+    # Algorithmic FLOPs from the event-counting build run on the same jobs
+    # (counts are per path, so they do not depend on what shares a launch),
+    # over the launch's kernel time from HIP events on the library stream.
+    roofline, events = None, None
+    if not args.no_count:
+        events = {k: 0 for k in volprim.EVENT_NAMES}
+        for r in reqs:
+            ev = ctx.count_events(*r)
+            for k in events:
+                events[k] += ev[k]
+        flops = sum(FLOP_WEIGHTS[k] * events[k] for k in FLOP_WEIGHTS)
+        kms = bst["kernel_ms"] / max(bst["launches"], 1)
+        achieved = flops / max(bst["launches"], 1) / (kms / 1e3) / 1e12
+        roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                    "traffic_note": "HBM bytes of this launch are not measured in the default run; "
+                                    "profiles/ holds rocprofv3 FETCH_SIZE/WRITE_SIZE runs per workload",
+                    "peak_nonpacked": PEAK_FP32_NONPACKED_TFLOPS, "kernel": "k_render (batched launch of the stream)",
+                    "avg_launch_ms": round(kms, 3), "flops_per_launch": flops / max(bst["launches"], 1),
+                    "weights_note": WEIGHTS_NOTE}
 
-        th = threading.Thread(target=feeder)
-        th.start()
-        th.join()
-        srv.drain()
-        dt = time.perf_counter() - t0
-        lat = [srv.poll(t[0]) for t in tickets]
-        st = srv.stats()
-        srv.close()
-        return dt, lat, st, [t[1] for t in tickets]
+    quality = None
+    if not args.no_quality:
+        quality = {}
+        for name in names:
+            i = next((k for k, j in enumerate(jobs) if j["scene"] == name), None)
+            if i is not None:
+                quality[name] = rel_rmse_est(ctx, torch, reqs[i])
+        quality["method"] = "per-pixel sample variance / spp of the first request of each scene"
 
-    t_sat, lat_sat, st_sat, served_imgs = serve(np.zeros(len(reqs)))
-    print(f"[mix] served, all at once {t_sat:.2f} s", file=sys.stderr, flush=True)
-    rate = args.mix_load * len(reqs) / t_batch
-    arr = np.cumsum(np.random.default_rng(5).exponential(1.0 / rate, size=len(reqs)))
-    t_served, lat, sst, served_imgs2 = serve(arr)
-    print(f"[mix] served, Poisson {t_served:.2f} s", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_mix(rows, env_rgba, handles, jobs, reqs, threads=min(os.cpu_count() or 1, 16))
 
-    same = all(np.array_equal(a, b) and np.array_equal(a, c) and np.array_equal(a, d)
-               for a, b, c, d in zip(plain_imgs, batch_imgs, served_imgs, served_imgs2))
     ms = lambda v: round(float(v), 2)
+    pct = lambda v, q: ms(np.percentile(v, q))
+    value = samples / t_served / 1e6
     line = {
-        "metric": "Msamples/s on the mixed request stream (plain vs batched vs served); p50/p99 request latency",
-        "value": round(samples / t_sat / 1e6, 4), "unit": "Msamples/s", "n_gpus": 1, "higher_is_better": True,
-        "dtype": "f32", "data": "synthetic", "device": dev_name,
-        "config": {"workload": f"{len(jobs)} jobs (seed 4): resolution U{{64..1024}}^2 / {args.mix_res_div}, spp "
-                               f"round(exp U[ln4, ln256]) / {args.mix_spp_div}, scenes U{{scattering 4096, cloud 1M, "
-                               f"torus 250k}}, max depth 64, RR 12, sky environment 2048x1024",
-                   "samples": samples, "jobs": len(jobs)},
+        "metric": "Msamples/s at ≤1% RMSE vs reference image; p50/p99 request latency",
+        "value": round(value, 4), "unit": "Msamples/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": round(t_served / K * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"served Poisson stream of {K} render requests (one step = one request) drawn from "
+                               f"the batched job mix (seed {args.mix_seed}): resolution U{{64,128,256,512,1024}}^2"
+                               + (f" / {args.res_div}" if args.res_div > 1 else "") +
+                               f", spp round(exp U[ln 4, ln 256]) / {args.spp_div} (at least 1), scene uniform over "
+                               f"scattering 4096 / cloud 10^6 / torus 2.5*10^5, camera on a sphere of radius 3.5, "
+                               f"FOV 40, max depth 64, RR from 12, MIS NEE, 2048x1024 sky + 0.5 deg sun; offered "
+                               f"load {args.load} x the batched capacity",
+                   "requests": K, "samples": samples, "spp_div": args.spp_div, "res_div": args.res_div,
+                   "lambda_req_s": round(lam, 4), "scenes": {k: MIX_SCENES[k] for k in names}},
+        "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "mean": ms(np.mean(lat))},
+        "speedup_vs_plain": round(t_plain / t_served, 3),
         "plain": {"seconds": round(t_plain, 3), "msamples_s": round(samples / t_plain / 1e6, 4),
-                  "p50_ms": ms(np.percentile(plain_lat, 50)), "p99_ms": ms(np.percentile(plain_lat, 99))},
+                  "p50_ms": pct(plain_ms, 50), "p99_ms": pct(plain_ms, 99),
+                  "note": "one vp_render per request in order; latency = the request's own render time"},
         "batched": {"seconds": round(t_batch, 3), "msamples_s": round(samples / t_batch / 1e6, 4),
+                    "speedup_vs_plain": round(t_plain / t_batch, 3), "autotune_s": round(t_tune, 2),
                     "launch": {"pipeline": "wavefront" if tuned.pipeline else "megakernel", "block": tuned.block,
                                "blocks_per_cu": tuned.blocks_per_cu, "active_cap": tuned.active_cap,
                                "regen_min": tuned.regen_min, "pool_paths": tuned.pool_paths},
-                    "autotune_s": round(t_tune, 2), "launches": bst["launches"],
-                    "speedup_vs_plain": round(t_plain / t_batch, 3)},
-        "served_saturated": {"seconds": round(t_sat, 3), "msamples_s": round(samples / t_sat / 1e6, 4),
-                             "p50_ms": ms(np.percentile(lat_sat, 50)), "p99_ms": ms(np.percentile(lat_sat, 99)),
-                             "speedup_vs_plain": round(t_plain / t_sat, 3), "server_stats": st_sat},
-        "served_poisson": {"seconds": round(t_served, 3), "msamples_s": round(samples / t_served / 1e6, 4),
-                           "lambda_req_s": round(rate, 3), "offered_load_vs_batched": args.mix_load,
-                           "requests": len(reqs), "p50_ms": ms(np.percentile(lat, 50)),
-                           "p99_ms": ms(np.percentile(lat, 99)), "server_stats": sst},
+                    "kernel_ms": round(bst["kernel_ms"], 2)},
+        "served": {"seconds": round(t_served, 3), "msamples_s": round(value, 4),
+                   "launches": sst["launches"], "busy_ms": round(sst["busy_ms"], 1)},
         "images_identical": bool(same),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "quality": quality,
+        "device": dev_name,
+        "events": events,
+        "mrays_per_s": (round((events["traces"] + events["shadow_rays"]) / t_served / 1e6, 2) if events else None),
     }
     print(json.dumps(line), flush=True)
     for h in handles.values():
@@ -259,17 +387,67 @@ def run_mix(args):
         sys.exit(1)
 
 
+def cpu_baseline_mix(rows, env_rgba, handles, jobs, reqs, threads, budget_s=20.0, repeats=3):
+    """The oracle (with its CPU BVH) on a bounded sample of the stream: for
+    each scene, the first request of that scene on every 16th row and
+    column, at a sample count sized to the budget.  Msamples/s is the
+    sample-weighted rate over the three scenes' samples in the stream."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import vpo
+    tabs = vpo.env_tables(env_rgba)
+    stride = 16
+    per_scene, notes = {}, []
+    for name in sorted(handles):
+        i = next((k for k, j in enumerate(jobs) if j["scene"] == name), None)
+        if i is None:
+            continue
+        rec = handles[name].export()[2]
+        _, _, cam, p = reqs[i]
+
+        def run(spp_used):
+            t0 = time.perf_counter()
+            vpo.render(rec, env_rgba, cam.pod(), p.width, p.height, p.spp, seed=p.seed, max_depth=p.max_depth,
+                       rr_depth=p.rr_depth, phase_g=p.phase_g, active_cap=p.active_cap, stride=stride,
+                       samples=(0, spp_used), threads=threads, env_tables_=tabs, bvh=True)
+            return time.perf_counter() - t0
+
+        npx = ((p.width + stride - 1) // stride) * ((p.height + stride - 1) // stride)
+        probe = run(1)
+        spp_used = int(max(1, min(p.spp, budget_s / len(handles) / repeats / max(probe, 1e-3))))
+        times = sorted(run(spp_used) for _ in range(repeats))
+        per_scene[name] = npx * spp_used / times[len(times) // 2] / 1e6
+        notes.append(f"{name}: {npx} px x {spp_used} spp")
+    # time per sample of the stream's scene mix
+    w = {k: sum(j["width"] * j["height"] * j["spp"] for j in jobs if j["scene"] == k) for k in per_scene}
+    tot = sum(w.values())
+    rate = tot / sum(w[k] / per_scene[k] for k in per_scene) if tot else 0.0
+    return {"value": round(rate, 5), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "per_scene": {k: round(v, 5) for k, v in per_scene.items()},
+            "sample": "the oracle (CPU BVH) on every 16th row and column of the first request of each scene ("
+                      + "; ".join(notes) + f"), 1 warm-up + {repeats} timed repeats (median); the stream's rate "
+                      "weights each scene by its share of the stream's samples"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="requests timed (--single: renders timed)")
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
     ap.add_argument("--no-quality", action="store_true", help="skip the image-error estimate")
-    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs for HBM traffic")
-    ap.add_argument("--rmse-ref-spp", type=int, default=0, help="also measure relative RMSE against a render "
-                                                                   "with this many spp (untimed)")
+    ap.add_argument("--spp-div", type=int, default=8, help="stream: every request's spp is divided by this")
+    ap.add_argument("--res-div", type=int, default=1, help="stream: every request's resolution is divided by this")
+    ap.add_argument("--load", type=float, default=1.0,
+                    help="stream: Poisson offered load as a multiple of the batched capacity")
+    ap.add_argument("--mix-seed", type=int, default=4)
+    ap.add_argument("--mem-gib", type=int, default=16, help="stream: server admission budget")
+    ap.add_argument("--single", choices=["scattering", "cloud", "torus"], default=None,
+                    help="render one scene whole per step instead of the request stream")
+    ap.add_argument("--traffic", action="store_true", help="--single: two rocprofv3 --pmc runs for HBM traffic")
+    ap.add_argument("--no-traffic", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--rmse-ref-spp", type=int, default=0, help="--single: also measure relative RMSE against a "
+                                                                   "render with this many spp (untimed)")
     ap.add_argument("--block", type=int, default=0, help="threads per workgroup (0 = library default)")
     ap.add_argument("--bpc", type=int, default=0, help="persistent workgroups per CU (0 = occupancy limit)")
     ap.add_argument("--regen", type=int, default=0, help="idle lanes per wavefront before claiming new paths")
@@ -283,13 +461,6 @@ def main():
                     help="run one 256 MiB elementwise copy kernel first (HBM counter calibration)")
     ap.add_argument("--ris", action="store_true",
                     help="NEE by resampled importance sampling (the reference's --ris) instead of MIS")
-    ap.add_argument("--mix", action="store_true", help="run the job mix (plain vs batched vs served) instead")
-    ap.add_argument("--mix-jobs", type=int, default=64)
-    ap.add_argument("--mix-res-div", type=int, default=4)
-    ap.add_argument("--mix-spp-div", type=int, default=4)
-    ap.add_argument("--mix-load", type=float, default=0.5,
-                    help="Poisson offered load as a multiple of the batched capacity")
-    ap.add_argument("--mix-mem-gib", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is in DESIGN.md",
@@ -297,13 +468,18 @@ def main():
         sys.exit(2)
 
     import torch
-    pkg = load_pkg()
-    if args.mix:
-        run_mix(args)
+    load_pkg()
+    if args.single is None:
+        run_stream(args)
         return
+    run_single(args, torch)
+
+
+def run_single(args, torch):
     from volprim_amd import scenes, volprim
 
-    rows, env, cam_args, pp = workload(scenes)
+    rows, cam_args, pp, desc = workload(scenes, args.single)
+    env = scenes.sky_env(2048, 1024)
     if args.ris:
         pp = dict(pp, use_ris=1)
     ctx = volprim.Context(0)
@@ -352,8 +528,8 @@ def main():
         avg_launch_s = kernel_ms / max(launches, 1) / 1e3
         flops_per_launch = flops * args.steps / max(launches, 1)
         achieved = flops_per_launch / avg_launch_s / 1e12
-        traffic, traffic_note = None, "not measured (--no-traffic)"
-        if not args.no_traffic and args.pipeline == "mega":
+        traffic, traffic_note = None, "not measured (pass --traffic)"
+        if args.traffic and not args.no_traffic and args.pipeline == "mega":
             tr, traffic_note = measure_traffic(args)
             if tr is not None:
                 traffic = int(tr["bytes"])
@@ -366,7 +542,8 @@ def main():
                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic, "traffic_note": traffic_note,
                     "peak_nonpacked": PEAK_FP32_NONPACKED_TFLOPS,
                     "kernel": "k_render" if args.pipeline == "mega" else "k_wf_* (one render step)",
-                    "avg_launch_ms": round(avg_launch_s * 1e3, 3), "flops_per_launch": flops_per_launch}
+                    "avg_launch_ms": round(avg_launch_s * 1e3, 3), "flops_per_launch": flops_per_launch,
+                    "weights_note": WEIGHTS_NOTE}
 
     # image error: relative RMSE of the timed image from its per-pixel sample
     # variance (an unbiased estimate of the MSE against the converged image),
@@ -404,15 +581,12 @@ def main():
         cpu = cpu_baseline(rows, env, cam.pod(), pp, threads=min(os.cpu_count() or 1, 16))
 
     line = {
-        "metric": "Msamples/s at <=1% RMSE vs reference image; p50/p99 request latency",
+        "metric": f"Msamples/s of one {args.single} render (profiling workload, not the request-stream metric)",
         "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "scattering: 4096 anisotropic Gaussians (seed 1), sigma multiplier 10, albedo 0.9, "
-                               "isotropic phase, 2048x1024 sky + sun environment, 512x512, 64 spp, max depth 64, "
-                               "RR from depth 12, " + ("RIS NEE (6 candidates)" if args.ris else "MIS NEE"),
-                   "width": 512, "height": 512, "spp": 64,
-                   "primitives": 4096},
+        "config": {"workload": desc + ", " + ("RIS NEE (6 candidates)" if args.ris else "MIS NEE"),
+                   "width": pp["width"], "height": pp["height"], "spp": pp["spp"], "primitives": int(rows.shape[0])},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "device": dev_name,

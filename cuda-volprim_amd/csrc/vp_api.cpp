@@ -162,6 +162,7 @@ vp_status vp_context_destroy(vp_context* ctx) {
     vp_free_wf(ctx);
     if (ctx->jobs_dev) (void)hipFree(ctx->jobs_dev);
     if (ctx->job_begin_dev) (void)hipFree(ctx->job_begin_dev);
+    if (ctx->job_done_dev) (void)hipFree(ctx->job_done_dev);
     if (ctx->launch_desc) (void)hipFree(ctx->launch_desc);
     if (ctx->jobs_pinned) (void)hipHostFree(ctx->jobs_pinned);
     if (ctx->begin_pinned) (void)hipHostFree(ctx->begin_pinned);

@@ -290,6 +290,31 @@ VP_HD float prim_uniform(uint64_t key, uint32_t bounce, uint32_t role, uint32_t 
 VP_HD float free_flight_tau(float xi) { return -logf_(maxf(1.0f - xi, 1e-10f)); }
 
 // ---------------------------------------------------------------------------
+// Per-pixel radiance accumulator.  The reference folds each sample into a
+// running Welford mean in sample order (raygen.cuh:214-238).  Here every
+// sample's radiance is added to an exact integer sum in 32.32 fixed point, so
+// the total does not depend on the order in which paths finish (any launch
+// shape, job mix or number of concurrent launches gives the same bits), and
+// the mean is formed once at the end.  Each channel is kept as two 64-bit
+// words, the low and high 32 bits of every sample's value summed separately,
+// which cannot overflow before 2^32 samples.  Eight words per pixel:
+//   [0..2] low sums r, g, b   [3] sample count   [4..6] high sums   [7] unused
+// A sample value is clamped to [0, 2^31] (finalize_sample already maps
+// non-finite radiance to 0); values below 2^-33 round to 0.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kAccumWords = 8;
+VP_HD uint64_t radiance_fixed(float v) {
+    const float c = minf(maxf(v, 0.0f), 2147483648.0f);
+    return static_cast<uint64_t>(rintf_(c * 4294967296.0f));
+}
+// Mean of one channel from its low and high sums over n samples.
+VP_HD float accum_mean(uint64_t lo, uint64_t hi, uint64_t n) {
+    if (n == 0) return 0.0f;
+    const double s = static_cast<double>(hi) * 4294967296.0 + static_cast<double>(lo);
+    return static_cast<float>((s * (1.0 / 4294967296.0)) / static_cast<double>(n));
+}
+
+// ---------------------------------------------------------------------------
 // Henyey–Greenstein phase (sampling.cuh:74-134, constants folded on the host
 // as renderer.cpp:224-235 does).
 // ---------------------------------------------------------------------------

@@ -52,6 +52,18 @@ __device__ __forceinline__ uint32_t lanes_below(unsigned long long m) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
 }
 
+// Hand-off point for data exchanged between lanes of one wavefront through
+// LDS (queues, compaction, per-ray tables, steal slots): the wavefront-scope
+// release/acquire pair keeps the compiler from moving LDS accesses across it
+// (a lane's reads of entries other lanes write, and the reverse), and the
+// wave barrier keeps the active lanes converged there.  The hardware already
+// executes a wavefront's LDS operations in order.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---- walk-tree traversal (tree layout and Walk4 in vp_walk.h) ----
 
 // Children of node n that the ray crosses within [0, tmax], as a 4-bit mask.
@@ -508,6 +520,7 @@ __device__ __forceinline__ bool wave_steal(const WaveQueue& wq, bool idle, bool 
             slot[64 + r] = ray | (part.lvl << 8);
         }
     }
+    wave_sync();
     bool got = false;
     if (idle) {
         const uint32_t r = lanes_below(im);
@@ -544,6 +557,7 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
     lds_store3(wq.rd + 3 * (2 * lane + 1), db);
     wq.acc[2 * lane] = 0ull;
     wq.acc[2 * lane + 1] = two ? 0ull : kTauLimitFixed;
+    wave_sync();
     // Chords of the origin set (every member's optical depth from the origin
     // to its exit), for all lanes' rays together: the wavefront's (member,
     // lane) pairs are numbered through a prefix sum of the set sizes and
@@ -566,6 +580,7 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
         set_of[rk] = act;
         pre_of[rk] = pre;
         wq.list[rk] = static_cast<uint8_t>(lane);
+        wave_sync();
         for (uint32_t g = rk; g < total; g += nl) {
             // the owner: the last rank whose prefix is at most g
             uint32_t lo = 0, hi = nl - 1u;
@@ -588,6 +603,7 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
             }
         }
     }
+    wave_sync();
     const bool full_a = wq.acc[2 * lane] >= kTauLimitFixed;
     const bool full_b = wq.acc[2 * lane + 1] >= kTauLimitFixed;
     // the rays still to walk: every ray a, then every ray b (a saturated ray
@@ -599,6 +615,7 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
     const uint32_t nr = npa + static_cast<uint32_t>(__popcll(pb));
     if (!full_a) wq.list[lanes_below(pa)] = static_cast<uint8_t>(2 * lane);
     if (!full_b) wq.list[npa + lanes_below(pb)] = static_cast<uint8_t>(2 * lane + 1);
+    wave_sync();
     const uint32_t nact = static_cast<uint32_t>(__popcll(am));
     uint32_t next = nact < nr ? nact : nr;  // list entries handed out
     constexpr uint32_t kNoRay = 0xffu;
@@ -646,6 +663,7 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
             if (step && fits) w = nw;
             if (__builtin_amdgcn_ballot_w64(!fits) != 0) break;
         }
+        wave_sync();
         // leaf phase, shared by all lanes: entry tests; the entered leaves
         // move down to the front of the queue (an entry is only overwritten
         // after it has been read); then their chords
@@ -664,6 +682,7 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
                 entered = tj >= 0.0f;
             }
             const unsigned long long hm = __builtin_amdgcn_ballot_w64(entered);
+            wave_sync();
             if (entered) {
                 const uint32_t h = nhit + lanes_below(hm);
                 wq.prim[h] = prim;
@@ -672,6 +691,7 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
             }
             nhit += static_cast<uint32_t>(__popcll(hm));
         }
+        wave_sync();
         for (uint32_t g = rank; g < nhit; g += nact) {
             const uint32_t own = wq.owner[g];
             const f3 ro = lds_load3(wq.ro + 3 * (own >> 1)), rd = lds_load3(wq.rd + 3 * own);
@@ -680,6 +700,7 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
             const float te = exit_from_entry(pr, ro, rd, tj, length2(to_local_dir(pr, rd)));
             if (te > tj && te < kTMax) atomicAdd(&wq.acc[own], tau_fixed(optical_depth(pr, ro, rd, tj, te)));
         }
+        wave_sync();
         // lanes whose ray is done (walked to the end, or saturated) take the
         // next ray of the list
         const bool fin = ray != kNoRay && (w.n == kWalkEnd || wq.acc[ray & 127u] >= kTauLimitFixed);
@@ -697,7 +718,9 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
         const bool can_give = ray != kNoRay && w.n != kWalkEnd && wq.acc[ray & 127u] < kTauLimitFixed;
         if (next >= nr) wave_steal(wq, ray == kNoRay, can_give, ray, w);
         setup();
+        wave_sync();
     }
+    wave_sync();
     const unsigned long long ta = wq.acc[2 * lane], tb = wq.acc[2 * lane + 1];
     Ta = ta >= kTauLimitFixed ? 0.0f : tau_to_T(ta);
     Tb = tb >= kTauLimitFixed ? 0.0f : tau_to_T(tb);
@@ -725,12 +748,24 @@ __device__ __forceinline__ uint32_t find_job(const uint64_t* begin, uint32_t njo
 }
 
 
+// finalize_sample (path_stages.cuh:52-62): optional luminance clamp,
+// non-finite radiance to zero.
+__device__ __forceinline__ float4 finalize_radiance(const JobDesc& J, f3 L) {
+    if (J.firefly_clamp > 0.0f) {
+        const float lum = dot(L, f3{kLumR, kLumG, kLumB});
+        if (lum > J.firefly_clamp) L = L * (J.firefly_clamp * rcpf(lum));
+    }
+    if (!isfinitef_(sum3(L))) L = mk(0.0f);
+    return make_float4(L.x, L.y, L.z, 1.0f);
+}
+
 // Path setup (raygen.cuh:72-113) for path q of job J's current slice: pixel
-// and sample from the 8x8-tile numbering, RNG stream, counter-hash key,
-// jittered camera ray and the per-sample output slot.  False for the padding
-// paths of partial tiles.
+// and sample from the 8x8-tile numbering, RNG stream, counter-hash key and
+// jittered camera ray; pix receives the pixel index (y * width + x) and sl the
+// sample's index within the slice.  False for the padding paths of partial
+// tiles.
 __device__ __forceinline__ bool path_begin(const JobDesc& J, uint64_t q, f3& o, f3& d, Pcg32& rng, uint64_t& key,
-                                           uint32_t& slot) {
+                                           uint32_t& pix_out, uint32_t& sl_out) {
     const uint32_t sl = static_cast<uint32_t>(q / J.npad);
     const uint32_t r = static_cast<uint32_t>(q - static_cast<uint64_t>(sl) * J.npad);
     const uint32_t tile = r >> 6, w = r & 63u;
@@ -750,20 +785,40 @@ __device__ __forceinline__ bool path_begin(const JobDesc& J, uint64_t q, f3& o, 
         jy = next_uniform(rng) - 0.5f;
     }
     camera_ray(J.cam, px, py, jx, jy, &o, &d);
-    slot = static_cast<uint32_t>(static_cast<uint64_t>(sl) * J.width * J.height + pix);
+    pix_out = static_cast<uint32_t>(pix);
+    sl_out = sl;
     return true;
 }
 
-// finalize_sample (path_stages.cuh:52-62): optional luminance clamp,
-// non-finite radiance to zero.
-__device__ __forceinline__ float4 finalize_radiance(const JobDesc& J, f3 L) {
-    if (J.firefly_clamp > 0.0f) {
-        const float lum = dot(L, f3{kLumR, kLumG, kLumB});
-        if (lum > J.firefly_clamp) L = L * (J.firefly_clamp * rcpf(lum));
+// A finished path (raygen.cuh:214-238): its radiance is added to its pixel's
+// exact fixed-point sums (fire-and-forget atomics: integer adds commute, so
+// the total does not depend on which path finishes first), written to its
+// per-sample slot if the caller asked for per-sample output, and counted
+// against its job-table entry if the entry is tracked (the server).  The
+// release fence orders the sums before that count, so a host that sees the
+// entry complete and then launches the resolve reads finished sums.
+__device__ __forceinline__ void path_finish(const JobDesc& J, uint32_t pix, uint32_t sl, f3 L) {
+    const float4 v = finalize_radiance(J, L);
+    unsigned long long* a = J.accum + static_cast<size_t>(pix) * kAccumWords;
+    const float c[3] = {v.x, v.y, v.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const uint64_t q = radiance_fixed(c[k]);
+        __hip_atomic_fetch_add(a + k, static_cast<unsigned long long>(q & 0xffffffffull), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        if ((q >> 32) != 0ull)
+            __hip_atomic_fetch_add(a + 4 + k, static_cast<unsigned long long>(q >> 32), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (!isfinitef_(sum3(L))) L = mk(0.0f);
-    return make_float4(L.x, L.y, L.z, 1.0f);
+    __hip_atomic_fetch_add(a + 3, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (J.samples != nullptr)
+        J.samples[(static_cast<size_t>(J.s_begin) + sl) * (static_cast<size_t>(J.width) * J.height) + pix] = v;
+    if (J.done != nullptr) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(J.done, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
+
 
 }  // namespace
 }  // namespace vp

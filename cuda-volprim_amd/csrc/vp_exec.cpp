@@ -3,10 +3,9 @@
 // A step renders one slice of samples of every job in a table: the table,
 // the slice start offsets and a two-word launch descriptor are staged in
 // pinned host memory, copied to the device, and a fixed sequence (counter
-// reset, k_render, k_welford_all) runs over them.  Because the sequence only
-// reads sizes from device memory, it can be captured once as a HIP graph and
-// replayed for every later step with the same launch shape (the serving loop
-// does this); the plain renderer launches it directly.
+// reset, k_render, k_resolve_all) runs over them.  Finished paths add into
+// per-pixel accumulators that live across the slices of a job, so slicing
+// (the reference's 16-spp batches) does not change the result.
 #include <algorithm>
 #include <string>
 
@@ -101,8 +100,20 @@ vp_status vp_ensure_wf(vp_context* ctx, uint32_t P, uint32_t cap) {
     w.ncand = reinterpret_cast<uint32_t*>(b + o_ncand);
     w.P = P;
     w.cap = cap;
-    VP_HIP_TRY(hipHostMalloc(&ctx->wf_done_pinned, 2 * sizeof(unsigned long long), hipHostMallocDefault));
-    for (auto& e : ctx->wf_ev) VP_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // on any failure below, drop the partial pool so the next call starts over
+    // instead of finding wf_mem set with the pinned counters or events missing
+    if (hipHostMalloc(&ctx->wf_done_pinned, 2 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+        ctx->wf_done_pinned = nullptr;
+        vp_free_wf(ctx);
+        return fail(VP_ERR_HIP, "wavefront pool: cannot allocate the pinned completion counters");
+    }
+    for (auto& e : ctx->wf_ev) {
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            e = nullptr;
+            vp_free_wf(ctx);
+            return fail(VP_ERR_HIP, "wavefront pool: cannot create its events");
+        }
+    }
     return VP_OK;
 }
 
@@ -204,7 +215,7 @@ vp_status vp_wf_prepare(vp_context* ctx, const RenderLaunch& L, hipStream_t s) {
 
 // Wavefront step: stage the tables, reset the pool, replay the captured
 // two-iteration graph until every path of the step has retired (checked one
-// replay behind, so the device never waits on the host), fold the slices.
+// replay behind, so the device never waits on the host), resolve the means.
 static vp_status vp_run_step_wf(vp_context* ctx, const RenderLaunch& L, uint64_t total_paths, uint32_t max_jobs,
                                 uint32_t max_npix, hipStream_t s) {
     vp_status st = vp_wf_prepare(ctx, L, s);
@@ -232,7 +243,7 @@ static vp_status vp_run_step_wf(vp_context* ctx, const RenderLaunch& L, uint64_t
             if (ctx->wf_done_pinned[(k - 1) & 1] >= total_paths) break;
         }
     }
-    VP_HIP_TRY(vp_launch_welford(ctx, max_jobs, max_npix, s));
+    VP_HIP_TRY(vp_launch_resolve(ctx, max_jobs, max_npix, s));
     return VP_OK;
 }
 
@@ -283,7 +294,17 @@ vp_status vp_run_step(vp_context* ctx, const RenderLaunch& L, uint32_t nslices, 
 
 vp_status vp_run_jobs(vp_context* ctx, std::vector<JobRun>& runs, const vp_launch_config* cfg, uint32_t active_cap,
                       int count, int measure, hipStream_t s, double* kernel_ms, uint64_t* paths) {
-    const RenderLaunch L = vp_resolve_launch(ctx, cfg, active_cap, count, measure);
+    RenderLaunch L = vp_resolve_launch(ctx, cfg, active_cap, count, measure);
+    // per-sample output is written by the megakernel only
+    bool per_sample = false;
+    for (auto& r : runs) per_sample |= r.samples_all != nullptr;
+    if (per_sample && L.pipeline == VP_PIPELINE_WAVEFRONT) {
+        vp_launch_config c{};
+        if (cfg) c = *cfg;
+        else if (ctx->have_tuned) c = ctx->tuned;
+        c.pipeline = VP_PIPELINE_MEGAKERNEL;
+        L = vp_resolve_launch(ctx, &c, active_cap, count, measure);
+    }
     const size_t lanes = static_cast<size_t>(L.grid) * L.block;
     vp_status st = VP_OK;
     if (L.pipeline == VP_PIPELINE_MEGAKERNEL)
@@ -292,29 +313,41 @@ vp_status vp_run_jobs(vp_context* ctx, std::vector<JobRun>& runs, const vp_launc
     st = vp_ensure_jobs(ctx, static_cast<uint32_t>(runs.size()));
     if (st != VP_OK) return st;
 
-    // slice sizes: spread the per-launch sample budget over the jobs
-    size_t total_pix = 0;
+    // per-pixel accumulators of every job, zeroed once: the slices of a job
+    // add into the same sums
+    size_t acc_words = 0;
     uint32_t max_npix = 1;
     for (auto& r : runs) {
-        total_pix += static_cast<size_t>(r.desc.width) * r.desc.height;
-        max_npix = std::max(max_npix, r.desc.width * r.desc.height);
+        const size_t npix = static_cast<size_t>(r.desc.width) * r.desc.height;
+        acc_words += npix * vp::kAccumWords;
+        max_npix = std::max(max_npix, static_cast<uint32_t>(npix));
     }
-    const size_t budget_samples = std::max<size_t>(1, kSliceBudgetBytes / (16 * std::max<size_t>(total_pix, 1)));
-    size_t slice_bytes = 0;
-    for (auto& r : runs) {
-        if (r.slice_spp == 0) r.slice_spp = static_cast<uint32_t>(std::min<size_t>(r.desc.spp, budget_samples));
-        if (!r.samples_all)
-            slice_bytes += static_cast<size_t>(r.slice_spp) * r.desc.width * r.desc.height * sizeof(float4);
-    }
-    void* slice_buf = nullptr;
-    if (slice_bytes && hipMalloc(&slice_buf, slice_bytes) != hipSuccess)
-        return fail(VP_ERR_OOM, "cannot allocate " + std::to_string(slice_bytes) + " bytes of sample slots");
+    void* acc_buf = nullptr;
+    if (hipMalloc(&acc_buf, acc_words * sizeof(unsigned long long)) != hipSuccess)
+        return fail(VP_ERR_OOM, "cannot allocate " + std::to_string(acc_words * 8) + " bytes of pixel accumulators");
     struct Free {
         void* p;
         ~Free() {
             if (p) (void)hipFree(p);
         }
-    } free_slice{slice_buf};
+    } free_acc{acc_buf};
+    {
+        size_t off = 0;
+        for (auto& r : runs) {
+            r.desc.accum = static_cast<unsigned long long*>(acc_buf) + off;
+            r.desc.mean = r.mean;
+            r.desc.samples = r.samples_all;
+            r.desc.done = nullptr;
+            r.issued = 0;
+            off += static_cast<size_t>(r.desc.width) * r.desc.height * vp::kAccumWords;
+        }
+        // fill kernels (not hipMemsetAsync): the launch sequences below may
+        // be captured into graphs by callers
+        for (size_t o = 0; o < acc_words; o += (size_t(1) << 30)) {
+            const uint32_t n = static_cast<uint32_t>(std::min<size_t>(acc_words - o, size_t(1) << 30));
+            VP_HIP_TRY(vp_launch_fill_u64(static_cast<unsigned long long*>(acc_buf) + o, n, 0ull, s));
+        }
+    }
 
     uint64_t total_paths_all = 0;
     double render_ms = 0.0;
@@ -322,27 +355,19 @@ vp_status vp_run_jobs(vp_context* ctx, std::vector<JobRun>& runs, const vp_launc
     const uint32_t max_jobs = static_cast<uint32_t>(runs.size());
     while (true) {
         uint64_t total = 0;
-        size_t off = 0;
         uint32_t k = 0;
         for (auto& r : runs) {
-            if (r.done >= r.desc.spp) continue;
+            if (r.issued >= r.desc.spp) continue;
             vp::JobDesc J = r.desc;
-            J.s_begin = r.done;
-            J.s_count = std::min(r.slice_spp, r.desc.spp - r.done);
+            const uint32_t slice = r.slice_spp ? r.slice_spp : r.desc.spp;
+            J.s_begin = r.issued;
+            J.s_count = std::min(slice, r.desc.spp - r.issued);
             J.path_begin = total;
             J.path_count = static_cast<uint64_t>(J.npad) * J.s_count;
-            J.mean = r.mean;
-            const size_t npix = static_cast<size_t>(J.width) * J.height;
-            if (r.samples_all) {
-                J.samples = r.samples_all + static_cast<size_t>(J.s_begin) * npix;
-            } else {
-                J.samples = reinterpret_cast<float4*>(static_cast<char*>(slice_buf) + off);
-                off += static_cast<size_t>(J.s_count) * npix * sizeof(float4);
-            }
             total += J.path_count;
             ctx->jobs_pinned[k] = J;
             ctx->begin_pinned[k] = J.path_begin;
-            r.done += J.s_count;
+            r.issued += J.s_count;
             ++k;
         }
         if (k == 0) break;
@@ -350,7 +375,8 @@ vp_status vp_run_jobs(vp_context* ctx, std::vector<JobRun>& runs, const vp_launc
         st = vp_run_step(ctx, L, k, total, max_jobs, max_npix, false, s);
         if (st != VP_OK) return st;
         VP_HIP_TRY(hipEventRecord(ctx->ev1, s));
-        // the pinned staging is rewritten next round: wait for this step
+        // the pinned staging is rewritten next round (and the accumulators
+        // are freed on return): wait for this step
         VP_HIP_TRY(hipEventSynchronize(ctx->ev1));
         float ms = 0.0f;
         VP_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));

@@ -75,7 +75,12 @@ struct EnvDev {
 // One job slice of the persistent renderer: samples [s_begin, s_begin+s_count)
 // of one image.  Path q of the slice (0 <= q < path_count) is sample
 // s_begin + q / npad of the pixel at tile position q % npad (8x8 tiles).
-struct JobDesc {
+// A finished path adds its radiance to its pixel's accumulator (kAccumWords
+// words per pixel, vp_core.h); the mean is resolved from it after the last
+// slice.  Entries are aligned to the 128-byte cache line so a render launch
+// that is reading table entries never shares a line with entries appended
+// after it started (the server appends while earlier launches run).
+struct alignas(128) JobDesc {
     SceneView scene;
     EnvDev env;
     CameraPod cam;
@@ -87,9 +92,12 @@ struct JobDesc {
     uint32_t tiles_x, npad;
     uint32_t s_begin, s_count;
     uint64_t path_begin, path_count;
-    float4* samples;  // s_count * width * height radiance values of this slice
-    float4* mean;     // width * height Welford mean of the job
+    unsigned long long* accum;  // width * height * kAccumWords radiance sums of the job
+    float4* mean;               // width * height mean, written by the resolve kernel
+    float4* samples;            // optional (null = none): per-sample radiance, sample-major, spp * W * H
+    unsigned long long* done;   // optional (null = none): finished paths of this entry (server)
 };
+static_assert(sizeof(JobDesc) % 128 == 0, "job-table entries fill whole cache lines");
 
 // Per-slot slab of the wavefront pipeline: two active-set buffers and one
 // buffer of rebuild candidates, each of the active-set capacity.
@@ -113,7 +121,7 @@ struct WfPool {
     float4* bt;    // throughput.xyz, scatter distance
     float4* lr;    // radiance.xyz, active-set size
     uint4* rng;    // PCG32 state, increment
-    uint4* misc;   // key (2 words), sample slot, flags (set buffer, state, pending terms)
+    uint4* misc;   // key (2 words), pixel index, flags (set buffer, state, pending terms)
     float4* pw;    // [3][P] pending term weight.xyz, s1
     float4* pv;    // [3][P] pending term factor.xyz, s2
     float4* po;    // [3][P] shadow-ray origin.xyz, set buffer | count << 1
@@ -263,16 +271,19 @@ struct RenderLaunch {
 };
 hipError_t vp_launch_step(vp_context* ctx, const RenderLaunch& L, uint32_t max_jobs, uint32_t max_npix,
                           hipStream_t s);
+// One render launch over an explicit job table, descriptor, path counter and
+// slab (the server runs up to two at once, each with its own slab).
+hipError_t vp_launch_render(const RenderLaunch& L, const vp::JobDesc* jobs, const uint64_t* job_begin,
+                            const uint64_t* launch_desc, unsigned long long* counter, uint32_t* slab,
+                            unsigned long long* counters, hipStream_t s);
 
 // execution of job lists (vp_exec.cpp)
-constexpr size_t kSliceBudgetBytes = size_t(1) << 30;  // per-sample radiance bytes per step
-
 struct JobRun {
     vp::JobDesc desc;
-    float4* mean = nullptr;         // device W*H Welford mean
-    float4* samples_all = nullptr;  // optional caller buffer spp*W*H
-    uint32_t done = 0;              // samples folded so far
-    uint32_t slice_spp = 0;         // samples per step (0 = chosen from the budget)
+    float4* mean = nullptr;         // device W*H mean (output)
+    float4* samples_all = nullptr;  // optional caller buffer spp*W*H of per-sample radiance
+    uint32_t issued = 0;            // samples launched so far
+    uint32_t slice_spp = 0;         // samples per launch (0 = all at once)
 };
 vp_status vp_ensure_slab(vp_context* ctx, size_t words);
 vp_status vp_ensure_jobs(vp_context* ctx, uint32_t n);
@@ -301,9 +312,10 @@ int vp_render_occupancy(uint32_t block, uint32_t active_cap, int count_events, i
 hipError_t vp_wf_launch_init(const vp::WfPool& w, hipStream_t s);
 hipError_t vp_wf_launch_iterations(vp_context* ctx, const vp::WfPool& w, const vp::WfLaunch& L, hipStream_t s);
 int vp_wf_occupancy(int which, int* blocks_per_cu);
-hipError_t vp_launch_welford(vp_context* ctx, uint32_t max_jobs, uint32_t max_npix, hipStream_t s);
-// Welford fold of every sample slot of job-table entry j (whole job at once).
-hipError_t vp_launch_welford_job(vp_context* ctx, uint32_t j, uint32_t npix, hipStream_t s);
+// Means of every entry of ctx's job table from their accumulators.
+hipError_t vp_launch_resolve(vp_context* ctx, uint32_t max_jobs, uint32_t max_npix, hipStream_t s);
+// Means of job-table entry j of the table at jobs.
+hipError_t vp_launch_resolve_job(const vp::JobDesc* jobs, uint32_t j, uint32_t npix, hipStream_t s);
 void vp_free_wf(vp_context* ctx);
 vp_status vp_ensure_wf(vp_context* ctx, uint32_t P, uint32_t cap);
 vp_status vp_wf_prepare(vp_context* ctx, const RenderLaunch& L, hipStream_t s);

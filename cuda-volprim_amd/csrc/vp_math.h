@@ -31,13 +31,30 @@
 #define VP_HD inline __attribute__((always_inline))
 #endif
 
+// VP_FAST_MATH=1 (device code only) replaces the correctly rounded
+// reciprocal, square root and reciprocal square root and the polynomial
+// exp/log with the hardware's approximate instructions (v_rcp_f32,
+// v_sqrt_f32, v_rsq_f32, v_exp_f32, v_log_f32; about 1 ulp).  Results then
+// agree with the host build statistically, not bit for bit.
+#if defined(VP_FAST_MATH) && VP_FAST_MATH && defined(__HIP_DEVICE_COMPILE__)
+#define VP_FAST_DEVICE 1
+#else
+#define VP_FAST_DEVICE 0
+#endif
+
 namespace vp {
 
 // ---------------------------------------------------------------------------
 // Primitive operations
 // ---------------------------------------------------------------------------
 VP_HD float fmaf_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-VP_HD float sqrtf_(float x) { return __builtin_sqrtf(x); }
+VP_HD float sqrtf_(float x) {
+#if VP_FAST_DEVICE
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return __builtin_sqrtf(x);
+#endif
+}
 VP_HD float fabsf_(float x) { return __builtin_fabsf(x); }
 VP_HD float floorf_(float x) { return __builtin_floorf(x); }
 VP_HD float rintf_(float x) { return __builtin_rintf(x); }
@@ -53,8 +70,13 @@ VP_HD bool isfinitef_(float x) { return (f2u(x) & 0x7f800000u) != 0x7f800000u; }
 VP_HD float minf(float a, float b) { return a < b ? a : b; }
 VP_HD float maxf(float a, float b) { return a > b ? a : b; }
 VP_HD float clampf(float x, float lo, float hi) { return maxf(lo, minf(x, hi)); }
+#if VP_FAST_DEVICE
+VP_HD float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+VP_HD float rsqrtf_(float x) { return __builtin_amdgcn_rsqf(x); }
+#else
 VP_HD float rcpf(float x) { return 1.0f / x; }
 VP_HD float rsqrtf_(float x) { return 1.0f / sqrtf_(x); }
+#endif
 
 constexpr float kInf = __builtin_huge_valf();
 constexpr float kPi = 3.14159265358979323846f;
@@ -75,6 +97,9 @@ constexpr float kDegToRad = kPi / 180.0f;
 VP_HD float exp2i_(int k) { return u2f(static_cast<uint32_t>(k + 127) << 23); }
 
 VP_HD float expf_(float x) {
+#if VP_FAST_DEVICE
+    return __builtin_amdgcn_exp2f(x * 1.44269502162933349609375f);
+#endif
     if (isnanf_(x)) return x;
     if (x > 88.72283935546875f) return kInf;
     if (x < -103.97208404541015625f) return 0.0f;
@@ -99,6 +124,9 @@ VP_HD float expf_(float x) {
 // log
 // ---------------------------------------------------------------------------
 VP_HD float logf_(float x) {
+#if VP_FAST_DEVICE
+    return __builtin_amdgcn_logf(x) * 0.693147182464599609375f;
+#endif
     if (isnanf_(x) || x < 0.0f) return __builtin_nanf("");
     if (x == 0.0f) return -kInf;
     if (x == kInf) return kInf;

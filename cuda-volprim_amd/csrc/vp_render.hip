@@ -48,7 +48,7 @@ struct PathState {
     f3 o, d, beta, L;
     Pcg32 rng;
     uint64_t key;
-    uint32_t bounce, job, na, cur, slot;
+    uint32_t bounce, job, na, cur, pix, sl;
 };
 
 // Minimum waves per SIMD the register allocator targets for the megakernel.
@@ -80,6 +80,7 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
     wq.acc[lane] = 0ull;
     wq.ts[lane] = __float_as_uint(kTMax);
     wq.nc[lane] = 0u;
+    wave_sync();
     // The origin set first (sampling.cuh:356-370), for all lanes' rays
     // together: the wavefront's (member, lane) pairs are numbered through a
     // prefix sum of the set sizes and shared out over the lanes.  Per member:
@@ -105,6 +106,7 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
         exit_of[rk] = act_exit;
         pre_of[rk] = pre;
         wq.list[rk] = static_cast<uint8_t>(lane);
+        wave_sync();
         for (uint32_t g = rk; g < total; g += nl) {
             // the owner: the last rank whose prefix is at most g
             uint32_t lo = 0, hi = nl - 1u;
@@ -127,6 +129,7 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
                 atomicAdd(&wq.acc[ol], tau_fixed(optical_depth(pr, ro, rd, 0.0f, te)));
         }
     }
+    wave_sync();
     const float ts_origin = __uint_as_float(wq.ts[lane]);
     const f3 inv = safe_inv(d), oi = o * inv;
     float tmax = direct ? kTMax : ts_origin;
@@ -147,6 +150,7 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
             if (step && fits) w = nw;
             if (__builtin_amdgcn_ballot_w64(!fits) != 0) break;
         }
+        wave_sync();
         const unsigned long long am = __builtin_amdgcn_ballot_w64(true);
         const uint32_t nact = static_cast<uint32_t>(__popcll(am));
         // leaf phase in two passes, each at full width: entry tests, keeping
@@ -172,6 +176,7 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
                 keep = tj >= 0.0f && (tj <= __uint_as_float(wq.ts[own]) || (wq.meta[own] >> 31) != 0u);
             }
             const unsigned long long km = __builtin_amdgcn_ballot_w64(keep);
+            wave_sync();
             if (keep) {
                 const uint32_t h = nk + lanes_below(km);
                 wq.prim[h] = prim;
@@ -180,6 +185,7 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
             }
             nk += static_cast<uint32_t>(__popcll(km));
         }
+        wave_sync();
         for (uint32_t g = rk; g < nk; g += nact) {
             const uint32_t own = wq.owner[g];
             const uint32_t prim = wq.prim[g];
@@ -210,8 +216,10 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
             if (t >= 0.0f && t < tj) t = tj;
             if (t >= 0.0f && t < ts_cur && t <= te) atomicMin(&wq.ts[own], __float_as_uint(t));
         }
+        wave_sync();
         if (!direct) tmax = __uint_as_float(wq.ts[lane]);
     }
+    wave_sync();
     const float wmin = __uint_as_float(wq.ts[lane]);
     ts = wmin < ts_origin ? wmin : ts_origin;
     nc = wq.nc[lane];
@@ -222,7 +230,10 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
 // launch_desc[0] = number of job slices, launch_desc[1] = total paths; read
 // from device memory so one captured graph serves every step of a job mix.
 // A wavefront claims new paths once at least regen_min of its lanes are idle
-// (or none is busy).
+// (or none is busy).  Claims never take the path counter past this launch's
+// total (a compare-and-swap on the counter), so launches that share one
+// counter with larger totals (the server starts a new launch over a longer
+// job table while earlier ones drain) hand out every path exactly once.
 template <bool kCount>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VP_RENDER_WAVES_PER_EU)))
 k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begin,
@@ -256,19 +267,33 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
         if (wmask && __popcll(wmask) < regen_min && __ballot(alive) != 0) wmask = 0;
         if (wmask) {
             const uint32_t leader = __ffsll(static_cast<long long>(wmask)) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(path_counter, static_cast<unsigned long long>(__popcll(wmask)));
+            unsigned long long base = 0, got = 0;
+            if (lane == leader) {
+                const unsigned long long n_want = static_cast<unsigned long long>(__popcll(wmask));
+                unsigned long long c = __hip_atomic_load(path_counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                while (c < total_paths) {
+                    const unsigned long long n = total_paths - c < n_want ? total_paths - c : n_want;
+                    if (__hip_atomic_compare_exchange_strong(path_counter, &c, c + n, __ATOMIC_RELAXED,
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        base = c;
+                        got = n;
+                        break;
+                    }
+                }
+            }
             base = __shfl(base, leader);
+            got = __shfl(got, leader);
             if (want) {
                 const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-                const uint64_t p = base + __popcll(wmask & lt);
-                if (p >= total_paths) {
+                const uint64_t rank = static_cast<uint64_t>(__popcll(wmask & lt));
+                const uint64_t p = base + rank;
+                if (rank >= got) {
                     exhausted = true;
                 } else {
                     // ---- path setup (raygen.cuh:72-113) ----
                     const uint32_t j = find_job(job_begin, njobs, p);
                     const JobDesc& J = jobs[j];
-                    if (path_begin(J, p - J.path_begin, st.o, st.d, st.rng, st.key, st.slot)) {
+                    if (path_begin(J, p - J.path_begin, st.o, st.d, st.rng, st.key, st.pix, st.sl)) {
                         st.beta = mk(1.0f);
                         st.L = mk(0.0f);
                         st.bounce = 0;
@@ -598,7 +623,7 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
         }
 
         if (finish) {
-            J.samples[st.slot] = finalize_radiance(J, st.L);
+            path_finish(J, st.pix, st.sl, st.L);
             alive = false;
             ev.flush(counters);
         }
@@ -606,38 +631,30 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
     if (overflow) atomicAdd(&counters[kCntActiveOverflow], static_cast<unsigned long long>(overflow));
 }
 
-// Fold every job slice of the launch into its Welford mean, in sample order
-// (raygen.cuh:214-238): n_inv = 1 / (samples before + 1).  blockIdx.y picks
-// the job; blocks beyond the launch's job count or the job's pixels exit.
-// Pixel i of job J: fold the slice's sample slots into the Welford mean in
-// sample order (raygen.cuh:214-238): n_inv = 1 / (samples before + 1).
-__device__ __forceinline__ void welford_fold(const JobDesc& J, uint32_t npix, uint32_t i) {
-    const float4 m = J.s_begin == 0 ? make_float4(0.0f, 0.0f, 0.0f, 1.0f) : J.mean[i];
-    f3 mu = f3{m.x, m.y, m.z};
-    for (uint32_t s = 0; s < J.s_count; ++s) {
-        const float4 v = J.samples[static_cast<size_t>(s) * npix + i];
-        const float ninv = rcpf(static_cast<float>(J.s_begin + s + 1));
-        mu = mu + (f3{v.x, v.y, v.z} - mu) * ninv;
-    }
-    J.mean[i] = make_float4(mu.x, mu.y, mu.z, 1.0f);
+// The mean of every pixel of job-table entry J from its accumulator
+// (raygen.cuh:214-238 keeps a running mean; here the sums are exact and the
+// division happens once).  Alpha is 1, as the reference writes it.
+__device__ __forceinline__ void resolve_pixel(const JobDesc& J, uint32_t i) {
+    const unsigned long long* a = J.accum + static_cast<size_t>(i) * kAccumWords;
+    const uint64_t n = a[3];
+    J.mean[i] = make_float4(accum_mean(a[0], a[4], n), accum_mean(a[1], a[5], n), accum_mean(a[2], a[6], n), 1.0f);
 }
 
-__global__ void k_welford_all(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ launch_desc) {
+// Resolve every entry of the launch's job table: blockIdx.y picks the entry;
+// blocks beyond the table's length or the entry's pixels exit.  Entries that
+// are slices of one job (the same accumulator) write the same values.
+__global__ void k_resolve_all(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ launch_desc) {
     const uint32_t j = blockIdx.y;
     if (j >= static_cast<uint32_t>(launch_desc[0])) return;
     const JobDesc& J = jobs[j];
     const uint32_t npix = J.width * J.height;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x) {
-        welford_fold(J, npix, i);
-    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x) resolve_pixel(J, i);
 }
 
-__global__ void k_welford_job(const JobDesc* __restrict__ jobs, uint32_t j) {
+__global__ void k_resolve_job(const JobDesc* __restrict__ jobs, uint32_t j) {
     const JobDesc& J = jobs[j];
     const uint32_t npix = J.width * J.height;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x) {
-        welford_fold(J, npix, i);
-    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x) resolve_pixel(J, i);
 }
 
 __global__ void k_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v) {
@@ -780,34 +797,43 @@ inline uint32_t cdiv(uint64_t a, uint32_t b) { return static_cast<uint32_t>((a +
 
 using namespace vp;
 
-// One step of the persistent renderer over the job table in ctx->jobs_dev:
-// reset the path counter (a fill kernel, so the sequence can be captured in a
-// graph), trace every path of every job slice, fold the slices into the
-// Welford means.  Job count and path total come from ctx->launch_desc.
-hipError_t vp_launch_step(vp_context* ctx, const RenderLaunch& L, uint32_t max_jobs, uint32_t max_npix,
-                          hipStream_t s) {
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctx->path_counter, 1u, 0ull);
+// One persistent render launch over a job table (device pointers): the
+// kernel claims paths from *counter up to launch_desc[1].
+hipError_t vp_launch_render(const RenderLaunch& L, const JobDesc* jobs, const uint64_t* job_begin,
+                            const uint64_t* launch_desc, unsigned long long* counter, uint32_t* slab,
+                            unsigned long long* counters, hipStream_t s) {
     const size_t lds = leaf_queue_bytes(L.block);
     if (L.count_events)
-        hipLaunchKernelGGL(k_render<true>, dim3(L.grid), dim3(L.block), lds, s, ctx->jobs_dev, ctx->job_begin_dev,
-                           ctx->launch_desc, ctx->path_counter, ctx->slab, L.active_cap, L.regen_min,
-                           ctx->counters, L.measure);
+        hipLaunchKernelGGL(k_render<true>, dim3(L.grid), dim3(L.block), lds, s, jobs, job_begin, launch_desc, counter,
+                           slab, L.active_cap, L.regen_min, counters, L.measure);
     else
-        hipLaunchKernelGGL(k_render<false>, dim3(L.grid), dim3(L.block), lds, s, ctx->jobs_dev, ctx->job_begin_dev,
-                           ctx->launch_desc, ctx->path_counter, ctx->slab, L.active_cap, L.regen_min,
-                           ctx->counters, L.measure);
-    return vp_launch_welford(ctx, max_jobs, max_npix, s);
-}
-
-hipError_t vp_launch_welford(vp_context* ctx, uint32_t max_jobs, uint32_t max_npix, hipStream_t s) {
-    const uint32_t gx = std::max<uint32_t>(1u, std::min<uint32_t>(cdiv(max_npix, 256), 512u));
-    hipLaunchKernelGGL(k_welford_all, dim3(gx, max_jobs), dim3(256), 0, s, ctx->jobs_dev, ctx->launch_desc);
+        hipLaunchKernelGGL(k_render<false>, dim3(L.grid), dim3(L.block), lds, s, jobs, job_begin, launch_desc, counter,
+                           slab, L.active_cap, L.regen_min, counters, L.measure);
     return hipGetLastError();
 }
 
-hipError_t vp_launch_welford_job(vp_context* ctx, uint32_t j, uint32_t npix, hipStream_t s) {
+// One step of the persistent renderer over the job table in ctx->jobs_dev:
+// reset the path counter (a fill kernel, so the sequence can be captured in a
+// graph), trace every path of every job slice, resolve the means.  Job count
+// and path total come from ctx->launch_desc.
+hipError_t vp_launch_step(vp_context* ctx, const RenderLaunch& L, uint32_t max_jobs, uint32_t max_npix,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctx->path_counter, 1u, 0ull);
+    const hipError_t e = vp_launch_render(L, ctx->jobs_dev, ctx->job_begin_dev, ctx->launch_desc, ctx->path_counter,
+                                          ctx->slab, ctx->counters, s);
+    if (e != hipSuccess) return e;
+    return vp_launch_resolve(ctx, max_jobs, max_npix, s);
+}
+
+hipError_t vp_launch_resolve(vp_context* ctx, uint32_t max_jobs, uint32_t max_npix, hipStream_t s) {
+    const uint32_t gx = std::max<uint32_t>(1u, std::min<uint32_t>(cdiv(max_npix, 256), 512u));
+    hipLaunchKernelGGL(k_resolve_all, dim3(gx, max_jobs), dim3(256), 0, s, ctx->jobs_dev, ctx->launch_desc);
+    return hipGetLastError();
+}
+
+hipError_t vp_launch_resolve_job(const JobDesc* jobs, uint32_t j, uint32_t npix, hipStream_t s) {
     const uint32_t gx = std::max<uint32_t>(1u, std::min<uint32_t>(cdiv(npix, 256), 1024u));
-    hipLaunchKernelGGL(k_welford_job, dim3(gx), dim3(256), 0, s, ctx->jobs_dev, j);
+    hipLaunchKernelGGL(k_resolve_job, dim3(gx), dim3(256), 0, s, jobs, j);
     return hipGetLastError();
 }
 

@@ -2,15 +2,25 @@
 //
 // Serving: a request loop on its own thread keeps scenes and environment
 // tables resident (requests reference vp_scene / vp_env handles created once)
-// and batches continuously: before every step it admits queued requests while
-// the bytes they need on the device (Welford image + one slice of per-sample
-// radiance) fit the budget and a slot is free; the step then advances every
-// admitted request by one slice of samples in a single persistent launch
-// (replayed from a captured HIP graph), and requests whose last slice landed
-// are copied out and retired, freeing their slot and bytes for the queue.
-// Each request's image depends only on its own (seed, pixel, sample) paths
-// and on folding its slices in sample order, so it is the same alone as in
-// any mix.
+// and batches continuously on the persistent megakernel.  An admitted request
+// becomes one entry of the server's job table (all its samples), appended
+// while earlier launches are still running; a new launch over the longer
+// table then starts on the second of two render streams.  All launches share
+// one path counter and claim from it only up to their own table's total, so
+// the new launch takes the new request's paths while the older one finishes
+// what it claimed: as the older launch's wavefronts run out of paths and
+// exit, the new launch's workgroups take their place, and no request waits
+// for a step barrier or another request's tail.  Each launch has its own
+// active-set slab, so two can run at once.  Finished paths add into their
+// request's per-pixel accumulator (exact fixed-point sums, so the image does
+// not depend on which launch or lane ran which path) and count against its
+// table entry; once the count reaches the request's paths, its accumulator
+// is copied to the host and the mean resolved there while rendering
+// continues.  The side stream that installs requests and copies results only
+// issues copies (DMA engines), never kernels, so it needs no compute unit
+// while the persistent launches hold them all.  Request memory is the
+// accumulator (64 bytes per pixel), whatever the sample count; admission is
+// by those bytes in flight and by a limit on requests in flight.
 //
 // Autotuning: vp_autotune times a reduced copy of a job mix for a grid of
 // launch shapes (megakernel: workgroup size, persistent workgroups per CU,
@@ -42,18 +52,22 @@ vp_status fail(vp_status code, const std::string& msg) {
 }
 
 // Job-table entries one server epoch can hold before it drains and restarts.
-constexpr uint32_t kServerTable = 4096;
+constexpr uint32_t kServerTable = 16384;
+// Device bytes per pixel of a request: its accumulator.
+constexpr size_t kRequestBytesPerPixel = vp::kAccumWords * sizeof(unsigned long long);
+
+enum class ReqState { Queued, Running, Resolving, Done };
 
 struct Request {
     uint64_t ticket = 0;
     vp_job job{};
     vp::JobDesc desc{};
-    float4* mean = nullptr;
-    float4* samples = nullptr;
+    unsigned long long* accum = nullptr;
     size_t bytes = 0;
+    uint64_t paths = 0;     // camera paths of the request (W * H * spp)
     uint32_t entry = 0;     // job-table entry in the current epoch
-    bool completing = false;
-    hipEvent_t ev = nullptr;  // recorded after the request's Welford fold
+    ReqState state = ReqState::Queued;
+    hipEvent_t ev = nullptr;  // recorded after the copy of the request's accumulator
     clk::time_point t_submit;
     double latency_ms = 0.0;
     bool done = false;
@@ -76,188 +90,284 @@ struct vp_server {
     size_t bytes_inflight = 0;
     bool stop = false;
     std::vector<double> latencies;
-    uint64_t completed = 0, samples = 0, steps = 0;
+    uint64_t completed = 0, samples = 0, launches = 0;
     double busy_ms = 0.0;
     std::string error;
+    // device resources, owned by the server (the context's tables stay free
+    // for other render calls)
+    vp::JobDesc* jobs_dev = nullptr;
+    uint64_t* begin_dev = nullptr;
+    unsigned long long* done_dev = nullptr;  // [kServerTable] finished paths per entry
+    unsigned long long* counter = nullptr;   // shared path counter of the epoch's launches
+    uint64_t* desc_dev = nullptr;            // [2][2] launch descriptors, one per render stream
+    uint32_t* slab[2] = {nullptr, nullptr};
+    vp::JobDesc* jobs_pinned = nullptr;
+    uint64_t* begin_pinned = nullptr;
+    unsigned long long* done_pinned = nullptr;
+    uint64_t* desc_pinned = nullptr;  // [2][2]
+    hipStream_t ks[2] = {nullptr, nullptr};  // render streams
+    hipStream_t side = nullptr;              // installs, progress copies, result copies (copies only)
+    unsigned long long* stage = nullptr;     // pinned: accumulator of the request being resolved
+    unsigned long long* zeros = nullptr;     // pinned zeros: accumulator and progress initialisation
+    Request* resolving = nullptr;            // request whose accumulator is being copied
+    hipEvent_t kev[2] = {nullptr, nullptr};  // end of the last launch on each render stream
+    hipEvent_t install_ev = nullptr, poll_ev = nullptr;
     // epoch state, server thread only
+    bool kbusy[2] = {false, false};
     uint32_t njobs = 0;
-    uint64_t total = 0;
-    unsigned long long* done_pinned = nullptr;  // [2][kServerTable] per-entry retired paths
-    uint64_t* desc_pinned = nullptr;            // [2][2] launch descriptors
-    uint32_t njobs_copied[2] = {0, 0};
-    hipEvent_t rep_ev[2] = {nullptr, nullptr};
-    hipStream_t copy_stream = nullptr;
+    uint64_t total = 0;  // paths installed in the epoch
+    uint64_t snap = 0;   // paths visible to the newest launch
+    bool poll_pending = false;
+    uint32_t poll_n = 0;
 };
 
 namespace {
 
-void release_request(vp_server* S, Request* r, hipStream_t s) {
-    if (r->mean) (void)hipFreeAsync(r->mean, s);
-    if (r->samples) (void)hipFreeAsync(r->samples, s);
-    r->mean = r->samples = nullptr;
+void release_request(vp_server* S, Request* r) {
+    if (r->accum) (void)hipFreeAsync(r->accum, S->side);
+    r->accum = nullptr;
     if (r->ev) (void)hipEventDestroy(r->ev);
     r->ev = nullptr;
-    (void)S;
 }
 
 // Finish request r (under S->mu): record its latency and wake pollers.
 void finish_request(vp_server* S, Request* r, vp_status st) {
     r->status = st;
+    r->state = ReqState::Done;
     r->latency_ms = std::chrono::duration<double, std::milli>(clk::now() - r->t_submit).count();
     r->done = true;
     S->bytes_inflight -= r->bytes;
-    S->latencies.push_back(r->latency_ms);
-    ++S->completed;
-    S->samples += static_cast<uint64_t>(r->job.params.width) * r->job.params.height * r->job.params.spp;
+    if (st == VP_OK) {
+        S->latencies.push_back(r->latency_ms);
+        ++S->completed;
+        S->samples += r->paths;
+    }
     S->inflight.erase(std::find(S->inflight.begin(), S->inflight.end(), r));
     S->cv_done.notify_all();
 }
 
-// Continuous batching over the wavefront pipeline.  Each admitted request
-// becomes one job-table entry (all its samples) appended between two graph
-// replays; the pipeline's free path slots pick its paths up at the next
-// iteration, next to whatever else is in flight.  A request is complete
-// when its entry's retired-path count reaches its path count; its samples
-// are then folded and copied out while the pipeline keeps running.
+bool streams_idle(vp_server* S) {
+    for (int i = 0; i < 2; ++i)
+        if (S->kbusy[i] && hipEventQuery(S->kev[i]) == hipSuccess) S->kbusy[i] = false;
+    return !S->kbusy[0] && !S->kbusy[1];
+}
+
+// Install request r as the next table entry: accumulator (stream-ordered
+// on the side stream, zeroed by a copy), table entry and progress counter.
+// On failure nothing of the epoch changes and only r fails.
+vp_status install_request(vp_server* S, Request* r) {
+    const size_t npix = static_cast<size_t>(r->job.params.width) * r->job.params.height;
+    const size_t acc_bytes = npix * vp::kAccumWords * sizeof(unsigned long long);
+    if (hipMallocAsync(reinterpret_cast<void**>(&r->accum), acc_bytes, S->side) != hipSuccess) {
+        (void)hipGetLastError();
+        r->accum = nullptr;
+        return fail(VP_ERR_OOM, "server: cannot allocate a request's accumulator");
+    }
+    const uint32_t e = S->njobs;
+    vp::JobDesc J = r->desc;
+    J.s_begin = 0;
+    J.s_count = J.spp;
+    J.path_begin = S->total;
+    J.path_count = static_cast<uint64_t>(J.npad) * J.spp;
+    J.accum = r->accum;
+    J.mean = nullptr;  // resolved on the host
+    J.samples = nullptr;
+    J.done = S->done_dev + e;
+    S->jobs_pinned[e] = J;
+    S->begin_pinned[e] = J.path_begin;
+    if (hipMemcpyAsync(r->accum, S->zeros, acc_bytes, hipMemcpyHostToDevice, S->side) != hipSuccess ||
+        hipMemcpyAsync(S->done_dev + e, S->zeros, sizeof(unsigned long long), hipMemcpyHostToDevice, S->side) !=
+            hipSuccess ||
+        hipMemcpyAsync(S->jobs_dev + e, S->jobs_pinned + e, sizeof(vp::JobDesc), hipMemcpyHostToDevice, S->side) !=
+            hipSuccess ||
+        hipMemcpyAsync(S->begin_dev + e, S->begin_pinned + e, sizeof(uint64_t), hipMemcpyHostToDevice, S->side) !=
+            hipSuccess)
+        return fail(VP_ERR_HIP, "server: cannot install a request");
+    r->desc = J;
+    r->entry = e;
+    r->state = ReqState::Running;
+    ++S->njobs;
+    S->total += J.path_count;
+    return VP_OK;
+}
+
+// Fail every request in flight after a device error; the next epoch starts
+// from an empty table once the render streams have drained.
+void fail_all(vp_server* S, const std::string& what) {
+    for (auto& k : S->ks) (void)hipStreamSynchronize(k);
+    (void)hipStreamSynchronize(S->side);
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->error = what;
+    while (!S->inflight.empty()) {
+        Request* r = S->inflight.back();
+        release_request(S, r);
+        finish_request(S, r, VP_ERR_HIP);
+    }
+    S->kbusy[0] = S->kbusy[1] = false;
+    S->poll_pending = false;
+    S->resolving = nullptr;
+}
+
 void server_loop(vp_server* S) {
     vp_context* ctx = S->ctx;
     (void)hipSetDevice(ctx->device);
-    hipStream_t s = ctx->stream;
-    uint64_t k = 0;
-    bool fresh = true;  // the pool must be reset before the next replay
+    bool fresh = true;  // the path counter must be reset before the next launch
     clk::time_point busy_since{};
     std::vector<Request*> admit;
     while (true) {
         admit.clear();
+        bool progress = false;
         {
             std::unique_lock<std::mutex> lk(S->mu);
             if (S->inflight.empty() && (S->pending.empty() || S->njobs >= kServerTable)) {
-                // nothing on the GPU: start a new epoch
+                // nothing on the GPU: start a new epoch once the last launches
+                // have exited (a launch still running would claim again from a
+                // reset counter)
+                lk.unlock();
+                for (auto& k : S->ks) (void)hipStreamSynchronize(k);
+                (void)hipStreamSynchronize(S->side);
+                lk.lock();
+                S->kbusy[0] = S->kbusy[1] = false;
+                S->poll_pending = false;
                 if (!fresh) S->busy_ms += std::chrono::duration<double, std::milli>(clk::now() - busy_since).count();
                 S->njobs = 0;
-                S->total = 0;
+                S->total = S->snap = 0;
                 fresh = true;
-                k = 0;
                 if (S->pending.empty()) {
                     if (S->stop) return;
                     S->cv.wait(lk);
                     continue;
                 }
             }
-            // admission: FIFO, by device bytes in flight and free slots
-            while (!S->pending.empty() && S->inflight.size() < S->cfg.max_inflight && S->njobs < kServerTable) {
+            // admission: FIFO, by device bytes in flight and requests in flight
+            while (!S->pending.empty() && S->inflight.size() < S->cfg.max_inflight &&
+                   S->njobs + admit.size() < kServerTable) {
                 Request* r = S->pending.front();
                 if (!S->inflight.empty() && S->bytes_inflight + r->bytes > S->cfg.mem_budget_bytes) break;
                 S->pending.pop_front();
                 S->inflight.push_back(r);
                 S->bytes_inflight += r->bytes;
-                r->entry = S->njobs++;
                 admit.push_back(r);
             }
         }
-        vp_status st = VP_OK;
-        if (fresh) {
-            // the previous epoch's staging copies must have run before its
-            // pinned entries are rewritten
-            if (hipStreamSynchronize(s) != hipSuccess) st = VP_ERR_HIP;
+        if (fresh && !admit.empty()) {
             busy_since = clk::now();
-            if (st == VP_OK) st = vp_wf_prepare(ctx, S->L, s);
-            if (st == VP_OK && (vp_launch_fill_u64(ctx->path_counter, 1u, 0ull, s) != hipSuccess ||
-                                vp_wf_launch_init(ctx->wf, s) != hipSuccess))
-                st = VP_ERR_HIP;
+            if (hipMemcpyAsync(S->counter, S->zeros, sizeof(unsigned long long), hipMemcpyHostToDevice, S->side) !=
+                hipSuccess) {
+                fail_all(S, "server: cannot reset the path counter");
+                continue;
+            }
             fresh = false;
         }
-        // install the admitted requests
+        // install the admitted requests (a request that cannot be installed
+        // fails alone; the table and path total are untouched by it)
+        bool installed = false;
         for (Request* r : admit) {
-            const size_t npix = static_cast<size_t>(r->job.params.width) * r->job.params.height;
-            vp::JobDesc J = r->desc;
-            J.s_begin = 0;
-            J.s_count = J.spp;
-            J.path_begin = S->total;
-            J.path_count = static_cast<uint64_t>(J.npad) * J.spp;
-            if (st == VP_OK && (hipMallocAsync(reinterpret_cast<void**>(&r->mean), npix * sizeof(float4), s) != hipSuccess ||
-                                hipMallocAsync(reinterpret_cast<void**>(&r->samples),
-                                               npix * J.spp * sizeof(float4), s) != hipSuccess))
-                st = VP_ERR_OOM;
-            J.mean = r->mean;
-            J.samples = r->samples;
-            r->desc = J;
-            ctx->jobs_pinned[r->entry] = J;
-            ctx->begin_pinned[r->entry] = J.path_begin;
-            if (st == VP_OK &&
-                (hipMemcpyAsync(ctx->jobs_dev + r->entry, ctx->jobs_pinned + r->entry, sizeof(vp::JobDesc),
-                                hipMemcpyHostToDevice, s) != hipSuccess ||
-                 hipMemcpyAsync(ctx->job_begin_dev + r->entry, ctx->begin_pinned + r->entry, sizeof(uint64_t),
-                                hipMemcpyHostToDevice, s) != hipSuccess ||
-                 vp_launch_fill_u64(ctx->job_done_dev + r->entry, 1u, 0ull, s) != hipSuccess))
-                st = VP_ERR_HIP;
-            S->total += J.path_count;
-        }
-        if (!admit.empty() && st == VP_OK) {
-            uint64_t* d = S->desc_pinned + 2 * (k & 1);
-            d[0] = S->njobs;
-            d[1] = S->total;
-            if (hipMemcpyAsync(ctx->launch_desc, d, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, s) != hipSuccess)
-                st = VP_ERR_HIP;
-        }
-        // one replay (two iterations), then the per-entry progress snapshot
-        if (st == VP_OK && hipGraphLaunch(ctx->wf_graph, s) != hipSuccess) st = VP_ERR_HIP;
-        S->njobs_copied[k & 1] = S->njobs;
-        if (st == VP_OK &&
-            (hipMemcpyAsync(S->done_pinned + static_cast<size_t>(k & 1) * kServerTable, ctx->job_done_dev,
-                            sizeof(unsigned long long) * S->njobs, hipMemcpyDeviceToHost, s) != hipSuccess ||
-             hipEventRecord(S->rep_ev[k & 1], s) != hipSuccess))
-            st = VP_ERR_HIP;
-        if (st != VP_OK) {
-            std::lock_guard<std::mutex> lk(S->mu);
-            S->error = "server step failed";
-            while (!S->inflight.empty()) {
-                Request* r = S->inflight.back();
-                release_request(S, r, s);
+            const vp_status st = install_request(S, r);
+            if (st != VP_OK) {
+                release_request(S, r);
+                std::lock_guard<std::mutex> lk(S->mu);
                 finish_request(S, r, st);
+                continue;
             }
-            continue;
+            installed = true;
         }
-        ++S->steps;
-        if (k > 0) {
-            // progress as of the previous replay (the current one keeps running)
-            const uint64_t pk = k - 1;
-            (void)hipEventSynchronize(S->rep_ev[pk & 1]);
-            const unsigned long long* dn = S->done_pinned + static_cast<size_t>(pk & 1) * kServerTable;
-            const uint32_t seen = S->njobs_copied[pk & 1];
+        if (installed) {
+            progress = true;
+            if (hipEventRecord(S->install_ev, S->side) != hipSuccess) {
+                fail_all(S, "server: event record failed");
+                continue;
+            }
+        }
+        // a launch over the grown table, on a render stream that is free
+        streams_idle(S);
+        if (S->total > S->snap) {
+            const int i = !S->kbusy[0] ? 0 : (!S->kbusy[1] ? 1 : -1);
+            if (i >= 0) {
+                uint64_t* d = S->desc_pinned + 2 * i;
+                d[0] = S->njobs;
+                d[1] = S->total;
+                if (hipStreamWaitEvent(S->ks[i], S->install_ev, 0) != hipSuccess ||
+                    hipMemcpyAsync(S->desc_dev + 2 * i, d, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, S->ks[i]) !=
+                        hipSuccess ||
+                    vp_launch_render(S->L, S->jobs_dev, S->begin_dev, S->desc_dev + 2 * i, S->counter, S->slab[i],
+                                     ctx->counters, S->ks[i]) != hipSuccess ||
+                    hipEventRecord(S->kev[i], S->ks[i]) != hipSuccess) {
+                    fail_all(S, "server: render launch failed");
+                    continue;
+                }
+                S->kbusy[i] = true;
+                S->snap = S->total;
+                ++S->launches;
+                progress = true;
+            }
+        }
+        // progress of the entries, one copy in flight at a time
+        if (!S->poll_pending && S->njobs > 0) {
+            S->poll_n = S->njobs;
+            if (hipMemcpyAsync(S->done_pinned, S->done_dev, sizeof(unsigned long long) * S->poll_n,
+                               hipMemcpyDeviceToHost, S->side) != hipSuccess ||
+                hipEventRecord(S->poll_ev, S->side) != hipSuccess) {
+                fail_all(S, "server: progress copy failed");
+                continue;
+            }
+            S->poll_pending = true;
+        } else if (S->poll_pending && hipEventQuery(S->poll_ev) == hipSuccess) {
+            S->poll_pending = false;
             std::vector<Request*> snapshot;
             {
                 std::lock_guard<std::mutex> lk(S->mu);
                 snapshot = S->inflight;
             }
+            bool err = false;
             for (Request* r : snapshot) {
-                if (!r->completing && r->entry < seen && dn[r->entry] >= r->desc.path_count) {
-                    const uint32_t npix = r->desc.width * r->desc.height;
-                    vp_status rs = VP_OK;
-                    if (vp_launch_welford_job(ctx, r->entry, npix, s) != hipSuccess ||
+                if (r->state == ReqState::Running && S->resolving == nullptr && r->entry < S->poll_n &&
+                    S->done_pinned[r->entry] >= r->paths) {
+                    // every path of the request has finished: copy its
+                    // accumulator out, behind the rendering (one at a time
+                    // through the pinned stage)
+                    const size_t npix = static_cast<size_t>(r->desc.width) * r->desc.height;
+                    if (hipMemcpyAsync(S->stage, r->accum, npix * vp::kAccumWords * sizeof(unsigned long long),
+                                       hipMemcpyDeviceToHost, S->side) != hipSuccess ||
                         hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess ||
-                        hipEventRecord(r->ev, s) != hipSuccess)
-                        rs = VP_ERR_HIP;
-                    r->completing = true;
-                    if (rs != VP_OK) {
-                        release_request(S, r, s);
-                        std::lock_guard<std::mutex> lk(S->mu);
-                        finish_request(S, r, rs);
+                        hipEventRecord(r->ev, S->side) != hipSuccess) {
+                        err = true;
+                        break;
                     }
-                } else if (r->completing && hipEventQuery(r->ev) == hipSuccess) {
-                    const size_t bytes = static_cast<size_t>(r->desc.width) * r->desc.height * sizeof(float4);
-                    vp_status rs = VP_OK;
-                    if (hipMemcpyAsync(r->job.out_rgba, r->mean, bytes, hipMemcpyDeviceToHost, S->copy_stream) !=
-                            hipSuccess ||
-                        hipStreamSynchronize(S->copy_stream) != hipSuccess)
-                        rs = VP_ERR_HIP;
-                    release_request(S, r, s);
-                    std::lock_guard<std::mutex> lk(S->mu);
-                    finish_request(S, r, rs);
+                    r->state = ReqState::Resolving;
+                    S->resolving = r;
+                    progress = true;
                 }
             }
+            Request* r = S->resolving;
+            if (!err && r != nullptr && hipEventQuery(r->ev) == hipSuccess) {
+                // the mean of every pixel, as k_resolve_* forms it on the device
+                const size_t npix = static_cast<size_t>(r->desc.width) * r->desc.height;
+                float* out = r->job.out_rgba;
+                for (size_t i = 0; i < npix; ++i) {
+                    const unsigned long long* a = S->stage + i * vp::kAccumWords;
+                    out[4 * i + 0] = vp::accum_mean(a[0], a[4], a[3]);
+                    out[4 * i + 1] = vp::accum_mean(a[1], a[5], a[3]);
+                    out[4 * i + 2] = vp::accum_mean(a[2], a[6], a[3]);
+                    out[4 * i + 3] = 1.0f;
+                }
+                S->resolving = nullptr;
+                release_request(S, r);
+                std::lock_guard<std::mutex> lk(S->mu);
+                finish_request(S, r, VP_OK);
+                progress = true;
+            }
+            if (err) {
+                fail_all(S, "server: result copy failed");
+                continue;
+            }
         }
-        ++k;
+        if (!progress) {
+            // nothing to do until a launch or copy advances or a request
+            // arrives: wait briefly for a submission
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv.wait_for(lk, std::chrono::microseconds(200));
+        }
     }
 }
 
@@ -270,43 +380,74 @@ double percentile(std::vector<double> v, double q) {
     return i + 1 < v.size() ? v[i] * (1.0 - f) + v[i + 1] * f : v[i];
 }
 
+void free_server(vp_server* S) {
+    if (S->jobs_dev) (void)hipFree(S->jobs_dev);
+    if (S->begin_dev) (void)hipFree(S->begin_dev);
+    if (S->done_dev) (void)hipFree(S->done_dev);
+    if (S->counter) (void)hipFree(S->counter);
+    if (S->desc_dev) (void)hipFree(S->desc_dev);
+    for (auto& p : S->slab)
+        if (p) (void)hipFree(p);
+    if (S->jobs_pinned) (void)hipHostFree(S->jobs_pinned);
+    if (S->begin_pinned) (void)hipHostFree(S->begin_pinned);
+    if (S->done_pinned) (void)hipHostFree(S->done_pinned);
+    if (S->desc_pinned) (void)hipHostFree(S->desc_pinned);
+    if (S->stage) (void)hipHostFree(S->stage);
+    if (S->zeros) (void)hipHostFree(S->zeros);
+    for (auto& k : S->ks)
+        if (k) (void)hipStreamDestroy(k);
+    if (S->side) (void)hipStreamDestroy(S->side);
+    for (auto& e : S->kev)
+        if (e) (void)hipEventDestroy(e);
+    if (S->install_ev) (void)hipEventDestroy(S->install_ev);
+    if (S->poll_ev) (void)hipEventDestroy(S->poll_ev);
+    for (auto& kv : S->by_ticket) delete kv.second;
+    delete S;
+}
+
 }  // namespace
 
 extern "C" {
 
 vp_status vp_server_create(vp_context* ctx, const vp_server_config* cfg, vp_server** out) {
     if (!ctx || !out) return fail(VP_ERR_INVALID, "null argument");
-    auto S = std::make_unique<vp_server>();
+    std::unique_ptr<vp_server, void (*)(vp_server*)> S(new vp_server(), free_server);
     S->ctx = ctx;
     if (cfg) S->cfg = *cfg;
     if (S->cfg.mem_budget_bytes == 0) S->cfg.mem_budget_bytes = size_t(8) << 30;
-    if (S->cfg.slice_spp == 0) S->cfg.slice_spp = 4;
     if (S->cfg.max_inflight == 0) S->cfg.max_inflight = 64;
     S->max_npix = 1024u * 1024u;
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(VP_ERR_HIP, "hipSetDevice failed");
-    // the tuned (or default) shape, on the wavefront pipeline
+    // the tuned (or default) megakernel shape; a tuned wavefront shape keeps
+    // its capacity and takes the megakernel's default shape
     vp_launch_config c{};
     if (ctx->have_tuned) c = ctx->tuned;
-    c.pipeline = VP_PIPELINE_WAVEFRONT;
+    if (c.pipeline != VP_PIPELINE_MEGAKERNEL) c = vp_launch_config{0, 0, c.active_cap, 0, 0, VP_PIPELINE_MEGAKERNEL, 0};
     const uint32_t cap = c.active_cap ? c.active_cap : 128u;
     S->L = vp_resolve_launch(ctx, &c, cap, 0, 0);
-    vp_status st = vp_ensure_jobs(ctx, kServerTable);
-    if (st == VP_OK) st = vp_wf_prepare(ctx, S->L, ctx->stream);
-    if (st == VP_OK) st = vp_reset_counters(ctx, ctx->stream);
-    if (st != VP_OK) return st;
-    if (hipHostMalloc(&S->done_pinned, 2 * kServerTable * sizeof(unsigned long long), hipHostMallocDefault) !=
-            hipSuccess ||
-        hipHostMalloc(&S->desc_pinned, 4 * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess ||
-        hipEventCreateWithFlags(&S->rep_ev[0], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&S->rep_ev[1], hipEventDisableTiming) != hipSuccess ||
-        hipStreamCreateWithFlags(&S->copy_stream, hipStreamNonBlocking) != hipSuccess) {
-        if (S->done_pinned) (void)hipHostFree(S->done_pinned);
-        if (S->desc_pinned) (void)hipHostFree(S->desc_pinned);
-        for (auto& e : S->rep_ev)
-            if (e) (void)hipEventDestroy(e);
-        if (S->copy_stream) (void)hipStreamDestroy(S->copy_stream);
-        return fail(VP_ERR_HIP, "server resources");
-    }
+    const size_t slab_words = vp::kMegaSlabBuffers * static_cast<size_t>(cap) * S->L.grid * S->L.block;
+    bool ok = hipMalloc(&S->jobs_dev, sizeof(vp::JobDesc) * kServerTable) == hipSuccess &&
+              hipMalloc(&S->begin_dev, sizeof(uint64_t) * kServerTable) == hipSuccess &&
+              hipMalloc(&S->done_dev, sizeof(unsigned long long) * kServerTable) == hipSuccess &&
+              hipMalloc(&S->counter, sizeof(unsigned long long)) == hipSuccess &&
+              hipMalloc(&S->desc_dev, 4 * sizeof(uint64_t)) == hipSuccess &&
+              hipMalloc(&S->slab[0], slab_words * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc(&S->slab[1], slab_words * sizeof(uint32_t)) == hipSuccess;
+    if (!ok) return fail(VP_ERR_OOM, "server: cannot allocate its tables and active-set slabs");
+    ok = hipHostMalloc(&S->jobs_pinned, sizeof(vp::JobDesc) * kServerTable, hipHostMallocDefault) == hipSuccess &&
+         hipHostMalloc(&S->begin_pinned, sizeof(uint64_t) * kServerTable, hipHostMallocDefault) == hipSuccess &&
+         hipHostMalloc(&S->done_pinned, sizeof(unsigned long long) * kServerTable, hipHostMallocDefault) ==
+             hipSuccess &&
+         hipHostMalloc(&S->desc_pinned, 4 * sizeof(uint64_t), hipHostMallocDefault) == hipSuccess &&
+         hipHostMalloc(&S->stage, S->max_npix * kRequestBytesPerPixel, hipHostMallocDefault) == hipSuccess &&
+         hipHostMalloc(&S->zeros, S->max_npix * kRequestBytesPerPixel, hipHostMallocDefault) == hipSuccess;
+    if (ok) std::memset(S->zeros, 0, S->max_npix * kRequestBytesPerPixel);
+    for (auto& k : S->ks) ok = ok && hipStreamCreateWithFlags(&k, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipStreamCreateWithFlags(&S->side, hipStreamNonBlocking) == hipSuccess;
+    for (auto& e : S->kev) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&S->install_ev, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&S->poll_ev, hipEventDisableTiming) == hipSuccess;
+    if (!ok) return fail(VP_ERR_HIP, "server: cannot create its streams, events and pinned staging");
     S->th = std::thread(server_loop, S.get());
     *out = S.release();
     return VP_OK;
@@ -317,17 +458,21 @@ vp_status vp_server_submit(vp_server* S, const vp_job* job, uint64_t* ticket) {
     if (!job->scene || !job->env || !job->out_rgba) return fail(VP_ERR_INVALID, "job has a null field");
     vp_status st = vp_check_params(&job->params);
     if (st != VP_OK) return st;
-    if (job->params.width * job->params.height > S->max_npix)
-        return fail(VP_ERR_INVALID, "request larger than the server's 1024x1024 pixel limit");
+    const uint64_t npix = static_cast<uint64_t>(job->params.width) * job->params.height;
+    if (npix > S->max_npix) return fail(VP_ERR_INVALID, "request larger than the server's 1024x1024 pixel limit");
     if (job->params.measure_caps) return fail(VP_ERR_INVALID, "measure_caps is not available through the server");
     if (job->params.active_cap > S->L.active_cap)
         return fail(VP_ERR_INVALID, "request asks for an active-set capacity above the server's (" +
                                         std::to_string(S->L.active_cap) + "): autotune with it first");
+    const size_t bytes = static_cast<size_t>(npix) * kRequestBytesPerPixel;
+    if (bytes > S->cfg.mem_budget_bytes)
+        return fail(VP_ERR_INVALID, "request needs " + std::to_string(bytes) +
+                                        " device bytes, more than the server's memory budget");
     auto* r = new Request();
     r->job = *job;
     r->desc = vp_make_job(job->scene, job->env, &job->camera, &job->params);
-    const size_t npix = static_cast<size_t>(job->params.width) * job->params.height;
-    r->bytes = npix * sizeof(float4) * (1 + static_cast<size_t>(job->params.spp));
+    r->bytes = bytes;
+    r->paths = npix * job->params.spp;
     r->t_submit = clk::now();
     {
         std::lock_guard<std::mutex> lk(S->mu);
@@ -359,7 +504,10 @@ vp_status vp_server_drain(vp_server* S) {
     if (!S) return fail(VP_ERR_INVALID, "server is null");
     std::unique_lock<std::mutex> lk(S->mu);
     S->cv_done.wait(lk, [&] { return S->pending.empty() && S->inflight.empty(); });
-    return S->error.empty() ? VP_OK : fail(VP_ERR_HIP, S->error);
+    // an error fails the requests it hit and is reported once
+    std::string err;
+    std::swap(err, S->error);
+    return err.empty() ? VP_OK : fail(VP_ERR_HIP, err);
 }
 
 vp_status vp_server_stats_get(vp_server* S, vp_server_stats* out) {
@@ -373,7 +521,7 @@ vp_status vp_server_stats_get(vp_server* S, vp_server_stats* out) {
     for (double v : S->latencies) sum += v;
     out->mean_ms = S->latencies.empty() ? 0.0 : sum / static_cast<double>(S->latencies.size());
     out->busy_ms = S->busy_ms;
-    out->steps = S->steps;
+    out->launches = S->launches;
     return VP_OK;
 }
 
@@ -392,14 +540,9 @@ vp_status vp_server_destroy(vp_server* S) {
     // the loop runs until its in-flight requests are done, then exits
     if (S->th.joinable()) S->th.join();
     (void)hipSetDevice(S->ctx->device);
-    (void)hipStreamSynchronize(S->ctx->stream);
-    for (auto& kv : S->by_ticket) delete kv.second;
-    if (S->done_pinned) (void)hipHostFree(S->done_pinned);
-    if (S->desc_pinned) (void)hipHostFree(S->desc_pinned);
-    for (auto& e : S->rep_ev)
-        if (e) (void)hipEventDestroy(e);
-    if (S->copy_stream) (void)hipStreamDestroy(S->copy_stream);
-    delete S;
+    for (auto& k : S->ks) (void)hipStreamSynchronize(k);
+    (void)hipStreamSynchronize(S->side);
+    free_server(S);
     return VP_OK;
 }
 

@@ -4,8 +4,8 @@
 // advances every live path by one bounce with six kernels, each over a
 // compacted queue so its wavefronts do one kind of work:
 //   k_wf_shade        per slot: add the previous bounce's shadow-ray terms to
-//                     L in the estimator's order, retire finished paths to
-//                     their sample slots (counted per job), queue free and
+//                     L in the estimator's order, retire finished paths
+//                     into their pixels' sums (counted per job), queue free and
 //                     live slots (ballot + popcount prefix compaction, one
 //                     atomic per wavefront);
 //   k_wf_claim        one thread: hands out min(free slots, unclaimed paths)
@@ -20,7 +20,7 @@
 //   k_wf_shadow       per shadow ray: fixed-point transmittance.
 // Each path draws its random numbers in the same order as the megakernel
 // (camera jitter, phase, NEE, roulette) and adds its radiance terms in the
-// same order, so both variants write identical per-sample radiance.
+// same order, so both variants produce identical per-sample radiance.
 //
 // Queue counters come in two parity sets; the first kernel of an iteration
 // clears the set the next iteration will use, so a two-iteration sequence
@@ -114,7 +114,7 @@ __global__ void VP_WF_LB k_wf_shade(const JobDesc* __restrict__ jobs, WfPool w, 
         if (state == kSlotDone) {
             job = __float_as_uint(w.dj[i].w);
             const JobDesc& J = jobs[job];
-            J.samples[m.z] = finalize_radiance(J, L);
+            path_finish(J, m.z, 0u, L);
             state = kSlotFree;
             retired = true;
         } else {
@@ -172,8 +172,8 @@ __global__ void VP_WF_LB k_wf_regen(const JobDesc* __restrict__ jobs, const uint
             f3 o, d;
             Pcg32 rng;
             uint64_t key;
-            uint32_t slot;
-            if (path_begin(J, p - J.path_begin, o, d, rng, key, slot)) {
+            uint32_t slot, sl;
+            if (path_begin(J, p - J.path_begin, o, d, rng, key, slot, sl)) {
                 // origin-inside set (sampling.cuh:322-341) by point query, into buffer 0
                 uint32_t na = 0, visits = 0;
                 uint32_t* act = w.slab + static_cast<size_t>(i) * kSlabBuffers * w.cap;
@@ -290,6 +290,7 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
             w4w = walk4_begin();
             have = true;
         }
+        wave_sync();
         if (__ballot(have) == 0) break;
         // node phase: crossed leaves go to the wave queue
         uint32_t count = 0;
@@ -305,6 +306,7 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
             if (step && fits) w4w = nw;
             if (__builtin_amdgcn_ballot_w64(!fits) != 0) break;
         }
+        wave_sync();
         // leaf phase, shared by all lanes (sampling.cuh:372-412)
         const unsigned long long am = __builtin_amdgcn_ballot_w64(true);
         const uint32_t nact = static_cast<uint32_t>(__popcll(am));
@@ -330,6 +332,7 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
                 if (t <= te) atomicMin(&wq.ts[own], __float_as_uint(t));
             }
         }
+        wave_sync();
         if (have) {
             const float wmin = __uint_as_float(wq.ts[lane]);
             tmax = wmin;
@@ -600,6 +603,7 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
             if (full) w4w.n = kWalkEnd;
             have = true;
         }
+        wave_sync();
         if (__ballot(have) == 0) break;
         // node phase: crossed leaves go to the wave queue
         uint32_t count = 0;
@@ -615,6 +619,7 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
             if (step && fits) w4w = nw;
             if (__builtin_amdgcn_ballot_w64(!fits) != 0) break;
         }
+        wave_sync();
         // leaf phase, shared by all lanes: entry tests, then the chords of
         // the entered leaves (fixed-point sums, order free)
         const unsigned long long am = __builtin_amdgcn_ballot_w64(true);
@@ -634,6 +639,7 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
                 entered = tj >= 0.0f;
             }
             const unsigned long long hm = __builtin_amdgcn_ballot_w64(entered);
+            wave_sync();
             if (entered) {
                 const uint32_t h = nhit + lanes_below(hm);
                 wq.prim[h] = prim;
@@ -642,6 +648,7 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
             }
             nhit += static_cast<uint32_t>(__popcll(hm));
         }
+        wave_sync();
         for (uint32_t g = lanes_below(am); g < nhit; g += nact) {
             const uint32_t own = wq.owner[g];
             const f3 ro = lds_load3(wq.ro + 3 * own), rd = lds_load3(wq.rd + 3 * own);
@@ -650,6 +657,7 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
             const float te = exit_from_entry(pr, ro, rd, tj, length2(to_local_dir(pr, rd)));
             if (te > tj && te < kTMax) atomicAdd(&wq.acc[own], tau_fixed(optical_depth(pr, ro, rd, tj, te)));
         }
+        wave_sync();
         if (have) {
             const unsigned long long acc = wq.acc[lane];
             const bool full = acc >= kTauLimitFixed;

@@ -77,14 +77,13 @@ class Job(ctypes.Structure):
 
 
 class ServerConfig(ctypes.Structure):
-    _fields_ = [("mem_budget_bytes", ctypes.c_uint64), ("slice_spp", ctypes.c_uint32),
-                ("max_inflight", ctypes.c_uint32)]
+    _fields_ = [("mem_budget_bytes", ctypes.c_uint64), ("max_inflight", ctypes.c_uint32)]
 
 
 class ServerStats(ctypes.Structure):
     _fields_ = [("completed", ctypes.c_uint64), ("samples", ctypes.c_uint64), ("p50_ms", ctypes.c_double),
                 ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double), ("busy_ms", ctypes.c_double),
-                ("steps", ctypes.c_uint64)]
+                ("launches", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -320,6 +319,10 @@ class Context:
                                      ctypes.byref(st)), "vp_render_jobs")
         return outs, st.as_dict()
 
+    def normalize_accumulator(self, out_ptr, mean_ptr, n, scale, stream=None):
+        """out[i] = mean[i] * scale over n RGBA pixels (device pointers)."""
+        _check(load().vp_normalize_accumulator(out_ptr, mean_ptr, n, scale, stream), "vp_normalize_accumulator")
+
     def count_events(self, scene, env, cam, params):
         ev = np.zeros(len(EVENT_NAMES), np.uint64)
         _check(load().vp_render_count_events(self.handle, scene.handle, env.handle, ctypes.byref(cam),
@@ -335,8 +338,8 @@ class Context:
         _check(load().vp_autotune(self.handle, arr, len(jobs), ctypes.byref(best)), "vp_autotune")
         return best
 
-    def server(self, mem_budget_bytes=8 << 30, slice_spp=4, max_inflight=64):
-        return Server(self, mem_budget_bytes, slice_spp, max_inflight)
+    def server(self, mem_budget_bytes=8 << 30, max_inflight=64):
+        return Server(self, mem_budget_bytes, max_inflight)
 
     def transmittance_rays(self, scene, rays6):
         rays6 = f32(rays6).reshape(-1, 6)
@@ -375,9 +378,9 @@ class Server:
     """Continuous-batching request loop (vp_server_*).  submit() returns a
     ticket and the host image the result lands in; poll() reports completion."""
 
-    def __init__(self, ctx, mem_budget_bytes, slice_spp, max_inflight):
+    def __init__(self, ctx, mem_budget_bytes, max_inflight):
         self.ctx = ctx
-        cfg = ServerConfig(mem_budget_bytes, slice_spp, max_inflight)
+        cfg = ServerConfig(mem_budget_bytes, max_inflight)
         h = _vp()
         _check(load().vp_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)), "vp_server_create")
         self.handle = h

@@ -154,7 +154,12 @@ vp_status vp_render(vp_context* ctx, vp_scene* scene, vp_env* env, const vp_came
                     const vp_render_params* params, float* out_rgba, vp_render_stats* stats);
 /* Same, output stays on the device: out_rgba_dev (W*H*4 floats), and, if not
  * null, out_samples_dev (spp*W*H*4 floats, sample-major) receives every
- * sample's radiance.  stream is a hipStream_t (null = the context stream). */
+ * sample's radiance (the megakernel writes it; a wavefront launch
+ * configuration falls back to the megakernel).  stream is a hipStream_t
+ * (null = the context stream).  A pixel's mean is the exact 32.32 fixed-point
+ * sum of its samples over their count (the reference keeps a running Welford
+ * mean, raygen.cuh:214-238; the sum makes the result independent of the order
+ * in which paths finish). */
 vp_status vp_render_device(vp_context* ctx, vp_scene* scene, vp_env* env, const vp_camera* cam,
                            const vp_render_params* params, const vp_launch_config* launch,
                            float* out_rgba_dev, float* out_samples_dev, void* stream, vp_render_stats* stats);
@@ -176,26 +181,31 @@ vp_status vp_render_jobs(vp_context* ctx, vp_job* jobs, uint32_t njobs, const vp
 vp_status vp_autotune(vp_context* ctx, vp_job* jobs, uint32_t njobs, vp_launch_config* best);
 
 /* ---- serving form ------------------------------------------------------ */
-/* Continuous batching on the wavefront pipeline: a server thread keeps the
- * pipeline running while requests are in flight; each admitted request joins
- * the job table between two iterations and its paths fill free path slots
- * next to the other requests'.  Scenes and environments stay resident in
- * their handles.  While a server exists its context belongs to it: other
- * render calls on that context are allowed only while the server is idle. */
+/* Continuous batching on the persistent megakernel: a server thread appends
+ * each admitted request to its job table while earlier render launches are
+ * still running and starts a launch over the longer table on a second
+ * stream; launches share one path counter, so the new one takes the new
+ * request's paths while the older one drains and the GPU stays full.
+ * Finished paths add into the request's per-pixel accumulator (64 bytes per
+ * pixel, whatever the spp); a request is resolved and copied out as soon as
+ * its last path finishes.  Scenes and environments stay resident in their
+ * handles.  The server owns its job table, path counter and slabs, so other
+ * render calls on the same context stay possible (they share the GPU). */
 typedef struct {
-    uint64_t mem_budget_bytes; /* admission limit: sample slots + image bytes of the requests in flight */
-    uint32_t slice_spp;        /* reserved (requests enter the pipeline with all their samples) */
-    uint32_t max_inflight;     /* requests on the GPU at once */
+    uint64_t mem_budget_bytes; /* admission limit: accumulator bytes of the requests in flight (0 = 8 GiB) */
+    uint32_t max_inflight;     /* requests on the GPU at once (0 = 64) */
 } vp_server_config;
 typedef struct {
     uint64_t completed;
     uint64_t samples;
     double p50_ms, p99_ms, mean_ms; /* submit-to-result latency */
     double busy_ms;                 /* time with requests in flight */
-    uint64_t steps;                 /* graph replays (two pipeline iterations each) */
+    uint64_t launches;              /* render launches started */
 } vp_server_stats;
-/* The launch shape is the context's autotuned one (else the defaults), on the
- * wavefront pipeline; its active-set capacity is the largest a request may ask for. */
+/* The launch shape is the context's autotuned megakernel one (else the
+ * defaults); its active-set capacity is the largest a request may ask for.
+ * A request above 1024x1024 pixels or above the memory budget is rejected
+ * at submit. */
 vp_status vp_server_create(vp_context* ctx, const vp_server_config* cfg, vp_server** out);
 /* Queue a request; *ticket identifies it.  The output buffer must stay valid
  * until vp_server_poll reports completion. */

@@ -23,7 +23,11 @@
  *     reference CDF is restated too (vpo_env_ref_cdf) so tests can pin the
  *     distribution;
  *   - bilinear environment lookup with exact float weights;
- *   - the entry hit list is unbounded (the reference drops beyond 128).
+ *   - the entry hit list is unbounded (the reference drops beyond 128);
+ *   - a pixel's mean is the exact 32.32 fixed-point sum of its samples
+ *     divided by their count (accum_add / accum_mean), where the reference
+ *     folds a running Welford mean in sample order (raygen.cuh:214-238): same
+ *     estimator, rounded once at the end instead of once per sample.
  * Parity status of each function is recorded in DESIGN.md.
  *
  * Build: oracle/Makefile (gcc -O2 -ffp-contract=off).
@@ -1214,6 +1218,29 @@ static void counters_add(counters_t* a, const counters_t* b) {
     if (b->max_overlap > a->max_overlap) a->max_overlap = b->max_overlap;
 }
 
+/* Exact per-pixel sum of sample radiance in 32.32 fixed point (the GPU's
+ * radiance_fixed / accum_mean): each channel's low and high 32-bit halves are
+ * summed in separate 64-bit words, so the total is an order-free integer. */
+typedef struct { uint64_t lo[3], hi[3], n; } accum_t;
+static inline uint64_t radiance_fixed(float v) {
+    float c = vpo_min(vpo_max(v, 0.0f), 2147483648.0f);
+    return (uint64_t)rintf(c * 4294967296.0f);
+}
+static inline void accum_add(accum_t* a, v3 L) {
+    const float c[3] = {L.x, L.y, L.z};
+    for (int i = 0; i < 3; ++i) {
+        uint64_t q = radiance_fixed(c[i]);
+        a->lo[i] += q & 0xffffffffu;
+        a->hi[i] += q >> 32;
+    }
+    a->n += 1;
+}
+static inline float accum_mean(uint64_t lo, uint64_t hi, uint64_t n) {
+    if (n == 0) return 0.0f;
+    double s = (double)hi * 4294967296.0 + (double)lo;
+    return (float)((s * (1.0 / 4294967296.0)) / (double)n);
+}
+
 static void* worker(void* arg) {
     job_t* J = (job_t*)arg;
     const vpo_params* P = J->p;
@@ -1229,8 +1256,10 @@ static void* worker(void* arg) {
         const uint32_t st = P->stride ? P->stride : 1u;
         if ((y - P->y0) % st) continue;
         for (uint32_t x = P->x0; x < P->x1; x += st) {
-            /* Welford mean over samples in order (raygen.cuh:214-219) */
-            v3 mean = V(0, 0, 0);
+            /* mean over the samples (raygen.cuh:214-219), as an exact
+             * fixed-point sum */
+            accum_t acc;
+            memset(&acc, 0, sizeof(acc));
             for (uint32_t s = P->sample_begin; s < P->sample_end; ++s) {
                 v3 Ls = trace_sample(J, x, y, s, buf, cand, &k);
                 if (J->out_samples) {
@@ -1240,9 +1269,10 @@ static void* worker(void* arg) {
                     J->out_samples[idx + 2] = Ls.z;
                     J->out_samples[idx + 3] = 1.0f;
                 }
-                float ninv = vpo_rcp((float)(s - P->sample_begin + 1));
-                mean = vadd(mean, vscale(vsub(Ls, mean), ninv));
+                accum_add(&acc, Ls);
             }
+            v3 mean = V(accum_mean(acc.lo[0], acc.hi[0], acc.n), accum_mean(acc.lo[1], acc.hi[1], acc.n),
+                        accum_mean(acc.lo[2], acc.hi[2], acc.n));
             if (J->out_mean) {
                 size_t idx = ((size_t)(y - P->y0) * (P->x1 - P->x0) + (x - P->x0)) * 4;
                 J->out_mean[idx] = mean.x;
