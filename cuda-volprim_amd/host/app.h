@@ -14,9 +14,14 @@
 // the facade; the C ABI underneath only returns codes.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <filesystem>
 #include <optional>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/vp_api.h"
@@ -24,24 +29,87 @@
 
 namespace vpapp {
 
+namespace fs = std::filesystem;
+using uint = unsigned int;
+
+namespace utils {
+// Error and Result of the reference's host utilities
+// (include/thesis/host/utils/result.h:48-121): a value or an (code, message)
+// error, with try_unwrap_or_exit logging the error and exiting with its code.
+struct Error {
+    int code_;
+    std::string msg_;
+};
+template <typename T>
+class Result {
+   public:
+    Result(T v) : v_(std::move(v)) {}
+    Result(Error e) : e_(std::move(e)) {}
+    explicit operator bool() const { return v_.has_value(); }
+    bool has_value() const { return v_.has_value(); }
+    T& operator*() { return *v_; }
+    const T& operator*() const { return *v_; }
+    T* operator->() { return &*v_; }
+    const Error& error() const { return *e_; }
+
+   private:
+    std::optional<T> v_;
+    std::optional<Error> e_;
+};
+inline Error make_error(int code, std::string msg) { return Error{code, std::move(msg)}; }
+// Logs "Fatal: <msg> (code <code>)" and exits with the error's code.
+template <typename T>
+T try_unwrap_or_exit(Result<T>&& res) {
+    if (!res) {
+        std::fprintf(stderr, "[error] Fatal: %s (code %d)\n", res.error().msg_.c_str(), res.error().code_);
+        std::exit(res.error().code_);
+    }
+    return std::move(*res);
+}
+}  // namespace utils
+
+// app::Config (include/thesis/host/app/config.h:14-50): the same field names,
+// types and defaults, so a caller that fills the reference's Config compiles
+// against this one.  The OptiX program and module names have no meaning here
+// (there is no SBT and no module); they are kept for source compatibility
+// and ignored.  active_cap_ is this build's: the reference fixes its
+// active-set capacity at compile time (device/core/constants.cuh:45-48).
 struct Config {
-    uint32_t image_width = 1920;
-    uint32_t image_height = 1080;
-    uint32_t num_samples_per_pixel = 64;
-    uint32_t seed = 42;
-    std::string output_path = "output.exr";
-    std::string env_map_path = "assets/environment_maps/white_constant.hdr";
-    bool denoise = false;
-    uint32_t max_bounces = 128;
-    uint32_t rr_depth = 12;
-    float rr_max_survival = 0.99f;
-    float firefly_clamp_luminance = 0.0f;
-    float pixel_filter_stddev = 0.0f;
-    float hg_g = 0.85f;
-    bool use_ris = false;
-    uint32_t ris_num_candidates = 6;
-    bool measure_caps = false;
-    uint32_t active_cap = 128;
+    std::string raygen_function_name_ = "__raygen__rg";
+    std::string miss_function_name_ = "__miss__ms";
+    std::string anyhit_function_name_ = "__anyhit__ah";
+    std::string launch_params_variable_name_ = "launch_params";
+
+    fs::path output_path_ = "output.exr";
+    fs::path module_blob_path_ = fs::path("build") / "device_program.optixir";
+    fs::path env_map_path_ = fs::path("assets") / "environment_maps" / "meadow_2_4k.hdr";
+
+    size_t num_samples_per_pixel_ = 1;
+    size_t image_width_ = 1000;
+    size_t image_height_ = 750;
+
+    uint seed_ = 42;
+    bool denoise_ = false;
+
+    size_t max_bounces_ = 128;
+    size_t rr_depth_ = 12;
+    float rr_max_survival_ = 0.99f;
+    float firefly_clamp_luminance_ = 0.0f;
+    float pixel_filter_stddev_ = 0.0f;
+    float hg_g_ = 0.85f;
+    bool use_ris_ = false;
+    uint32_t ris_num_candidates_ = 6;
+    bool measure_caps_ = false;
+
+    uint32_t active_cap_ = 128;
+
+    // Config::parse (src/thesis/host/app/config.cpp:11-69): the same options
+    // (--output --height --width --samples_per_pixel --raygen --miss --anyhit
+    // --launch_params --module_blob --env_map --seed --denoise --max-depth
+    // --rr-depth --rr-max-survival --firefly-clamp --filter-width --phase-g
+    // --measure-caps, plus --ris --ris-candidates --active-cap), the same
+    // range checks, an error Result on a bad argument, never an exception.
+    [[nodiscard]] static utils::Result<Config> parse(int argc, char* argv[]) noexcept;
 };
 
 struct ViewCamera {

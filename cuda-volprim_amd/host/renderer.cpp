@@ -73,7 +73,7 @@ bool load_environment(const std::string& path, std::vector<float>& rgba, uint32_
 
 Renderer::Renderer(const Config& config, std::vector<vp_primitive>&& primitives, std::optional<vp_camera> camera)
     : config_(config), num_primitives_(static_cast<uint32_t>(primitives.size())) {
-    if (config_.denoise)
+    if (config_.denoise_)
         die("--denoise: the OptiX AI denoiser has no counterpart in this build (the AOVs it needs are not produced)");
     check(vp_context_create(0, &ctx_), "vp_context_create");
     // the walk tree over the median-split leaf order: a host build (about 1 s for 10^6 Gaussians)
@@ -83,14 +83,15 @@ Renderer::Renderer(const Config& config, std::vector<vp_primitive>&& primitives,
     std::vector<float> env;
     uint32_t ew = 0, eh = 0;
     std::string err;
-    if (!load_environment(config_.env_map_path, env, ew, eh, err)) die(err);
+    if (!load_environment(config_.env_map_path_.string(), env, ew, eh, err)) die(err);
     check(vp_env_create(ctx_, env.data(), ew, eh, &env_), "vp_env_create");
     if (camera) {
         camera_ = *camera;
     } else {
         // Camera::getDefaultCamera: lookfrom (0,0,-1) at origin, up +y, 90 degree vertical fov
         const float eye[3] = {0.0f, 0.0f, -1.0f}, at[3] = {0.0f, 0.0f, 0.0f}, up[3] = {0.0f, 1.0f, 0.0f};
-        camera_ = create_perspective(config_.image_width, config_.image_height, eye, at, up, 90.0f);
+        camera_ = create_perspective(static_cast<uint32_t>(config_.image_width_),
+                                     static_cast<uint32_t>(config_.image_height_), eye, at, up, 90.0f);
     }
 }
 
@@ -104,20 +105,24 @@ void Renderer::render() {
     const auto t0 = std::chrono::steady_clock::now();
     vp_render_params p{};
     vp_default_params(&p);
-    p.width = config_.image_width;
-    p.height = config_.image_height;
-    p.spp = config_.num_samples_per_pixel;
-    p.seed = config_.seed;
-    p.max_depth = config_.max_bounces;
-    p.rr_depth = config_.rr_depth;
-    p.rr_max_survival = config_.rr_max_survival;
-    p.firefly_clamp = config_.firefly_clamp_luminance;
-    p.filter_stddev = config_.pixel_filter_stddev;
-    p.phase_g = config_.hg_g;
-    p.use_ris = config_.use_ris ? 1u : 0u;
-    p.ris_candidates = config_.ris_num_candidates;
-    p.measure_caps = config_.measure_caps ? 1u : 0u;
-    p.active_cap = config_.active_cap;
+    auto u32 = [](size_t v, const char* what) {
+        if (v > 0xffffffffull) die(std::string(what) + " is too large");
+        return static_cast<uint32_t>(v);
+    };
+    p.width = u32(config_.image_width_, "image width");
+    p.height = u32(config_.image_height_, "image height");
+    p.spp = u32(config_.num_samples_per_pixel_, "samples per pixel");
+    p.seed = config_.seed_;
+    p.max_depth = u32(config_.max_bounces_, "max depth");
+    p.rr_depth = u32(config_.rr_depth_, "rr depth");
+    p.rr_max_survival = config_.rr_max_survival_;
+    p.firefly_clamp = config_.firefly_clamp_luminance_;
+    p.filter_stddev = config_.pixel_filter_stddev_;
+    p.phase_g = config_.hg_g_;
+    p.use_ris = config_.use_ris_ ? 1u : 0u;
+    p.ris_candidates = config_.ris_num_candidates_;
+    p.measure_caps = config_.measure_caps_ ? 1u : 0u;
+    p.active_cap = config_.active_cap_;
     std::printf("[info] Render params: max_depth=%u rr_depth=%u rr_max_survival=%.3f hg_g=%.3f%s firefly_clamp=%.3g "
                 "filter_stddev=%.3g%s seed=%u ris=%d\n",
                 p.max_depth, p.rr_depth, p.rr_max_survival, p.phase_g,
@@ -132,16 +137,17 @@ void Renderer::render() {
                     static_cast<unsigned long long>(stats_.active_overflow));
     else
         std::printf("[info] Cap check: 0 overflows (active-set capacity %u sufficient for this run).\n", p.active_cap);
-    if (config_.measure_caps) {
+    if (config_.measure_caps_) {
         auto suggest = [](uint32_t v) { return (((v * 9 + 7) / 8) + 15) / 16 * 16; };
         std::printf("[info] Cap measurement: max hits/ray = %u, max point-overlap = %u\n", stats_.max_hits_per_ray,
                     stats_.max_point_overlap);
         std::printf("[info] Suggested caps: hit list unbounded (streamed), --active-cap %u\n",
                     suggest(stats_.max_point_overlap));
     }
-    if (vp_io_save_exr(config_.output_path.c_str(), image_.data(), p.width, p.height) != 0)
+    const std::string out = config_.output_path_.string();
+    if (vp_io_save_exr(out.c_str(), image_.data(), p.width, p.height) != 0)
         die(std::string("EXR save failed: ") + vp_io_last_error());
-    std::printf("[info] Saved '%s'\n", config_.output_path.c_str());
+    std::printf("[info] Saved '%s'\n", out.c_str());
     const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::printf("[info] Rendering complete - Total time: %.3fs (%lld ms)\n", sec, static_cast<long long>(sec * 1000.0));
 }

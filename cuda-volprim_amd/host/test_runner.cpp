@@ -136,24 +136,24 @@ bool parse(int argc, char** argv, TestConfig& c, std::string& err) {
 
 vpapp::Config renderer_config(const TestConfig& t, uint32_t w, uint32_t h, const std::string& out,
                               const std::string& env) {
-    vpapp::Config c;
-    c.image_width = w;
-    c.image_height = h;
-    c.num_samples_per_pixel = t.spp;
-    c.output_path = out;
-    c.seed = t.seed;
-    c.denoise = t.denoise;
-    c.max_bounces = t.max_depth;
-    c.rr_depth = t.rr_depth;
-    c.rr_max_survival = t.rr_max_survival;
-    c.firefly_clamp_luminance = t.firefly_clamp;
-    c.pixel_filter_stddev = t.filter_stddev;
-    c.hg_g = t.hg_g;
-    c.use_ris = t.use_ris;
-    c.ris_num_candidates = t.ris_candidates;
-    c.measure_caps = t.measure_caps;
-    c.env_map_path = env;
-    c.active_cap = t.active_cap;
+    vpapp::Config c;  // test_runner.cpp:188-191 fills the renderer's Config the same way
+    c.image_width_ = w;
+    c.image_height_ = h;
+    c.num_samples_per_pixel_ = t.spp;
+    c.output_path_ = out;
+    c.seed_ = t.seed;
+    c.denoise_ = t.denoise;
+    c.max_bounces_ = t.max_depth;
+    c.rr_depth_ = t.rr_depth;
+    c.rr_max_survival_ = t.rr_max_survival;
+    c.firefly_clamp_luminance_ = t.firefly_clamp;
+    c.pixel_filter_stddev_ = t.filter_stddev;
+    c.hg_g_ = t.hg_g;
+    c.use_ris_ = t.use_ris;
+    c.ris_num_candidates_ = t.ris_candidates;
+    c.measure_caps_ = t.measure_caps;
+    c.env_map_path_ = env;
+    c.active_cap_ = t.active_cap;
     return c;
 }
 
