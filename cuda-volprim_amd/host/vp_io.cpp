@@ -11,7 +11,9 @@
 #include <cstring>
 #include <fstream>
 #include <sstream>
+#include <functional>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/vp_io.h"
@@ -184,71 +186,105 @@ int vp_io_load_ply(const char* path, float sigma_multiplier, const float albedo_
     }
     const uint64_t N = V.count;
     if (N == 0 || N > (1ull << 30)) return fail("PLY vertex count out of range");
-    std::vector<float> vals(static_cast<size_t>(N) * 14);
-    if (ascii) {
-        for (uint64_t r = 0; r < N; ++r) {
-            std::istringstream ss(line());
-            std::vector<double> v(V.props.size());
-            std::string tok;
-            for (auto& x : v) {
-                // strtod, unlike operator>>, also reads "nan" and "inf"
-                char* end = nullptr;
-                if (!(ss >> tok)) return fail("PLY ascii row " + std::to_string(r) + " is short");
-                x = std::strtod(tok.c_str(), &end);
-                if (*end) return fail("PLY ascii row " + std::to_string(r) + ": bad number '" + tok + "'");
-            }
-            for (int k = 0; k < 14; ++k) vals[r * 14 + k] = static_cast<float>(v[idx[k]]);
-        }
-    } else {
-        if (pos + row * N > buf.size()) return fail("PLY file truncated");
-        for (uint64_t r = 0; r < N; ++r) {
-            const unsigned char* p = buf.data() + pos + r * row;
-            for (int k = 0; k < 14; ++k) vals[r * 14 + k] = static_cast<float>(pread(V.props[idx[k]].type, p + off[idx[k]]));
-        }
-    }
+    // Rows are decoded, converted and validated in parallel chunks, as the
+    // reference does with std::execution::par (ply.cpp:97,123); the first
+    // failing row (lowest index) is the one reported.
     const bool override_alb = albedo_override && albedo_override[0] >= 0.0f && albedo_override[1] >= 0.0f &&
                               albedo_override[2] >= 0.0f;
+    std::vector<size_t> line_start;  // ascii: offset of each vertex row
+    if (ascii) {
+        line_start.resize(static_cast<size_t>(N));
+        for (uint64_t r = 0; r < N; ++r) {
+            if (pos >= buf.size()) return fail("PLY ascii row " + std::to_string(r) + " is short");
+            line_start[r] = pos;
+            line();
+        }
+    } else if (pos + row * N > buf.size()) {
+        return fail("PLY file truncated");
+    }
     auto* prims = static_cast<vp_primitive*>(std::malloc(sizeof(vp_primitive) * N));
     if (!prims) return fail("out of host memory");
-    for (uint64_t i = 0; i < N; ++i) {
-        const float* r = &vals[i * 14];
-        vp_primitive& p = prims[i];
-        for (int k = 0; k < 3; ++k) p.center[k] = r[k];
-        for (int k = 0; k < 4; ++k) p.quat_wxyz[k] = r[3 + k];  // normalised by vp_scene_create
-        for (int k = 0; k < 3; ++k) p.scale[k] = std::exp(r[7 + k]);
-        for (int k = 0; k < 3; ++k) {
-            float a = override_alb ? albedo_override[k] : r[10 + k];
-            if (!std::isfinite(a)) a = 0.0f;  // ply.cpp:126-137: NaN -> 0, clamp to [0, 1]
-            a = a < 0.0f ? 0.0f : (a > 1.0f ? 1.0f : a);
-            p.albedo[k] = a;
+    struct RowError {
+        uint64_t row = ~0ull;
+        std::string msg;
+    };
+    // decode, convert (ply.cpp:107-142) and validate (ply.cpp:146-172) rows [b, e)
+    auto work = [&](uint64_t b, uint64_t e, RowError& err) {
+        std::vector<double> v(V.props.size());
+        for (uint64_t i = b; i < e; ++i) {
+            float r[14];
+            if (ascii) {
+                const char* c = reinterpret_cast<const char*>(buf.data() + line_start[i]);
+                const char* lim = reinterpret_cast<const char*>(buf.data() + buf.size());
+                for (size_t q = 0; q < v.size(); ++q) {
+                    while (c < lim && (*c == ' ' || *c == '\t' || *c == '\r')) ++c;
+                    if (c >= lim || *c == '\n') {
+                        err = {i, "PLY ascii row " + std::to_string(i) + " is short"};
+                        return;
+                    }
+                    // strtod, unlike operator>>, also reads "nan" and "inf"
+                    const char* tok = c;
+                    while (c < lim && *c != ' ' && *c != '\t' && *c != '\r' && *c != '\n') ++c;
+                    const std::string t(tok, c);
+                    char* end = nullptr;
+                    v[q] = std::strtod(t.c_str(), &end);
+                    if (*end) {
+                        err = {i, "PLY ascii row " + std::to_string(i) + ": bad number '" + t + "'"};
+                        return;
+                    }
+                }
+                for (int k = 0; k < 14; ++k) r[k] = static_cast<float>(v[idx[k]]);
+            } else {
+                const unsigned char* p = buf.data() + pos + i * row;
+                for (int k = 0; k < 14; ++k) r[k] = static_cast<float>(pread(V.props[idx[k]].type, p + off[idx[k]]));
+            }
+            vp_primitive& p = prims[i];
+            for (int k = 0; k < 3; ++k) p.center[k] = r[k];
+            for (int k = 0; k < 4; ++k) p.quat_wxyz[k] = r[3 + k];  // normalised by vp_scene_create
+            for (int k = 0; k < 3; ++k) p.scale[k] = std::exp(r[7 + k]);
+            for (int k = 0; k < 3; ++k) {
+                float a = override_alb ? albedo_override[k] : r[10 + k];
+                if (!std::isfinite(a)) a = 0.0f;  // ply.cpp:126-137: NaN -> 0, clamp to [0, 1]
+                a = a < 0.0f ? 0.0f : (a > 1.0f ? 1.0f : a);
+                p.albedo[k] = a;
+            }
+            p.mass = r[13] * sigma_multiplier;
+            const std::string id = "Primitive " + std::to_string(i) + ": ";
+            const char* bad = nullptr;
+            if (!(p.scale[0] > 0.0f && p.scale[1] > 0.0f && p.scale[2] > 0.0f))
+                bad = "Invalid scale - all components must be > 0";
+            else if (!std::isfinite(p.scale[0]) || !std::isfinite(p.scale[1]) || !std::isfinite(p.scale[2]))
+                bad = "NaN/Inf in scale";
+            else if (!(p.mass > 0.0f))
+                bad = "Invalid optical_thickness - must be > 0";
+            else if (!std::isfinite(p.mass))
+                bad = "NaN/Inf in optical_thickness";
+            else if (!std::isfinite(p.center[0]) || !std::isfinite(p.center[1]) || !std::isfinite(p.center[2]))
+                bad = "NaN/Inf in center";
+            if (bad) {
+                err = {i, "Primitive " + std::to_string(i) + ": " + bad};
+                return;
+            }
         }
-        p.mass = r[13] * sigma_multiplier;
+    };
+    const uint64_t nthreads = std::max<uint64_t>(
+        1, std::min<uint64_t>({static_cast<uint64_t>(std::max(1u, std::thread::hardware_concurrency())), 16,
+                               (N + 65535) / 65536}));
+    std::vector<RowError> errs(nthreads);
+    {
+        std::vector<std::thread> th;
+        const uint64_t chunk = (N + nthreads - 1) / nthreads;
+        for (uint64_t t = 0; t < nthreads; ++t) {
+            const uint64_t b = t * chunk, e = std::min<uint64_t>(N, b + chunk);
+            if (b < e) th.emplace_back(work, b, e, std::ref(errs[t]));
+        }
+        for (auto& x : th) x.join();
     }
-    // validation (ply.cpp:146-172), first failing primitive reported
-    for (uint64_t i = 0; i < N; ++i) {
-        const vp_primitive& p = prims[i];
-        const std::string id = "Primitive " + std::to_string(i) + ": ";
-        if (!(p.scale[0] > 0.0f && p.scale[1] > 0.0f && p.scale[2] > 0.0f)) {
+    for (const RowError& e : errs)  // chunks in row order: the first error is the lowest row
+        if (e.row != ~0ull) {
             std::free(prims);
-            return fail(id + "Invalid scale - all components must be > 0");
+            return fail(e.msg);
         }
-        if (!std::isfinite(p.scale[0]) || !std::isfinite(p.scale[1]) || !std::isfinite(p.scale[2])) {
-            std::free(prims);
-            return fail(id + "NaN/Inf in scale");
-        }
-        if (!(p.mass > 0.0f)) {
-            std::free(prims);
-            return fail(id + "Invalid optical_thickness - must be > 0");
-        }
-        if (!std::isfinite(p.mass)) {
-            std::free(prims);
-            return fail(id + "NaN/Inf in optical_thickness");
-        }
-        if (!std::isfinite(p.center[0]) || !std::isfinite(p.center[1]) || !std::isfinite(p.center[2])) {
-            std::free(prims);
-            return fail(id + "NaN/Inf in center");
-        }
-    }
     *out = prims;
     *n_out = static_cast<uint32_t>(N);
     return 0;
