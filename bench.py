@@ -198,7 +198,8 @@ def serve(ctx, reqs, arrivals, mem_gib=16, max_inflight=64):
     vp_server; returns (seconds from the first submission to the last
     result, per-request latencies in ms, server stats, images)."""
     import threading
-    srv = ctx.server(mem_budget_bytes=mem_gib << 30, max_inflight=max_inflight)
+    srv = ctx.server(mem_budget_bytes=mem_gib << 30, max_inflight=max_inflight,
+                     active_cap=max(r[3].active_cap for r in reqs))
     tickets = [None] * len(reqs)
     t0 = time.perf_counter()
 
@@ -312,10 +313,13 @@ def run_stream(args):
     roofline, events = None, None
     if not args.no_count:
         events = {k: 0 for k in volprim.EVENT_NAMES}
-        for r in reqs:
+        t0 = time.perf_counter()
+        for i, r in enumerate(reqs):
             ev = ctx.count_events(*r)
             for k in events:
                 events[k] += ev[k]
+            print(f"[stream] event counts: request {i + 1}/{K} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
+                  flush=True)
         flops = sum(FLOP_WEIGHTS[k] * events[k] for k in FLOP_WEIGHTS)
         kms = bst["kernel_ms"] / max(bst["launches"], 1)
         achieved = flops / max(bst["launches"], 1) / (kms / 1e3) / 1e12

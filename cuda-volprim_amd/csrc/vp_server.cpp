@@ -423,7 +423,9 @@ vp_status vp_server_create(vp_context* ctx, const vp_server_config* cfg, vp_serv
     vp_launch_config c{};
     if (ctx->have_tuned) c = ctx->tuned;
     if (c.pipeline != VP_PIPELINE_MEGAKERNEL) c = vp_launch_config{0, 0, c.active_cap, 0, 0, VP_PIPELINE_MEGAKERNEL, 0};
-    const uint32_t cap = c.active_cap ? c.active_cap : 128u;
+    uint32_t cap = c.active_cap ? c.active_cap : 128u;
+    if (S->cfg.active_cap > cap) cap = S->cfg.active_cap;
+    c.active_cap = cap;
     S->L = vp_resolve_launch(ctx, &c, cap, 0, 0);
     const size_t slab_words = vp::kMegaSlabBuffers * static_cast<size_t>(cap) * S->L.grid * S->L.block;
     bool ok = hipMalloc(&S->jobs_dev, sizeof(vp::JobDesc) * kServerTable) == hipSuccess &&

@@ -194,6 +194,8 @@ vp_status vp_autotune(vp_context* ctx, vp_job* jobs, uint32_t njobs, vp_launch_c
 typedef struct {
     uint64_t mem_budget_bytes; /* admission limit: accumulator bytes of the requests in flight (0 = 8 GiB) */
     uint32_t max_inflight;     /* requests on the GPU at once (0 = 64) */
+    uint32_t active_cap;       /* largest active-set capacity a request may ask for (0 = the context's
+                                  autotuned capacity, else 128); sizes the server's slabs */
 } vp_server_config;
 typedef struct {
     uint64_t completed;
@@ -203,7 +205,8 @@ typedef struct {
     uint64_t launches;              /* render launches started */
 } vp_server_stats;
 /* The launch shape is the context's autotuned megakernel one (else the
- * defaults); its active-set capacity is the largest a request may ask for.
+ * defaults); the larger of cfg->active_cap and the tuned capacity is the
+ * largest a request may ask for.
  * A request above 1024x1024 pixels or above the memory budget is rejected
  * at submit. */
 vp_status vp_server_create(vp_context* ctx, const vp_server_config* cfg, vp_server** out);

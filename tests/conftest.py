@@ -3,6 +3,10 @@ import os
 import sys
 
 import pytest
+# torch first: its bundled HIP runtime then serves libvolprim.so too (both
+# carry the soname libamdhip64.so.7).  Loading libvolprim.so first would map
+# the system runtime, and torch's own copy, loaded beside it, finds no GPU.
+import torch  # noqa: F401
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG_DIR = os.path.join(ROOT, "cuda-volprim_amd")

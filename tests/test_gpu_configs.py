@@ -225,7 +225,7 @@ def test_c4_served_poisson_stream_bitexact(ctx, sky, resident):
     reqs = _requests(jobs, hs, envh)
     arrivals = np.cumsum(np.random.default_rng(5).exponential(1.0 / 400.0, size=len(reqs)))
     print(f"[case] C4 served stream: {len(reqs)} requests, lambda 400/s", flush=True)
-    srv = ctx.server(mem_budget_bytes=4 << 30, max_inflight=64)
+    srv = ctx.server(mem_budget_bytes=4 << 30, max_inflight=64, active_cap=DENSE_CAP)
     tickets = [None] * len(reqs)
     t0 = time.perf_counter()
 
