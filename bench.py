@@ -310,16 +310,23 @@ def run_stream(args):
     # Algorithmic FLOPs from the event-counting build run on the same jobs
     # (counts are per path, so they do not depend on what shares a launch),
     # over the launch's kernel time from HIP events on the library stream.
+    # Each request is counted on its first ceil(spp / count_div) samples of
+    # every pixel and scaled by the spp ratio (samples are independent and
+    # identically distributed per pixel).
     roofline, events = None, None
     if not args.no_count:
-        events = {k: 0 for k in volprim.EVENT_NAMES}
+        events = {k: 0.0 for k in volprim.EVENT_NAMES}
         t0 = time.perf_counter()
-        for i, r in enumerate(reqs):
-            ev = ctx.count_events(*r)
+        for i, (sc, ev_h, cam, p) in enumerate(reqs):
+            spp_c = max(1, -(-p.spp // args.count_div))
+            pc = type(p).from_buffer_copy(p)
+            pc.spp = spp_c
+            ev = ctx.count_events(sc, ev_h, cam, pc)
             for k in events:
-                events[k] += ev[k]
+                events[k] += ev[k] * (p.spp / spp_c)
             print(f"[stream] event counts: request {i + 1}/{K} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
                   flush=True)
+        events = {k: int(round(v)) for k, v in events.items()}
         flops = sum(FLOP_WEIGHTS[k] * events[k] for k in FLOP_WEIGHTS)
         kms = bst["kernel_ms"] / max(bst["launches"], 1)
         achieved = flops / max(bst["launches"], 1) / (kms / 1e3) / 1e12
@@ -329,7 +336,9 @@ def run_stream(args):
                                     "profiles/ holds rocprofv3 FETCH_SIZE/WRITE_SIZE runs per workload",
                     "peak_nonpacked": PEAK_FP32_NONPACKED_TFLOPS, "kernel": "k_render (batched launch of the stream)",
                     "avg_launch_ms": round(kms, 3), "flops_per_launch": flops / max(bst["launches"], 1),
-                    "weights_note": WEIGHTS_NOTE}
+                    "weights_note": WEIGHTS_NOTE,
+                    "counts_note": f"event counts of every request on its first ceil(spp/{args.count_div}) samples "
+                                   f"per pixel, scaled by the spp ratio"}
 
     quality = None
     if not args.no_quality:
@@ -439,6 +448,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--count-div", type=int, default=8,
+                    help="stream: event counts on ceil(spp / this) samples per pixel, scaled up")
     ap.add_argument("--no-quality", action="store_true", help="skip the image-error estimate")
     ap.add_argument("--spp-div", type=int, default=8, help="stream: every request's spp is divided by this")
     ap.add_argument("--res-div", type=int, default=1, help="stream: every request's resolution is divided by this")

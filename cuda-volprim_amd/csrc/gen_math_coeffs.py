@@ -12,6 +12,10 @@ digits), rounded to float32 and printed as C hex-float literals:
   (m, s) of each interval are printed beside the table.
 * ERFINV_A: erfinv(x)/x as a polynomial in w - 2.5, w = -log(1-x^2) < 5    (degree 9)
 * ERFINV_B: erfinv(x)/x as a polynomial in sqrt(w) - 3, w >= 5            (degree 9)
+* CHORD_S : S(d) = sum_n d^n / (2n+1)!! = sqrt(pi/2) e^(d/2) erf(sqrt(d/2)) / sqrt(d)
+            on [0, 9] (degree 10); the optical depth of a whole chord through a
+            3-sigma-truncated Gaussian is 2 e^(-9/2) M dnf sqrt(d)/|w| S(d), with d
+            the chord's squared half-length in the primitive's local frame
 
 The erfinv variable substitution is the one of M. Giles, "Approximating the
 erfinv function" (GPU Computing Gems, 2011); the coefficients are refitted
@@ -66,6 +70,10 @@ def main():
     ws = np.linspace(5.0, 16.7, 5000)
     ys = np.array([float(mp.erfinv(mp.sqrt(1 - mp.exp(-mp.mpf(w)))) / mp.sqrt(1 - mp.exp(-mp.mpf(w)))) for w in ws])
     emit("ERFINV_B", *fit_rel(np.sqrt(ws) - 3.0, ys, 9))
+    xs = cheb(0.0, 9.0, 4000)
+    ys = np.array([1.0 if x == 0 else float(mp.sqrt(mp.pi / 2) * mp.exp(mp.mpf(x) / 2) * mp.erf(mp.sqrt(mp.mpf(x) / 2))
+                                            / mp.sqrt(mp.mpf(x))) for x in xs])
+    emit("CHORD_S", *fit_rel(xs, ys, 10, iters=80))
 
 
 if __name__ == "__main__":

@@ -199,39 +199,69 @@ VP_HD float exit_from_inside(const PrimRecord& pr, f3 o, f3 d) {
     return (-b + sqrtf_(disc)) * rcpf(a);
 }
 
-// Exit from a surface entry point (intersection.h:142-173).
-VP_HD float exit_from_entry(const PrimRecord& pr, f3 o, f3 d, float t_entry, float w_len2) {
-    const f3 pe = o + t_entry * d;
-    const f3 p = to_local_pos(pr, pe);
-    const f3 w = to_local_dir(pr, d);
-    const float pdw = dot(p, w);
-    const float exit_dist = -2.0f * pdw * rcpf(w_len2);
-    return t_entry + exit_dist;
-}
+// 2 e^(-9/2): the chord depth's constant factor (chord_poly_ in vp_math.h).
+constexpr float kChordScale = 0x1.6c0504p-6f;
 
-// Entry of the ray into the 3 sigma ellipsoid over [0, kTMax): the contract
-// of the reference's built-in-sphere traversal (front face only, no hit when
-// the origin is inside or on the bound, tangents excluded).  The quadratic
-// uses the stable form of intersection.h:66-101 with radius^2 = 9.  Returns
-// a negative value when there is no entry; *w_len2 receives |w|^2.
-VP_HD float entry_t(const PrimRecord& pr, f3 o, f3 d, float* w_len2) {
+// A ray's crossing of a primitive's 3 sigma bound from outside, over
+// [0, kTMax): the contract of the reference's built-in-sphere traversal
+// (front face only, no entry when the origin is inside or on the bound,
+// tangents excluded; src/thesis/host/app/renderer.cpp:142-157,
+// device/entry/anyhit.cuh:68-86, intersection.h:47-55), with the entry, the
+// exit and the optical depth of the whole chord from one transform of the
+// ray into the primitive's frame.  In that frame the ray is p + t w, and the
+// quadratic |p + t w|^2 = 9 is solved in the stable form of
+// intersection.h:66-101: b = -p.w, q0 = p + (b/a) w the point closest to the
+// centre, delta = 9 - |q0|^2, q = b + sqrt(a delta); the entry is c/q and the
+// exit the other root q/a (compute_exit_from_entry, intersection.h:142-173,
+// gives the same root from the entry point).  The chord's optical depth
+// (optical_depth over [entry, exit], primitive.h:249-318) is, with the
+// Gaussian factored about q0,
+//   M dnf sqrt(2 pi)/|w| e^(-|q0|^2/2) erf(sqrt(delta/2))
+//     = 2 e^(-9/2) M dnf (sqrt(a delta)/a) S(delta),
+// S from chord_poly_ (no exp, no erf).  Chords of length <= kSegMin have
+// depth 0, as in optical_depth.  t0 < 0 when the ray does not enter.
+struct Crossing {
+    float t0, t1, tau;
+};
+VP_HD Crossing crossing(const PrimRecord& pr, f3 o, f3 d) {
+    Crossing r{-1.0f, -1.0f, 0.0f};
     const f3 w = to_local_dir(pr, d);
     const f3 p = to_local_pos(pr, o);
     const float a = length2(w);
-    *w_len2 = a;
-    if (a < kDirMin2) return -1.0f;
+    if (a < kDirMin2) return r;
     const float c = length2(p) - kExtent2;
-    if (!(c > 0.0f)) return -1.0f;
+    if (!(c > 0.0f)) return r;
     const float b = -dot(p, w);
-    if (!(b > 0.0f)) return -1.0f;  // moving away: both roots behind the origin
+    if (!(b > 0.0f)) return r;  // moving away: both roots behind the origin
     const float a_inv = rcpf(a);
     const f3 q0 = fma3(b * a_inv, w, p);
     const float delta = kExtent2 - length2(q0);
-    if (delta < kDiscEps) return -1.0f;
-    const float q = b + sqrtf_(a * delta);
+    if (delta < kDiscEps) return r;
+    const float sq = sqrtf_(a * delta);
+    const float q = b + sq;
     const float t = c * rcpf(q);
-    if (!(t >= 0.0f && t < kTMax)) return -1.0f;
-    return t;
+    if (!(t >= 0.0f && t < kTMax)) return r;
+    r.t0 = t;
+    r.t1 = q * a_inv;
+    if (r.t1 - t > kSegMin && r.t1 < kTMax)
+        r.tau = ((kChordScale * (pr.mass * pr.dnf)) * (sq * a_inv)) * chord_poly_(delta);
+    return r;
+}
+
+// Entry distance alone (negative when the ray does not enter).
+VP_HD float entry_t(const PrimRecord& pr, f3 o, f3 d) { return crossing(pr, o, d).t0; }
+
+// Free flight of a ray that entered a primitive (argmin over the entered
+// primitives, sampling.cuh:372-412): with tau the path's optical-depth sample
+// for the primitive, the flight ends inside its chord exactly when tau is at
+// most the chord's depth; the distance is then the inverse CDF from the entry
+// (inv_cdf_span, primitive.h:203-246), kept inside [entry, exit] as the
+// reference keeps it (a result below the entry is clamped up to it; the test
+// against the exit is the depth comparison).  kTMax when the flight passes
+// through the chord.
+VP_HD float entered_flight(const PrimRecord& pr, f3 o, f3 d, const Crossing& x, float tau) {
+    if (!(tau <= x.tau)) return kTMax;
+    return clampf(inv_cdf_span(pr, o, d, x.t0, tau), x.t0, x.t1);
 }
 
 // Point inside the 3 sigma bound (intersection.h:56-61).

@@ -455,16 +455,12 @@ __device__ float transmittance(const SceneView& s, f3 o, f3 d, const uint32_t* a
     walk_ray_q(s, o, inv, tmax, visits, lq, [&](uint32_t prim) -> bool {
         const PrimRecord pr = load_prim(s.prims, prim);
         ++leaves;
-        float wl2;
-        const float tj = entry_t(pr, o, d, &wl2);
-        if (tj < 0.0f) return true;
-        const float te = exit_from_entry(pr, o, d, tj, wl2);
+        const Crossing x = crossing(pr, o, d);
+        if (x.t0 < 0.0f) return true;
         ++sh;
-        if (te > tj && te < kTMax) {
-            if (tau_add(acc, optical_depth(pr, o, d, tj, te))) {
-                full = true;
-                return false;
-            }
+        if (tau_add(acc, x.tau)) {
+            full = true;
+            return false;
         }
         return true;
     });
@@ -664,41 +660,14 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
             if (__builtin_amdgcn_ballot_w64(!fits) != 0) break;
         }
         wave_sync();
-        // leaf phase, shared by all lanes: entry tests; the entered leaves
-        // move down to the front of the queue (an entry is only overwritten
-        // after it has been read); then their chords
-        uint32_t nhit = 0;
-        for (uint32_t base = 0; base < count; base += nact) {
-            const uint32_t g = base + rank;
-            bool entered = false;
-            uint32_t own = 0, prim = 0;
-            float tj = -1.0f;
-            if (g < count) {
-                own = wq.owner[g];
-                prim = wq.prim[g];
-                const PrimRecord pr = load_prim(wq.prims[own >> 1], prim);
-                float wl2;
-                tj = entry_t(pr, lds_load3(wq.ro + 3 * (own >> 1)), lds_load3(wq.rd + 3 * own), &wl2);
-                entered = tj >= 0.0f;
-            }
-            const unsigned long long hm = __builtin_amdgcn_ballot_w64(entered);
-            wave_sync();
-            if (entered) {
-                const uint32_t h = nhit + lanes_below(hm);
-                wq.prim[h] = prim;
-                wq.owner[h] = static_cast<uint8_t>(own);
-                wq.tj[h] = tj;
-            }
-            nhit += static_cast<uint32_t>(__popcll(hm));
-        }
-        wave_sync();
-        for (uint32_t g = rank; g < nhit; g += nact) {
+        // leaf phase, shared by all lanes: each queued leaf's crossing (entry
+        // test and whole-chord depth from one transform) goes into its ray's
+        // fixed-point sum
+        for (uint32_t g = rank; g < count; g += nact) {
             const uint32_t own = wq.owner[g];
-            const f3 ro = lds_load3(wq.ro + 3 * (own >> 1)), rd = lds_load3(wq.rd + 3 * own);
             const PrimRecord pr = load_prim(wq.prims[own >> 1], wq.prim[g]);
-            const float tj = wq.tj[g];
-            const float te = exit_from_entry(pr, ro, rd, tj, length2(to_local_dir(pr, rd)));
-            if (te > tj && te < kTMax) atomicAdd(&wq.acc[own], tau_fixed(optical_depth(pr, ro, rd, tj, te)));
+            const Crossing x = crossing(pr, lds_load3(wq.ro + 3 * (own >> 1)), lds_load3(wq.rd + 3 * own));
+            if (x.tau > 0.0f) atomicAdd(&wq.acc[own], tau_fixed(x.tau));
         }
         wave_sync();
         // lanes whose ray is done (walked to the end, or saturated) take the

@@ -153,27 +153,44 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
         wave_sync();
         const unsigned long long am = __builtin_amdgcn_ballot_w64(true);
         const uint32_t nact = static_cast<uint32_t>(__popcll(am));
-        // leaf phase in two passes, each at full width: entry tests, keeping
-        // the entries that can matter (entered, and before the running
-        // minimum or on a direct ray) at the front of the queue; then their
-        // exits, direct-term chords, candidates and free flights.  The second
-        // pass reads the running minimum again; it can only have dropped,
-        // and a candidate it then skips lies beyond the final minimum, which
-        // the rebuild would not keep.
+        // leaf phase in two passes, each at full width.  First every queued
+        // leaf's crossing (entry, exit and whole-chord depth from one
+        // transform): on a direct ray its depth joins the ray's sum; if it is
+        // entered before the running minimum it is a rebuild candidate and
+        // its free flight is sampled, which ends inside the chord when the
+        // sampled depth is at most the chord's.  Those leaves move down to
+        // the front of the queue (an entry is only overwritten after it has
+        // been read) with their sampled depth, and the second pass inverts
+        // their flights into distances.
         const uint32_t rk = lanes_below(am);
         uint32_t nk = 0;
         for (uint32_t base = 0; base < count; base += nact) {
             const uint32_t g = base + rk;
             bool keep = false;
             uint32_t own = 0, prim = 0;
-            float tj = -1.0f;
+            float tau = 0.0f;
             if (g < count) {
                 own = wq.owner[g];
                 prim = wq.prim[g];
                 const PrimRecord pr = load_prim(wq.prims[own], prim);
-                float wl2;
-                tj = entry_t(pr, lds_load3(wq.ro + 3 * own), lds_load3(wq.rd + 3 * own), &wl2);
-                keep = tj >= 0.0f && (tj <= __uint_as_float(wq.ts[own]) || (wq.meta[own] >> 31) != 0u);
+                const Crossing x = crossing(pr, lds_load3(wq.ro + 3 * own), lds_load3(wq.rd + 3 * own));
+                if (x.t0 >= 0.0f) {
+                    const uint32_t meta = wq.meta[own];
+                    if ((meta >> 31) != 0u && x.tau > 0.0f) atomicAdd(&wq.acc[own], tau_fixed(x.tau));
+                    // entered before the running minimum: a rebuild candidate.
+                    // Beyond it nothing can change (t >= entry > minimum).
+                    if (x.t0 <= __uint_as_float(wq.ts[own])) {
+                        const uint32_t pos = atomicAdd(&wq.nc[own], 1u);
+                        if (pos < cap) {
+                            uint32_t* mine = slab + static_cast<size_t>(gbase + own) * kMegaSlabBuffers * cap;
+                            mine[2 * cap + pos] = prim;
+                            mine[3 * cap + pos] = __float_as_uint(x.t0);
+                            mine[4 * cap + pos] = __float_as_uint(x.t1);
+                        }
+                        tau = free_flight_tau(prim_uniform(wq.key[own], meta & 0x7fffffffu, 1u, prim));
+                        keep = tau <= x.tau;
+                    }
+                }
             }
             const unsigned long long km = __builtin_amdgcn_ballot_w64(keep);
             wave_sync();
@@ -181,40 +198,17 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
                 const uint32_t h = nk + lanes_below(km);
                 wq.prim[h] = prim;
                 wq.owner[h] = static_cast<uint8_t>(own);
-                wq.tj[h] = tj;
+                wq.tj[h] = tau;
             }
             nk += static_cast<uint32_t>(__popcll(km));
         }
         wave_sync();
         for (uint32_t g = rk; g < nk; g += nact) {
             const uint32_t own = wq.owner[g];
-            const uint32_t prim = wq.prim[g];
             const f3 ro = lds_load3(wq.ro + 3 * own), rd = lds_load3(wq.rd + 3 * own);
-            const PrimRecord pr = load_prim(wq.prims[own], prim);
-            const float tj = wq.tj[g];
-            const float wl2 = length2(to_local_dir(pr, rd));
-            const uint32_t meta = wq.meta[own];
-            const float ts_cur = __uint_as_float(wq.ts[own]);
-            // entered before the running minimum: a rebuild candidate.
-            // Beyond it nothing below can change (t >= tj > ts), except the
-            // direct term's optical depth.
-            const bool cand = tj <= ts_cur;
-            const bool dir = (meta >> 31) != 0u;
-            if (!cand && !dir) continue;
-            const float te = exit_from_entry(pr, ro, rd, tj, wl2);
-            if (dir && te > tj && te < kTMax) atomicAdd(&wq.acc[own], tau_fixed(optical_depth(pr, ro, rd, tj, te)));
-            if (!cand) continue;
-            const uint32_t pos = atomicAdd(&wq.nc[own], 1u);
-            if (pos < cap) {
-                uint32_t* mine = slab + static_cast<size_t>(gbase + own) * kMegaSlabBuffers * cap;
-                mine[2 * cap + pos] = prim;
-                mine[3 * cap + pos] = __float_as_uint(tj);
-                mine[4 * cap + pos] = __float_as_uint(te);
-            }
-            const float tau = free_flight_tau(prim_uniform(wq.key[own], meta & 0x7fffffffu, 1u, prim));
-            float t = inv_cdf_span(pr, ro, rd, tj, tau);
-            if (t >= 0.0f && t < tj) t = tj;
-            if (t >= 0.0f && t < ts_cur && t <= te) atomicMin(&wq.ts[own], __float_as_uint(t));
+            const PrimRecord pr = load_prim(wq.prims[own], wq.prim[g]);
+            const float t = entered_flight(pr, ro, rd, crossing(pr, ro, rd), wq.tj[g]);
+            atomicMin(&wq.ts[own], t == 0.0f ? 0u : __float_as_uint(t));
         }
         wave_sync();
         if (!direct) tmax = __uint_as_float(wq.ts[lane]);
@@ -388,32 +382,24 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
             walk_ray_q(sc, o, inv, tmax, visits, lq, [&](uint32_t prim) -> bool {
                 const PrimRecord pr = load_prim(sc.prims, prim);
                 ++leaves;
-                float wl2;
-                const float tj = entry_t(pr, o, d, &wl2);
-                if (tj < 0.0f) return true;
+                const Crossing x = crossing(pr, o, d);
+                if (x.t0 < 0.0f) return true;
                 ++hits;
-                if (direct) {
-                    const float te = exit_from_entry(pr, o, d, tj, wl2);
-                    if (!dfull && te > tj && te < kTMax) dfull = tau_add(dacc, optical_depth(pr, o, d, tj, te));
-                    // beyond the running minimum nothing below can change
-                    // (t >= tj >= ts)
-                    if (tj > ts) return true;
+                if (direct && !dfull) dfull = tau_add(dacc, x.tau);
+                // beyond the running minimum nothing below can change
+                // (t >= entry > ts)
+                if (x.t0 > ts) return true;
+                if (nc < cap) {
+                    cnd[nc * stride] = prim;
+                    my[(3 * cap + nc) * stride] = __float_as_uint(x.t0);
+                    my[(4 * cap + nc) * stride] = __float_as_uint(x.t1);
                 }
-                if (tj <= ts) {
-                    if (nc < cap) {
-                        cnd[nc * stride] = prim;
-                        my[(3 * cap + nc) * stride] = __float_as_uint(tj);
-                        my[(4 * cap + nc) * stride] = __float_as_uint(exit_from_entry(pr, o, d, tj, wl2));
-                    }
-                    ++nc;
-                }
+                ++nc;
                 const float tau = free_flight_tau(prim_uniform(st.key, b, 1u, prim));
-                float t = inv_cdf_span(pr, o, d, tj, tau);
-                if (t >= 0.0f && t < tj) t = tj;
-                if (t >= 0.0f && t < ts) {
+                if (tau <= x.tau) {
                     ++accept;
-                    const float te = exit_from_entry(pr, o, d, tj, wl2);
-                    if (t <= te) {
+                    const float t = entered_flight(pr, o, d, x, tau);
+                    if (t < ts) {
                         ts = t;
                         if (prune) tmax = ts;
                     }
@@ -474,10 +460,9 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                 if (!keep) {
                     ++rb_h;
                     if (!stored) {
-                        const PrimRecord pr = load_prim(sc.prims, c);
-                        float wl2;
-                        tj = entry_t(pr, o, d, &wl2);
-                        te = tj >= 0.0f ? exit_from_entry(pr, o, d, tj, wl2) : -1.0f;
+                        const Crossing cx = crossing(load_prim(sc.prims, c), o, d);
+                        tj = cx.t0;
+                        te = cx.t1;
                     }
                     keep = tj >= 0.0f && tj <= ts && ts <= te;
                 }
@@ -692,16 +677,11 @@ __global__ void k_transmittance_rays(SceneView sc, const float* __restrict__ ray
         const f3 inv = safe_inv(d);
         const float tmax = kTMax;
         walk_ray(sc, o, inv, tmax, visits, [&](uint32_t prim) -> bool {
-            const PrimRecord pr = load_prim(sc.prims, prim);
-            float wl2;
-            const float tj = entry_t(pr, o, d, &wl2);
-            if (tj < 0.0f) return true;
-            const float te = exit_from_entry(pr, o, d, tj, wl2);
-            if (te > tj && te < kTMax) {
-                if (tau_add(acc, optical_depth(pr, o, d, tj, te))) {
-                    full = true;
-                    return false;
-                }
+            const Crossing x = crossing(load_prim(sc.prims, prim), o, d);
+            if (x.t0 < 0.0f) return true;
+            if (tau_add(acc, x.tau)) {
+                full = true;
+                return false;
             }
             return true;
         });
@@ -722,8 +702,7 @@ __global__ void k_overlaps(SceneView sc, const float* __restrict__ rays, uint32_
     const f3 inv = safe_inv(d);
     const float tmax = kTMax;
     walk_ray(sc, o, inv, tmax, visits, [&](uint32_t prim) -> bool {
-        float wl2;
-        const float tj = entry_t(load_prim(sc.prims, prim), o, d, &wl2);
+        const float tj = entry_t(load_prim(sc.prims, prim), o, d);
         if (tj < 0.0f) return true;
         if (c < cap) {
             uint32_t k = c;
@@ -760,6 +739,7 @@ __global__ void k_math(int fn, const float* __restrict__ x, const float* __restr
         case 9: r = erf_diff_(a, b); break;
         case 10: r = sqrtf_(a); break;
         case 11: r = a / b; break;
+        case 12: r = chord_poly_(a); break;
         default: r = __builtin_nanf("");
     }
     out[i] = r;
@@ -772,19 +752,21 @@ __global__ void k_prim(int fn, const PrimRecord* __restrict__ rec, const float* 
     const PrimRecord p = *rec;
     const float* a = args + 9 * i;
     const f3 o = f3{a[0], a[1], a[2]}, d = f3{a[3], a[4], a[5]};
-    float wl2, r;
+    float r;
     switch (fn) {
         case 0: r = inv_cdf(p, o, d, a[8]); break;
         case 1: r = inv_cdf_span(p, o, d, a[6], a[8]); break;
         case 2: r = optical_depth(p, o, d, a[6], a[7]); break;
         case 3: r = exit_from_inside(p, o, d); break;
-        case 4: r = entry_t(p, o, d, &wl2); break;
+        case 4: r = crossing(p, o, d).t0; break;
         case 5: {
-            const float t = entry_t(p, o, d, &wl2);
-            r = t >= 0.0f ? exit_from_entry(p, o, d, t, wl2) : -1.0f;
+            const Crossing x = crossing(p, o, d);
+            r = x.t0 >= 0.0f ? x.t1 : -1.0f;
         } break;
         case 6: r = point_inside(p, o) ? 1.0f : 0.0f; break;
         case 7: r = prim_pdf(p, o); break;
+        case 8: r = crossing(p, o, d).tau; break;
+        case 9: r = entered_flight(p, o, d, crossing(p, o, d), a[8]); break;
         default: r = __builtin_nanf("");
     }
     out[i] = r;

@@ -22,9 +22,10 @@
 // accumulator (64 bytes per pixel), whatever the sample count; admission is
 // by those bytes in flight and by a limit on requests in flight.
 //
-// Autotuning: vp_autotune times a reduced copy of a job mix for a grid of
-// launch shapes (megakernel: workgroup size, persistent workgroups per CU,
-// regeneration threshold; wavefront pipeline: path-pool size) and sizes the active-set capacity from the measured maximum
+// Autotuning: vp_autotune times a reduced copy of a job mix (about four
+// GPU-fills of paths) for a grid of launch shapes (megakernel: workgroup
+// size, regeneration threshold; wavefront pipeline: path-pool size) and
+// sizes the active-set capacity from the measured maximum
 // point overlap (ceil(1.125 v) rounded up to 16, the rule of the reference's
 // --measure-caps, src/thesis/host/app/renderer.cpp:405-418, and at least the
 // largest capacity the jobs request); the best shape
@@ -552,22 +553,34 @@ vp_status vp_autotune(vp_context* ctx, vp_job* jobs, uint32_t njobs, vp_launch_c
     if (!ctx || !jobs || njobs == 0) return fail(VP_ERR_INVALID, "no jobs");
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(VP_ERR_HIP, "hipSetDevice failed");
     hipStream_t s = ctx->stream;
-    // reduced copy of the mix: every view downscaled by an integer factor k
-    // to at most kTuneRes pixels a side (same field of view, k-times wider
-    // pixels) at one sample per pixel and depth at most 16
-    constexpr uint32_t kTuneRes = 64;
+    // Reduced copy of the mix, sized to keep the GPU full several times
+    // over (about four waves of the megakernel's lanes): every job's samples
+    // are scaled by the same factor f, half through fewer samples per pixel
+    // and half through a view downscaled by an integer factor k (same field
+    // of view, k-times wider pixels), at depth at most 16.  A mix much smaller
+    // than the GPU would time launch tails only, where the wavefront pool
+    // (all paths in flight at once) wins, though it loses on full-size work.
+    int occ_max = 1;
+    vp_render_occupancy(256u, 128u, 0, &occ_max);
+    const double target = 4.0 * static_cast<double>(ctx->prop.multiProcessorCount) * occ_max * 256.0;
+    double total = 0.0;
+    for (uint32_t i = 0; i < njobs; ++i)
+        total += static_cast<double>(jobs[i].params.width) * jobs[i].params.height * jobs[i].params.spp;
+    const double f = std::min(1.0, target / std::max(total, 1.0));
     std::vector<vp_render_params> params(njobs);
     std::vector<vp_camera> cams(njobs);
     for (uint32_t i = 0; i < njobs; ++i) {
         vp_status st = vp_check_params(&jobs[i].params);
         if (st != VP_OK) return st;
         params[i] = jobs[i].params;
-        params[i].spp = 1;
         params[i].measure_caps = 0;
         // long paths only lengthen each timing run's tail
         params[i].max_depth = std::min<uint32_t>(params[i].max_depth, 16u);
-        const uint32_t side = std::max(params[i].width, params[i].height);
-        const uint32_t k = (side + kTuneRes - 1) / kTuneRes;
+        const double want = static_cast<double>(params[i].spp) * f;  // samples per full-size pixel
+        const uint32_t spp = static_cast<uint32_t>(std::max(1.0, std::floor(params[i].spp * std::sqrt(f) + 0.5)));
+        const double area = want / spp;  // fraction of the pixels kept
+        const uint32_t k = std::max<uint32_t>(1u, static_cast<uint32_t>(std::floor(1.0 / std::sqrt(area) + 0.5)));
+        params[i].spp = spp;
         params[i].width = (params[i].width + k - 1) / k;
         params[i].height = (params[i].height + k - 1) / k;
         vp_camera c = jobs[i].camera;
@@ -615,17 +628,13 @@ vp_status vp_autotune(vp_context* ctx, vp_job* jobs, uint32_t njobs, vp_launch_c
         cap = std::min(cap, vp::kMaxActiveCap);  // beyond it sets overflow and are counted
     }
     // candidate shapes: megakernel workgroup size x regeneration threshold
-    // at the occupancy limit, one at half occupancy, and the wavefront
-    // pipeline at two pool sizes
+    // at the occupancy limit, and the wavefront pipeline at two pool sizes
     std::vector<vp_launch_config> cands;
-    for (uint32_t b : {128u, 256u}) {
+    for (uint32_t b : {256u, 128u}) {
         int occ = 1;
         vp_render_occupancy(b, cap, 0, &occ);
-        for (uint32_t rg : {1u, 8u, 16u})
+        for (uint32_t rg : {8u, 16u})
             cands.push_back(vp_launch_config{b, static_cast<uint32_t>(occ), cap, 0, rg, VP_PIPELINE_MEGAKERNEL, 0});
-        if (b == 256u && occ > 1)
-            cands.push_back(
-                vp_launch_config{b, static_cast<uint32_t>(occ) / 2u, cap, 0, 8u, VP_PIPELINE_MEGAKERNEL, 0});
     }
     for (uint32_t pool : {1u << 20, 1u << 21})
         if (static_cast<uint64_t>(pool) * cap * vp::kSlabBuffers < (uint64_t(1) << 32))

@@ -315,21 +315,16 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
             const uint32_t prim = wq.prim[g];
             const f3 ro = lds_load3(wq.ro + 3 * own), rd = lds_load3(wq.rd + 3 * own);
             const PrimRecord pr = load_prim(wq.prims[own], prim);
-            float wl2;
-            const float tj = entry_t(pr, ro, rd, &wl2);
-            if (tj < 0.0f) continue;
-            const float ts_cur = __uint_as_float(wq.ts[own]);
-            // entered before the running minimum: a rebuild candidate
-            if (tj <= ts_cur) {
-                const uint32_t pos = atomicAdd(&wq.nc[own], 1u);
-                if (pos < w.cap) w.slab[(static_cast<size_t>(slot[own]) * kSlabBuffers + 2) * w.cap + pos] = prim;
-            }
+            const Crossing x = crossing(pr, ro, rd);
+            // entered before the running minimum: a rebuild candidate.
+            // Beyond it nothing can change (t >= entry > minimum).
+            if (x.t0 < 0.0f || x.t0 > __uint_as_float(wq.ts[own])) continue;
+            const uint32_t pos = atomicAdd(&wq.nc[own], 1u);
+            if (pos < w.cap) w.slab[(static_cast<size_t>(slot[own]) * kSlabBuffers + 2) * w.cap + pos] = prim;
             const float tau = free_flight_tau(prim_uniform(wq.key[own], wq.meta[own], 1u, prim));
-            float t = inv_cdf_span(pr, ro, rd, tj, tau);
-            if (t >= 0.0f && t < tj) t = tj;
-            if (t >= 0.0f && t < ts_cur) {
-                const float te = exit_from_entry(pr, ro, rd, tj, wl2);
-                if (t <= te) atomicMin(&wq.ts[own], __float_as_uint(t));
+            if (tau <= x.tau) {
+                const float t = entered_flight(pr, ro, rd, x, tau);
+                atomicMin(&wq.ts[own], t == 0.0f ? 0u : __float_as_uint(t));
             }
         }
         wave_sync();
@@ -428,9 +423,8 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
                         ++ia;
                     }
                     if (!keep) {
-                        float wl2;
-                        const float tj = entry_t(pr, o, d, &wl2);
-                        keep = tj >= 0.0f && tj <= ts && ts <= exit_from_entry(pr, o, d, tj, wl2);
+                        const Crossing cx = crossing(pr, o, d);
+                        keep = cx.t0 >= 0.0f && cx.t0 <= ts && ts <= cx.t1;
                     }
                     if (keep) push(c);
                 };
@@ -620,42 +614,16 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
             if (__builtin_amdgcn_ballot_w64(!fits) != 0) break;
         }
         wave_sync();
-        // leaf phase, shared by all lanes: entry tests, then the chords of
-        // the entered leaves (fixed-point sums, order free)
+        // leaf phase, shared by all lanes: each queued leaf's crossing (entry
+        // test and whole-chord depth from one transform) goes into its ray's
+        // fixed-point sum (order free)
         const unsigned long long am = __builtin_amdgcn_ballot_w64(true);
         const uint32_t nact = static_cast<uint32_t>(__popcll(am));
-        uint32_t nhit = 0;
-        for (uint32_t base = 0; base < count; base += nact) {
-            const uint32_t g = base + lanes_below(am);
-            bool entered = false;
-            uint32_t own = 0, prim = 0;
-            float tj = -1.0f;
-            if (g < count) {
-                own = wq.owner[g];
-                prim = wq.prim[g];
-                const PrimRecord pr = load_prim(wq.prims[own], prim);
-                float wl2;
-                tj = entry_t(pr, lds_load3(wq.ro + 3 * own), lds_load3(wq.rd + 3 * own), &wl2);
-                entered = tj >= 0.0f;
-            }
-            const unsigned long long hm = __builtin_amdgcn_ballot_w64(entered);
-            wave_sync();
-            if (entered) {
-                const uint32_t h = nhit + lanes_below(hm);
-                wq.prim[h] = prim;
-                wq.owner[h] = static_cast<uint8_t>(own);
-                wq.tj[h] = tj;
-            }
-            nhit += static_cast<uint32_t>(__popcll(hm));
-        }
-        wave_sync();
-        for (uint32_t g = lanes_below(am); g < nhit; g += nact) {
+        for (uint32_t g = lanes_below(am); g < count; g += nact) {
             const uint32_t own = wq.owner[g];
-            const f3 ro = lds_load3(wq.ro + 3 * own), rd = lds_load3(wq.rd + 3 * own);
             const PrimRecord pr = load_prim(wq.prims[own], wq.prim[g]);
-            const float tj = wq.tj[g];
-            const float te = exit_from_entry(pr, ro, rd, tj, length2(to_local_dir(pr, rd)));
-            if (te > tj && te < kTMax) atomicAdd(&wq.acc[own], tau_fixed(optical_depth(pr, ro, rd, tj, te)));
+            const Crossing x = crossing(pr, lds_load3(wq.ro + 3 * own), lds_load3(wq.rd + 3 * own));
+            if (x.tau > 0.0f) atomicAdd(&wq.acc[own], tau_fixed(x.tau));
         }
         wave_sync();
         if (have) {

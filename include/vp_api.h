@@ -236,10 +236,12 @@ vp_status vp_collect_overlaps(vp_context* ctx, vp_scene* scene, const float* ray
 vp_status vp_render_count_events(vp_context* ctx, vp_scene* scene, vp_env* env, const vp_camera* cam,
                                  const vp_render_params* params, uint64_t* events, uint32_t n_events);
 /* Evaluate math function fn (0 exp 1 log 2 sin 3 cos 4 atan2 5 acos 6 erf
- * 7 erfc 8 erfinv 9 erf_diff 10 sqrt 11 div) on the device. */
+ * 7 erfc 8 erfinv 9 erf_diff 10 sqrt 11 div 12 chord_poly) on the device. */
 vp_status vp_math_eval(vp_context* ctx, int fn, const float* x, const float* y, uint32_t n, float* out);
 /* Evaluate primitive function fn (0 inv_cdf 1 inv_cdf_span 2 optical_depth
- * 3 exit_inside 4 entry_t 5 exit_entry 6 point_inside 7 pdf) for one record
+ * 3 exit_inside 4 entry_t 5 exit_entry 6 point_inside 7 pdf 8 chord_tau (the
+ * optical depth of the whole entered chord) 9 entered_flight (free flight of
+ * depth tau into the entered primitive; 1e20 when it passes)) for one record
  * over m argument rows (o3 d3 t0 t1 tau). */
 vp_status vp_prim_eval(vp_context* ctx, int fn, const float* record16, const float* args9, uint32_t m,
                        float* out);

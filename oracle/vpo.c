@@ -173,37 +173,50 @@ static float exit_inside(const prim_t* pr, v3 o, v3 d) {
     return (-b + sqrtf(disc)) * vpo_rcp(a);
 }
 
-/* intersection.h:142-173 */
-static float exit_entry(const prim_t* pr, v3 o, v3 d, float te, float wl2) {
-    v3 pe = vadd(o, vscale(d, te));
-    v3 p = local_pos(pr, pe);
-    v3 w = local_dir(pr, d);
-    float pdw = vdot(p, w);
-    float ed = -2.0f * pdw * vpo_rcp(wl2);
-    return te + ed;
-}
-
-/* Entry into the 3-sigma bound over [0, 1e20): the built-in sphere contract
- * (renderer.cpp:142-157 instance transform T R S(3 s); anyhit.cuh:68-86;
- * intersection.h:47-55 front face only), solved with the stable quadratic of
- * intersection.h:66-101 at radius^2 = 9.  Negative = no entry. */
-static float entry_t(const prim_t* pr, v3 o, v3 d, float* wl2) {
+/* Crossing of the 3-sigma bound from outside over [0, 1e20): the built-in
+ * sphere contract (renderer.cpp:142-157 instance transform T R S(3 s);
+ * anyhit.cuh:68-86; intersection.h:47-55 front face only), solved with the
+ * stable quadratic of intersection.h:66-101 at radius^2 = 9.  Entry c/q; exit
+ * the far root q/a (compute_exit_from_entry, intersection.h:142-173, is the
+ * same root); optical depth of the whole chord (optical_depth over
+ * [entry, exit], primitive.h:249-318) in the closed form
+ *   M dnf sqrt(2 pi)/|w| e^(-|q0|^2/2) erf(sqrt(delta/2))
+ *     = 2 e^(-9/2) M dnf (sqrt(a delta)/a) S(delta)
+ * with S(d) = sum d^n/(2n+1)!! (vpo_chord_poly).  Chords of length <= 1e-6
+ * have depth 0.  t0 < 0: no entry. */
+typedef struct { float t0, t1, tau; } crossing_t;
+#define CHORD_SCALE 0x1.6c0504p-6f /* 2 e^(-9/2) */
+static crossing_t crossing(const prim_t* pr, v3 o, v3 d) {
+    crossing_t r = {-1.0f, -1.0f, 0.0f};
     v3 w = local_dir(pr, d), p = local_pos(pr, o);
     float a = vdot(w, w);
-    *wl2 = a;
-    if (a < 1e-12f) return -1.0f;
+    if (a < 1e-12f) return r;
     float c = vdot(p, p) - EXTENT2;
-    if (!(c > 0.0f)) return -1.0f;
+    if (!(c > 0.0f)) return r;
     float b = -vdot(p, w);
-    if (!(b > 0.0f)) return -1.0f;
+    if (!(b > 0.0f)) return r;
     float ai = vpo_rcp(a);
     v3 q0 = vfma(b * ai, w, p);
     float delta = EXTENT2 - vdot(q0, q0);
-    if (delta < 1e-12f) return -1.0f;
-    float q = b + sqrtf(a * delta);
+    if (delta < 1e-12f) return r;
+    float sq = sqrtf(a * delta);
+    float q = b + sq;
     float t = c * vpo_rcp(q);
-    if (!(t >= 0.0f && t < INF_F)) return -1.0f;
-    return t;
+    if (!(t >= 0.0f && t < INF_F)) return r;
+    r.t0 = t;
+    r.t1 = q * ai;
+    if (r.t1 - t > 1e-6f && r.t1 < INF_F) r.tau = ((CHORD_SCALE * (pr->mass * pr->dnf)) * (sq * ai)) * vpo_chord_poly(delta);
+    return r;
+}
+
+/* Free flight into an entered primitive (sampling.cuh:372-412): it ends in
+ * the chord iff the sampled depth is at most the chord's; the distance is
+ * inv_cdf_span from the entry (primitive.h:203-246) kept in [entry, exit]
+ * (the reference clamps a result below the entry up to it and rejects one
+ * beyond the exit).  INF when the flight passes the chord. */
+static float entered_flight(const prim_t* pr, v3 o, v3 d, crossing_t x, float tau) {
+    if (!(tau <= x.tau)) return INF_F;
+    return vpo_clamp(inv_cdf_span(pr, o, d, x.t0, tau), x.t0, x.t1);
 }
 
 /* intersection.h:56-61 */
@@ -218,16 +231,17 @@ void vpo_prim_eval(int fn, const float* prim16, const float* args, long m, float
     for (long i = 0; i < m; ++i) {
         const float* a = args + 9 * i; /* o(3) d(3) t0 t1 tau */
         v3 o = V(a[0], a[1], a[2]), d = V(a[3], a[4], a[5]);
-        float wl2;
         switch (fn) {
             case 0: out[i] = inv_cdf(p, o, d, a[8]); break;
             case 1: out[i] = inv_cdf_span(p, o, d, a[6], a[8]); break;
             case 2: out[i] = optical_depth(p, o, d, a[6], a[7]); break;
             case 3: out[i] = exit_inside(p, o, d); break;
-            case 4: out[i] = entry_t(p, o, d, &wl2); break;
-            case 5: { float t = entry_t(p, o, d, &wl2); out[i] = t >= 0.0f ? exit_entry(p, o, d, t, wl2) : -1.0f; } break;
+            case 4: out[i] = crossing(p, o, d).t0; break;
+            case 5: { crossing_t x = crossing(p, o, d); out[i] = x.t0 >= 0.0f ? x.t1 : -1.0f; } break;
             case 6: out[i] = (float)point_inside(p, o); break;
             case 7: out[i] = prim_pdf(p, o); break;
+            case 8: out[i] = crossing(p, o, d).tau; break;
+            case 9: out[i] = entered_flight(p, o, d, crossing(p, o, d), a[8]); break;
             default: out[i] = NAN;
         }
     }
@@ -248,6 +262,7 @@ void vpo_math_eval(int fn, const float* x, const float* y, long n, float* out) {
             case 9: out[i] = vpo_erf_diff(x[i], y[i]); break;
             case 10: out[i] = sqrtf(x[i]); break;
             case 11: out[i] = x[i] / y[i]; break;
+            case 12: out[i] = vpo_chord_poly(x[i]); break;
             default: out[i] = NAN;
         }
     }
@@ -530,15 +545,10 @@ static float transmittance(const prim_t* prims, long n, v3 o, v3 d, const uint32
     cbvh_it it;
     if (B) cbvh_ray_begin(&it, B, o, d);
     for (long j = B ? cbvh_ray_next(&it, INF_F) : 0; B ? j >= 0 : j < n; j = B ? cbvh_ray_next(&it, INF_F) : j + 1) {
-        const prim_t* pr = prims + j;
-        float wl2;
-        float t = entry_t(pr, o, d, &wl2);
-        if (t < 0.0f) continue;
-        float te = exit_entry(pr, o, d, t, wl2);
+        crossing_t x = crossing(prims + j, o, d);
+        if (x.t0 < 0.0f) continue;
         if (k) k->n_sh_hit++;
-        if (te > t && te < INF_F) {
-            if (tau_add(&acc, optical_depth(pr, o, d, t, te))) return 0.0f;
-        }
+        if (tau_add(&acc, x.tau)) return 0.0f;
     }
     float tau = (float)acc * (1.0f / 1099511627776.0f);
     return tau >= 88.0f ? 0.0f : vpo_expf(-tau);
@@ -572,8 +582,7 @@ long vpo_overlaps(const float* prims16, long n, const float* rays, long m, long 
         float* tr = tout + r * cap;
         uint32_t* ir = iout + r * cap;
         for (long j = 0; j < n; ++j) {
-            float wl2;
-            float t = entry_t(prims + j, o, d, &wl2);
+            float t = crossing(prims + j, o, d).t0;
             if (t < 0.0f) continue;
             if (c < cap) {
                 /* insertion in (t, index) order */
@@ -950,18 +959,15 @@ static float argmin_scatter(const prim_t* prims, long n, v3 o, v3 d, const uint3
 #define ARG_NEXT(j) (B ? cbvh_ray_next(&it, prune ? ts_min : INF_F) : (j) + 1)
     for (long j = B ? ARG_NEXT(-1) : 0; B ? j >= 0 : j < n; j = ARG_NEXT(j)) {
         const prim_t* pr = prims + j;
-        float wl2;
-        float tj = entry_t(pr, o, d, &wl2);
-        if (tj < 0.0f) continue;
+        crossing_t x = crossing(pr, o, d);
+        if (x.t0 < 0.0f) continue;
         ++hits;
         k->n_arg_hit++;
         float tau = ff_tau(prim_u(key, bounce, 1u, (uint32_t)j));
-        float ts = inv_cdf_span(pr, o, d, tj, tau);
-        if (ts >= 0.0f && ts < tj) ts = tj;
-        if (ts >= 0.0f && ts < ts_min) {
+        if (tau <= x.tau) {
             k->n_arg_accept++;
-            float te = exit_entry(pr, o, d, tj, wl2);
-            if (ts <= te) ts_min = ts;
+            float ts = entered_flight(pr, o, d, x, tau);
+            if (ts < ts_min) ts_min = ts;
         }
     }
 #undef ARG_NEXT
@@ -1096,11 +1102,10 @@ static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, 
                     if (ts_min <= exit_inside(pr, o, d)) keep = 1;
                 }
                 if (!keep) {
-                    float wl2;
-                    float tj = entry_t(pr, o, d, &wl2);
-                    if (tj >= 0.0f && tj <= ts_min) {
+                    crossing_t x = crossing(pr, o, d);
+                    if (x.t0 >= 0.0f && x.t0 <= ts_min) {
                         k->n_rb_hit++;
-                        if (ts_min <= exit_entry(pr, o, d, tj, wl2)) keep = 1;
+                        if (ts_min <= x.t1) keep = 1;
                     }
                 }
                 if (keep) set_push(nxt, &nn, cap, (uint32_t)j, k);

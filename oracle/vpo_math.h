@@ -200,6 +200,19 @@ static inline float vpo_erf_diff(float a, float b) {
     if (b <= 0.0f) return vpo_erfcf(-b) - vpo_erfcf(-a);
     return vpo_erff(b) - vpo_erff(a);
 }
+/* S(d) = sum_n d^n/(2n+1)!! = sqrt(pi/2) e^(d/2) erf(sqrt(d/2))/sqrt(d) on
+ * [0, 9]: degree-10 minimax fit (cuda-volprim_amd/csrc/gen_math_coeffs.py),
+ * the same float32 coefficients and Horner order as the product. */
+static const float VPO_CHORD_S[11] = {
+    0x1.000002p+0f, 0x1.5554b8p-2f, 0x1.111e7ap-4f, 0x1.373554p-7f, 0x1.1c9208p-10f, 0x1.504234p-14f,
+    0x1.b774c6p-17f, -0x1.9bb328p-21f, 0x1.af65b8p-23f, -0x1.9ee29ap-27f, 0x1.452d92p-31f,
+};
+static inline float vpo_chord_poly(float d) {
+    float p = VPO_CHORD_S[10];
+    for (int i = 9; i >= 0; --i) p = fmaf(p, d, VPO_CHORD_S[i]);
+    return p;
+}
+
 static inline float vpo_erfinvf(float x) {
     if (x != x || x > 1.0f || x < -1.0f) return NAN;
     if (x == 1.0f) return VPO_INF;

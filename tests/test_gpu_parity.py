@@ -42,6 +42,7 @@ MATH_CASES = {
     "erf_diff": lambda rng, n: tuple(np.sort(rng.uniform(-6, 6, (2, n)), axis=0)),
     "sqrt": lambda rng, n: (np.exp(rng.uniform(-80, 80, n)), None),
     "div": lambda rng, n: (rng.normal(size=n), rng.normal(size=n)),
+    "chord_poly": lambda rng, n: (rng.uniform(0, 9, n), None),
 }
 SPECIALS = np.array([0.0, -0.0, 1.0, -1.0, 0.5, 1e-30, -1e-30, 1e-40, 88.7, -87.5, -103.9, 3.92, 10.1, 10.2,
                      0.9999999, -0.9999999, np.inf, -np.inf, np.nan], np.float32)

@@ -162,6 +162,97 @@ def test_optical_depth_matches_quadrature():
         assert abs(od[i] - q) <= 2e-5 * q + 1e-7, (i, od[i], q)
 
 
+def test_chord_poly_matches_mpmath():
+    # S(d) = sqrt(pi/2) e^(d/2) erf(sqrt(d/2)) / sqrt(d) = sum d^n / (2n+1)!!
+    # on [0, 9], the whole-chord depth factor: relative error below 5e-7
+    d = np.concatenate([np.linspace(0.0, 9.0, 1001), np.random.default_rng(4).uniform(0.0, 9.0, 1000)])
+    d = d.astype(np.float32)
+    got = vpo.math_eval("chord_poly", d).astype(np.float64)
+
+    def S(x):
+        x = mp.mpf(float(x))
+        return mp.mpf(1) if x == 0 else mp.sqrt(mp.pi / 2) * mp.exp(x / 2) * mp.erf(mp.sqrt(x / 2)) / mp.sqrt(x)
+    ref = np.array([float(S(x)) for x in d])
+    assert np.max(np.abs(got - ref) / ref) < 5e-7
+    # the series' first terms: S(0) = 1, S'(0) = 1/3
+    assert abs(got[0] - 1.0) < 3e-7
+    h = np.float32(1e-2)
+    s1 = vpo.math_eval("chord_poly", np.array([h], np.float32))[0]
+    assert abs((s1 - 1.0) / h - (1.0 / 3.0 + h / 15.0)) < 1e-3
+
+
+def test_chord_depth_matches_quadrature_and_erf_form():
+    # The whole-chord optical depth of an entered primitive (one transform,
+    # S(delta), no exp/erf) against FP64 quadrature of M N(x) between the
+    # FP64 entry and exit, and against the erf-difference optical_depth over
+    # the oracle's own entry and exit.  A dropped factor (2, e^(-9/2), M or
+    # dnf) or delta confused with the closest distance fails both.
+    rng = np.random.default_rng(19)
+    from golden.make_golden import chord, density_along
+    for seed in (31, 32, 33):
+        rows = scenes.random_scene(1, seed, smin=0.1, smax=0.6)
+        rec = vpo.make_prims(rows)[0]
+        c = rows[0, 0:3].astype(np.float64)
+        args = []
+        for _ in range(200):
+            d = rng.normal(size=3)
+            d /= np.linalg.norm(d)
+            o = c - 4.0 * d + rng.normal(size=3) * 0.3
+            args.append([*o, *d, 0.0, 0.0, 0.0])
+        args = np.array(args, np.float32)
+        tau = vpo.prim_eval("chord_tau", rec, args).astype(np.float64)
+        t_en = vpo.prim_eval("entry_t", rec, args)
+        t_ex = vpo.prim_eval("exit_entry", rec, args)
+        args2 = args.copy()
+        args2[:, 6], args2[:, 7] = t_en, t_ex
+        od = vpo.prim_eval("optical_depth", rec, args2).astype(np.float64)
+        n = 0
+        for i in range(args.shape[0]):
+            ch, org_in = chord(rows[0], args[i, :3].astype(np.float64), args[i, 3:6].astype(np.float64))
+            if org_in or ch is None:
+                assert tau[i] == 0.0
+                continue
+            q, _ = integrate.quad(lambda t: density_along(rows[0], args[i, :3].astype(np.float64),
+                                                          args[i, 3:6].astype(np.float64), t),
+                                  ch[0], ch[1], epsrel=1e-12, epsabs=0)
+            assert abs(tau[i] - q) <= 2e-5 * q + 1e-7, (seed, i, tau[i], q)
+            assert abs(tau[i] - od[i]) <= 2e-5 * q + 1e-7, (seed, i, tau[i], od[i])
+            n += 1
+        assert n > 100
+
+
+def test_entered_flight_inverts_the_chord_depth():
+    # entered_flight(tau) lands where the optical depth from the entry equals
+    # tau when tau is at most the chord's depth, and passes (1e20) otherwise
+    rng = np.random.default_rng(23)
+    rows = scenes.random_scene(1, 37, smin=0.2, smax=0.5)
+    rows[:, 13] = 5.0  # mass: chords of depth ~0.5-5
+    rec = vpo.make_prims(rows)[0]
+    c = rows[0, 0:3].astype(np.float64)
+    args = []
+    for _ in range(600):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        o = c - 4.0 * d + rng.normal(size=3) * 0.2
+        args.append([*o, *d, 0.0, 0.0, 0.0])
+    args = np.array(args, np.float32)
+    full = vpo.prim_eval("chord_tau", rec, args)
+    t_en = vpo.prim_eval("entry_t", rec, args)
+    t_ex = vpo.prim_eval("exit_entry", rec, args)
+    hit = t_en >= 0
+    args[:, 8] = (rng.uniform(0.02, 1.3, args.shape[0]) * full).astype(np.float32)
+    t = vpo.prim_eval("entered_flight", rec, args)
+    inside = hit & (args[:, 8] <= full)
+    assert inside.sum() > 300 and (hit & ~inside).sum() > 50
+    assert np.all(t[hit & ~inside] >= 1e20)
+    assert np.all((t[inside] >= t_en[inside]) & (t[inside] <= t_ex[inside]))
+    a2 = args.copy()
+    a2[:, 6], a2[:, 7] = t_en, t
+    od = vpo.prim_eval("optical_depth", rec, a2)
+    sel = inside & (args[:, 8] > 1e-3)
+    assert np.allclose(od[sel], args[sel, 8], rtol=3e-4, atol=3e-6)
+
+
 def test_exits_and_entry_match_fp64_quadratic():
     rng = np.random.default_rng(10)
     rows = scenes.random_scene(1, 41, smin=0.1, smax=0.6)
