@@ -224,7 +224,7 @@ static vp_status vp_run_step_wf(vp_context* ctx, const RenderLaunch& L, uint64_t
                               s));
     VP_HIP_TRY(hipMemcpyAsync(ctx->job_begin_dev, ctx->begin_pinned, sizeof(uint64_t) * max_jobs,
                               hipMemcpyHostToDevice, s));
-    VP_HIP_TRY(hipMemcpyAsync(ctx->launch_desc, ctx->desc_pinned, sizeof(uint64_t) * 2, hipMemcpyHostToDevice, s));
+    VP_HIP_TRY(hipMemcpyAsync(ctx->launch_desc, ctx->desc_pinned, sizeof(uint64_t) * 3, hipMemcpyHostToDevice, s));
     VP_HIP_TRY(vp_launch_fill_u64(ctx->path_counter, 1u, 0ull, s));
     VP_HIP_TRY(vp_launch_fill_u64(ctx->job_done_dev, max_jobs, 0ull, s));
     VP_HIP_TRY(vp_wf_launch_init(ctx->wf, s));
@@ -251,6 +251,7 @@ vp_status vp_run_step(vp_context* ctx, const RenderLaunch& L, uint32_t nslices, 
                       uint32_t max_jobs, uint32_t max_npix, bool use_graph, hipStream_t s) {
     ctx->desc_pinned[0] = nslices;
     ctx->desc_pinned[1] = total_paths;
+    ctx->desc_pinned[2] = 0;  // the launch's paths start at 0
     if (L.pipeline == VP_PIPELINE_WAVEFRONT) return vp_run_step_wf(ctx, L, total_paths, max_jobs, max_npix, s);
     auto enqueue = [&]() -> hipError_t {
         hipError_t e = hipMemcpyAsync(ctx->jobs_dev, ctx->jobs_pinned, sizeof(vp::JobDesc) * max_jobs,
@@ -259,7 +260,7 @@ vp_status vp_run_step(vp_context* ctx, const RenderLaunch& L, uint32_t nslices, 
             e = hipMemcpyAsync(ctx->job_begin_dev, ctx->begin_pinned, sizeof(uint64_t) * max_jobs,
                                hipMemcpyHostToDevice, s);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(ctx->launch_desc, ctx->desc_pinned, sizeof(uint64_t) * 2, hipMemcpyHostToDevice, s);
+            e = hipMemcpyAsync(ctx->launch_desc, ctx->desc_pinned, sizeof(uint64_t) * 3, hipMemcpyHostToDevice, s);
         if (e == hipSuccess) e = vp_launch_step(ctx, L, max_jobs, max_npix, s);
         return e;
     };

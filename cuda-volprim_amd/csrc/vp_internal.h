@@ -185,7 +185,7 @@ struct vp_context {
     uint32_t jobs_cap = 0;
     uint64_t* job_begin_dev = nullptr;
     unsigned long long* job_done_dev = nullptr;  // per job-table entry (wavefront pipeline)
-    uint64_t* launch_desc = nullptr;  // [0] job slices, [1] total paths
+    uint64_t* launch_desc = nullptr;  // [0] job slices, [1] end of the launch's path range, [2] its start
     // pinned staging of the job table, the slice begins and the launch descriptor
     vp::JobDesc* jobs_pinned = nullptr;
     uint64_t* begin_pinned = nullptr;

@@ -221,13 +221,16 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
     Tdirect = acc >= kTauLimitFixed ? 0.0f : tau_to_T(acc);
 }
 
-// launch_desc[0] = number of job slices, launch_desc[1] = total paths; read
-// from device memory so one captured graph serves every step of a job mix.
-// A wavefront claims new paths once at least regen_min of its lanes are idle
-// (or none is busy).  Claims never take the path counter past this launch's
-// total (a compare-and-swap on the counter), so launches that share one
-// counter with larger totals (the server starts a new launch over a longer
-// job table while earlier ones drain) hand out every path exactly once.
+// launch_desc[0] = number of job slices, launch_desc[1] = end of the
+// launch's path range, launch_desc[2] = its start; read from device memory so
+// one captured graph serves every step of a job mix.  A wavefront claims new
+// paths once at least regen_min of its lanes are idle (or none is busy), with
+// one fetch-and-add on the launch's own counter: path start + counter.  The
+// counter may run past the range (later claims then get nothing), and each
+// launch has its own counter and range, so launches that run at once (the
+// server starts a launch over the paths a longer job table adds while the
+// earlier ones drain) hand out every path exactly once, without the retries a
+// compare-and-swap on one shared counter costs when paths are short.
 template <bool kCount>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VP_RENDER_WAVES_PER_EU)))
 k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begin,
@@ -237,6 +240,7 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
     extern __shared__ __align__(16) uint32_t s_leafq[];  // [blockDim.x][kLeafRow]
     const uint32_t njobs = static_cast<uint32_t>(launch_desc[0]);
     const uint64_t total_paths = launch_desc[1];
+    const uint64_t path_lo = launch_desc[2];
     const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
     // the lane's two active-set buffers, contiguous (lanes read and append
@@ -264,15 +268,12 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
             unsigned long long base = 0, got = 0;
             if (lane == leader) {
                 const unsigned long long n_want = static_cast<unsigned long long>(__popcll(wmask));
-                unsigned long long c = __hip_atomic_load(path_counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                while (c < total_paths) {
-                    const unsigned long long n = total_paths - c < n_want ? total_paths - c : n_want;
-                    if (__hip_atomic_compare_exchange_strong(path_counter, &c, c + n, __ATOMIC_RELAXED,
-                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                        base = c;
-                        got = n;
-                        break;
-                    }
+                const unsigned long long c =
+                    __hip_atomic_fetch_add(path_counter, n_want, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t b = path_lo + c;
+                if (b < total_paths) {
+                    base = b;
+                    got = total_paths - b < n_want ? total_paths - b : n_want;
                 }
             }
             base = __shfl(base, leader);
@@ -780,7 +781,7 @@ inline uint32_t cdiv(uint64_t a, uint32_t b) { return static_cast<uint32_t>((a +
 using namespace vp;
 
 // One persistent render launch over a job table (device pointers): the
-// kernel claims paths from *counter up to launch_desc[1].
+// kernel claims paths launch_desc[2] + *counter below launch_desc[1].
 hipError_t vp_launch_render(const RenderLaunch& L, const JobDesc* jobs, const uint64_t* job_begin,
                             const uint64_t* launch_desc, unsigned long long* counter, uint32_t* slab,
                             unsigned long long* counters, hipStream_t s) {

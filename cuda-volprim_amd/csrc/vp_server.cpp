@@ -5,12 +5,12 @@
 // and batches continuously on the persistent megakernel.  An admitted request
 // becomes one entry of the server's job table (all its samples), appended
 // while earlier launches are still running; a new launch over the longer
-// table then starts on the second of two render streams.  All launches share
-// one path counter and claim from it only up to their own table's total, so
-// the new launch takes the new request's paths while the older one finishes
-// what it claimed: as the older launch's wavefronts run out of paths and
-// exit, the new launch's workgroups take their place, and no request waits
-// for a step barrier or another request's tail.  Each launch has its own
+// table then starts on the second of two render streams.  Each launch
+// claims, with its own counter, the range of paths its table adds to the
+// previous launch's, so the new launch takes the new request's paths while
+// the older one finishes its own: as the older launch's wavefronts run out of
+// paths and exit, the new launch's workgroups take their place, and no
+// request waits for a step barrier or another request's tail.  Each launch has its own
 // active-set slab, so two can run at once.  Finished paths add into their
 // request's per-pixel accumulator (exact fixed-point sums, so the image does
 // not depend on which launch or lane ran which path) and count against its
@@ -99,13 +99,13 @@ struct vp_server {
     vp::JobDesc* jobs_dev = nullptr;
     uint64_t* begin_dev = nullptr;
     unsigned long long* done_dev = nullptr;  // [kServerTable] finished paths per entry
-    unsigned long long* counter = nullptr;   // shared path counter of the epoch's launches
-    uint64_t* desc_dev = nullptr;            // [2][2] launch descriptors, one per render stream
+    unsigned long long* counter = nullptr;   // [2] path counter of the launch on each render stream
+    uint64_t* desc_dev = nullptr;            // [2][4] launch descriptors, one per render stream
     uint32_t* slab[2] = {nullptr, nullptr};
     vp::JobDesc* jobs_pinned = nullptr;
     uint64_t* begin_pinned = nullptr;
     unsigned long long* done_pinned = nullptr;
-    uint64_t* desc_pinned = nullptr;  // [2][2]
+    uint64_t* desc_pinned = nullptr;  // [2][4]
     hipStream_t ks[2] = {nullptr, nullptr};  // render streams
     hipStream_t side = nullptr;              // installs, progress copies, result copies (copies only)
     unsigned long long* stage = nullptr;     // pinned: accumulator of the request being resolved
@@ -212,7 +212,7 @@ void fail_all(vp_server* S, const std::string& what) {
 void server_loop(vp_server* S) {
     vp_context* ctx = S->ctx;
     (void)hipSetDevice(ctx->device);
-    bool fresh = true;  // the path counter must be reset before the next launch
+    bool fresh = true;  // no launch in this epoch yet
     clk::time_point busy_since{};
     std::vector<Request*> admit;
     while (true) {
@@ -221,9 +221,8 @@ void server_loop(vp_server* S) {
         {
             std::unique_lock<std::mutex> lk(S->mu);
             if (S->inflight.empty() && (S->pending.empty() || S->njobs >= kServerTable)) {
-                // nothing on the GPU: start a new epoch once the last launches
-                // have exited (a launch still running would claim again from a
-                // reset counter)
+                // nothing on the GPU: start a new epoch, with the table refilled
+                // from entry 0, once the last launches have exited
                 lk.unlock();
                 for (auto& k : S->ks) (void)hipStreamSynchronize(k);
                 (void)hipStreamSynchronize(S->side);
@@ -253,11 +252,6 @@ void server_loop(vp_server* S) {
         }
         if (fresh && !admit.empty()) {
             busy_since = clk::now();
-            if (hipMemcpyAsync(S->counter, S->zeros, sizeof(unsigned long long), hipMemcpyHostToDevice, S->side) !=
-                hipSuccess) {
-                fail_all(S, "server: cannot reset the path counter");
-                continue;
-            }
             fresh = false;
         }
         // install the admitted requests (a request that cannot be installed
@@ -280,18 +274,23 @@ void server_loop(vp_server* S) {
                 continue;
             }
         }
-        // a launch over the grown table, on a render stream that is free
+        // a launch over the paths the grown table adds, [snap, total), on a
+        // render stream that is free, with that stream's counter reset
         streams_idle(S);
         if (S->total > S->snap) {
             const int i = !S->kbusy[0] ? 0 : (!S->kbusy[1] ? 1 : -1);
             if (i >= 0) {
-                uint64_t* d = S->desc_pinned + 2 * i;
+                uint64_t* d = S->desc_pinned + 4 * i;
                 d[0] = S->njobs;
                 d[1] = S->total;
+                d[2] = S->snap;
+                d[3] = 0;
                 if (hipStreamWaitEvent(S->ks[i], S->install_ev, 0) != hipSuccess ||
-                    hipMemcpyAsync(S->desc_dev + 2 * i, d, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, S->ks[i]) !=
+                    hipMemcpyAsync(S->desc_dev + 4 * i, d, 4 * sizeof(uint64_t), hipMemcpyHostToDevice, S->ks[i]) !=
                         hipSuccess ||
-                    vp_launch_render(S->L, S->jobs_dev, S->begin_dev, S->desc_dev + 2 * i, S->counter, S->slab[i],
+                    hipMemcpyAsync(S->counter + i, S->zeros, sizeof(unsigned long long), hipMemcpyHostToDevice,
+                                   S->ks[i]) != hipSuccess ||
+                    vp_launch_render(S->L, S->jobs_dev, S->begin_dev, S->desc_dev + 4 * i, S->counter + i, S->slab[i],
                                      ctx->counters, S->ks[i]) != hipSuccess ||
                     hipEventRecord(S->kev[i], S->ks[i]) != hipSuccess) {
                     fail_all(S, "server: render launch failed");
@@ -432,8 +431,8 @@ vp_status vp_server_create(vp_context* ctx, const vp_server_config* cfg, vp_serv
     bool ok = hipMalloc(&S->jobs_dev, sizeof(vp::JobDesc) * kServerTable) == hipSuccess &&
               hipMalloc(&S->begin_dev, sizeof(uint64_t) * kServerTable) == hipSuccess &&
               hipMalloc(&S->done_dev, sizeof(unsigned long long) * kServerTable) == hipSuccess &&
-              hipMalloc(&S->counter, sizeof(unsigned long long)) == hipSuccess &&
-              hipMalloc(&S->desc_dev, 4 * sizeof(uint64_t)) == hipSuccess &&
+              hipMalloc(&S->counter, 2 * sizeof(unsigned long long)) == hipSuccess &&
+              hipMalloc(&S->desc_dev, 8 * sizeof(uint64_t)) == hipSuccess &&
               hipMalloc(&S->slab[0], slab_words * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc(&S->slab[1], slab_words * sizeof(uint32_t)) == hipSuccess;
     if (!ok) return fail(VP_ERR_OOM, "server: cannot allocate its tables and active-set slabs");
@@ -441,7 +440,7 @@ vp_status vp_server_create(vp_context* ctx, const vp_server_config* cfg, vp_serv
          hipHostMalloc(&S->begin_pinned, sizeof(uint64_t) * kServerTable, hipHostMallocDefault) == hipSuccess &&
          hipHostMalloc(&S->done_pinned, sizeof(unsigned long long) * kServerTable, hipHostMallocDefault) ==
              hipSuccess &&
-         hipHostMalloc(&S->desc_pinned, 4 * sizeof(uint64_t), hipHostMallocDefault) == hipSuccess &&
+         hipHostMalloc(&S->desc_pinned, 8 * sizeof(uint64_t), hipHostMallocDefault) == hipSuccess &&
          hipHostMalloc(&S->stage, S->max_npix * kRequestBytesPerPixel, hipHostMallocDefault) == hipSuccess &&
          hipHostMalloc(&S->zeros, S->max_npix * kRequestBytesPerPixel, hipHostMallocDefault) == hipSuccess;
     if (ok) std::memset(S->zeros, 0, S->max_npix * kRequestBytesPerPixel);
