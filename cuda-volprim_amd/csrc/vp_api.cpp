@@ -147,6 +147,12 @@ vp_status vp_context_create(int device, vp_context** out) {
         vp_context_destroy(ctx);
         return fail(VP_ERR_HIP, "context setup failed");
     }
+    const vp_status st = vp_noalloc_attach(ctx);
+    if (st != VP_OK) {
+        const std::string msg = vp_last_error();
+        vp_context_destroy(ctx);
+        return fail(st, msg);
+    }
     *out = ctx;
     return VP_OK;
 }

@@ -207,6 +207,9 @@ struct vp_context {
     uint32_t last_launches = 0;
     vp_launch_config tuned{};
     bool have_tuned = false;
+    // device address of this device's no-allocation scratch array
+    // (vp_noalloc.hip), resolved once at context creation
+    void* noalloc_scratch = nullptr;
 };
 
 struct vp_scene {
@@ -299,6 +302,8 @@ vp_status vp_read_stats(vp_context* ctx, hipStream_t s, vp_render_stats* stats);
 vp_status vp_reset_counters(vp_context* ctx, hipStream_t s);
 vp::JobDesc vp_make_job(const vp_scene* sc, const vp_env* env, const vp_camera* cam, const vp_render_params* p);
 vp_status vp_check_params(const vp_render_params* p);
+// Resolves ctx->noalloc_scratch on ctx's device (which must be current).
+vp_status vp_noalloc_attach(vp_context* ctx);
 hipError_t vp_launch_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v, hipStream_t s);
 hipError_t vp_launch_normalize(float4* out, const float4* in, size_t n, float scale, hipStream_t s);
 hipError_t vp_launch_transmittance(vp::SceneView sc, const float* rays, uint32_t m, float* T, hipStream_t s);

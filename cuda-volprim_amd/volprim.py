@@ -98,7 +98,7 @@ EXPORTED = [
     "vp_render_device", "vp_normalize_accumulator", "vp_render_jobs", "vp_autotune", "vp_server_create",
     "vp_server_submit", "vp_server_poll", "vp_server_drain", "vp_server_stats_get", "vp_server_destroy",
     "vp_transmittance_rays", "vp_collect_overlaps", "vp_render_count_events", "vp_math_eval", "vp_prim_eval",
-    "vp_scene_create_ex",
+    "vp_scene_create_ex", "vp_render_device_noalloc",
 ]
 SCENE_BVH_SAH = 1
 SCENE_BVH_MEDIAN = 2
@@ -142,6 +142,8 @@ def load():
     lib.vp_render_device.argtypes = [_vp, _vp, _vp, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams),
                                      ctypes.POINTER(LaunchConfig), _vp, _vp, _vp, ctypes.POINTER(RenderStats)]
     lib.vp_normalize_accumulator.argtypes = [_vp, _vp, ctypes.c_size_t, ctypes.c_float, _vp]
+    lib.vp_render_device_noalloc.argtypes = [_vp, _vp, _vp, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams),
+                                             _vp, _vp, ctypes.POINTER(RenderStats)]
     lib.vp_render_jobs.argtypes = [_vp, ctypes.POINTER(Job), ctypes.c_uint32, ctypes.POINTER(LaunchConfig),
                                    ctypes.POINTER(RenderStats)]
     lib.vp_autotune.argtypes = [_vp, ctypes.POINTER(Job), ctypes.c_uint32, ctypes.POINTER(LaunchConfig)]
@@ -302,6 +304,14 @@ class Context:
                                        ctypes.byref(launch) if launch is not None else None, out_ptr, samples_ptr,
                                        stream, ctypes.byref(st)), "vp_render_device")
         return st.as_dict()
+
+    def render_device_noalloc(self, scene, env, cam, params, out_ptr, stream=None, stats=True):
+        """vp_render_device_noalloc: no allocation, scratch in a static device array."""
+        st = RenderStats()
+        _check(load().vp_render_device_noalloc(self.handle, scene.handle, env.handle, ctypes.byref(cam),
+                                               ctypes.byref(params), out_ptr, stream,
+                                               ctypes.byref(st) if stats else None), "vp_render_device_noalloc")
+        return st.as_dict() if stats else None
 
     def render_jobs(self, jobs, launch=None):
         """jobs: list of (scene, env, cam, params).  Returns (images, stats)."""

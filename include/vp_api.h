@@ -163,6 +163,23 @@ vp_status vp_render(vp_context* ctx, vp_scene* scene, vp_env* env, const vp_came
 vp_status vp_render_device(vp_context* ctx, vp_scene* scene, vp_env* env, const vp_camera* cam,
                            const vp_render_params* params, const vp_launch_config* launch,
                            float* out_rgba_dev, float* out_samples_dev, void* stream, vp_render_stats* stats);
+/* vp_render_device without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * buffer beyond the caller's scene, environment and output image lives in a
+ * file-scope device array of 128 MiB + 4 KiB that each device holds from the
+ * library's first use until the process ends: the pixel accumulators (at most
+ * 512x512 pixels; larger images are rejected with VP_ERR_INVALID), a one-entry
+ * job table with its counters, and the megakernel's active-set slab, which
+ * bounds the lanes of the launch (fewer lanes at larger active_cap; a
+ * capacity that leaves no room for one workgroup is rejected).  The image is
+ * bit-identical to vp_render_device's.  Calls on one device share the array
+ * and race if they overlap; each call clears what must start at zero with
+ * kernels (no hipMemsetAsync).  With stats null nothing synchronises the
+ * stream; with stats it reads the counters back and synchronises. */
+vp_status vp_render_device_noalloc(vp_context* ctx, vp_scene* scene, vp_env* env, const vp_camera* cam,
+                                   const vp_render_params* params, float* out_rgba_dev, void* stream,
+                                   vp_render_stats* stats);
 /* out[i] = mean[i] * scale over n RGBA pixels, device pointers. */
 vp_status vp_normalize_accumulator(float* out_rgba_dev, const float* mean_rgba_dev, size_t n, float scale,
                                    void* stream);
