@@ -106,7 +106,8 @@ def cpu_baseline(rows, env, cam_pod, params, threads, budget_s=20.0, repeats=5):
         vpo.render(rec, env, cam_pod, params["width"], params["height"], params["spp"], seed=params["seed"],
                    max_depth=params["max_depth"], rr_depth=params["rr_depth"], phase_g=params["phase_g"],
                    use_ris=bool(params.get("use_ris", 0)), stride=stride, samples=(0, spp_used), threads=threads,
-                   env_tables_=tabs, bvh=True, active_cap=params.get("active_cap", 128))
+                   env_tables_=tabs, bvh=True, active_cap=params.get("active_cap", 128),
+                   shadow_rr=params.get("shadow_rr", 0.0))
         return time.perf_counter() - t0
 
     probe = run(1)
@@ -184,11 +185,12 @@ def mix_scenes(ctx, scenes, bvh, names):
     return {k: ctx.scene(r, bvh=bvh) for k, r in rows.items()}, rows
 
 
-def mix_requests(volprim, jobs, handles, envh):
+def mix_requests(volprim, jobs, handles, envh, shadow_rr=0.0):
     reqs = []
     for j in jobs:
         p = volprim.default_params(width=j["width"], height=j["height"], spp=j["spp"], seed=j["seed"], max_depth=64,
-                                   rr_depth=12, phase_g=j["phase_g"], active_cap=MIX_ACTIVE_CAP[j["scene"]])
+                                   rr_depth=12, phase_g=j["phase_g"], active_cap=MIX_ACTIVE_CAP[j["scene"]],
+                                   shadow_rr=shadow_rr)
         reqs.append((handles[j["scene"]], envh, volprim.camera_perspective(**j["cam"]), p))
     return reqs
 
@@ -257,8 +259,8 @@ def run_stream(args):
     handles, rows = mix_scenes(ctx, scenes, args.bvh, names)
     env_rgba = scenes.sky_env(2048, 1024)
     envh = ctx.env(env_rgba)
-    reqs = mix_requests(volprim, jobs, handles, envh)
-    wreqs = mix_requests(volprim, wjobs, handles, envh)
+    reqs = mix_requests(volprim, jobs, handles, envh, args.shadow_rr)
+    wreqs = mix_requests(volprim, wjobs, handles, envh, args.shadow_rr)
     samples = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
     print(f"[stream] {K} requests ({samples} samples) over {names}, scenes resident after "
           f"{time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
@@ -421,6 +423,7 @@ def cpu_baseline_mix(rows, env_rgba, handles, jobs, reqs, threads, budget_s=20.0
             t0 = time.perf_counter()
             vpo.render(rec, env_rgba, cam.pod(), p.width, p.height, p.spp, seed=p.seed, max_depth=p.max_depth,
                        rr_depth=p.rr_depth, phase_g=p.phase_g, active_cap=p.active_cap, stride=stride,
+                       shadow_rr=p.shadow_rr,
                        samples=(0, spp_used), threads=threads, env_tables_=tabs, bvh=True)
             return time.perf_counter() - t0
 
@@ -476,6 +479,8 @@ def main():
                     help="run one 256 MiB elementwise copy kernel first (HBM counter calibration)")
     ap.add_argument("--ris", action="store_true",
                     help="NEE by resampled importance sampling (the reference's --ris) instead of MIS")
+    ap.add_argument("--shadow-rr", type=float, default=0.0,
+                    help="shadow-ray roulette threshold eps (0 = off, the reference's estimator)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is in DESIGN.md",
@@ -497,6 +502,8 @@ def run_single(args, torch):
     env = scenes.sky_env(2048, 1024)
     if args.ris:
         pp = dict(pp, use_ris=1)
+    if args.shadow_rr:
+        pp = dict(pp, shadow_rr=args.shadow_rr)
     ctx = volprim.Context(0)
     dev_name, n_cu, clk = ctx.device_info()
     scene = ctx.scene(rows, bvh=args.bvh)
@@ -600,7 +607,8 @@ def run_single(args, torch):
         "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": desc + ", " + ("RIS NEE (6 candidates)" if args.ris else "MIS NEE"),
+        "config": {"workload": desc + ", " + ("RIS NEE (6 candidates)" if args.ris else "MIS NEE")
+                               + (f", shadow-ray roulette eps {args.shadow_rr}" if args.shadow_rr else ""),
                    "width": pp["width"], "height": pp["height"], "spp": pp["spp"], "primitives": int(rows.shape[0])},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -46,6 +46,7 @@ vp_status check_params_impl(const vp_render_params* p) {
     if (!(p->filter_stddev >= 0.0f)) return fail(VP_ERR_INVALID, "filter_stddev must be >= 0");
     if (p->use_ris && p->ris_candidates == 0) return fail(VP_ERR_INVALID, "ris_candidates must be positive");
     if (p->active_cap > vp::kMaxActiveCap) return fail(VP_ERR_INVALID, "active_cap must be at most 65535");
+    if (!(p->shadow_rr >= 0.0f && p->shadow_rr < 1e30f)) return fail(VP_ERR_INVALID, "shadow_rr must be >= 0");
     return VP_OK;
 }
 
@@ -73,6 +74,7 @@ vp::JobDesc make_job_impl(const vp_scene* sc, const vp_env* env, const vp_camera
     J.filter_stddev = p->filter_stddev;
     J.use_ris = p->use_ris ? 1u : 0u;
     J.ris_candidates = p->ris_candidates ? p->ris_candidates : 6u;
+    J.shadow_rr = p->shadow_rr;
     J.tiles_x = (p->width + 7) / 8;
     const uint32_t tiles_y = (p->height + 7) / 8;
     J.npad = J.tiles_x * tiles_y * 64u;

@@ -320,6 +320,56 @@ VP_HD float prim_uniform(uint64_t key, uint32_t bounce, uint32_t role, uint32_t 
 VP_HD float free_flight_tau(float xi) { return -logf_(maxf(1.0f - xi, 1e-10f)); }
 
 // ---------------------------------------------------------------------------
+// Shadow-ray roulette, an estimator option of this build (the reference walks
+// every shadow ray to the tau = 88 saturation, sampling.cuh:548-583).  A NEE
+// shadow ray whose contribution is w * T(tau) (w: the largest channel of the
+// term T multiplies) plays one Russian roulette when its optical depth
+// reaches tau_c = log(w / eps), where w e^(-tau) has fallen below eps: it
+// survives with probability 1/4 and then counts 4 T, else it counts 0.  The
+// expectation is T, so the estimator stays unbiased, and the added variance
+// per ray is below 3 eps^2.  Whether the depth reaches tau_c does not depend
+// on the order chords are summed in, and the survival draw is a counter hash
+// of (path, bounce, ray), so a walk that does not survive stops at tau_c and
+// the result matches a full walk bit for bit.  eps = 0 turns it off.
+// Thresholds are 24.40 fixed-point depths, multiples of 2^16.
+// ---------------------------------------------------------------------------
+VP_HD uint64_t rr_threshold(float w, float eps) {
+    if (!(eps > 0.0f)) return kTauLimitFixed;
+    if (!(w > eps)) return 0ull;
+    const float tc = logf_(w * rcpf(eps));
+    if (!(tc > 0.0f)) return 0ull;
+    if (tc >= kMaxTau) return kTauLimitFixed;
+    return static_cast<uint64_t>(tc * kTauScaleF) & ~0xffffull;
+}
+VP_HD float rr_uniform(uint64_t key, uint32_t bounce, uint32_t ray) {
+    uint64_t x = (key ^ 0xd1b54a32d192ed03ull) + 0x9e3779b97f4a7c15ull * ((static_cast<uint64_t>(bounce) << 8) | ray);
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return u32_to_unit(static_cast<uint32_t>(x >> 32));
+}
+// The roulette of one shadow ray: the depth at which its walk may stop
+// (kTauLimitFixed when it survives or the roulette is off) and the threshold.
+struct ShadowRr {
+    uint64_t thr, limit;
+    bool survive;
+};
+VP_HD ShadowRr shadow_rr(float w, float eps, uint64_t key, uint32_t bounce, uint32_t ray) {
+    ShadowRr r;
+    r.thr = rr_threshold(w, eps);
+    r.survive = r.thr < kTauLimitFixed && rr_uniform(key, bounce, ray) < 0.25f;
+    r.limit = r.survive ? kTauLimitFixed : r.thr;
+    return r;
+}
+// Transmittance from the summed depth acc under the roulette rr.
+VP_HD float shadow_rr_T(uint64_t acc, const ShadowRr& rr, float T) {
+    if (acc >= rr.thr) return rr.survive ? T * 4.0f : 0.0f;
+    return T;
+}
+
+// ---------------------------------------------------------------------------
 // Per-pixel radiance accumulator.  The reference folds each sample into a
 // running Welford mean in sample order (raygen.cuh:214-238).  Here every
 // sample's radiance is added to an exact integer sum in 32.32 fixed point, so

@@ -231,13 +231,14 @@ struct WaveQueue {
     uint32_t* nn;                  // [64] its level count (or, in the wavefront argmin
                                    //      kernel, the pool slot of each lane's path)
     uint8_t* list;                 // [128] the rays still to be walked
+    uint32_t* lim;                 // [128] depth at which each ray's walk may stop, >> 16 (shadow roulette)
     // argmin walks (one ray per lane); they lie over rd[64..128) and acc[64..128)
     unsigned long long* key;       // [64] path key of each lane
     uint32_t* ts;                  // [64] running minimum of each lane's argmin, as float bits
     uint32_t* nc;                  // [64] candidates recorded for each lane's argmin
     uint32_t* meta;                // [64] bounce | direct-term flag << 31
 };
-constexpr uint32_t kWaveQueueBytes = kWaveQ * 9 + 64 * 12 + 64 * 8 + 128 * 12 + 128 * 8 + 64 * 8 + 64 * 4 + 128;
+constexpr uint32_t kWaveQueueBytes = kWaveQ * 9 + 64 * 12 + 64 * 8 + 128 * 12 + 128 * 8 + 64 * 8 + 64 * 4 + 128 + 128 * 4;
 static_assert(kWaveQ % 8 == 0, "queue arrays keep the per-lane arrays 8-byte aligned");
 static_assert(64 * (8 + 4) <= 64 * 12 && 64 * (4 + 4) <= 64 * 8, "argmin arrays fit over the upper rays");
 // LDS bytes per wavefront: the larger of the two uses, in 16-byte units
@@ -258,6 +259,7 @@ __device__ __forceinline__ WaveQueue wave_queue(uint32_t* lds) {
     q.nodes = reinterpret_cast<const float4**>(q.acc + 128);
     q.nn = reinterpret_cast<uint32_t*>(q.nodes + 64);
     q.list = reinterpret_cast<uint8_t*>(q.nn + 64);
+    q.lim = reinterpret_cast<uint32_t*>(q.list + 128);
     q.key = reinterpret_cast<unsigned long long*>(q.rd + 64 * 3);
     q.ts = reinterpret_cast<uint32_t*>(q.key + 64);
     q.nc = reinterpret_cast<uint32_t*>(q.acc + 64);
@@ -411,6 +413,8 @@ __device__ __forceinline__ float tau_to_T(uint64_t acc) {
     const float tau = static_cast<float>(acc) * kTauInvScale;
     return tau >= kMaxTau ? 0.0f : expf_(-tau);
 }
+// Transmittance of a fixed-point depth (0 once saturated).
+__device__ __forceinline__ float depth_T(uint64_t acc) { return acc >= kTauLimitFixed ? 0.0f : tau_to_T(acc); }
 
 struct EvCount {
     uint32_t v[kNumCounters];
@@ -436,21 +440,25 @@ struct Ev {
 };
 
 // compute_transmittance_to_env (sampling.cuh:548-583) with the fused
-// any-hit integration (anyhit.cuh:53-66).
+// any-hit integration (anyhit.cuh:53-66), as a fixed-point optical depth: the
+// walk stops once the sum reaches limit (at most the tau = 88 saturation,
+// which it returns as kTauLimitFixed).
 template <bool kCount>
-__device__ float transmittance(const SceneView& s, f3 o, f3 d, const uint32_t* act, uint32_t na, uint32_t stride,
-                               const LeafQueue& lq, Ev<kCount>& ev) {
+__device__ uint64_t shadow_depth(const SceneView& s, f3 o, f3 d, const uint32_t* act, uint32_t na, uint32_t stride,
+                                 const LeafQueue& lq, Ev<kCount>& ev, uint64_t limit) {
     uint64_t acc = 0;
+    if (acc >= limit) return acc;
     for (uint32_t k = 0; k < na; ++k) {
         const PrimRecord pr = load_prim(s.prims, act[k * stride]);
         const float te = exit_from_inside(pr, o, d);
         ev.add(kCntShActive);
-        if (te > 0.0f && te < kTMax)
-            if (tau_add(acc, optical_depth(pr, o, d, 0.0f, te))) return 0.0f;
+        if (te > 0.0f && te < kTMax) {
+            if (tau_add(acc, optical_depth(pr, o, d, 0.0f, te))) return kTauLimitFixed;
+            if (acc >= limit) return acc;
+        }
     }
     const f3 inv = safe_inv(d);
     const float tmax = kTMax;
-    bool full = false;
     uint32_t visits = 0, leaves = 0, sh = 0;
     walk_ray_q(s, o, inv, tmax, visits, lq, [&](uint32_t prim) -> bool {
         const PrimRecord pr = load_prim(s.prims, prim);
@@ -459,17 +467,22 @@ __device__ float transmittance(const SceneView& s, f3 o, f3 d, const uint32_t* a
         if (x.t0 < 0.0f) return true;
         ++sh;
         if (tau_add(acc, x.tau)) {
-            full = true;
+            acc = kTauLimitFixed;
             return false;
         }
-        return true;
+        return acc < limit;
     });
     ev.add(kCntNodeVisits, visits);
     ev.add(kCntVisitsShadow, visits);
     ev.add(kCntLeafTests, leaves);
     ev.add(kCntShHit, sh);
-    if (full) return 0.0f;
-    return tau_to_T(acc);
+    return acc;
+}
+
+template <bool kCount>
+__device__ float transmittance(const SceneView& s, f3 o, f3 d, const uint32_t* act, uint32_t na, uint32_t stride,
+                               const LeafQueue& lq, Ev<kCount>& ev) {
+    return depth_T(shadow_depth(s, o, d, act, na, stride, lq, ev, kTauLimitFixed));
 }
 
 // one chord in fixed point, as tau_add adds it; a chord of tau >= 88 alone
@@ -539,11 +552,14 @@ __device__ __forceinline__ bool wave_steal(const WaveQueue& wq, bool idle, bool 
 // saturated once its sum reaches the tau = 88 limit, in whatever order the
 // chords arrive), so Ta and Tb equal two transmittance() calls bit for bit.
 // Every lane of the calling wavefront that is active here takes part.
-// With two = false only ray a is walked (the one shadow ray of RIS); Tb is
-// then 0.
+// With two = false only ray a is walked (the one shadow ray of RIS).  Each
+// ray's walk may stop once its depth reaches its limit (lim_a, lim_b: the
+// tau = 88 saturation, or a shadow-roulette threshold; multiples of 2^16);
+// acc_a and acc_b receive the fixed-point depths.
 __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 da, f3 db, const uint32_t* act,
-                                                   uint32_t na, uint32_t stride, const WaveQueue& wq, float& Ta,
-                                                   float& Tb, bool two = true) {
+                                                   uint32_t na, uint32_t stride, const WaveQueue& wq,
+                                                   uint64_t lim_a, uint64_t lim_b, uint64_t& acc_a,
+                                                   uint64_t& acc_b, bool two = true) {
     const uint32_t lane = __lane_id();
     lds_store3(wq.ro + 3 * lane, o);
     wq.prims[lane] = s.prims;
@@ -553,6 +569,12 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
     lds_store3(wq.rd + 3 * (2 * lane + 1), db);
     wq.acc[2 * lane] = 0ull;
     wq.acc[2 * lane + 1] = two ? 0ull : kTauLimitFixed;
+    wq.lim[2 * lane] = static_cast<uint32_t>(lim_a >> 16);
+    wq.lim[2 * lane + 1] = two ? static_cast<uint32_t>(lim_b >> 16) : 0u;
+    // a ray is done once its depth reaches its limit
+    auto reached = [&](uint32_t r) -> bool {
+        return wq.acc[r] >= (static_cast<unsigned long long>(wq.lim[r]) << 16);
+    };
     wave_sync();
     // Chords of the origin set (every member's optical depth from the origin
     // to its exit), for all lanes' rays together: the wavefront's (member,
@@ -600,8 +622,8 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
         }
     }
     wave_sync();
-    const bool full_a = wq.acc[2 * lane] >= kTauLimitFixed;
-    const bool full_b = wq.acc[2 * lane + 1] >= kTauLimitFixed;
+    const bool full_a = reached(2 * lane);
+    const bool full_b = reached(2 * lane + 1);
     // the rays still to walk: every ray a, then every ray b (a saturated ray
     // is done already)
     const unsigned long long am = __builtin_amdgcn_ballot_w64(true);
@@ -672,7 +694,7 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
         wave_sync();
         // lanes whose ray is done (walked to the end, or saturated) take the
         // next ray of the list
-        const bool fin = ray != kNoRay && (w.n == kWalkEnd || wq.acc[ray & 127u] >= kTauLimitFixed);
+        const bool fin = ray != kNoRay && (w.n == kWalkEnd || reached(ray & 127u));
         const unsigned long long fm = __builtin_amdgcn_ballot_w64(fin);
         const uint32_t k = next + lanes_below(fm);
         const uint32_t listed = k < nr ? wq.list[k < 128u ? k : 0u] : kNoRay;
@@ -684,15 +706,14 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
         // lane that kept its ray): the refill and the handoff stay free of
         // branches around the setup, so every lane of the wavefront stays in
         // the loop and takes part in its ballots and leaf passes.
-        const bool can_give = ray != kNoRay && w.n != kWalkEnd && wq.acc[ray & 127u] < kTauLimitFixed;
+        const bool can_give = ray != kNoRay && w.n != kWalkEnd && !reached(ray & 127u);
         if (next >= nr) wave_steal(wq, ray == kNoRay, can_give, ray, w);
         setup();
         wave_sync();
     }
     wave_sync();
-    const unsigned long long ta = wq.acc[2 * lane], tb = wq.acc[2 * lane + 1];
-    Ta = ta >= kTauLimitFixed ? 0.0f : tau_to_T(ta);
-    Tb = tb >= kTauLimitFixed ? 0.0f : tau_to_T(tb);
+    acc_a = wq.acc[2 * lane];
+    acc_b = wq.acc[2 * lane + 1];
 }
 
 __device__ __forceinline__ EnvView env_view(const EnvDev& e) {

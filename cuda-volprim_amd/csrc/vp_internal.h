@@ -87,7 +87,7 @@ struct alignas(128) JobDesc {
     PhaseParams hg;
     uint32_t width, height, spp, seed;
     uint32_t max_depth, rr_depth;
-    float rr_max_survival, firefly_clamp, filter_stddev;
+    float rr_max_survival, firefly_clamp, filter_stddev, shadow_rr;
     uint32_t use_ris, ris_candidates;
     uint32_t tiles_x, npad;
     uint32_t s_begin, s_count;

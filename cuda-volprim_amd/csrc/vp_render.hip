@@ -556,11 +556,15 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                     const float lumy = dot(yenv, luma);
                     if (lumy > 0.0f) {
                         ev.add(kCntShadowRays);
-                        float T, T_unused;
-                        if constexpr (kCount) T = transmittance(sc, x, ydir, act, nn, stride, lq, ev);
-                        else transmittance_pool(sc, x, ydir, ydir, act, nn, stride, wq, T, T_unused, false);
                         const float Wr = wsum * rcpf(static_cast<float>(J.ris_candidates));
-                        st.L = st.L + ((base * (yenv * rcpf(lumy))) * T) * Wr;
+                        const f3 cy = base * (yenv * rcpf(lumy));
+                        const ShadowRr rr = shadow_rr(max3(cy) * Wr, J.shadow_rr, st.key, b, 0u);
+                        uint64_t acc, acc_unused;
+                        if constexpr (kCount) acc = shadow_depth(sc, x, ydir, act, nn, stride, lq, ev, rr.limit);
+                        else transmittance_pool(sc, x, ydir, ydir, act, nn, stride, wq, rr.limit, 0ull, acc, acc_unused,
+                                                false);
+                        const float T = shadow_rr_T(acc, rr, depth_T(acc));
+                        st.L = st.L + (cy * T) * Wr;
                     }
                 }
             } else {
@@ -578,17 +582,24 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                 const float wb = sb > 0.0f ? bpdf / sb : 0.0f;
                 const f3 envb = env_lookup(env, bwo);
                 ev.add(kCntShadowRays, 2);
+                // the terms the two transmittances multiply, and each ray's
+                // roulette (a no-op unless J.shadow_rr > 0)
+                const f3 ca = base * enva, cb = base * envb;
+                const ShadowRr rra = shadow_rr(max3(ca) * wa, J.shadow_rr, st.key, b, 0u);
+                const ShadowRr rrb = shadow_rr(((max3(cb) * wb) * ph_b) * rcpf(bpdf), J.shadow_rr, st.key, b, 1u);
                 // the counting build walks the two rays with separate calls,
                 // so its counts are per-ray work
-                float Ta, Tb;
+                uint64_t acc_a, acc_b;
                 if constexpr (kCount) {
-                    Ta = transmittance(sc, x, awo, act, nn, stride, lq, ev);
-                    Tb = transmittance(sc, x, bwo, act, nn, stride, lq, ev);
+                    acc_a = shadow_depth(sc, x, awo, act, nn, stride, lq, ev, rra.limit);
+                    acc_b = shadow_depth(sc, x, bwo, act, nn, stride, lq, ev, rrb.limit);
                 } else {
-                    transmittance_pool(sc, x, awo, bwo, act, nn, stride, wq, Ta, Tb);
+                    transmittance_pool(sc, x, awo, bwo, act, nn, stride, wq, rra.limit, rrb.limit, acc_a, acc_b);
                 }
-                st.L = st.L + ((base * enva) * Ta) * wa;
-                st.L = st.L + (((((base * envb) * Tb) * wb) * ph_b) * rcpf(bpdf));
+                const float Ta = shadow_rr_T(acc_a, rra, depth_T(acc_a));
+                const float Tb = shadow_rr_T(acc_b, rrb, depth_T(acc_b));
+                st.L = st.L + (ca * Ta) * wa;
+                st.L = st.L + ((((cb * Tb) * wb) * ph_b) * rcpf(bpdf));
             }
             st.beta = st.beta * albedo;
 

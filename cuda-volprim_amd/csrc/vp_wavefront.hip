@@ -341,15 +341,29 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
 }
 
 // Pending radiance term k of slot i: contribution (((w*T)*v)*s1)*s2 once the
-// shadow kernel has written T.
+// shadow kernel has written T.  Until then pd.w carries the ray's shadow
+// roulette: its threshold >> 16, and the survival flag in bit 31.
+__device__ __forceinline__ uint32_t rr_bits(const ShadowRr& rr) {
+    return static_cast<uint32_t>(rr.thr >> 16) | (rr.survive ? 0x80000000u : 0u);
+}
+__device__ __forceinline__ ShadowRr rr_unpack(uint32_t bits) {
+    ShadowRr rr;
+    rr.thr = static_cast<uint64_t>(bits & 0x7fffffffu) << 16;
+    rr.survive = (bits >> 31) != 0u;
+    rr.limit = rr.survive ? kTauLimitFixed : rr.thr;
+    return rr;
+}
 __device__ __forceinline__ void put_pending(const WfPool& w, uint32_t i, uint32_t k, f3 org, uint32_t set,
-                                            uint32_t count, f3 dir, f3 wt, f3 v, float s1, float s2) {
+                                            uint32_t count, f3 dir, f3 wt, f3 v, float s1, float s2,
+                                            uint32_t rrbits) {
     const uint32_t e = k * w.P + i;
     w.po[e] = make_float4(org.x, org.y, org.z, __uint_as_float(set | (count << 1)));
-    w.pd[e] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+    w.pd[e] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(rrbits));
     w.pw[e] = make_float4(wt.x, wt.y, wt.z, s1);
     w.pv[e] = make_float4(v.x, v.y, v.z, s2);
 }
+// the bounce-0 direct term has no roulette
+constexpr uint32_t kRrOff = static_cast<uint32_t>(kTauLimitFixed >> 16);
 
 __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par,
                                       unsigned long long* __restrict__ counters) {
@@ -376,6 +390,7 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
             const SceneView sc = J.scene;
             const EnvView env = env_view(J.env);
             uint4 m = w.misc[i];
+            const uint64_t key = static_cast<uint64_t>(m.x) | (static_cast<uint64_t>(m.y) << 32);
             uint32_t cur = f_cur(m.w);
             const uint32_t na = __float_as_uint(w.lr[i].w);
             f3 beta = f3{bt.x, bt.y, bt.z};
@@ -389,7 +404,7 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
 
             // analytic direct term at bounce 0 (raygen.cuh:148-160): T along the
             // camera ray with the origin set
-            if (b == 0) put_pending(w, i, npend++, o, cur, na, d, beta, env_lookup(env, d), 1.0f, 1.0f);
+            if (b == 0) put_pending(w, i, npend++, o, cur, na, d, beta, env_lookup(env, d), 1.0f, 1.0f, kRrOff);
 
             if (ts >= kTMax) {
                 finish = true;
@@ -498,8 +513,9 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
                         const float lumy = dot(yenv, luma);
                         if (lumy > 0.0f) {
                             const float Wr = wsum * rcpf(static_cast<float>(J.ris_candidates));
-                            put_pending(w, i, npend++, x, cur, nn, ydir, base3 * (yenv * rcpf(lumy)), mk(Wr), 1.0f,
-                                        1.0f);
+                            const f3 cy = base3 * (yenv * rcpf(lumy));
+                            put_pending(w, i, npend++, x, cur, nn, ydir, cy, mk(Wr), 1.0f, 1.0f,
+                                        rr_bits(shadow_rr(max3(cy) * Wr, J.shadow_rr, key, b, 0u)));
                         }
                     }
                 } else {
@@ -509,7 +525,9 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
                     const float sa = apdf + pb_a;
                     const float wa = sa > 0.0f ? apdf / sa : 0.0f;
                     const f3 enva = env_lookup(env, awo);
-                    put_pending(w, i, npend++, x, cur, nn, awo, base3 * enva, mk(wa), 1.0f, 1.0f);
+                    const f3 ca = base3 * enva;
+                    put_pending(w, i, npend++, x, cur, nn, awo, ca, mk(wa), 1.0f, 1.0f,
+                                rr_bits(shadow_rr(max3(ca) * wa, J.shadow_rr, key, b, 0u)));
 
                     float bpdf;
                     const f3 bwo = env_sample(env, rng, &bpdf);
@@ -517,7 +535,9 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
                     const float sb = bpdf + ph_b;
                     const float wb = sb > 0.0f ? bpdf / sb : 0.0f;
                     const f3 envb = env_lookup(env, bwo);
-                    put_pending(w, i, npend++, x, cur, nn, bwo, base3 * envb, mk(wb), ph_b, rcpf(bpdf));
+                    const f3 cb = base3 * envb;
+                    put_pending(w, i, npend++, x, cur, nn, bwo, cb, mk(wb), ph_b, rcpf(bpdf),
+                                rr_bits(shadow_rr(((max3(cb) * wb) * ph_b) * rcpf(bpdf), J.shadow_rr, key, b, 1u)));
                 }
                 beta = beta * albedo;
 
@@ -567,6 +587,7 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
     Walk4 w4w = walk4_begin();
     f3 inv = mk(0.0f), oi = mk(0.0f);
     SceneView sc{};
+    ShadowRr rr{};
     while (true) {
         const uint32_t idx = wave_fetch(&qc[5], !have, n, fetch_min);
         if (!have && idx < n) {
@@ -580,16 +601,20 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
             const uint32_t set = bits & 1u, count = bits >> 1;
             sc = jobs[__float_as_uint(w.dj[i].w)].scene;
             const uint32_t* act = w.slab + (static_cast<size_t>(i) * kSlabBuffers + set) * w.cap;
+            rr = rr_unpack(__float_as_uint(pd.w));
             uint64_t acc = 0;
-            bool full = false;
+            bool full = acc >= rr.limit;
             for (uint32_t c = 0; c < count && !full; ++c) {
                 const PrimRecord pr = load_prim(sc.prims, act[c]);
                 const float te = exit_from_inside(pr, o, d);
-                if (te > 0.0f && te < kTMax) full = tau_add(acc, optical_depth(pr, o, d, 0.0f, te));
+                if (te > 0.0f && te < kTMax) {
+                    if (tau_add(acc, optical_depth(pr, o, d, 0.0f, te))) acc = kTauLimitFixed;
+                    full = acc >= rr.limit;
+                }
             }
             lds_store3(wq.ro + 3 * lane, o);
             lds_store3(wq.rd + 3 * lane, d);
-            wq.acc[lane] = full ? kTauLimitFixed : acc;
+            wq.acc[lane] = acc;
             wq.prims[lane] = sc.prims;
             inv = safe_inv(d);
             oi = o * inv;
@@ -628,9 +653,8 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
         wave_sync();
         if (have) {
             const unsigned long long acc = wq.acc[lane];
-            const bool full = acc >= kTauLimitFixed;
-            if (full || w4w.n == kWalkEnd) {
-                w.pd[e].w = full ? 0.0f : tau_to_T(acc);
+            if (acc >= rr.limit || w4w.n == kWalkEnd) {
+                w.pd[e].w = shadow_rr_T(acc, rr, depth_T(acc));
                 have = false;
             }
         }

@@ -72,8 +72,9 @@ T try_unwrap_or_exit(Result<T>&& res) {
 // types and defaults, so a caller that fills the reference's Config compiles
 // against this one.  The OptiX program and module names have no meaning here
 // (there is no SBT and no module); they are kept for source compatibility
-// and ignored.  active_cap_ is this build's: the reference fixes its
-// active-set capacity at compile time (device/core/constants.cuh:45-48).
+// and ignored.  active_cap_ and shadow_rr_ are this build's: the reference
+// fixes its active-set capacity at compile time
+// (device/core/constants.cuh:45-48) and has no shadow-ray roulette.
 struct Config {
     std::string raygen_function_name_ = "__raygen__rg";
     std::string miss_function_name_ = "__miss__ms";
@@ -102,12 +103,15 @@ struct Config {
     bool measure_caps_ = false;
 
     uint32_t active_cap_ = 128;
+    // NEE shadow-ray roulette threshold (0 = off, the reference's estimator);
+    // this build's option, --shadow-rr
+    float shadow_rr_ = 0.0f;
 
     // Config::parse (src/thesis/host/app/config.cpp:11-69): the same options
     // (--output --height --width --samples_per_pixel --raygen --miss --anyhit
     // --launch_params --module_blob --env_map --seed --denoise --max-depth
     // --rr-depth --rr-max-survival --firefly-clamp --filter-width --phase-g
-    // --measure-caps, plus --ris --ris-candidates --active-cap), the same
+    // --measure-caps, plus --ris --ris-candidates --active-cap --shadow-rr), the same
     // range checks, an error Result on a bad argument, never an exception.
     [[nodiscard]] static utils::Result<Config> parse(int argc, char* argv[]) noexcept;
 };

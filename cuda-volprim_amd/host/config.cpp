@@ -19,7 +19,7 @@ const char* kUsage =
     "File paths:        --module_blob PATH (accepted, unused) --env_map PATH\n"
     "Runtime tweaks:    --seed N --denoise\n"
     "Render parameters: --max-depth N --rr-depth N --rr-max-survival X --firefly-clamp X --filter-width X\n"
-    "                   --phase-g X --measure-caps --ris --ris-candidates N --active-cap N\n";
+    "                   --phase-g X --measure-caps --ris --ris-candidates N --active-cap N --shadow-rr X\n";
 
 bool to_size(const std::string& v, size_t& out) {
     if (v.empty() || v[0] == '-') return false;
@@ -67,7 +67,7 @@ utils::Result<Config> Config::parse(int argc, char* argv[]) noexcept {
                 "--output", "--env_map", "--module_blob", "--raygen", "--miss", "--anyhit", "--launch_params",
                 "--height", "--width", "--samples_per_pixel", "--seed", "--max-depth", "--rr-depth",
                 "--ris-candidates", "--active-cap", "--rr-max-survival", "--firefly-clamp", "--filter-width",
-                "--phase-g"};
+                "--phase-g", "--shadow-rr"};
             bool known = false;
             for (const char* k : kValued) known = known || a == k;
             if (!known)
@@ -102,7 +102,7 @@ utils::Result<Config> Config::parse(int argc, char* argv[]) noexcept {
                 else if (a == "--ris-candidates") c.ris_num_candidates_ = static_cast<uint32_t>(n);
                 else c.active_cap_ = static_cast<uint32_t>(n);
             } else if (a == "--rr-max-survival" || a == "--firefly-clamp" || a == "--filter-width" ||
-                       a == "--phase-g") {
+                       a == "--phase-g" || a == "--shadow-rr") {
                 if (!to_float(val, f)) return bad("not a number");
                 if (a == "--rr-max-survival") {
                     if (!(f >= 0.0f && f <= 1.0f)) return bad("not in [0, 1]");  // CLI::Range(0, 1)
@@ -113,6 +113,9 @@ utils::Result<Config> Config::parse(int argc, char* argv[]) noexcept {
                 } else if (a == "--filter-width") {
                     if (!(f >= 0.0f)) return bad("must be non-negative");
                     c.pixel_filter_stddev_ = f;
+                } else if (a == "--shadow-rr") {
+                    if (!(f >= 0.0f)) return bad("must be non-negative");
+                    c.shadow_rr_ = f;
                 } else {
                     if (!(f >= -0.999f && f <= 0.999f)) return bad("not in [-0.999, 0.999]");  // CLI::Range
                     c.hg_g_ = f;

@@ -36,6 +36,7 @@ struct TestConfig {
     bool measure_caps = false;
     std::string env;
     uint32_t active_cap = 128;
+    float shadow_rr = 0.0f;
 };
 
 void usage() {
@@ -59,7 +60,8 @@ void usage() {
         "  --ris-candidates N      RIS candidate count K (6)\n"
         "  --measure-caps          measure max hits/ray and point overlap, print suggested caps\n"
         "  --env FILE.hdr          environment map override\n"
-        "  --active-cap N          active-set capacity (128)\n");
+        "  --active-cap N          active-set capacity (128)\n"
+        "  --shadow-rr F           shadow-ray roulette threshold, 0 = off (unbiased when > 0)\n");
 }
 
 bool parse(int argc, char** argv, TestConfig& c, std::string& err) {
@@ -126,6 +128,7 @@ bool parse(int argc, char** argv, TestConfig& c, std::string& err) {
         else if (a == "--measure-caps") c.measure_caps = true;
         else if (a == "--env") { const char* v = next("--env"); if (!v) return false; c.env = v; }
         else if (a == "--active-cap") { if (!num_u("--active-cap", c.active_cap, true)) return false; }
+        else if (a == "--shadow-rr") { if (!num_f("--shadow-rr", c.shadow_rr, 0.0f, 3e38f)) return false; }
         else {
             err = "unknown argument '" + a + "'";
             return false;
@@ -154,6 +157,7 @@ vpapp::Config renderer_config(const TestConfig& t, uint32_t w, uint32_t h, const
     c.measure_caps_ = t.measure_caps;
     c.env_map_path_ = env;
     c.active_cap_ = t.active_cap;
+    c.shadow_rr_ = t.shadow_rr;
     return c;
 }
 

@@ -12,6 +12,7 @@
 //    "width": 256, "height": 256, "spp": 16, "seed": 42, "max_depth": 128,
 //    "rr_depth": 12, "rr_max_survival": 0.99, "phase_g": 0.85, "ris": false,
 //    "ris_candidates": 6, "firefly_clamp": 0, "filter_width": 0, "active_cap": 128,
+//    "shadow_rr": 0 (shadow-ray roulette threshold, 0 = off),
 //    "camera": {"eye": [0,0,-3.5], "target": [0,0,0], "up": [0,1,0], "fov": 40}
 //              or {"eye": ..., "target": ..., "up": ..., "ortho_height": 6},
 //    "output": "out.exr"}
@@ -252,14 +253,14 @@ class Service {
         if (!output || output->kind != Json::Str) return err = "field 'output' (an EXR path) is required", false;
         P->output = output->s;
         double sigma = 7.5, width = 256, height = 256, spp = 16, seed = 42, max_depth = 128, rr_depth = 12;
-        double rr_surv = 0.99, g = 0.85, ris_k = 6, clamp = 0, filt = 0, cap = 128;
+        double rr_surv = 0.99, g = 0.85, ris_k = 6, clamp = 0, filt = 0, cap = 128, srr = 0;
         if (!num(r, "sigma_multiplier", sigma, err, 1e-30, 3e38) || !num(r, "width", width, err, 1, 1024) ||
             !num(r, "height", height, err, 1, 1024) || !num(r, "spp", spp, err, 1, 1 << 20) ||
             !num(r, "seed", seed, err, 0, 4294967295.0) || !num(r, "max_depth", max_depth, err, 1, 1 << 20) ||
             !num(r, "rr_depth", rr_depth, err, 0, 1 << 20) || !num(r, "rr_max_survival", rr_surv, err, 0, 1) ||
             !num(r, "phase_g", g, err, -0.999, 0.999) || !num(r, "ris_candidates", ris_k, err, 1, 64) ||
             !num(r, "firefly_clamp", clamp, err, 0, 3e38) || !num(r, "filter_width", filt, err, 0, 3e38) ||
-            !num(r, "active_cap", cap, err, 1, 4096))
+            !num(r, "active_cap", cap, err, 1, 4096) || !num(r, "shadow_rr", srr, err, 0, 3e38))
             return false;
         bool ris = false;
         if (const Json* x = r.get("ris")) ris = x->kind == Json::Bool && x->b;
@@ -302,6 +303,7 @@ class Service {
         job.params.firefly_clamp = static_cast<float>(clamp);
         job.params.filter_stddev = static_cast<float>(filt);
         job.params.active_cap = static_cast<uint32_t>(cap);
+        job.params.shadow_rr = static_cast<float>(srr);
         P->image.assign(static_cast<size_t>(P->w) * P->h * 4, 0.0f);
         job.out_rgba = P->image.data();
         if (vp_server_submit(srv_, &job, &P->ticket) != VP_OK) return err = vp_last_error(), false;

@@ -48,7 +48,7 @@ class RenderParams(ctypes.Structure):
                 ("rr_max_survival", ctypes.c_float), ("firefly_clamp", ctypes.c_float),
                 ("filter_stddev", ctypes.c_float), ("phase_g", ctypes.c_float),
                 ("use_ris", ctypes.c_uint32), ("ris_candidates", ctypes.c_uint32),
-                ("measure_caps", ctypes.c_uint32), ("active_cap", ctypes.c_uint32)]
+                ("measure_caps", ctypes.c_uint32), ("active_cap", ctypes.c_uint32), ("shadow_rr", ctypes.c_float)]
 
 
 class RenderStats(ctypes.Structure):

@@ -86,6 +86,9 @@ typedef struct {
     uint32_t ris_candidates;  /* --ris-candidates (default 6) */
     uint32_t measure_caps;    /* --measure-caps */
     uint32_t active_cap;      /* origin/scatter active-set capacity (0 = 128) */
+    float shadow_rr;          /* --shadow-rr: NEE shadow-ray roulette threshold eps (0 = off, the reference's
+                                 estimator); a shadow ray whose term has fallen below eps survives with
+                                 probability 1/4 and counts 4x (unbiased; no counterpart in the reference) */
 } vp_render_params;
 
 typedef struct {

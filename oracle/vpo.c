@@ -530,15 +530,21 @@ static inline int tau_add(uint64_t* acc, float od) {
     return *acc >= TAU_LIMIT_FIXED;
 }
 
-static float transmittance(const prim_t* prims, long n, v3 o, v3 d, const uint32_t* set, int nset, counters_t* k,
-                           const cbvh_t* B) {
+/* Fixed-point optical depth along a ray (sampling.cuh:548-583): origin set
+ * (optical_depth from the origin to each member's exit) plus every entered
+ * chord; the sum stops once it reaches `limit` (at most the tau = 88
+ * saturation), which is order independent because every term is >= 0. */
+static uint64_t shadow_depth(const prim_t* prims, long n, v3 o, v3 d, const uint32_t* set, int nset, counters_t* k,
+                             const cbvh_t* B, uint64_t limit) {
     uint64_t acc = 0;
+    if (acc >= limit) return acc;
     for (int s = 0; s < nset; ++s) {
         const prim_t* pr = prims + set[s];
         float te = exit_inside(pr, o, d);
         if (k) k->n_sh_active++;
         if (te > 0.0f && te < INF_F) {
-            if (tau_add(&acc, optical_depth(pr, o, d, 0.0f, te))) return 0.0f;
+            if (tau_add(&acc, optical_depth(pr, o, d, 0.0f, te))) return TAU_LIMIT_FIXED;
+            if (acc >= limit) return acc;
         }
     }
     /* entered primitives: every index (brute force) or the BVH's leaves along the ray */
@@ -548,10 +554,57 @@ static float transmittance(const prim_t* prims, long n, v3 o, v3 d, const uint32
         crossing_t x = crossing(prims + j, o, d);
         if (x.t0 < 0.0f) continue;
         if (k) k->n_sh_hit++;
-        if (tau_add(&acc, x.tau)) return 0.0f;
+        if (tau_add(&acc, x.tau)) return TAU_LIMIT_FIXED;
+        if (acc >= limit) return acc;
     }
+    return acc;
+}
+
+static inline float depth_to_T(uint64_t acc) {
+    if (acc >= TAU_LIMIT_FIXED) return 0.0f;
     float tau = (float)acc * (1.0f / 1099511627776.0f);
     return tau >= 88.0f ? 0.0f : vpo_expf(-tau);
+}
+
+static float transmittance(const prim_t* prims, long n, v3 o, v3 d, const uint32_t* set, int nset, counters_t* k,
+                           const cbvh_t* B) {
+    return depth_to_T(shadow_depth(prims, n, o, d, set, nset, k, B, TAU_LIMIT_FIXED));
+}
+
+/* Shadow-ray roulette (an option of this build; the reference walks every
+ * shadow ray to the tau = 88 saturation).  A shadow ray whose contribution
+ * is w * T(tau) plays a Russian roulette once its optical depth passes
+ * tau_c = log(w / eps), where w e^(-tau) has fallen below eps: it survives
+ * with probability 1/4 and then counts 4 T.  The expected value is T, so the
+ * estimator stays unbiased; the added variance per ray is at most
+ * 3 eps^2.  Whether tau passes tau_c does not depend on the order chords are
+ * summed in, and the survival draw is a counter hash of (path, bounce, ray),
+ * so the walk can stop at tau_c when the ray does not survive. */
+static inline uint64_t rr_threshold(float w, float eps) {
+    if (!(eps > 0.0f)) return TAU_LIMIT_FIXED;
+    if (!(w > eps)) return 0;
+    float tc = vpo_logf(w * vpo_rcp(eps));
+    if (!(tc > 0.0f)) return 0;
+    if (tc >= 88.0f) return TAU_LIMIT_FIXED;
+    return ((uint64_t)(tc * 1099511627776.0f)) & ~(uint64_t)0xffff;
+}
+static inline float rr_uniform(uint64_t key, uint32_t bounce, uint32_t ray) {
+    uint64_t x = (key ^ 0xd1b54a32d192ed03ull) + 0x9e3779b97f4a7c15ull * (((uint64_t)bounce << 8) | ray);
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return u01((uint32_t)(x >> 32));
+}
+static float shadow_T(const prim_t* prims, long n, v3 o, v3 d, const uint32_t* set, int nset, counters_t* k,
+                      const cbvh_t* B, float w, float eps, uint64_t key, uint32_t bounce, uint32_t ray) {
+    uint64_t thr = rr_threshold(w, eps);
+    int survive = thr < TAU_LIMIT_FIXED && rr_uniform(key, bounce, ray) < 0.25f;
+    uint64_t acc = shadow_depth(prims, n, o, d, set, nset, k, B, survive ? TAU_LIMIT_FIXED : thr);
+    float T = depth_to_T(acc);
+    if (acc >= thr) T = survive ? T * 4.0f : 0.0f;
+    return T;
 }
 
 /* Rays with origin-inside set = every primitive whose bound contains the
@@ -921,6 +974,7 @@ typedef struct {
     uint32_t sample_begin, sample_end; /* samples [begin, end) of spp */
     uint32_t stride;                   /* render every stride-th row and column of the crop (0 = 1) */
     uint32_t use_bvh;                  /* queries through a CPU BVH instead of loops over all primitives */
+    float shadow_rr;                   /* NEE shadow-ray roulette threshold (0 = off) */
 } vpo_params;
 
 typedef struct {
@@ -1159,9 +1213,10 @@ static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, 
                 float lumy = vdot(yenv, V(0.2126f, 0.7152f, 0.0722f));
                 if (lumy > 0.0f) {
                     k->n_shadow_rays++;
-                    float T = transmittance(prims, n, x, ydir, act, na, k, B);
                     float Wr = wsum * vpo_rcp((float)P->ris_candidates);
-                    L = vadd(L, vscale(vscale(vmul(base, vscale(yenv, vpo_rcp(lumy))), T), Wr));
+                    v3 cy = vmul(base, vscale(yenv, vpo_rcp(lumy)));
+                    float T = shadow_T(prims, n, x, ydir, act, na, k, B, vmax3(cy) * Wr, P->shadow_rr, key, bounce, 0u);
+                    L = vadd(L, vscale(vscale(cy, T), Wr));
                 }
             }
         } else {
@@ -1172,8 +1227,9 @@ static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, 
             float wa = sa > 0.0f ? apdf / sa : 0.0f;
             v3 enva = env_lookup(&J->env, awo);
             k->n_shadow_rays++;
-            float Ta = transmittance(prims, n, x, awo, act, na, k, B);
-            L = vadd(L, vscale(vscale(vmul(base, enva), Ta), wa));
+            v3 ca = vmul(base, enva);
+            float Ta = shadow_T(prims, n, x, awo, act, na, k, B, vmax3(ca) * wa, P->shadow_rr, key, bounce, 0u);
+            L = vadd(L, vscale(vscale(ca, Ta), wa));
 
             float bpdf;
             v3 bwo = env_sample(&J->env, &rng, &bpdf);
@@ -1182,8 +1238,10 @@ static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, 
             float wb = sb > 0.0f ? bpdf / sb : 0.0f;
             v3 envb = env_lookup(&J->env, bwo);
             k->n_shadow_rays++;
-            float Tb = transmittance(prims, n, x, bwo, act, na, k, B);
-            L = vadd(L, vscale(vscale(vscale(vscale(vmul(base, envb), Tb), wb), ph_b), vpo_rcp(bpdf)));
+            v3 cb = vmul(base, envb);
+            float Tb = shadow_T(prims, n, x, bwo, act, na, k, B, ((vmax3(cb) * wb) * ph_b) * vpo_rcp(bpdf),
+                                P->shadow_rr, key, bounce, 1u);
+            L = vadd(L, vscale(vscale(vscale(vscale(cb, Tb), wb), ph_b), vpo_rcp(bpdf)));
         }
         beta = vmul(beta, albedo);
 

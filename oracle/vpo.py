@@ -37,7 +37,7 @@ class Params(ctypes.Structure):
                 ("x0", ctypes.c_uint32), ("y0", ctypes.c_uint32),
                 ("x1", ctypes.c_uint32), ("y1", ctypes.c_uint32),
                 ("sample_begin", ctypes.c_uint32), ("sample_end", ctypes.c_uint32),
-                ("stride", ctypes.c_uint32), ("use_bvh", ctypes.c_uint32)]
+                ("stride", ctypes.c_uint32), ("use_bvh", ctypes.c_uint32), ("shadow_rr", ctypes.c_float)]
 
 
 def build():
@@ -226,7 +226,7 @@ def camera_orthographic(w, h, eye, target, up, ortho_h):
 def render(prims16, env_rgba, cam_pod, width, height, spp, seed=42, max_depth=128, rr_depth=12,
            rr_max_survival=0.99, firefly_clamp=0.0, filter_stddev=0.0, phase_g=0.85,
            use_ris=False, ris_candidates=6, active_cap=128, measure=False, crop=None,
-           samples=None, keep_samples=False, threads=None, env_tables_=None, stride=1, bvh=False):
+           samples=None, keep_samples=False, threads=None, env_tables_=None, stride=1, bvh=False, shadow_rr=0.0):
     """CPU restatement of the estimator.  Returns (mean[h,w,4], samples or None,
     counters dict).  crop = (x0, y0, x1, y1); samples = (begin, end).
     bvh=True answers the ray and point queries through a CPU BVH instead of
@@ -241,7 +241,7 @@ def render(prims16, env_rgba, cam_pod, width, height, spp, seed=42, max_depth=12
     s0, s1 = samples if samples is not None else (0, spp)
     p = Params(width, height, spp, seed, max_depth, rr_depth, rr_max_survival, firefly_clamp,
                filter_stddev, phase_g, int(use_ris), ris_candidates, active_cap, int(measure),
-               x0, y0, x1, y1, s0, s1, stride, int(bvh))
+               x0, y0, x1, y1, s0, s1, stride, int(bvh), float(shadow_rr))
     mean = np.zeros((y1 - y0, x1 - x0, 4), np.float32)
     smp = np.zeros((s1 - s0, y1 - y0, x1 - x0, 4), np.float32) if keep_samples else None
     counters = np.zeros(len(COUNTER_NAMES), np.uint64)

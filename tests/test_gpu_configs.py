@@ -48,7 +48,7 @@ def oracle_render(rec, env, cam, p, tables=None, **kw):
     ref, _, k = vpo.render(rec, env, cam.pod(), p.width, p.height, p.spp, seed=p.seed, max_depth=p.max_depth,
                            rr_depth=p.rr_depth, rr_max_survival=p.rr_max_survival, firefly_clamp=p.firefly_clamp,
                            filter_stddev=p.filter_stddev, phase_g=p.phase_g, use_ris=bool(p.use_ris),
-                           ris_candidates=p.ris_candidates, active_cap=p.active_cap, threads=THREADS,
+                           ris_candidates=p.ris_candidates, active_cap=p.active_cap, threads=THREADS, shadow_rr=p.shadow_rr,
                            env_tables_=tables, **kw)
     return ref, k
 

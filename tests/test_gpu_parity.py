@@ -151,7 +151,7 @@ def _render_both(ctx, rows, env, cam, **kw):
     ref, _, k = vpo.render(rec, env, cam.pod(), p.width, p.height, p.spp, seed=p.seed, max_depth=p.max_depth,
                            rr_depth=p.rr_depth, rr_max_survival=p.rr_max_survival, firefly_clamp=p.firefly_clamp,
                            filter_stddev=p.filter_stddev, phase_g=p.phase_g, use_ris=bool(p.use_ris),
-                           ris_candidates=p.ris_candidates, active_cap=p.active_cap)
+                           ris_candidates=p.ris_candidates, active_cap=p.active_cap, shadow_rr=p.shadow_rr)
     return img, ref, st, k
 
 
@@ -165,6 +165,12 @@ RENDER_CASES = {
                              kw=dict(phase_g=0.0, filter_stddev=0.7, firefly_clamp=5.0, rr_depth=2)),
     "small_cap": dict(rows=lambda: scenes.random_scene(80, 8, smax=0.5), env="const", w=16, h=16, spp=2,
                       kw=dict(active_cap=2)),
+    # shadow-ray roulette on (the build's option): the walks that stop at the
+    # threshold must give the oracle's full-walk result bit for bit
+    "mis_rr": dict(rows=lambda: scenes.random_scene(90, 21, smax=0.4, albedo=0.95), env="sky", w=24, h=20, spp=4,
+                   kw=dict(shadow_rr=0.2, phase_g=0.3)),
+    "ris_rr": dict(rows=lambda: scenes.random_scene(90, 22, smax=0.4, albedo=0.95), env="sky", w=20, h=20, spp=4,
+                   kw=dict(shadow_rr=0.2, use_ris=1, ris_candidates=4)),
 }
 
 
@@ -199,7 +205,7 @@ def test_render_wavefront_bitexact(ctx, case):
     ref, _, k = vpo.render(rec, env, cam.pod(), p.width, p.height, p.spp, seed=p.seed, max_depth=p.max_depth,
                            rr_depth=p.rr_depth, rr_max_survival=p.rr_max_survival, firefly_clamp=p.firefly_clamp,
                            filter_stddev=p.filter_stddev, phase_g=p.phase_g, use_ris=bool(p.use_ris),
-                           ris_candidates=p.ris_candidates, active_cap=p.active_cap)
+                           ris_candidates=p.ris_candidates, active_cap=p.active_cap, shadow_rr=p.shadow_rr)
     assert_bitwise(imgs[0], ref, f"wavefront render {case}")
     assert st["active_overflow"] == k["active_overflow"]
 
@@ -375,7 +381,7 @@ def test_render_sah_bitexact(ctx, case, bvh):
     ref, _, k = vpo.render(rec, env, cam.pod(), p.width, p.height, p.spp, seed=p.seed, max_depth=p.max_depth,
                            rr_depth=p.rr_depth, rr_max_survival=p.rr_max_survival, firefly_clamp=p.firefly_clamp,
                            filter_stddev=p.filter_stddev, phase_g=p.phase_g, use_ris=bool(p.use_ris),
-                           ris_candidates=p.ris_candidates, active_cap=p.active_cap)
+                           ris_candidates=p.ris_candidates, active_cap=p.active_cap, shadow_rr=p.shadow_rr)
     assert_bitwise(img, ref, f"{bvh} render {case}")
     assert st["active_overflow"] == k["active_overflow"]
 

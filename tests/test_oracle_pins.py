@@ -483,6 +483,42 @@ def test_mis_and_ris_agree_in_expectation():
     assert abs(ma - mb) < 4 * se + 1e-4
 
 
+def test_shadow_roulette_is_unbiased_and_stops_walks():
+    # The shadow-ray roulette (shadow_rr = eps) keeps the estimator's
+    # expectation: the image mean with eps = 0.3 matches eps = 0 within the
+    # Monte Carlo error, on independent samples (different seeds), while the
+    # shadow walks enter fewer primitives.  A roulette that dropped the 4x
+    # survivor weight would darken the image by the killed share of the
+    # direct light and fail the comparison.
+    rows = scenes.random_scene(90, 21, smax=0.4, albedo=0.95)
+    rec = vpo.make_prims(rows)[vpo.scene_order(rows)]
+    env = _env()
+    tabs = vpo.env_tables(env)
+    kw = dict(max_depth=24, rr_depth=6, phase_g=0.3, env_tables_=tabs, keep_samples=True)
+    a, sa, ka = vpo.render(rec, env, _cam(12, 12), 12, 12, 512, seed=1, **kw)
+    b, sb, kb = vpo.render(rec, env, _cam(12, 12), 12, 12, 512, seed=2, shadow_rr=0.3, **kw)
+    assert kb["n_sh_hit"] < 0.9 * ka["n_sh_hit"], (ka["n_sh_hit"], kb["n_sh_hit"])
+    for c in range(3):
+        xa, xb = sa[..., c].astype(np.float64), sb[..., c].astype(np.float64)
+        se = math.sqrt(xa.var() / xa.size + xb.var() / xb.size)
+        assert abs(xa.mean() - xb.mean()) < 4 * se, (c, xa.mean(), xb.mean(), se)
+    # per pixel as well (3 sigma of the difference, allowing 1 % outliers)
+    d = (sa[..., 1].mean(0) - sb[..., 1].mean(0)).astype(np.float64)
+    s = np.sqrt((sa[..., 1].var(0) + sb[..., 1].var(0)) / sa.shape[0])
+    assert np.mean(np.abs(d) > 3 * s + 1e-6) < 0.02
+
+
+def test_shadow_roulette_keeps_the_furnace():
+    rows = scenes.random_scene(24, 12, albedo=1.0)
+    rows[:, 10:13] = 1.0
+    rec = vpo.make_prims(rows)[vpo.scene_order(rows)]
+    img, _, k = vpo.render(rec, scenes.constant_env(), _cam(24, 24), 24, 24, 24, max_depth=1000, rr_depth=1000,
+                           phase_g=0.5, shadow_rr=0.05)
+    m = img[..., 0].astype(np.float64)
+    assert k["n_scatter"] > 5000
+    assert abs(m.mean() - 1.0) < max(5 * m.std() / math.sqrt(m.size), 2e-3)
+
+
 def test_argmin_free_flight_distribution():
     # The argmin of per-primitive closed-form free flights must sample the
     # first collision of the whole mixture: P[t* > t] = exp(-tau(0, t)) with
