@@ -4,15 +4,18 @@ Default: the served request stream.  Render requests drawn from the batched
 job mix (resolution uniform over {64,...,1024}^2, spp round(exp U[ln 4,
 ln 256]) divided by --spp-div, scene uniform over the 4096-Gaussian
 scattering scene, the 10^6-Gaussian cloud and the 2.5*10^5-Gaussian torus,
-camera on a sphere of radius 3.5) arrive as a Poisson stream at an offered
-load of the batched capacity; one step is one request (--steps requests are
-timed, --warmup smaller ones go first).  value = the served aggregate
-Msamples/s from the first submission to the last result, with scenes, BVHs
-and environment tables resident.  Beside it: plain one-launch-per-request
+camera on a sphere of radius 3.5) go to the server all at once; one step is
+one request (--steps requests are timed, --warmup smaller ones go first).
+value = the served aggregate Msamples/s from the first submission to the
+last result, with scenes, BVHs and environment tables resident.  The same
+requests then arrive as a Poisson stream at --load times that capacity for
+the p50/p99 latencies.  Requests use the unbiased shadow-ray
+roulette at eps = 1e-3 (--shadow-rr; 0 is the reference's estimator, which
+is also served once for comparison).  Beside it: plain one-launch-per-request
 rendering and one batched launch of the same requests (all three images are
 checked bit-identical), p50/p99 latency, the roofline of the batched
 k_render launch (FP32 VALU: algorithmic FLOPs from the event-counting kernel
-variant weighted as documented in DESIGN.md, over the launch's kernel time
+variant with per-event weights derived from the reference's formulas, over the launch's kernel time
 from HIP events on the library stream), a per-scene image-error estimate and
 the CPU baseline (the oracle's C restatement of the estimator with its CPU
 BVH, on a bounded pixel sample, all host cores).
@@ -40,7 +43,7 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_FP32_NONPACKED_TFLOPS = 78.6
 
 # FLOP weight of each counted event: FMA = 2, add/mul = 1, each transcendental
-# or rcp/sqrt = 1, erf = 16, erfinv = 20 (derivations in DESIGN.md).  A node
+# or rcp/sqrt = 1, erf = 16, erfinv = 20 (from the reference's formulas).  A node
 # visit loads one 4-wide walk-tree record and slab-tests its 4 child boxes
 # (6 FMA each).
 FLOP_WEIGHTS = {
@@ -296,17 +299,33 @@ def run_stream(args):
     print(f"[stream] batched {t_batch:.2f} s (autotune {t_tune:.1f} s, "
           f"{'wavefront' if tuned.pipeline else 'megakernel'})", file=sys.stderr, flush=True)
 
-    # served: a Poisson stream at the stated offered load of the batched
-    # capacity (load 1: requests arrive as fast as the batched form finishes
-    # them, so the GPU has work but the queue does not start full)
-    lam = args.load * K / t_batch
-    arrivals = np.cumsum(np.random.default_rng(args.mix_seed + 1).exponential(1.0 / lam, size=K))
-    arrivals -= arrivals[0]
-    t_served, lat, sst, served_imgs = serve(ctx, reqs, arrivals, mem_gib=args.mem_gib)
-    print(f"[stream] served {t_served:.2f} s, {sst['launches']} launches", file=sys.stderr, flush=True)
+    # served, saturated: every request submitted at once, so the server's
+    # continuous batching, not the arrival process, sets the pace (the value)
+    t_served, _, sst, served_imgs = serve(ctx, reqs, np.zeros(K), mem_gib=args.mem_gib)
+    print(f"[stream] served (saturated) {t_served:.2f} s, {sst['launches']} launches", file=sys.stderr, flush=True)
     same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) and
                np.array_equal(a.view(np.uint32), c.view(np.uint32))
                for a, b, c in zip(plain_imgs, batch_imgs, served_imgs))
+
+    # served, Poisson arrivals at the stated offered load of that capacity:
+    # the request latencies (p50/p99) under load
+    lam = args.load * K / t_served
+    arrivals = np.cumsum(np.random.default_rng(args.mix_seed + 1).exponential(1.0 / lam, size=K))
+    arrivals -= arrivals[0]
+    t_poisson, lat, pst, poisson_imgs = serve(ctx, reqs, arrivals, mem_gib=args.mem_gib)
+    print(f"[stream] served (Poisson, load {args.load}) {t_poisson:.2f} s", file=sys.stderr, flush=True)
+    same = same and all(np.array_equal(a.view(np.uint32), b.view(np.uint32))
+                        for a, b in zip(plain_imgs, poisson_imgs))
+
+    # the saturated served stream with the reference's estimator (no
+    # shadow-ray roulette), for comparison on the same requests
+    ref_served = None
+    if args.shadow_rr > 0.0 and not args.no_reference:
+        rreqs = mix_requests(volprim, jobs, handles, envh, 0.0)
+        t_ref, _, rst, _ = serve(ctx, rreqs, np.zeros(K), mem_gib=args.mem_gib)
+        ref_served = {"seconds": round(t_ref, 3), "msamples_s": round(samples / t_ref / 1e6, 4),
+                      "launches": rst["launches"], "note": "saturated, shadow_rr 0"}
+        print(f"[stream] served (saturated), reference estimator {t_ref:.2f} s", file=sys.stderr, flush=True)
 
     # roofline of the dominant kernel: k_render over the batched table.
     # Algorithmic FLOPs from the event-counting build run on the same jobs
@@ -349,7 +368,16 @@ def run_stream(args):
             i = next((k for k, j in enumerate(jobs) if j["scene"] == name), None)
             if i is not None:
                 quality[name] = rel_rmse_est(ctx, torch, reqs[i])
-        quality["method"] = "per-pixel sample variance / spp of the first request of each scene"
+                if args.shadow_rr > 0.0:
+                    sc, ev_h, cam, p = reqs[i]
+                    p0 = type(p).from_buffer_copy(p)
+                    p0.shadow_rr = 0.0
+                    quality[name]["rel_rmse_est_reference_estimator"] = rel_rmse_est(ctx, torch,
+                                                                                      (sc, ev_h, cam, p0))[
+                        "rel_rmse_est"]
+        quality["method"] = ("per-pixel sample variance / spp of the first request of each scene (an unbiased "
+                             "estimate of its MSE against the converged image), with the stream's estimator and "
+                             "with the reference's (no shadow-ray roulette)")
 
     cpu = None
     if not args.no_cpu:
@@ -368,11 +396,15 @@ def run_stream(args):
                                + (f" / {args.res_div}" if args.res_div > 1 else "") +
                                f", spp round(exp U[ln 4, ln 256]) / {args.spp_div} (at least 1), scene uniform over "
                                f"scattering 4096 / cloud 10^6 / torus 2.5*10^5, camera on a sphere of radius 3.5, "
-                               f"FOV 40, max depth 64, RR from 12, MIS NEE, 2048x1024 sky + 0.5 deg sun; offered "
-                               f"load {args.load} x the batched capacity",
+                               f"FOV 40, max depth 64, RR from 12, MIS NEE"
+                               + (f" with shadow-ray roulette eps {args.shadow_rr}" if args.shadow_rr > 0 else "")
+                               + ", 2048x1024 sky + 0.5 deg sun; all requests submitted at once (latency from a "
+                                 f"Poisson stream at {args.load} x that capacity)",
                    "requests": K, "samples": samples, "spp_div": args.spp_div, "res_div": args.res_div,
-                   "lambda_req_s": round(lam, 4), "scenes": {k: MIX_SCENES[k] for k in names}},
-        "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "mean": ms(np.mean(lat))},
+                   "lambda_req_s_latency_run": round(lam, 4), "scenes": {k: MIX_SCENES[k] for k in names}},
+        "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "mean": ms(np.mean(lat)),
+                       "note": f"served Poisson stream at load {args.load} x the saturated served capacity "
+                               f"(lambda {lam:.4f} requests/s), submit to result"},
         "speedup_vs_plain": round(t_plain / t_served, 3),
         "plain": {"seconds": round(t_plain, 3), "msamples_s": round(samples / t_plain / 1e6, 4),
                   "p50_ms": pct(plain_ms, 50), "p99_ms": pct(plain_ms, 99),
@@ -384,7 +416,12 @@ def run_stream(args):
                                "regen_min": tuned.regen_min, "pool_paths": tuned.pool_paths},
                     "kernel_ms": round(bst["kernel_ms"], 2)},
         "served": {"seconds": round(t_served, 3), "msamples_s": round(value, 4),
-                   "launches": sst["launches"], "busy_ms": round(sst["busy_ms"], 1)},
+                   "launches": sst["launches"], "busy_ms": round(sst["busy_ms"], 1),
+                   "note": "every request submitted at once; value"},
+        "served_poisson": {"seconds": round(t_poisson, 3), "msamples_s": round(samples / t_poisson / 1e6, 4),
+                           "lambda_req_s": round(lam, 4), "load": args.load, "launches": pst["launches"],
+                           "note": "arrival-bound: the last request arrives near K / lambda"},
+        "served_reference_estimator": ref_served,
         "images_identical": bool(same),
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -456,8 +493,8 @@ def main():
     ap.add_argument("--no-quality", action="store_true", help="skip the image-error estimate")
     ap.add_argument("--spp-div", type=int, default=8, help="stream: every request's spp is divided by this")
     ap.add_argument("--res-div", type=int, default=1, help="stream: every request's resolution is divided by this")
-    ap.add_argument("--load", type=float, default=1.0,
-                    help="stream: Poisson offered load as a multiple of the batched capacity")
+    ap.add_argument("--load", type=float, default=0.7,
+                    help="stream: Poisson offered load as a multiple of the saturated served capacity")
     ap.add_argument("--mix-seed", type=int, default=4)
     ap.add_argument("--mem-gib", type=int, default=16, help="stream: server admission budget")
     ap.add_argument("--single", choices=["scattering", "cloud", "torus"], default=None,
@@ -479,11 +516,13 @@ def main():
                     help="run one 256 MiB elementwise copy kernel first (HBM counter calibration)")
     ap.add_argument("--ris", action="store_true",
                     help="NEE by resampled importance sampling (the reference's --ris) instead of MIS")
-    ap.add_argument("--shadow-rr", type=float, default=0.0,
+    ap.add_argument("--shadow-rr", type=float, default=1e-3,
                     help="shadow-ray roulette threshold eps (0 = off, the reference's estimator)")
+    ap.add_argument("--no-reference", action="store_true",
+                    help="stream: skip the served run with the reference's estimator")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is in DESIGN.md",
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); this build renders on one GPU",
               file=sys.stderr)
         sys.exit(2)
 

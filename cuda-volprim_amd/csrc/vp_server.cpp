@@ -16,9 +16,10 @@
 // not depend on which launch or lane ran which path) and count against its
 // table entry; once the count reaches the request's paths, its accumulator
 // is copied to the host and the mean resolved there while rendering
-// continues.  The side stream that installs requests and copies results only
-// issues copies (DMA engines), never kernels, so it needs no compute unit
-// while the persistent launches hold them all.  Request memory is the
+// continues.  The side stream installs requests and copies results; its
+// copies run as blit kernels on this runtime, so the render streams are
+// masked to every CU but one and their launches sized for that, which keeps
+// one CU free for them while a persistent launch holds the rest.  Request memory is the
 // accumulator (64 bytes per pixel), whatever the sample count; admission is
 // by those bytes in flight and by a limit on requests in flight.
 //
@@ -427,6 +428,24 @@ vp_status vp_server_create(vp_context* ctx, const vp_server_config* cfg, vp_serv
     if (S->cfg.active_cap > cap) cap = S->cfg.active_cap;
     c.active_cap = cap;
     S->L = vp_resolve_launch(ctx, &c, cap, 0, 0);
+    // The render streams leave one CU of each of the 8 XCDs free, and their
+    // launches are sized for the rest: the side stream's copies are blit
+    // kernels on this runtime, and a persistent launch holding every CU kept
+    // them (progress polls, result copies) waiting until it drained, so a
+    // finished request waited for its whole launch.  Workgroups are dealt out
+    // to the XCDs in turn, so a free CU in one XCD alone would not be found by
+    // a copy dealt to another.  The free CUs are 33 k (k = 0..7), one in each
+    // block of 32 CUs and one of each residue mod 8, whichever of the two
+    // numberings maps CUs to XCDs.
+    const uint32_t cus = static_cast<uint32_t>(ctx->prop.multiProcessorCount);
+    std::vector<uint32_t> cu_mask((cus + 31) / 32, 0u);
+    uint32_t free_cus = 0;
+    for (uint32_t cu = 0; cu < cus; ++cu) {
+        const bool spare = cus >= 64 && cus % 8 == 0 && cu % (cus / 8 + 1) == 0 && cu / (cus / 8 + 1) < 8;
+        if (spare) ++free_cus;
+        else cu_mask[cu / 32] |= 1u << (cu % 32);
+    }
+    if (free_cus > 0 && S->L.grid >= cus) S->L.grid = S->L.grid / cus * (cus - free_cus);
     const size_t slab_words = vp::kMegaSlabBuffers * static_cast<size_t>(cap) * S->L.grid * S->L.block;
     bool ok = hipMalloc(&S->jobs_dev, sizeof(vp::JobDesc) * kServerTable) == hipSuccess &&
               hipMalloc(&S->begin_dev, sizeof(uint64_t) * kServerTable) == hipSuccess &&
@@ -444,7 +463,18 @@ vp_status vp_server_create(vp_context* ctx, const vp_server_config* cfg, vp_serv
          hipHostMalloc(&S->stage, S->max_npix * kRequestBytesPerPixel, hipHostMallocDefault) == hipSuccess &&
          hipHostMalloc(&S->zeros, S->max_npix * kRequestBytesPerPixel, hipHostMallocDefault) == hipSuccess;
     if (ok) std::memset(S->zeros, 0, S->max_npix * kRequestBytesPerPixel);
-    for (auto& k : S->ks) ok = ok && hipStreamCreateWithFlags(&k, hipStreamNonBlocking) == hipSuccess;
+    {
+        for (auto& k : S->ks) {
+            if (!ok) break;
+            if (free_cus == 0 ||
+                hipExtStreamCreateWithCUMask(&k, static_cast<uint32_t>(cu_mask.size()), cu_mask.data()) !=
+                    hipSuccess) {
+                (void)hipGetLastError();
+                k = nullptr;
+                ok = hipStreamCreateWithFlags(&k, hipStreamNonBlocking) == hipSuccess;
+            }
+        }
+    }
     ok = ok && hipStreamCreateWithFlags(&S->side, hipStreamNonBlocking) == hipSuccess;
     for (auto& e : S->kev) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
     ok = ok && hipEventCreateWithFlags(&S->install_ev, hipEventDisableTiming) == hipSuccess &&

@@ -28,7 +28,7 @@
  *     divided by their count (accum_add / accum_mean), where the reference
  *     folds a running Welford mean in sample order (raygen.cuh:214-238): same
  *     estimator, rounded once at the end instead of once per sample.
- * Parity status of each function is recorded in DESIGN.md.
+ * Each function is pinned by tests/test_oracle_pins.py.
  *
  * Build: oracle/Makefile (gcc -O2 -ffp-contract=off).
  */
