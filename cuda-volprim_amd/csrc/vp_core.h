@@ -251,6 +251,53 @@ VP_HD Crossing crossing(const PrimRecord& pr, f3 o, f3 d) {
 // Entry distance alone (negative when the ray does not enter).
 VP_HD float entry_t(const PrimRecord& pr, f3 o, f3 d) { return crossing(pr, o, d).t0; }
 
+// e^(-9/2)
+constexpr float kExpHalfExtent2 = 0x1.6c0504p-7f;
+
+// The chord of a ray whose origin lies inside the 3 sigma bound, from the
+// origin to the exit, from one transform: the exit of exit_from_inside
+// (intersection.h:115-126) and the optical depth over [0, exit]
+// (optical_depth, primitive.h:249-318).  With b = p.w, u0 = b/|w| the
+// origin's coordinate along the ray from the closest point q0,
+// delta = 9 - |q0|^2 = (b^2 - a c)/a and |p|^2 = |q0|^2 + u0^2, the depth
+//   M dnf sqrt(pi/2)/|w| e^(-|q0|^2/2) [erf(sqrt(delta/2)) - erf(u0/sqrt 2)]
+// is, by e^(x^2) erf(x) = (2/sqrt(pi)) x S(2x^2),
+//   M dnf [e^(-9/2) (sqrt(a delta)/a) S(delta) - e^(-|p|^2/2) (b/a) S(b^2/a)]
+// with S from chord_poly_: one exp and no erf.  Exits at or below kSegMin (and
+// origins the rounding puts outside) have depth 0, as optical_depth gives.
+struct InsideChord {
+    float t1, tau;
+};
+VP_HD InsideChord inside_chord(const PrimRecord& pr, f3 o, f3 d) {
+    const f3 w = to_local_dir(pr, d);
+    const f3 p = to_local_pos(pr, o);
+    const float a = length2(w);
+    const float b = dot(p, w);
+    const float pp = length2(p);
+    const float c = pp - kExtent2;
+    const float disc = fmaf_(-a, c, b * b);
+    const float sq = sqrtf_(disc);
+    const float a_inv = rcpf(a);
+    InsideChord r;
+    r.t1 = (-b + sq) * a_inv;
+    r.tau = 0.0f;
+    if (r.t1 > kSegMin && r.t1 < kTMax) {
+        const float far = (kExpHalfExtent2 * (sq * a_inv)) * chord_poly_(minf(disc * a_inv, kExtent2));
+        const float near = (expf_(-0.5f * pp) * (b * a_inv)) * chord_poly_(minf((b * b) * a_inv, kExtent2));
+        r.tau = maxf((pr.mass * pr.dnf) * (far - near), 0.0f);
+    }
+    return r;
+}
+
+// Free flight from an origin inside a primitive (argmin over the origin set,
+// sampling.cuh:356-370): it ends inside the chord exactly when the sampled
+// depth tau is at most the chord's; the distance is then inv_cdf
+// (primitive.h:140-186) kept inside [0, exit].  kTMax when it passes.
+VP_HD float origin_flight(const PrimRecord& pr, f3 o, f3 d, const InsideChord& x, float tau) {
+    if (!(tau <= x.tau)) return kTMax;
+    return clampf(inv_cdf(pr, o, d, tau), 0.0f, x.t1);
+}
+
 // Free flight of a ray that entered a primitive (argmin over the entered
 // primitives, sampling.cuh:372-412): with tau the path's optical-depth sample
 // for the primitive, the flight ends inside its chord exactly when tau is at

@@ -450,10 +450,10 @@ __device__ uint64_t shadow_depth(const SceneView& s, f3 o, f3 d, const uint32_t*
     if (acc >= limit) return acc;
     for (uint32_t k = 0; k < na; ++k) {
         const PrimRecord pr = load_prim(s.prims, act[k * stride]);
-        const float te = exit_from_inside(pr, o, d);
+        const InsideChord x = inside_chord(pr, o, d);
         ev.add(kCntShActive);
-        if (te > 0.0f && te < kTMax) {
-            if (tau_add(acc, optical_depth(pr, o, d, 0.0f, te))) return kTauLimitFixed;
+        if (x.tau > 0.0f) {
+            if (tau_add(acc, x.tau)) return kTauLimitFixed;
             if (acc >= limit) return acc;
         }
     }
@@ -611,13 +611,12 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
             const PrimRecord pr = load_prim(wq.prims[ol], set_of[lo][(g - pre_of[lo]) * stride]);
             const f3 xo = lds_load3(wq.ro + 3 * ol);
             const f3 dra = lds_load3(wq.rd + 3 * (2 * ol));
-            const float tea = exit_from_inside(pr, xo, dra);
-            if (tea > 0.0f && tea < kTMax) atomicAdd(&wq.acc[2 * ol], tau_fixed(optical_depth(pr, xo, dra, 0.0f, tea)));
+            const float ta = inside_chord(pr, xo, dra).tau;
+            if (ta > 0.0f) atomicAdd(&wq.acc[2 * ol], tau_fixed(ta));
             if (two) {
                 const f3 drb = lds_load3(wq.rd + 3 * (2 * ol + 1));
-                const float teb = exit_from_inside(pr, xo, drb);
-                if (teb > 0.0f && teb < kTMax)
-                    atomicAdd(&wq.acc[2 * ol + 1], tau_fixed(optical_depth(pr, xo, drb, 0.0f, teb)));
+                const float tb = inside_chord(pr, xo, drb).tau;
+                if (tb > 0.0f) atomicAdd(&wq.acc[2 * ol + 1], tau_fixed(tb));
             }
         }
     }

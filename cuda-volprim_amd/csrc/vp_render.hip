@@ -119,14 +119,15 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
             const uint32_t i = set_of[lo][k];
             const PrimRecord pr = load_prim(wq.prims[ol], i);
             const f3 ro = lds_load3(wq.ro + 3 * ol), rd = lds_load3(wq.rd + 3 * ol);
-            const float te = exit_from_inside(pr, ro, rd);
-            exit_of[lo][k] = te;
+            const InsideChord x = inside_chord(pr, ro, rd);
+            exit_of[lo][k] = x.t1;
             const uint32_t mo = wq.meta[ol];
             const float tau = free_flight_tau(prim_uniform(wq.key[ol], mo & 0x7fffffffu, 0u, i));
-            const float t = inv_cdf(pr, ro, rd, tau);
-            if (t >= 0.0f && t <= te) atomicMin(&wq.ts[ol], t == 0.0f ? 0u : __float_as_uint(t));
-            if ((mo >> 31) != 0u && te > 0.0f && te < kTMax)
-                atomicAdd(&wq.acc[ol], tau_fixed(optical_depth(pr, ro, rd, 0.0f, te)));
+            if (tau <= x.tau) {
+                const float t = origin_flight(pr, ro, rd, x, tau);
+                atomicMin(&wq.ts[ol], t == 0.0f ? 0u : __float_as_uint(t));
+            }
+            if ((mo >> 31) != 0u && x.tau > 0.0f) atomicAdd(&wq.acc[ol], tau_fixed(x.tau));
         }
     }
     wave_sync();
@@ -355,13 +356,11 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
             for (uint32_t k = 0; k < na; ++k) {
                 const uint32_t i = act[k * stride];
                 const PrimRecord pr = load_prim(sc.prims, i);
-                const float te = exit_from_inside(pr, o, d);
-                act_exit[k * stride] = te;
+                const InsideChord x = inside_chord(pr, o, d);
+                act_exit[k * stride] = x.t1;
                 const float tau = free_flight_tau(prim_uniform(st.key, b, 0u, i));
-                const float t = inv_cdf(pr, o, d, tau);
-                if (t >= 0.0f && t < ts && t <= te) ts = t;
-                if (direct && !dfull && te > 0.0f && te < kTMax)
-                    dfull = tau_add(dacc, optical_depth(pr, o, d, 0.0f, te));
+                if (tau <= x.tau) ts = minf(ts, origin_flight(pr, o, d, x, tau));
+                if (direct && !dfull) dfull = tau_add(dacc, x.tau);
             }
         }
         ev.add(kCntArgActive, na);
@@ -681,9 +680,7 @@ __global__ void k_transmittance_rays(SceneView sc, const float* __restrict__ ray
     walk_point(sc, o, visits, [&](uint32_t prim) {
         const PrimRecord pr = load_prim(sc.prims, prim);
         if (full || !point_inside(pr, o)) return;
-        const float te = exit_from_inside(pr, o, d);
-        if (te > 0.0f && te < kTMax)
-            if (tau_add(acc, optical_depth(pr, o, d, 0.0f, te))) full = true;
+        if (tau_add(acc, inside_chord(pr, o, d).tau)) full = true;
     });
     if (!full) {
         const f3 inv = safe_inv(d);
@@ -779,6 +776,8 @@ __global__ void k_prim(int fn, const PrimRecord* __restrict__ rec, const float* 
         case 7: r = prim_pdf(p, o); break;
         case 8: r = crossing(p, o, d).tau; break;
         case 9: r = entered_flight(p, o, d, crossing(p, o, d), a[8]); break;
+        case 10: r = inside_chord(p, o, d).tau; break;
+        case 11: r = origin_flight(p, o, d, inside_chord(p, o, d), a[8]); break;
         default: r = __builtin_nanf("");
     }
     out[i] = r;

@@ -270,10 +270,9 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
             for (uint32_t k = 0; k < na; ++k) {
                 const uint32_t pi = act[k];
                 const PrimRecord pr = load_prim(sc.prims, pi);
-                const float te = exit_from_inside(pr, o, d);
+                const InsideChord x = inside_chord(pr, o, d);
                 const float tau = free_flight_tau(prim_uniform(key, b, 0u, pi));
-                const float t = inv_cdf(pr, o, d, tau);
-                if (t >= 0.0f && t < ts && t <= te) ts = t;
+                if (tau <= x.tau) ts = minf(ts, origin_flight(pr, o, d, x, tau));
             }
             lds_store3(wq.ro + 3 * lane, o);
             lds_store3(wq.rd + 3 * lane, d);
@@ -605,10 +604,9 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
             uint64_t acc = 0;
             bool full = acc >= rr.limit;
             for (uint32_t c = 0; c < count && !full; ++c) {
-                const PrimRecord pr = load_prim(sc.prims, act[c]);
-                const float te = exit_from_inside(pr, o, d);
-                if (te > 0.0f && te < kTMax) {
-                    if (tau_add(acc, optical_depth(pr, o, d, 0.0f, te))) acc = kTauLimitFixed;
+                const float tc = inside_chord(load_prim(sc.prims, act[c]), o, d).tau;
+                if (tc > 0.0f) {
+                    if (tau_add(acc, tc)) acc = kTauLimitFixed;
                     full = acc >= rr.limit;
                 }
             }

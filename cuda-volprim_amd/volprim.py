@@ -19,7 +19,8 @@ _STATUS = {0: "VP_OK", 1: "VP_ERR_INVALID", 2: "VP_ERR_HIP", 3: "VP_ERR_OOM", 4:
 MATH_FNS = {"exp": 0, "log": 1, "sin": 2, "cos": 3, "atan2": 4, "acos": 5, "erf": 6, "erfc": 7,
             "erfinv": 8, "erf_diff": 9, "sqrt": 10, "div": 11, "chord_poly": 12}
 PRIM_FNS = {"inv_cdf": 0, "inv_cdf_span": 1, "optical_depth": 2, "exit_inside": 3, "entry_t": 4,
-            "exit_entry": 5, "point_inside": 6, "pdf": 7, "chord_tau": 8, "entered_flight": 9}
+            "exit_entry": 5, "point_inside": 6, "pdf": 7, "chord_tau": 8, "entered_flight": 9,
+            "inside_tau": 10, "origin_flight": 11}
 
 
 class VpError(RuntimeError):

@@ -261,8 +261,10 @@ vp_status vp_math_eval(vp_context* ctx, int fn, const float* x, const float* y, 
 /* Evaluate primitive function fn (0 inv_cdf 1 inv_cdf_span 2 optical_depth
  * 3 exit_inside 4 entry_t 5 exit_entry 6 point_inside 7 pdf 8 chord_tau (the
  * optical depth of the whole entered chord) 9 entered_flight (free flight of
- * depth tau into the entered primitive; 1e20 when it passes)) for one record
- * over m argument rows (o3 d3 t0 t1 tau). */
+ * depth tau into the entered primitive; 1e20 when it passes) 10 inside_tau
+ * (optical depth from an origin inside the primitive to its exit)
+ * 11 origin_flight (free flight of depth tau from such an origin)) for one
+ * record over m argument rows (o3 d3 t0 t1 tau). */
 vp_status vp_prim_eval(vp_context* ctx, int fn, const float* record16, const float* args9, uint32_t m,
                        float* out);
 

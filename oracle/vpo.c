@@ -209,6 +209,41 @@ static crossing_t crossing(const prim_t* pr, v3 o, v3 d) {
     return r;
 }
 
+/* Chord of a ray whose origin is inside the 3-sigma bound, origin to exit,
+ * from one transform: the exit of exit_inside (intersection.h:115-126) and
+ * optical_depth over [0, exit] (primitive.h:249-318), written with
+ * e^(x^2) erf(x) = (2/sqrt(pi)) x S(2x^2) as
+ *   M dnf [e^(-9/2) (sqrt(a delta)/a) S(delta) - e^(-|p|^2/2) (b/a) S(b^2/a)]
+ * with b = p.w and delta = (b^2 - a c)/a.  Exits <= 1e-6 have depth 0. */
+typedef struct { float t1, tau; } inside_t;
+static inside_t inside_chord(const prim_t* pr, v3 o, v3 d) {
+    v3 w = local_dir(pr, d), p = local_pos(pr, o);
+    float a = vdot(w, w);
+    float b = vdot(p, w);
+    float pp = vdot(p, p);
+    float c = pp - EXTENT2;
+    float disc = fmaf(-a, c, b * b);
+    float sq = sqrtf(disc);
+    float ai = vpo_rcp(a);
+    inside_t r;
+    r.t1 = (-b + sq) * ai;
+    r.tau = 0.0f;
+    if (r.t1 > 1e-6f && r.t1 < INF_F) {
+        float fr = ((CHORD_SCALE * 0.5f) * (sq * ai)) * vpo_chord_poly(vpo_min(disc * ai, EXTENT2));
+        float nr = (vpo_expf(-0.5f * pp) * (b * ai)) * vpo_chord_poly(vpo_min((b * b) * ai, EXTENT2));
+        r.tau = vpo_max((pr->mass * pr->dnf) * (fr - nr), 0.0f);
+    }
+    return r;
+}
+
+/* Free flight from an origin inside the primitive (sampling.cuh:356-370):
+ * inv_cdf (primitive.h:140-186) kept in [0, exit] when the sampled depth is
+ * at most the chord's, INF otherwise. */
+static float origin_flight(const prim_t* pr, v3 o, v3 d, inside_t x, float tau) {
+    if (!(tau <= x.tau)) return INF_F;
+    return vpo_clamp(inv_cdf(pr, o, d, tau), 0.0f, x.t1);
+}
+
 /* Free flight into an entered primitive (sampling.cuh:372-412): it ends in
  * the chord iff the sampled depth is at most the chord's; the distance is
  * inv_cdf_span from the entry (primitive.h:203-246) kept in [entry, exit]
@@ -242,6 +277,8 @@ void vpo_prim_eval(int fn, const float* prim16, const float* args, long m, float
             case 7: out[i] = prim_pdf(p, o); break;
             case 8: out[i] = crossing(p, o, d).tau; break;
             case 9: out[i] = entered_flight(p, o, d, crossing(p, o, d), a[8]); break;
+            case 10: out[i] = inside_chord(p, o, d).tau; break;
+            case 11: out[i] = origin_flight(p, o, d, inside_chord(p, o, d), a[8]); break;
             default: out[i] = NAN;
         }
     }
@@ -539,11 +576,10 @@ static uint64_t shadow_depth(const prim_t* prims, long n, v3 o, v3 d, const uint
     uint64_t acc = 0;
     if (acc >= limit) return acc;
     for (int s = 0; s < nset; ++s) {
-        const prim_t* pr = prims + set[s];
-        float te = exit_inside(pr, o, d);
+        float tc = inside_chord(prims + set[s], o, d).tau;
         if (k) k->n_sh_active++;
-        if (te > 0.0f && te < INF_F) {
-            if (tau_add(&acc, optical_depth(pr, o, d, 0.0f, te))) return TAU_LIMIT_FIXED;
+        if (tc > 0.0f) {
+            if (tau_add(&acc, tc)) return TAU_LIMIT_FIXED;
             if (acc >= limit) return acc;
         }
     }
@@ -1028,10 +1064,9 @@ static float argmin_scatter(const prim_t* prims, long n, v3 o, v3 d, const uint3
     for (int s = 0; s < na; ++s) {
         const prim_t* pr = prims + act[s];
         k->n_arg_active++;
-        float te = exit_inside(pr, o, d);
+        inside_t x = inside_chord(pr, o, d);
         float tau = ff_tau(prim_u(key, bounce, 0u, act[s]));
-        float ts = inv_cdf(pr, o, d, tau);
-        if (ts >= 0.0f && ts < ts_min && ts <= te) ts_min = ts;
+        if (tau <= x.tau) ts_min = vpo_min(ts_min, origin_flight(pr, o, d, x, tau));
     }
     *hits_out = hits;
     return ts_min;

@@ -20,7 +20,8 @@ _u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
 MATH_FNS = {"exp": 0, "log": 1, "sin": 2, "cos": 3, "atan2": 4, "acos": 5, "erf": 6,
             "erfc": 7, "erfinv": 8, "erf_diff": 9, "sqrt": 10, "div": 11, "chord_poly": 12}
 PRIM_FNS = {"inv_cdf": 0, "inv_cdf_span": 1, "optical_depth": 2, "exit_inside": 3,
-            "entry_t": 4, "exit_entry": 5, "point_inside": 6, "pdf": 7, "chord_tau": 8, "entered_flight": 9}
+            "entry_t": 4, "exit_entry": 5, "point_inside": 6, "pdf": 7, "chord_tau": 8, "entered_flight": 9,
+            "inside_tau": 10, "origin_flight": 11}
 COUNTER_NAMES = ["n_sh_active", "n_sh_hit", "n_arg_active", "n_arg_hit", "n_arg_accept",
                  "n_rb_active", "n_rb_hit", "n_alb", "n_scatter", "n_sample", "n_shadow_rays",
                  "n_trace", "active_overflow", "max_hits", "max_overlap"]

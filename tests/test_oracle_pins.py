@@ -221,6 +221,83 @@ def test_chord_depth_matches_quadrature_and_erf_form():
         assert n > 100
 
 
+def test_inside_chord_matches_quadrature_and_erf_form():
+    # Optical depth from an origin inside the 3-sigma bound to the exit (one
+    # transform, one exp, two S polynomials) against FP64 quadrature of M N(x)
+    # over [0, FP64 exit], and against the erf-difference optical_depth over
+    # the oracle's own [0, exit_inside].  Origins near the centre, near the
+    # bound, moving inwards and outwards.
+    rng = np.random.default_rng(29)
+    from golden.make_golden import chord, density_along
+    n = 0
+    for seed in (41, 42, 43):
+        rows = scenes.random_scene(1, seed, smin=0.1, smax=0.6)
+        rec = vpo.make_prims(rows)[0]
+        c = rows[0, 0:3].astype(np.float64)
+        s_max = float(np.max(rows[0, 7:10]))
+        args = []
+        for _ in range(150):
+            d = rng.normal(size=3)
+            d /= np.linalg.norm(d)
+            o = c + rng.normal(size=3) * s_max * rng.uniform(0.1, 1.5)
+            args.append([*o, *d, 0.0, 0.0, 0.0])
+        args = np.array(args, np.float32)
+        tau = vpo.prim_eval("inside_tau", rec, args).astype(np.float64)
+        inside = vpo.prim_eval("point_inside", rec, args) > 0
+        ex = vpo.prim_eval("exit_inside", rec, args)
+        a2 = args.copy()
+        a2[:, 6], a2[:, 7] = 0.0, ex
+        od = vpo.prim_eval("optical_depth", rec, a2).astype(np.float64)
+        for i in np.flatnonzero(inside):
+            ch, org_in = chord(rows[0], args[i, :3].astype(np.float64), args[i, 3:6].astype(np.float64))
+            if not org_in:
+                continue
+            dens = lambda t: density_along(rows[0], args[i, :3].astype(np.float64), args[i, 3:6].astype(np.float64), t)
+            q, _ = integrate.quad(dens, 0.0, ch[1], epsrel=1e-12, epsabs=0)
+            # the whole chord's depth scales the absolute bound: near the exit
+            # the two terms of the closed form cancel to a few ulp of it
+            from golden.make_golden import whiten
+            pw, ww = whiten(rows[0], args[i, :3].astype(np.float64), args[i, 3:6].astype(np.float64))
+            a_, b_, c_ = ww @ ww, pw @ ww, pw @ pw - 9.0
+            t_back = (-b_ - math.sqrt(b_ * b_ - a_ * c_)) / a_  # the bound behind the origin
+            full, _ = integrate.quad(dens, t_back, ch[1], epsrel=1e-12, epsabs=0)
+            assert abs(tau[i] - q) <= 2e-5 * q + 2e-6 * full + 1e-7, (seed, i, tau[i], q)
+            assert abs(tau[i] - od[i]) <= 2e-5 * q + 2e-6 * full + 1e-7, (seed, i, tau[i], od[i])
+            n += 1
+    assert n > 150
+
+
+def test_origin_flight_inverts_the_inside_depth():
+    # origin_flight(tau) lands where the depth from the origin equals tau
+    # when tau is at most the depth to the exit, and passes (1e20) otherwise
+    rng = np.random.default_rng(31)
+    rows = scenes.random_scene(1, 47, smin=0.2, smax=0.5)
+    rows[:, 13] = 5.0
+    rec = vpo.make_prims(rows)[0]
+    c = rows[0, 0:3].astype(np.float64)
+    args = []
+    for _ in range(600):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        o = c + rng.normal(size=3) * 0.15
+        args.append([*o, *d, 0.0, 0.0, 0.0])
+    args = np.array(args, np.float32)
+    inside = vpo.prim_eval("point_inside", rec, args) > 0
+    full = vpo.prim_eval("inside_tau", rec, args)
+    ex = vpo.prim_eval("exit_inside", rec, args)
+    args[:, 8] = (rng.uniform(0.02, 1.3, args.shape[0]) * full).astype(np.float32)
+    t = vpo.prim_eval("origin_flight", rec, args)
+    hit = inside & (args[:, 8] <= full)
+    assert hit.sum() > 300 and (inside & ~hit).sum() > 50
+    assert np.all(t[inside & ~hit] >= 1e20)
+    assert np.all((t[hit] >= 0.0) & (t[hit] <= ex[hit]))
+    a2 = args.copy()
+    a2[:, 6], a2[:, 7] = 0.0, t
+    od = vpo.prim_eval("optical_depth", rec, a2)
+    sel = hit & (args[:, 8] > 1e-3)
+    assert np.allclose(od[sel], args[sel, 8], rtol=3e-4, atol=3e-6)
+
+
 def test_entered_flight_inverts_the_chord_depth():
     # entered_flight(tau) lands where the optical depth from the entry equals
     # tau when tau is at most the chord's depth, and passes (1e20) otherwise
