@@ -593,6 +593,8 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
             pre += lanes_below(bm) << bit;
             total += static_cast<uint32_t>(__popcll(bm)) << bit;
         }
+        static_assert(kWaveQ * sizeof(float) >= 64 * (sizeof(void*) + sizeof(uint32_t)),
+                      "the shared queue must hold the origin-set tables");
         const uint32_t** set_of = reinterpret_cast<const uint32_t**>(wq.tj);  // [64] by rank
         uint32_t* pre_of = reinterpret_cast<uint32_t*>(wq.tj) + 2 * 64;     // [64] by rank
         set_of[rk] = act;

@@ -44,6 +44,11 @@
 namespace vp {
 namespace {
 
+// The argmin's origin-set tables (64 set pointers, 64 exit pointers, 64
+// prefix sums) borrow wq.tj between leaf phases.
+static_assert(kWaveQ * sizeof(float) >= 64 * (2 * sizeof(void*) + sizeof(uint32_t)),
+              "the shared queue must hold the argmin's origin-set tables");
+
 struct PathState {
     f3 o, d, beta, L;
     Pcg32 rng;
