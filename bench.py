@@ -180,12 +180,12 @@ MIX_SCENES = {"scattering": "4096 anisotropic Gaussians (seed 1, sigma multiplie
               "torus": "2.5*10^5 Gaussians on a torus shell R 1 r 0.3 (seed 3, sigma multiplier 10, albedo 0.95)"}
 
 
-def mix_scenes(ctx, scenes, bvh, names):
+def mix_scenes(ctx, scenes, bvh, names, walk="auto"):
     gens = {"scattering": lambda: scenes.c1_scattering(n=4096, seed=1, sigma_multiplier=10.0, albedo=0.9),
             "cloud": lambda: scenes.c2_cloud(n=1_000_000, seed=2, sigma_multiplier=7.5, albedo=0.99),
             "torus": lambda: scenes.torus_shell(n=250_000, seed=3, sigma_multiplier=10.0, albedo=0.95)}
     rows = {k: gens[k]() for k in names}
-    return {k: ctx.scene(r, bvh=bvh) for k, r in rows.items()}, rows
+    return {k: ctx.scene(r, bvh=bvh, walk=walk) for k, r in rows.items()}, rows
 
 
 def mix_requests(volprim, jobs, handles, envh, shadow_rr=0.0):
@@ -259,7 +259,7 @@ def run_stream(args):
     # (first launches load code objects and grow the allocator pools)
     wjobs = scenes.mix_jobs(n_jobs=W, seed=args.mix_seed + 100, res_div=8, spp_div=256) if W else []
     names = sorted({j["scene"] for j in jobs + wjobs})
-    handles, rows = mix_scenes(ctx, scenes, args.bvh, names)
+    handles, rows = mix_scenes(ctx, scenes, args.bvh, names, args.walk)
     env_rgba = scenes.sky_env(2048, 1024)
     envh = ctx.env(env_rgba)
     reqs = mix_requests(volprim, jobs, handles, envh, args.shadow_rr)
@@ -512,6 +512,8 @@ def main():
     ap.add_argument("--bvh", choices=["lbvh", "sah", "median"], default="median",
                     help="leaf order of the walk tree: host median splits on the walk tree's record boundaries "
                          "(default), host SAH tree, or the GPU LBVH's Morton order")
+    ap.add_argument("--walk", choices=["auto", "f16", "f32"], default="auto",
+                    help="walk-tree records: binary16 when the leaf boxes grow little (auto), or forced")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB elementwise copy kernel first (HBM counter calibration)")
     ap.add_argument("--ris", action="store_true",
@@ -545,7 +547,8 @@ def run_single(args, torch):
         pp = dict(pp, shadow_rr=args.shadow_rr)
     ctx = volprim.Context(0)
     dev_name, n_cu, clk = ctx.device_info()
-    scene = ctx.scene(rows, bvh=args.bvh)
+    scene = ctx.scene(rows, bvh=args.bvh, walk=args.walk)
+    walk_half, walk_growth = scene.walk_info()
     envh = ctx.env(env)
     cam = volprim.camera_perspective(**cam_args)
     params = volprim.default_params(**pp)
@@ -655,6 +658,8 @@ def run_single(args, torch):
         "kernel_ms_per_step": round(kernel_ms / args.steps, 3),
         "pipeline": args.pipeline,
         "bvh": args.bvh,
+        "walk_records": {"form": "binary16" if walk_half else "binary32", "leaf_box_surface_growth":
+                         round(walk_growth, 5)},
         "events_per_step": events,
         # rays traced per second: camera and continuation rays (one argmin
         # walk each) plus shadow rays, from the counted events of one render

@@ -199,9 +199,12 @@ vp_status vp_scene_create(vp_context* ctx, const vp_primitive* prims, uint32_t n
 vp_status vp_scene_create_ex(vp_context* ctx, const vp_primitive* prims, uint32_t n, uint32_t flags,
                              vp_scene** out) {
     if (!ctx || !prims || !out) return fail(VP_ERR_INVALID, "null argument");
-    if (flags & ~(VP_SCENE_BVH_SAH | VP_SCENE_BVH_MEDIAN)) return fail(VP_ERR_INVALID, "unknown scene flags");
+    if (flags & ~(VP_SCENE_BVH_SAH | VP_SCENE_BVH_MEDIAN | VP_SCENE_WALK_F32 | VP_SCENE_WALK_F16))
+        return fail(VP_ERR_INVALID, "unknown scene flags");
     if ((flags & VP_SCENE_BVH_SAH) && (flags & VP_SCENE_BVH_MEDIAN))
         return fail(VP_ERR_INVALID, "VP_SCENE_BVH_SAH and VP_SCENE_BVH_MEDIAN exclude each other");
+    if ((flags & VP_SCENE_WALK_F32) && (flags & VP_SCENE_WALK_F16))
+        return fail(VP_ERR_INVALID, "VP_SCENE_WALK_F32 and VP_SCENE_WALK_F16 exclude each other");
     if (n == 0) return fail(VP_ERR_INVALID, "scene has no primitives");
     if (n > (1u << 30)) return fail(VP_ERR_INVALID, "scene has more than 2^30 primitives");
     // the validation of loadPrimitivesFromPLY (ply.cpp:146-172)
@@ -224,7 +227,7 @@ vp_status vp_scene_create_ex(vp_context* ctx, const vp_primitive* prims, uint32_
     vp_status st = vp_build_scene_device(ctx, prims, n, sc);
     if (st == VP_OK && (flags & (VP_SCENE_BVH_SAH | VP_SCENE_BVH_MEDIAN)))
         st = vp_build_scene_sah(ctx, prims, n, sc, (flags & VP_SCENE_BVH_MEDIAN) != 0u);
-    if (st == VP_OK) st = vp_build_walk_tree(ctx, sc);
+    if (st == VP_OK) st = vp_build_walk_tree(ctx, sc, flags & (VP_SCENE_WALK_F32 | VP_SCENE_WALK_F16));
     if (st != VP_OK) {
         vp_scene_destroy(sc);
         return st;
@@ -254,6 +257,13 @@ vp_status vp_scene_export(vp_scene* sc, uint32_t* order, uint32_t* morton, float
         VP_HIP_TRY(hipMemcpyAsync(records16, sc->prims, sizeof(vp::PrimRecord) * sc->n, hipMemcpyDeviceToHost, s));
     VP_HIP_TRY(hipStreamSynchronize(s));
     if (depth) *depth = sc->depth;
+    return VP_OK;
+}
+
+vp_status vp_scene_walk_info(vp_scene* sc, uint32_t* half_records, double* box_growth) {
+    if (!sc) return fail(VP_ERR_INVALID, "scene is null");
+    if (half_records) *half_records = sc->w4_half;
+    if (box_growth) *box_growth = sc->w4_growth;
     return VP_OK;
 }
 

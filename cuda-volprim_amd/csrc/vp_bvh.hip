@@ -393,6 +393,58 @@ __global__ void k_w4_level(uint32_t l, float4* w4) {
     w4_store(w4, w4_level_base(l) + k, lo, hi);
 }
 
+// 16-bit records from the 32-bit ones (one thread per record): min bounds
+// rounded toward -inf, max bounds toward +inf, packed in the byte order of
+// vp_walk.h.  For the last level's records (index >= leaf_base) it also sums
+// the surfaces ex ey + ey ez + ez ex of the non-empty leaf boxes, of both
+// forms, into sums[0] (32-bit) and sums[1] (16-bit).
+__global__ void k_w4_half(const float4* w4, uint32_t records, uint32_t leaf_base, uint4* out, double* sums) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    double s32 = 0.0, s16 = 0.0;
+    if (k < records) {
+        const float4* r = w4 + static_cast<size_t>(kW4RecordF4) * k;
+        float b[6][4];
+        uint32_t words[12];
+        for (int g = 0; g < 6; ++g) {
+            const float4 v = r[g];
+            b[g][0] = v.x;
+            b[g][1] = v.y;
+            b[g][2] = v.z;
+            b[g][3] = v.w;
+            for (int c = 0; c < 4; c += 2) {
+                const uint32_t h0 = g < 3 ? f16_down(b[g][c]) : f16_up(b[g][c]);
+                const uint32_t h1 = g < 3 ? f16_down(b[g][c + 1]) : f16_up(b[g][c + 1]);
+                words[2 * g + c / 2] = h0 | (h1 << 16);
+            }
+        }
+        uint4* o = out + 4 * static_cast<size_t>(k);
+        o[0] = make_uint4(words[0], words[1], words[2], words[3]);
+        o[1] = make_uint4(words[4], words[5], words[6], words[7]);
+        o[2] = make_uint4(words[8], words[9], words[10], words[11]);
+        o[3] = make_uint4(0u, 0u, 0u, 0u);
+        if (k >= leaf_base) {
+            for (int c = 0; c < 4; ++c) {
+                if (!(b[0][c] < INFINITY)) continue;
+                double e32[3], e16[3];
+                for (int a = 0; a < 3; ++a) {
+                    e32[a] = static_cast<double>(b[3 + a][c]) - b[a][c];
+                    e16[a] = static_cast<double>(f16_value(f16_up(b[3 + a][c]))) - f16_value(f16_down(b[a][c]));
+                }
+                s32 += e32[0] * e32[1] + e32[1] * e32[2] + e32[2] * e32[0];
+                s16 += e16[0] * e16[1] + e16[1] * e16[2] + e16[2] * e16[0];
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        s32 += __shfl_xor(s32, off);
+        s16 += __shfl_xor(s16, off);
+    }
+    if ((threadIdx.x & 63u) == 0u && s16 != 0.0) {
+        atomicAdd(sums, s32);
+        atomicAdd(sums + 1, s16);
+    }
+}
+
 inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 struct DevBuf {
@@ -495,7 +547,7 @@ vp_status vp_build_scene_device(vp_context* ctx, const vp_primitive* host_prims,
     return VP_OK;
 }
 
-vp_status vp_build_walk_tree(vp_context* ctx, vp_scene* sc) {
+vp_status vp_build_walk_tree(vp_context* ctx, vp_scene* sc, uint32_t walk_format) {
     using namespace vp;
     hipStream_t s = ctx->stream;
     const uint32_t n = sc->n;
@@ -518,7 +570,34 @@ vp_status vp_build_walk_tree(vp_context* ctx, vp_scene* sc) {
     for (uint32_t l = levels - 1u; l-- > 0;)
         hipLaunchKernelGGL(k_w4_level, dim3(cdiv(1u << (2u * l), 256u)), dim3(256), 0, s, l, sc->w4);
     VP_HIP_TRY(hipGetLastError());
-    VP_HIP_TRY(hipStreamSynchronize(s));
     sc->w4_levels = levels;
+    sc->w4_half = 0u;
+    sc->w4_growth = 0.0;
+    if (walk_format == VP_SCENE_WALK_F32) {
+        VP_HIP_TRY(hipStreamSynchronize(s));
+        return VP_OK;
+    }
+    // the 16-bit records, kept when forced or when the leaf boxes' summed
+    // surface grows by at most 1/16 (more leaf tests and node visits, against
+    // half the bytes per node visit)
+    DevBuf d_half, d_sums;
+    VP_HIP_TRY(hipMalloc(&d_half.p, static_cast<size_t>(kW4hRecordB) * records));
+    VP_HIP_TRY(hipMalloc(&d_sums.p, 2 * sizeof(double)));
+    VP_HIP_TRY(hipMemsetAsync(d_sums.p, 0, 2 * sizeof(double), s));
+    hipLaunchKernelGGL(k_w4_half, dim3(cdiv(static_cast<uint32_t>(records), 256u)), dim3(256), 0, s,
+                       (const float4*)sc->w4, static_cast<uint32_t>(records), w4_level_base(levels - 1u),
+                       (uint4*)d_half.p, (double*)d_sums.p);
+    VP_HIP_TRY(hipGetLastError());
+    double sums[2] = {0.0, 0.0};
+    VP_HIP_TRY(hipMemcpyAsync(sums, d_sums.p, sizeof(sums), hipMemcpyDeviceToHost, s));
+    VP_HIP_TRY(hipStreamSynchronize(s));
+    const double growth = sums[0] > 0.0 ? sums[1] / sums[0] - 1.0 : 0.0;
+    sc->w4_growth = growth;
+    if (walk_format == VP_SCENE_WALK_F16 || (growth <= 1.0 / 16.0)) {
+        VP_HIP_TRY(hipFree(sc->w4));
+        sc->w4 = static_cast<float4*>(d_half.p);
+        d_half.p = nullptr;
+        sc->w4_half = 1u;
+    }
     return VP_OK;
 }

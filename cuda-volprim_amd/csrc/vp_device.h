@@ -121,26 +121,98 @@ __device__ __forceinline__ uint32_t w4_ray_mask_nf(const float4* w4, uint32_t n,
            (hit(nx.z, ny.z, nz.z, gx.z, gy.z, gz.z) ? 4u : 0u) | (hit(nx.w, ny.w, nz.w, gx.w, gy.w, gz.w) ? 8u : 0u);
 }
 
+// ---- the same tests on 16-bit records (layout in vp_walk.h) ----
+// The low and high binary16 of word w as float (exact).  As an FMA operand
+// the conversion folds into v_fma_mix_f32, so a slab distance costs one
+// instruction, as on 32-bit records, for half the bytes per node.
+__device__ __forceinline__ float h16lo(uint32_t w) {
+    return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w)));
+}
+__device__ __forceinline__ float h16hi(uint32_t w) {
+    return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w >> 16)));
+}
+// record n of a 16-bit tree (kW4hRecordB = 4 float4)
+__device__ __forceinline__ const VP_GLOBAL_AS uint4* w4h_record(const float4* w4, uint32_t n) {
+    return reinterpret_cast<const VP_GLOBAL_AS uint4*>(as_global(w4) + 4u * static_cast<size_t>(n));
+}
+__device__ __forceinline__ uint32_t w4h_ray_mask(const float4* w4, uint32_t n, f3 inv, f3 oi, float tmax) {
+    const auto* r = w4h_record(w4, n);
+    // a = min.x (children 0,1 | 2,3), min.y; b = min.z, max.x; c = max.y, max.z
+    const uint4 a = r[0], b = r[1], c = r[2];
+    return (slab_hit(h16lo(a.x), h16lo(a.z), h16lo(b.x), h16lo(b.z), h16lo(c.x), h16lo(c.z), inv, oi, tmax) ? 1u
+                                                                                                             : 0u) |
+           (slab_hit(h16hi(a.x), h16hi(a.z), h16hi(b.x), h16hi(b.z), h16hi(c.x), h16hi(c.z), inv, oi, tmax) ? 2u
+                                                                                                             : 0u) |
+           (slab_hit(h16lo(a.y), h16lo(a.w), h16lo(b.y), h16lo(b.w), h16lo(c.y), h16lo(c.w), inv, oi, tmax) ? 4u
+                                                                                                             : 0u) |
+           (slab_hit(h16hi(a.y), h16hi(a.w), h16hi(b.y), h16hi(b.w), h16hi(c.y), h16hi(c.w), inv, oi, tmax) ? 8u
+                                                                                                             : 0u);
+}
+// near/far form: the bounds of one axis and side are 8 bytes (min.x at 0,
+// min.y 8, min.z 16, max.x 24, max.y 32, max.z 40)
+__device__ __forceinline__ uint32_t w4h_ray_mask_nf(const float4* w4, uint32_t n, f3 inv, f3 oi, float tmax) {
+    const uint32_t fx = (__float_as_uint(inv.x) >> 31) * 24u;
+    const uint32_t fy = (__float_as_uint(inv.y) >> 31) * 24u;
+    const uint32_t fz = (__float_as_uint(inv.z) >> 31) * 24u;
+    const auto* rec = reinterpret_cast<const VP_GLOBAL_AS char*>(w4h_record(w4, n));
+    const uint2 nx = *reinterpret_cast<const VP_GLOBAL_AS uint2*>(rec + fx);
+    const uint2 ny = *reinterpret_cast<const VP_GLOBAL_AS uint2*>(rec + 8u + fy);
+    const uint2 nz = *reinterpret_cast<const VP_GLOBAL_AS uint2*>(rec + 16u + fz);
+    const uint2 gx = *reinterpret_cast<const VP_GLOBAL_AS uint2*>(rec + 24u - fx);
+    const uint2 gy = *reinterpret_cast<const VP_GLOBAL_AS uint2*>(rec + 32u - fy);
+    const uint2 gz = *reinterpret_cast<const VP_GLOBAL_AS uint2*>(rec + 40u - fz);
+    auto hit = [&](float ax, float ay, float az, float bx, float by, float bz) {
+        const float tn = fmaxf(fmaxf(fmaf_(ax, inv.x, -oi.x), fmaf_(ay, inv.y, -oi.y)),
+                               fmaxf(fmaf_(az, inv.z, -oi.z), 0.0f));
+        const float tf = fminf(fminf(fmaf_(bx, inv.x, -oi.x), fmaf_(by, inv.y, -oi.y)),
+                               fminf(fmaf_(bz, inv.z, -oi.z), tmax));
+        return tn <= tf;
+    };
+    return (hit(h16lo(nx.x), h16lo(ny.x), h16lo(nz.x), h16lo(gx.x), h16lo(gy.x), h16lo(gz.x)) ? 1u : 0u) |
+           (hit(h16hi(nx.x), h16hi(ny.x), h16hi(nz.x), h16hi(gx.x), h16hi(gy.x), h16hi(gz.x)) ? 2u : 0u) |
+           (hit(h16lo(nx.y), h16lo(ny.y), h16lo(nz.y), h16lo(gx.y), h16lo(gy.y), h16lo(gz.y)) ? 4u : 0u) |
+           (hit(h16hi(nx.y), h16hi(ny.y), h16hi(nz.y), h16hi(gx.y), h16hi(gy.y), h16hi(gz.y)) ? 8u : 0u);
+}
+
+// The ray tests on either record form (half = the scene's w4_half).
+__device__ __forceinline__ uint32_t ray_mask(const float4* w4, uint32_t half, uint32_t n, f3 inv, f3 oi,
+                                             float tmax) {
+    return half ? w4h_ray_mask(w4, n, inv, oi, tmax) : w4_ray_mask(w4, n, inv, oi, tmax);
+}
+__device__ __forceinline__ uint32_t ray_mask_nf(const float4* w4, uint32_t half, uint32_t n, f3 inv, f3 oi,
+                                                float tmax) {
+    return half ? w4h_ray_mask_nf(w4, n, inv, oi, tmax) : w4_ray_mask_nf(w4, n, inv, oi, tmax);
+}
+
 // The node test of the megakernel's shared-queue walks: 1 selects the
 // near/far form (vp_render.hip sets it).
 #ifndef VP_MEGA_NEAR_FAR
 #define VP_MEGA_NEAR_FAR 0
 #endif
-__device__ __forceinline__ uint32_t mega_ray_mask(const float4* w4, uint32_t n, f3 inv, f3 oi, float tmax) {
+__device__ __forceinline__ uint32_t mega_ray_mask(const float4* w4, uint32_t half, uint32_t n, f3 inv, f3 oi,
+                                                  float tmax) {
 #if VP_MEGA_NEAR_FAR
-    return w4_ray_mask_nf(w4, n, inv, oi, tmax);
+    return ray_mask_nf(w4, half, n, inv, oi, tmax);
 #else
-    return w4_ray_mask(w4, n, inv, oi, tmax);
+    return ray_mask(w4, half, n, inv, oi, tmax);
 #endif
 }
 
 // children of node n whose box contains x
-__device__ __forceinline__ uint32_t w4_point_mask(const float4* w4, uint32_t n, f3 x) {
-    const auto* r = as_global(w4) + static_cast<size_t>(kW4RecordF4) * n;
-    const float4 lx = r[0], ly = r[1], lz = r[2], hx = r[3], hy = r[4], hz = r[5];
+__device__ __forceinline__ uint32_t point_mask(const float4* w4, uint32_t half, uint32_t n, f3 x) {
     auto in = [&](float l0, float l1, float l2, float h0, float h1, float h2) {
         return x.x >= l0 && x.x <= h0 && x.y >= l1 && x.y <= h1 && x.z >= l2 && x.z <= h2;
     };
+    if (half) {
+        const auto* r = w4h_record(w4, n);
+        const uint4 a = r[0], b = r[1], c = r[2];
+        return (in(h16lo(a.x), h16lo(a.z), h16lo(b.x), h16lo(b.z), h16lo(c.x), h16lo(c.z)) ? 1u : 0u) |
+               (in(h16hi(a.x), h16hi(a.z), h16hi(b.x), h16hi(b.z), h16hi(c.x), h16hi(c.z)) ? 2u : 0u) |
+               (in(h16lo(a.y), h16lo(a.w), h16lo(b.y), h16lo(b.w), h16lo(c.y), h16lo(c.w)) ? 4u : 0u) |
+               (in(h16hi(a.y), h16hi(a.w), h16hi(b.y), h16hi(b.w), h16hi(c.y), h16hi(c.w)) ? 8u : 0u);
+    }
+    const auto* r = as_global(w4) + static_cast<size_t>(kW4RecordF4) * n;
+    const float4 lx = r[0], ly = r[1], lz = r[2], hx = r[3], hy = r[4], hz = r[5];
     return (in(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x) ? 1u : 0u) | (in(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y) ? 2u : 0u) |
            (in(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z) ? 4u : 0u) | (in(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w) ? 8u : 0u);
 }
@@ -154,7 +226,7 @@ __device__ __forceinline__ void walk_ray(const SceneView& s, f3 o, f3 inv, const
     Walk4 w = walk4_begin();
     while (w.n != kWalkEnd) {
         ++visits;
-        const uint32_t m = w4_ray_mask(s.w4, w.n, inv, oi, tmax);
+        const uint32_t m = ray_mask(s.w4, s.w4_half, w.n, inv, oi, tmax);
         uint32_t first;
         for (uint32_t lm = walk4_advance(w, s.w4_levels, m, first); lm != 0u; lm &= lm - 1u)
             if (!leaf(first + static_cast<uint32_t>(__builtin_ctz(lm)))) return;
@@ -168,7 +240,7 @@ __device__ __forceinline__ void walk_point(const SceneView& s, f3 x, uint32_t& v
     Walk4 w = walk4_begin();
     while (w.n != kWalkEnd) {
         ++visits;
-        const uint32_t m = w4_point_mask(s.w4, w.n, x);
+        const uint32_t m = point_mask(s.w4, s.w4_half, w.n, x);
         uint32_t first;
         for (uint32_t lm = walk4_advance(w, s.w4_levels, m, first); lm != 0u; lm &= lm - 1u)
             leaf(first + static_cast<uint32_t>(__builtin_ctz(lm)));
@@ -327,7 +399,7 @@ __device__ __forceinline__ void queue_leaves(const LeafQueue& q, uint32_t& cnt, 
 // the walk moves on.
 __device__ __forceinline__ void ray_node_step(const SceneView& s, f3 inv, f3 oi, float tmax, Walk4& w, uint32_t& cnt,
                                               const LeafQueue& q) {
-    const uint32_t m = w4_ray_mask(s.w4, w.n, inv, oi, tmax);
+    const uint32_t m = ray_mask(s.w4, s.w4_half, w.n, inv, oi, tmax);
     uint32_t first;
     const uint32_t lm = walk4_advance(w, s.w4_levels, m, first);
     queue_leaves(q, cnt, lm, first);
@@ -361,7 +433,7 @@ __device__ __forceinline__ void walk_point_q(const SceneView& s, f3 x, uint32_t&
         uint32_t cnt = 0;
         while (w.n != kWalkEnd && queue_room(cnt)) {
             ++visits;
-            const uint32_t m = w4_point_mask(s.w4, w.n, x);
+            const uint32_t m = point_mask(s.w4, s.w4_half, w.n, x);
             uint32_t first;
             const uint32_t lm = walk4_advance(w, s.w4_levels, m, first);
             if (lm != 0u) queue_leaves(q, cnt, lm, first);
@@ -564,7 +636,7 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
     lds_store3(wq.ro + 3 * lane, o);
     wq.prims[lane] = s.prims;
     wq.nodes[lane] = s.w4;
-    wq.nn[lane] = s.w4_levels;
+    wq.nn[lane] = s.w4_levels | (s.w4_half << 8);
     lds_store3(wq.rd + 3 * (2 * lane), da);
     lds_store3(wq.rd + 3 * (2 * lane + 1), db);
     wq.acc[2 * lane] = 0ull;
@@ -642,7 +714,7 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
     const uint32_t rank = lanes_below(am);
     uint32_t ray = rank < nr ? wq.list[rank] : kNoRay;
     const float4* w4 = nullptr;
-    uint32_t levels = 0;
+    uint32_t levels = 0, half = 0;
     Walk4 w = walk4_begin();
     f3 inv = mk(0.0f), oi = mk(0.0f);
     // the walk setup of the lane's ray (any lane may run it: an idle lane
@@ -653,7 +725,8 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
         inv = safe_inv(rd);
         oi = ro * inv;
         w4 = wq.nodes[own];
-        levels = wq.nn[own];
+        levels = wq.nn[own] & 0xffu;
+        half = wq.nn[own] >> 8;
     };
     setup();
     while (__builtin_amdgcn_ballot_w64(ray != kNoRay) != 0) {
@@ -677,7 +750,7 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
             ++nsteps;
             uint32_t lm = 0, first = 0;
             Walk4 nw = w;
-            if (step) lm = walk4_advance(nw, levels, mega_ray_mask(w4, w.n, inv, oi, kTMax), first);
+            if (step) lm = walk4_advance(nw, levels, mega_ray_mask(w4, half, w.n, inv, oi, kTMax), first);
             const bool fits = wave_push(wq, count, lm, first, ray);
             if (step && fits) w = nw;
             if (__builtin_amdgcn_ballot_w64(!fits) != 0) break;

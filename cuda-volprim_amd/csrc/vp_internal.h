@@ -59,8 +59,9 @@ struct SceneView {
     const float4* nodes;
     uint32_t n_prims;
     uint32_t n_nodes;
-    const float4* w4;    // walk-tree records (kW4RecordF4 float4 each)
+    const float4* w4;    // walk-tree records (kW4RecordF4 float4 each, or 16-bit records)
     uint32_t w4_levels;  // its level count L
+    uint32_t w4_half;    // 1: w4 holds 16-bit records (kW4hRecordB bytes each)
 };
 
 struct EnvDev {
@@ -220,9 +221,13 @@ struct vp_scene {
     float4* nodes = nullptr;           // 2 * (2n - 1)
     uint32_t* order = nullptr;         // order[k] = input index
     uint32_t* morton = nullptr;        // codes in input order
-    float4* w4 = nullptr;              // walk tree, kW4RecordF4 * w4_records(w4_levels)
+    float4* w4 = nullptr;              // walk tree, w4_records(w4_levels) records
     uint32_t w4_levels = 0;
-    vp::SceneView view() const { return vp::SceneView{prims, nodes, n, n ? 2 * n - 1 : 0, w4, w4_levels}; }
+    uint32_t w4_half = 0;              // 1: 16-bit records
+    double w4_growth = 0.0;            // summed leaf-box surface of the 16-bit records / 32-bit - 1
+    vp::SceneView view() const {
+        return vp::SceneView{prims, nodes, n, n ? 2 * n - 1 : 0, w4, w4_levels, w4_half};
+    }
 };
 
 struct vp_env {
@@ -258,7 +263,9 @@ vp_status vp_build_scene_device(vp_context* ctx, const vp_primitive* host_prims,
 vp_status vp_build_scene_sah(vp_context* ctx, const vp_primitive* host_prims, uint32_t n, vp_scene* scene,
                              bool median);
 // walk tree over the scene's stored order, from its rope tree's leaf boxes (vp_bvh.hip)
-vp_status vp_build_walk_tree(vp_context* ctx, vp_scene* scene);
+// walk_format: 0 (16-bit records when the leaf boxes grow little),
+// VP_SCENE_WALK_F32 or VP_SCENE_WALK_F16
+vp_status vp_build_walk_tree(vp_context* ctx, vp_scene* scene, uint32_t walk_format);
 
 // launchers (vp_render.hip)
 struct RenderLaunch {

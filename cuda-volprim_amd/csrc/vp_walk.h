@@ -14,14 +14,74 @@
 // An empty child has min = max = +inf on every axis, which no ray or point
 // test accepts.  Parents and siblings follow from the index, so walks need no
 // stack (see Walk4).
+//
+// A scene whose boxes lose little to it stores the tree instead in 64-byte
+// records of binary16 bounds (the 16-bit records), each bound rounded
+// outward (min toward -inf, max toward +inf) from the 32-bit record:
+//   bytes  0.. 7 min.x of children 0..3, 8..15 min.y, 16..23 min.z,
+//   bytes 24..31 max.x,                  32..39 max.y, 40..47 max.z,
+//   bytes 48..63 unused.
+// A decoded 16-bit box contains the 32-bit one, and since the rounding is
+// monotone a decoded parent box still contains its decoded children, so a
+// walk of the 16-bit tree meets every leaf a walk of the 32-bit tree meets
+// (and perhaps a few more, which the leaf tests reject).  The device node
+// tests decode in the FMA (binary16 operands of v_fma_mix_f32 convert
+// exactly), so the slab distances are those of the decoded box.
 #pragma once
 
 #include "vp_math.h"
 
 namespace vp {
 
-constexpr uint32_t kW4RecordF4 = 8;   // float4 per node record
-constexpr uint32_t kW4MaxLevels = 15; // 4^15 = 2^30 leaves
+constexpr uint32_t kW4RecordF4 = 8;    // float4 per node record
+constexpr uint32_t kW4hRecordB = 64;   // bytes per 16-bit node record
+constexpr uint32_t kW4MaxLevels = 15;  // 4^15 = 2^30 leaves
+
+// ---- binary16 with directed rounding (16-bit records) ----
+
+// Bits of the largest binary16 magnitude <= |x| (x not NaN); inexact tells
+// whether it differs from |x|.  Magnitudes above the largest finite value
+// 65504 give 65504 (inexact), +inf gives +inf.
+VP_HD uint32_t f16_mag_floor(float x, bool& inexact) {
+    const uint32_t u = f2u(x) & 0x7fffffffu;
+    const float a = u2f(u);
+    if (u >= 0x7f800000u) {
+        inexact = false;
+        return 0x7c00u;
+    }
+    if (a >= 65504.0f) {
+        inexact = a != 65504.0f;
+        return 0x7bffu;
+    }
+    if (a >= 0x1p-14f) {
+        // normal: biased exponent 1..30 and the top 10 bits of the fraction
+        inexact = (u & 0x1fffu) != 0u;
+        return (((u >> 23) - 112u) << 10) | ((u >> 13) & 0x3ffu);
+    }
+    // subnormal: m 2^-24 with m < 1024 (the scaling is exact)
+    const float s = a * 0x1p24f;
+    const uint32_t m = static_cast<uint32_t>(s);
+    inexact = static_cast<float>(m) != s;
+    return m;
+}
+// largest binary16 <= x, as bits (x not NaN)
+VP_HD uint16_t f16_down(float x) {
+    bool inexact = false;
+    const uint32_t m = f16_mag_floor(x, inexact);
+    if (!(x < 0.0f)) return static_cast<uint16_t>(m);
+    // negative: the magnitude rounds up (65504 + 1 ulp is the bit pattern of inf)
+    return static_cast<uint16_t>(0x8000u | (m + (inexact ? 1u : 0u)));
+}
+// smallest binary16 >= x, as bits (x not NaN)
+VP_HD uint16_t f16_up(float x) { return static_cast<uint16_t>(f16_down(-x) ^ 0x8000u); }
+// the value of binary16 bits h (not NaN), exactly
+VP_HD float f16_value(uint16_t h) {
+    const uint32_t sign = static_cast<uint32_t>(h & 0x8000u) << 16;
+    const uint32_t e = (h >> 10) & 0x1fu, f = h & 0x3ffu;
+    if (e == 31u) return u2f(sign | 0x7f800000u);
+    if (e == 0u) return u2f(sign | f2u(static_cast<float>(f) * 0x1p-24f));
+    return u2f(sign | ((e + 112u) << 23) | (f << 13));
+}
 
 VP_HD uint32_t w4_levels(uint32_t n) {
     uint32_t l = 1;

@@ -300,7 +300,7 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
             if (count >= kWaveQ / VP_SHARED_END_DIV && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
             uint32_t lm = 0, first = 0;
             Walk4 nw = w4w;
-            if (step) lm = walk4_advance(nw, sc.w4_levels, w4_ray_mask_nf(sc.w4, w4w.n, inv, oi, tmax), first);
+            if (step) lm = walk4_advance(nw, sc.w4_levels, ray_mask_nf(sc.w4, sc.w4_half, w4w.n, inv, oi, tmax), first);
             const bool fits = wave_push(wq, count, lm, first, lane);
             if (step && fits) w4w = nw;
             if (__builtin_amdgcn_ballot_w64(!fits) != 0) break;
@@ -631,7 +631,7 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
             if (count >= kWaveQ / VP_SHARED_END_DIV && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
             uint32_t lm = 0, first = 0;
             Walk4 nw = w4w;
-            if (step) lm = walk4_advance(nw, sc.w4_levels, w4_ray_mask_nf(sc.w4, w4w.n, inv, oi, kTMax), first);
+            if (step) lm = walk4_advance(nw, sc.w4_levels, ray_mask_nf(sc.w4, sc.w4_half, w4w.n, inv, oi, kTMax), first);
             const bool fits = wave_push(wq, count, lm, first, lane);
             if (step && fits) w4w = nw;
             if (__builtin_amdgcn_ballot_w64(!fits) != 0) break;

@@ -99,10 +99,12 @@ EXPORTED = [
     "vp_render_device", "vp_normalize_accumulator", "vp_render_jobs", "vp_autotune", "vp_server_create",
     "vp_server_submit", "vp_server_poll", "vp_server_drain", "vp_server_stats_get", "vp_server_destroy",
     "vp_transmittance_rays", "vp_collect_overlaps", "vp_render_count_events", "vp_math_eval", "vp_prim_eval",
-    "vp_scene_create_ex", "vp_render_device_noalloc",
+    "vp_scene_create_ex", "vp_render_device_noalloc", "vp_scene_walk_info",
 ]
 SCENE_BVH_SAH = 1
 SCENE_BVH_MEDIAN = 2
+SCENE_WALK_F32 = 4
+SCENE_WALK_F16 = 8
 EVENT_NAMES = ["active_overflow", "max_hits", "max_overlap", "paths", "traces", "scatters", "shadow_rays",
                "arg_hit", "arg_active", "arg_accept", "rb_active", "rb_hit", "alb", "sh_active", "sh_hit",
                "leaf_tests", "node_visits", "contain", "visits_shadow", "visits_point"]
@@ -129,6 +131,7 @@ def load():
     lib.vp_scene_create_ex.argtypes = [_vp, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_vp)]
     lib.vp_scene_destroy.argtypes = [_vp]
     lib.vp_scene_export.argtypes = [_vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_uint32)]
+    lib.vp_scene_walk_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_double)]
     lib.vp_env_create.argtypes = [_vp, _f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_vp)]
     lib.vp_env_destroy.argtypes = [_vp]
     lib.vp_env_export.argtypes = [_vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_float)]
@@ -207,13 +210,14 @@ def camera_orthographic(w, h, eye, target, up, ortho_h):
 
 
 class Scene:
-    def __init__(self, ctx, rows14, bvh="lbvh"):
+    def __init__(self, ctx, rows14, bvh="lbvh", walk="auto"):
         self.ctx = ctx
         self.rows = f32(rows14).reshape(-1, 14)
         self.n = self.rows.shape[0]
         prims = primitives_from_rows(self.rows)
         h = _vp()
         flags = {"lbvh": 0, "sah": SCENE_BVH_SAH, "median": SCENE_BVH_MEDIAN}[bvh]
+        flags |= {"auto": 0, "f32": SCENE_WALK_F32, "f16": SCENE_WALK_F16}[walk]
         _check(load().vp_scene_create_ex(ctx.handle, ctypes.cast(prims, _vp), self.n, flags, ctypes.byref(h)),
                "vp_scene_create_ex")
         self.handle = h
@@ -226,6 +230,15 @@ class Scene:
         _check(load().vp_scene_export(self.handle, order.ctypes.data, morton.ctypes.data, rec.ctypes.data,
                                       ctypes.byref(depth)), "vp_scene_export")
         return order, morton, rec, depth.value
+
+    def walk_info(self):
+        """(half_records, box_growth): whether the walk tree holds binary16
+        records, and the growth of the summed leaf-box surface they bring."""
+        half = ctypes.c_uint32(0)
+        growth = ctypes.c_double(0.0)
+        _check(load().vp_scene_walk_info(self.handle, ctypes.byref(half), ctypes.byref(growth)),
+               "vp_scene_walk_info")
+        return bool(half.value), growth.value
 
     def close(self):
         if self.handle:
@@ -284,10 +297,12 @@ class Context:
         _check(load().vp_device_info(self.handle, buf, 256, ctypes.byref(cus), ctypes.byref(mhz)), "vp_device_info")
         return buf.value.decode(), cus.value, mhz.value
 
-    def scene(self, rows14, bvh="lbvh"):
+    def scene(self, rows14, bvh="lbvh", walk="auto"):
         """bvh: "lbvh" (GPU build, default), "sah" (host SAH tree, same layout) or
-        "median" (host tree of median splits on the walk tree's record boundaries)."""
-        return Scene(self, rows14, bvh)
+        "median" (host tree of median splits on the walk tree's record boundaries).
+        walk: "auto" (binary16 walk-tree records when the leaf boxes grow by at
+        most 1/16), "f16" or "f32"; renders are bit-identical for all three."""
+        return Scene(self, rows14, bvh, walk)
 
     def env(self, rgba):
         return Env(self, rgba)

@@ -131,9 +131,18 @@ vp_status vp_scene_create(vp_context* ctx, const vp_primitive* prims, uint32_t n
  * splits that fall on the 4-wide walk tree's record boundaries (each split
  * on the axis whose halves have the smaller summed box surface), so each
  * walk-tree record's children are spatial quarters of its primitives.  The
- * two flags exclude each other. */
+ * two flags exclude each other.
+ * The 4-wide walk tree the render kernels traverse is stored in 64-byte
+ * records of binary16 bounds, rounded outward, when that grows the leaf
+ * boxes' summed surface by at most 1/16 (scenes whose primitives are small
+ * against their distance from the origin keep 128-byte records of binary32
+ * bounds).  VP_SCENE_WALK_F32 and VP_SCENE_WALK_F16 force one form; they
+ * exclude each other.  The form decides only which primitives a walk tests,
+ * never a result: renders are bit-identical either way. */
 #define VP_SCENE_BVH_SAH 1u
 #define VP_SCENE_BVH_MEDIAN 2u
+#define VP_SCENE_WALK_F32 4u
+#define VP_SCENE_WALK_F16 8u
 vp_status vp_scene_create_ex(vp_context* ctx, const vp_primitive* prims, uint32_t n, uint32_t flags,
                              vp_scene** out);
 vp_status vp_scene_destroy(vp_scene* scene);
@@ -141,6 +150,11 @@ vp_status vp_scene_destroy(vp_scene* scene);
  * their 30-bit Morton codes (input order) and the 64-byte device records
  * (16 floats each, stored order).  Any pointer may be null. */
 vp_status vp_scene_export(vp_scene* scene, uint32_t* order, uint32_t* morton, float* records16, uint32_t* depth);
+/* Walk-tree form of a scene (no counterpart): half_records = 1 for the
+ * binary16 records; box_growth = the summed leaf-box surface of the binary16
+ * records over that of the binary32 ones, minus 1 (0 when the scene was
+ * forced to VP_SCENE_WALK_F32).  Either pointer may be null. */
+vp_status vp_scene_walk_info(vp_scene* scene, uint32_t* half_records, double* box_growth);
 vp_status vp_env_create(vp_context* ctx, const float* rgba, uint32_t width, uint32_t height, vp_env** out);
 vp_status vp_env_destroy(vp_env* env);
 /* joint weights (W*H), alias thresholds (W*H), alias indices (W*H), total. */

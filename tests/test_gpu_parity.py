@@ -278,7 +278,8 @@ def _three_sigma_boxes(rows):
 
 
 @pytest.mark.parametrize("flags,msg", [(volprim.SCENE_BVH_SAH | volprim.SCENE_BVH_MEDIAN, "exclude each other"),
-                                       (4, "unknown scene flags")])
+                                       (volprim.SCENE_WALK_F32 | volprim.SCENE_WALK_F16, "exclude each other"),
+                                       (16, "unknown scene flags")])
 def test_scene_flags_rejected(ctx, flags, msg):
     import ctypes
     rows = scenes.random_scene(8, 1)
@@ -384,6 +385,92 @@ def test_render_sah_bitexact(ctx, case, bvh):
                            ris_candidates=p.ris_candidates, active_cap=p.active_cap, shadow_rr=p.shadow_rr)
     assert_bitwise(img, ref, f"{bvh} render {case}")
     assert st["active_overflow"] == k["active_overflow"]
+
+
+def _oracle_render(rec, env, cam, p):
+    return vpo.render(rec, env, cam.pod(), p.width, p.height, p.spp, seed=p.seed, max_depth=p.max_depth,
+                      rr_depth=p.rr_depth, rr_max_survival=p.rr_max_survival, firefly_clamp=p.firefly_clamp,
+                      filter_stddev=p.filter_stddev, phase_g=p.phase_g, use_ris=bool(p.use_ris),
+                      ris_candidates=p.ris_candidates, active_cap=p.active_cap, shadow_rr=p.shadow_rr)
+
+
+@pytest.mark.parametrize("walk", ["f16", "f32"])
+@pytest.mark.parametrize("case", ["mis_rr", "ris_sky", "small_cap", "c0_scatter"])
+def test_render_walk_formats_bitexact(ctx, case, walk):
+    # the walk tree in binary16 records (bounds rounded outward) and in
+    # binary32 records: both pipelines give the oracle's image bit for bit,
+    # and the unit-sized test scenes pick the binary16 form by themselves
+    c = RENDER_CASES[case]
+    env = scenes.constant_env() if c["env"] == "const" else scenes.sky_env(64, 32, sun_radius_deg=4.0, sun_scale=50.0)
+    cam = volprim.camera_perspective(**scenes.default_camera_args(c["w"], c["h"]))
+    print(f"[case] walk {walk} render {case}", flush=True)
+    rows = c["rows"]()
+    auto = ctx.scene(rows)
+    assert auto.walk_info()[0], "a unit-sized scene should take the binary16 records"
+    growth = auto.walk_info()[1]
+    auto.close()
+    assert 0.0 <= growth < 1.0 / 16.0
+    sc = ctx.scene(rows, walk=walk)
+    assert sc.walk_info()[0] == (walk == "f16")
+    ev = ctx.env(env)
+    _, _, rec, _ = sc.export()
+    p = volprim.default_params(width=c["w"], height=c["h"], spp=c["spp"], **c["kw"])
+    img, st = ctx.render(sc, ev, cam, p)
+    launch = volprim.LaunchConfig(256, 0, 0, 0, 1, volprim.PIPELINE_WAVEFRONT, 4096)
+    imgs_wf, _ = ctx.render_jobs([(sc, ev, cam, p)], launch=launch)
+    sc.close()
+    ev.close()
+    ref, _, k = _oracle_render(rec, env, cam, p)
+    assert_bitwise(img, ref, f"walk {walk} render {case}")
+    assert_bitwise(imgs_wf[0], ref, f"walk {walk} wavefront render {case}")
+    assert st["active_overflow"] == k["active_overflow"]
+
+
+@pytest.mark.parametrize("walk", ["f16", "f32"])
+def test_walk_formats_overlaps_and_transmittance_bitexact(ctx, walk):
+    rng = np.random.default_rng(91)
+    rows = scenes.random_scene(500, 31)
+    sc = ctx.scene(rows, walk=walk)
+    _, _, rec, _ = sc.export()
+    rays = _random_rays(rng, 3000, 1.5)
+    print(f"[case] walk {walk} overlaps + transmittance n=500", flush=True)
+    cg, tg, ig = ctx.collect_overlaps(sc, rays, 512)
+    T = ctx.transmittance_rays(sc, rays)
+    sc.close()
+    co, to, io = vpo.overlaps(rec, rays, 512)
+    assert np.array_equal(cg, co)
+    for r in range(rays.shape[0]):
+        k = int(cg[r])
+        assert np.array_equal(ig[r, :k], io[r, :k]) and np.array_equal(tg[r, :k], to[r, :k])
+    assert_bitwise(T, vpo.transmittance_rays(rec, rays), "transmittance")
+
+
+@pytest.mark.parametrize("walk", ["auto", "f16"])
+def test_far_scene_walk_format(ctx, walk):
+    # a scene 3000 units from the origin: binary16 bounds there are 2 apart,
+    # so the automatic choice keeps binary32 records; forced binary16 records
+    # (boxes grown to the binary16 grid, so many more leaf tests) still give
+    # the oracle's image bit for bit
+    off = np.array([3000.0, -1000.0, 500.0], np.float32)
+    rows = scenes.random_scene(40, 33)
+    rows[:, 0:3] += off
+    env = scenes.sky_env(64, 32, sun_radius_deg=4.0, sun_scale=50.0)
+    cam = volprim.camera_perspective(16, 16, off + np.array([0, 0, -3.5], np.float32), off,
+                                     np.array([0, 1, 0], np.float32), 40.0)
+    print(f"[case] far scene walk {walk}", flush=True)
+    sc = ctx.scene(rows, walk=walk)
+    half, growth = sc.walk_info()
+    assert growth > 1.0 / 16.0
+    assert half == (walk == "f16")
+    ev = ctx.env(env)
+    _, _, rec, _ = sc.export()
+    p = volprim.default_params(width=16, height=16, spp=2)
+    img, st = ctx.render(sc, ev, cam, p)
+    sc.close()
+    ev.close()
+    ref, _, k = _oracle_render(rec, env, cam, p)
+    assert_bitwise(img, ref, f"far scene walk {walk}")
+    assert img[..., :3].mean() > 0.0
 
 
 def test_full_size_scattering_workload(ctx):

@@ -13,14 +13,22 @@
 //   * a walk split among several walkers (walk4_split, as the wavefront
 //     shares out running walks) meets every leaf exactly once;
 //   * the near/far form of the node test decides every child box (empty
-//     ones included) exactly as the min/max form.
+//     ones included) exactly as the min/max form;
+//   * the 16-bit tree (every bound rounded outward to binary16, as the
+//     device build stores it) contains each 32-bit box, and its walks meet
+//     exactly the leaves whose rounded box passes, a superset of the 32-bit
+//     walk's leaves.
 // Prints "ok <walks>" and exits 0, or prints the first mismatch and exits 1.
 // Usage: walk_tree_check <seed> <walks per size>
+//        walk_tree_check f16 <in> <out>: reads float32 values from <in> and
+//        writes f16_down and f16_up of each (uint16 pairs) and f16_value of
+//        both (float32 pairs) to <out>.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <random>
+#include <string>
 #include <vector>
 
 #include "vp_walk.h"
@@ -107,6 +115,36 @@ Tree build(const std::vector<Box>& leaves) {
     return t;
 }
 
+// the 16-bit tree of t: every bound rounded outward to binary16, decoded
+Tree half_of(const Tree& t) {
+    Tree h = t;
+    for (Box& b : h.child)
+        for (int a = 0; a < 3; ++a) {
+            b.lo[a] = vp::f16_value(vp::f16_down(b.lo[a]));
+            b.hi[a] = vp::f16_value(vp::f16_up(b.hi[a]));
+        }
+    return h;
+}
+
+int f16_mode(const char* in, const char* out) {
+    FILE* f = std::fopen(in, "rb");
+    if (!f) return 2;
+    std::vector<float> x;
+    float v;
+    while (std::fread(&v, sizeof v, 1, f) == 1) x.push_back(v);
+    std::fclose(f);
+    FILE* g = std::fopen(out, "wb");
+    if (!g) return 2;
+    for (float xv : x) {
+        const uint16_t h[2] = {vp::f16_down(xv), vp::f16_up(xv)};
+        const float d[2] = {vp::f16_value(h[0]), vp::f16_value(h[1])};
+        std::fwrite(h, sizeof h, 1, g);
+        std::fwrite(d, sizeof d, 1, g);
+    }
+    std::fclose(g);
+    return 0;
+}
+
 int fail(const char* what, uint32_t n, uint32_t walk) {
     std::printf("FAIL %s: n=%u walk=%u\n", what, n, walk);
     return 1;
@@ -135,6 +173,7 @@ int walk(const Tree& t, Mask&& mask, std::vector<uint32_t>& met) {
 }  // namespace
 
 int main(int argc, char** argv) {
+    if (argc > 3 && std::string(argv[1]) == "f16") return f16_mode(argv[2], argv[3]);
     const uint32_t seed = argc > 1 ? static_cast<uint32_t>(std::atoi(argv[1])) : 1u;
     const uint32_t walks = argc > 2 ? static_cast<uint32_t>(std::atoi(argv[2])) : 200u;
     std::mt19937 rng(seed);
@@ -150,6 +189,11 @@ int main(int argc, char** argv) {
                 b.hi[a] = c + h;
             }
         const Tree t = build(leaves);
+        const Tree th = half_of(t);
+        for (size_t i = 0; i < t.child.size(); ++i)
+            for (int a = 0; a < 3; ++a)
+                if (!(th.child[i].lo[a] <= t.child[i].lo[a] && th.child[i].hi[a] >= t.child[i].hi[a]))
+                    return fail("16-bit box does not contain the 32-bit box", n, static_cast<uint32_t>(i));
         for (uint32_t k = 0; k < walks; ++k) {
             // ray: origin inside or outside the leaves' region, direction random
             float o[3], d[3], inv[3];
@@ -184,6 +228,28 @@ int main(int argc, char** argv) {
                 return fail("ray walk index", n, k);
             if (!nf_same) return fail("near/far node test decides differently", n, k);
             if (met != want) return fail("ray walk leaves", n, k);
+            // the 16-bit tree: exactly the leaves whose rounded box passes,
+            // which include every leaf of the 32-bit walk
+            {
+                const uint32_t lb = vp::w4_level_base(th.levels - 1);
+                std::vector<uint32_t> met16, want16;
+                for (uint32_t j = 0; j < n; ++j)
+                    if (slab(th.child[4 * lb + j], o, inv, tmax)) want16.push_back(j);
+                if (walk(th, [&](uint32_t node) {
+                        uint32_t m = 0;
+                        for (uint32_t c = 0; c < 4; ++c) {
+                            const bool h = slab(th.child[4 * node + c], o, inv, tmax);
+                            nf_same &= h == slab_nf(th.child[4 * node + c], o, inv, tmax);
+                            m |= h ? (1u << c) : 0u;
+                        }
+                        return m;
+                    }, met16))
+                    return fail("16-bit ray walk index", n, k);
+                if (!nf_same) return fail("near/far node test decides differently on the 16-bit tree", n, k);
+                if (met16 != want16) return fail("16-bit ray walk leaves", n, k);
+                if (!std::includes(met16.begin(), met16.end(), want.begin(), want.end()))
+                    return fail("16-bit ray walk lost a leaf of the 32-bit walk", n, k);
+            }
             // the same ray with the end lowered as the walk goes (argmin style):
             // every leaf passing at the final end must be met
             float tcur = 1e20f;
@@ -244,6 +310,15 @@ int main(int argc, char** argv) {
                 }, pm))
                 return fail("point walk index", n, k);
             if (pm != pw) return fail("point walk leaves", n, k);
+            std::vector<uint32_t> pm16;
+            if (walk(th, [&](uint32_t node) {
+                    uint32_t m = 0;
+                    for (uint32_t c = 0; c < 4; ++c) m |= inside(th.child[4 * node + c], x) ? (1u << c) : 0u;
+                    return m;
+                }, pm16))
+                return fail("16-bit point walk index", n, k);
+            if (!std::includes(pm16.begin(), pm16.end(), pw.begin(), pw.end()))
+                return fail("16-bit point walk lost a leaf", n, k);
             total += 3;
         }
     }
