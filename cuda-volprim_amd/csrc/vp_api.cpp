@@ -47,6 +47,12 @@ vp_status check_params_impl(const vp_render_params* p) {
     if (p->use_ris && p->ris_candidates == 0) return fail(VP_ERR_INVALID, "ris_candidates must be positive");
     if (p->active_cap > vp::kMaxActiveCap) return fail(VP_ERR_INVALID, "active_cap must be at most 65535");
     if (!(p->shadow_rr >= 0.0f && p->shadow_rr < 1e30f)) return fail(VP_ERR_INVALID, "shadow_rr must be >= 0");
+    if (!(p->adaptive_threshold >= 0.0f && p->adaptive_threshold <= 3.4e38f))
+        return fail(VP_ERR_INVALID, "adaptive_threshold must be finite and >= 0");
+    if (p->adaptive_threshold > 0.0f && p->adaptive_min_samples == 0)
+        return fail(VP_ERR_INVALID, "adaptive_min_samples must be positive");
+    if (p->adaptive_threshold > 0.0f && p->spp >= (1u << 31))
+        return fail(VP_ERR_INVALID, "adaptive sampling counts samples in 31 bits: spp must be below 2^31");
     return VP_OK;
 }
 
@@ -75,6 +81,8 @@ vp::JobDesc make_job_impl(const vp_scene* sc, const vp_env* env, const vp_camera
     J.use_ris = p->use_ris ? 1u : 0u;
     J.ris_candidates = p->ris_candidates ? p->ris_candidates : 6u;
     J.shadow_rr = p->shadow_rr;
+    J.adapt_threshold = p->adaptive_threshold;
+    J.adapt_min = p->adaptive_min_samples;
     J.tiles_x = (p->width + 7) / 8;
     const uint32_t tiles_y = (p->height + 7) / 8;
     J.npad = J.tiles_x * tiles_y * 64u;
@@ -414,6 +422,8 @@ void vp_default_params(vp_render_params* p) {
     p->ris_candidates = 6;
     p->measure_caps = 0;
     p->active_cap = 128;
+    p->adaptive_threshold = 0.0f;   // ENABLE_ADAPTIVE_SAMPLING = false (constants.cuh:196-199)
+    p->adaptive_min_samples = 32;  // ADAPTIVE_MIN_SAMPLES (constants.cuh:204)
 }
 
 vp_status vp_render_device(vp_context* ctx, vp_scene* sc, vp_env* env, const vp_camera* cam,
@@ -442,7 +452,7 @@ vp_status vp_render_device(vp_context* ctx, vp_scene* sc, vp_env* env, const vp_
     if (stats) {
         stats->kernel_ms = kms;
         stats->launches = ctx->last_launches;
-        stats->paths = static_cast<uint64_t>(params->width) * params->height * params->spp;
+        stats->paths = runs[0].traced;
         stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     return VP_OK;
@@ -534,9 +544,11 @@ vp_status vp_render_jobs(vp_context* ctx, vp_job* jobs, uint32_t njobs, const vp
     double kms = 0.0;
     st = vp_run_jobs(ctx, runs, launch, cap, 0, measure, ctx->stream, &kms, nullptr);
     if (st != VP_OK) return st;
+    samples = 0;
     for (uint32_t i = 0; i < njobs; ++i) {
         const size_t bytes = static_cast<size_t>(jobs[i].params.width) * jobs[i].params.height * sizeof(float4);
         VP_HIP_TRY(hipMemcpyAsync(jobs[i].out_rgba, means[i].p, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        samples += runs[i].traced;
     }
     st = vp_read_stats(ctx, ctx->stream, stats);
     if (st != VP_OK) return st;

@@ -827,7 +827,7 @@ __device__ __forceinline__ float4 finalize_radiance(const JobDesc& J, f3 L) {
 // and sample from the 8x8-tile numbering, RNG stream, counter-hash key and
 // jittered camera ray; pix receives the pixel index (y * width + x) and sl the
 // sample's index within the slice.  False for the padding paths of partial
-// tiles.
+// tiles and for the paths of pixels that adaptive sampling has stopped.
 __device__ __forceinline__ bool path_begin(const JobDesc& J, uint64_t q, f3& o, f3& d, Pcg32& rng, uint64_t& key,
                                            uint32_t& pix_out, uint32_t& sl_out) {
     const uint32_t sl = static_cast<uint32_t>(q / J.npad);
@@ -838,6 +838,9 @@ __device__ __forceinline__ bool path_begin(const JobDesc& J, uint64_t q, f3& o, 
     if (px >= J.width || py >= J.height) return false;
     const uint32_t sample = J.s_begin + sl;
     const uint64_t pix = static_cast<uint64_t>(py) * J.width + px;
+    // adaptive sampling: a converged pixel traces nothing more
+    // (raygen.cuh:53-69); its paths end like padding
+    if (J.adapt_state != nullptr && (J.adapt_state[pix] >> 31) != 0u) return false;
     const uint64_t seq = hash64(pix * J.spp + sample);
     rng = pcg_init(J.seed, seq);
     key = path_key(J.seed, seq);
@@ -875,8 +878,7 @@ __device__ __forceinline__ void path_finish(const JobDesc& J, uint32_t pix, uint
                                    __HIP_MEMORY_SCOPE_AGENT);
     }
     __hip_atomic_fetch_add(a + 3, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (J.samples != nullptr)
-        J.samples[(static_cast<size_t>(J.s_begin) + sl) * (static_cast<size_t>(J.width) * J.height) + pix] = v;
+    if (J.samples != nullptr) J.samples[static_cast<size_t>(sl) * (static_cast<size_t>(J.width) * J.height) + pix] = v;
     if (J.done != nullptr) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __hip_atomic_fetch_add(J.done, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -296,9 +296,15 @@ vp_status vp_run_step(vp_context* ctx, const RenderLaunch& L, uint32_t nslices, 
 vp_status vp_run_jobs(vp_context* ctx, std::vector<JobRun>& runs, const vp_launch_config* cfg, uint32_t active_cap,
                       int count, int measure, hipStream_t s, double* kernel_ms, uint64_t* paths) {
     RenderLaunch L = vp_resolve_launch(ctx, cfg, active_cap, count, measure);
-    // per-sample output is written by the megakernel only
+    // per-sample output (and adaptive sampling, which folds the per-sample
+    // radiance of each batch) is written by the megakernel only
     bool per_sample = false;
-    for (auto& r : runs) per_sample |= r.samples_all != nullptr;
+    for (auto& r : runs) {
+        const bool adaptive = r.desc.adapt_threshold > 0.0f;
+        if (adaptive && r.samples_all != nullptr)
+            return fail(VP_ERR_INVALID, "adaptive sampling and per-sample output exclude each other");
+        per_sample |= r.samples_all != nullptr || adaptive;
+    }
     if (per_sample && L.pipeline == VP_PIPELINE_WAVEFRONT) {
         vp_launch_config c{};
         if (cfg) c = *cfg;
@@ -332,15 +338,51 @@ vp_status vp_run_jobs(vp_context* ctx, std::vector<JobRun>& runs, const vp_launc
             if (p) (void)hipFree(p);
         }
     } free_acc{acc_buf};
+    // adaptive sampling state of every adaptive job, in one allocation: per
+    // pixel the Welford M2 (float4) and the sample count / converged flag,
+    // and the per-sample radiance of one batch (the reference's batch of 16
+    // unless the launch sets batch_spp); the Welford mean lives in the
+    // job's output image
+    size_t adapt_bytes = 0;
+    for (auto& r : runs) {
+        if (!(r.desc.adapt_threshold > 0.0f)) continue;
+        if (r.slice_spp == 0) r.slice_spp = kAdaptiveBatch;
+        const size_t npix = static_cast<size_t>(r.desc.width) * r.desc.height;
+        adapt_bytes += npix * (sizeof(float4) * (1 + std::min(r.slice_spp, r.desc.spp)) + sizeof(uint32_t));
+        adapt_bytes = (adapt_bytes + 255) & ~size_t(255);
+    }
+    void* adapt_buf = nullptr;
+    if (adapt_bytes && hipMalloc(&adapt_buf, adapt_bytes) != hipSuccess)
+        return fail(VP_ERR_OOM, "cannot allocate " + std::to_string(adapt_bytes) + " bytes of adaptive-sampling state");
+    Free free_adapt{adapt_buf};
     {
-        size_t off = 0;
+        size_t off = 0, aoff = 0;
         for (auto& r : runs) {
+            const size_t npix = static_cast<size_t>(r.desc.width) * r.desc.height;
             r.desc.accum = static_cast<unsigned long long*>(acc_buf) + off;
             r.desc.mean = r.mean;
-            r.desc.samples = r.samples_all;
+            r.desc.samples = nullptr;
             r.desc.done = nullptr;
+            r.desc.adapt_m2 = nullptr;
+            r.desc.adapt_state = nullptr;
+            r.batch = nullptr;
             r.issued = 0;
-            off += static_cast<size_t>(r.desc.width) * r.desc.height * vp::kAccumWords;
+            off += npix * vp::kAccumWords;
+            if (r.desc.adapt_threshold > 0.0f) {
+                char* b = static_cast<char*>(adapt_buf) + aoff;
+                r.desc.adapt_m2 = reinterpret_cast<float4*>(b);
+                r.batch = r.desc.adapt_m2 + npix;
+                r.desc.adapt_state =
+                    reinterpret_cast<uint32_t*>(r.batch + npix * std::min(r.slice_spp, r.desc.spp));
+                aoff += npix * (sizeof(float4) * (1 + std::min(r.slice_spp, r.desc.spp)) + sizeof(uint32_t));
+                aoff = (aoff + 255) & ~size_t(255);
+                // Welford state from zero (raygen.cuh:30-36 reads zeroed buffers)
+                VP_HIP_TRY(vp_launch_fill_u64(reinterpret_cast<unsigned long long*>(r.desc.adapt_m2),
+                                              static_cast<uint32_t>(2 * npix), 0ull, s));
+                VP_HIP_TRY(vp_launch_fill_u64(reinterpret_cast<unsigned long long*>(r.mean),
+                                              static_cast<uint32_t>(2 * npix), 0ull, s));
+                VP_HIP_TRY(vp_launch_fill_u32(r.desc.adapt_state, static_cast<uint32_t>(npix), 0u, s));
+            }
         }
         // fill kernels (not hipMemsetAsync): the launch sequences below may
         // be captured into graphs by callers
@@ -363,6 +405,9 @@ vp_status vp_run_jobs(vp_context* ctx, std::vector<JobRun>& runs, const vp_launc
             const uint32_t slice = r.slice_spp ? r.slice_spp : r.desc.spp;
             J.s_begin = r.issued;
             J.s_count = std::min(slice, r.desc.spp - r.issued);
+            const size_t npix = static_cast<size_t>(J.width) * J.height;
+            if (r.samples_all) J.samples = r.samples_all + static_cast<size_t>(J.s_begin) * npix;
+            if (r.batch) J.samples = r.batch;
             J.path_begin = total;
             J.path_count = static_cast<uint64_t>(J.npad) * J.s_count;
             total += J.path_count;
@@ -388,6 +433,19 @@ vp_status vp_run_jobs(vp_context* ctx, std::vector<JobRun>& runs, const vp_launc
     if (kernel_ms) *kernel_ms = render_ms;
     ctx->last_launches = launches;
     if (paths) *paths = total_paths_all;
+    // samples traced per job: every sample, or under adaptive sampling the
+    // per-pixel counts
+    for (auto& r : runs) {
+        const size_t npix = static_cast<size_t>(r.desc.width) * r.desc.height;
+        r.traced = static_cast<uint64_t>(npix) * r.desc.spp;
+        if (r.desc.adapt_state == nullptr) continue;
+        std::vector<uint32_t> state(npix);
+        VP_HIP_TRY(hipMemcpyAsync(state.data(), r.desc.adapt_state, npix * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        VP_HIP_TRY(hipStreamSynchronize(s));
+        uint64_t t = 0;
+        for (uint32_t v : state) t += v & 0x7fffffffu;
+        r.traced = t;
+    }
     return VP_OK;
 }
 

@@ -95,8 +95,19 @@ struct alignas(128) JobDesc {
     uint64_t path_begin, path_count;
     unsigned long long* accum;  // width * height * kAccumWords radiance sums of the job
     float4* mean;               // width * height mean, written by the resolve kernel
-    float4* samples;            // optional (null = none): per-sample radiance, sample-major, spp * W * H
+    float4* samples;            // optional (null = none): per-sample radiance of this slice, sample-major,
+                                // s_count * W * H (sample s_begin + k of pixel i at k * W * H + i)
     unsigned long long* done;   // optional (null = none): finished paths of this entry (server)
+    // adaptive sampling (null adapt_state = off): the slices are the batches
+    // of the reference's launch loop; a pixel whose state has bit 31 set has
+    // converged and traces no more paths.  The resolve then folds the slice's
+    // samples (samples, required) into the pixel's Welford mean (in mean) and
+    // M2, and sets bit 31 for the next slice once the criterion holds; the low
+    // 31 bits count the pixel's samples.
+    float adapt_threshold;
+    uint32_t adapt_min;
+    float4* adapt_m2;
+    uint32_t* adapt_state;
 };
 static_assert(sizeof(JobDesc) % 128 == 0, "job-table entries fill whole cache lines");
 
@@ -294,7 +305,11 @@ struct JobRun {
     float4* samples_all = nullptr;  // optional caller buffer spp*W*H of per-sample radiance
     uint32_t issued = 0;            // samples launched so far
     uint32_t slice_spp = 0;         // samples per launch (0 = all at once)
+    float4* batch = nullptr;        // adaptive sampling: per-sample radiance of one batch
+    uint64_t traced = 0;            // samples traced (set when the run completes)
 };
+// samples per batch of adaptive sampling (BATCH_SIZE, renderer.cpp:32)
+constexpr uint32_t kAdaptiveBatch = 16;
 vp_status vp_ensure_slab(vp_context* ctx, size_t words);
 vp_status vp_ensure_jobs(vp_context* ctx, uint32_t n);
 void vp_drop_graph(vp_context* ctx);
@@ -312,6 +327,7 @@ vp_status vp_check_params(const vp_render_params* p);
 // Resolves ctx->noalloc_scratch on ctx's device (which must be current).
 vp_status vp_noalloc_attach(vp_context* ctx);
 hipError_t vp_launch_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v, hipStream_t s);
+hipError_t vp_launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
 hipError_t vp_launch_normalize(float4* out, const float4* in, size_t n, float scale, hipStream_t s);
 hipError_t vp_launch_transmittance(vp::SceneView sc, const float* rays, uint32_t m, float* T, hipStream_t s);
 hipError_t vp_launch_overlaps(vp::SceneView sc, const float* rays, uint32_t m, uint32_t cap, uint32_t* counts,

@@ -104,6 +104,9 @@ extern "C" vp_status vp_render_device_noalloc(vp_context* ctx, vp_scene* sc, vp_
     if (!ctx || !sc || !env || !cam || !params || !out_rgba_dev) return fail(VP_ERR_INVALID, "null argument");
     vp_status st = vp_check_params(params);
     if (st != VP_OK) return st;
+    if (params->adaptive_threshold > 0.0f)
+        return fail(VP_ERR_INVALID, "vp_render_device_noalloc: adaptive sampling needs per-pixel state beyond the "
+                                    "scratch array; use vp_render_device");
     const uint64_t npix = static_cast<uint64_t>(params->width) * params->height;
     if (npix > kNoAllocMaxPixels)
         return fail(VP_ERR_INVALID, "vp_render_device_noalloc: " + std::to_string(params->width) + "x" +

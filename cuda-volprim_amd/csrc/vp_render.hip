@@ -632,10 +632,58 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
     if (overflow) atomicAdd(&counters[kCntActiveOverflow], static_cast<unsigned long long>(overflow));
 }
 
+// Adaptive sampling, after a batch (the slice of J): the batch's samples of
+// pixel i, in sample order, are folded into its Welford mean and M2 as the
+// reference's raygen does per sample (raygen.cuh:214-233: n_inv =
+// 1/(prev + s + 1), delta1 = x - mean, mean += delta1 n_inv, M2 += delta1
+// (x - mean)), and the convergence test the reference runs at the start of
+// the next batch (raygen.cuh:53-69: n >= min samples and max over channels
+// of sqrt(M2/(n-1)) / max(mean, 1e-6) below the threshold) sets bit 31 of
+// the state, so the next batch skips the pixel.  A converged pixel traced
+// nothing in this batch and keeps its state.
+__device__ __forceinline__ void fold_pixel(const JobDesc& J, uint32_t i) {
+    const uint32_t st = J.adapt_state[i];
+    if ((st >> 31) != 0u) return;
+    const size_t npix = static_cast<size_t>(J.width) * J.height;
+    float4 m = J.mean[i], m2 = J.adapt_m2[i];
+    float mean[3] = {m.x, m.y, m.z}, M2[3] = {m2.x, m2.y, m2.z};
+    for (uint32_t k = 0; k < J.s_count; ++k) {
+        const float4 v = J.samples[k * npix + i];
+        const float x[3] = {v.x, v.y, v.z};
+        const float n_inv = 1.0f / static_cast<float>(st + k + 1u);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float d1 = x[c] - mean[c];
+            mean[c] = mean[c] + d1 * n_inv;
+            M2[c] = M2[c] + d1 * (x[c] - mean[c]);
+        }
+    }
+    const uint32_t n = st + J.s_count;
+    bool conv = false;
+    if (n >= J.adapt_min) {
+        const float r = 1.0f / static_cast<float>(n - 1u);
+        float err = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float e = sqrtf_(M2[c] * r) / maxf(mean[c], 1e-6f);
+            err = c == 0 ? e : maxf(err, e);
+        }
+        conv = err < J.adapt_threshold;
+    }
+    J.mean[i] = make_float4(mean[0], mean[1], mean[2], 1.0f);
+    J.adapt_m2[i] = make_float4(M2[0], M2[1], M2[2], 0.0f);
+    J.adapt_state[i] = n | (conv ? 0x80000000u : 0u);
+}
+
 // The mean of every pixel of job-table entry J from its accumulator
 // (raygen.cuh:214-238 keeps a running mean; here the sums are exact and the
-// division happens once).  Alpha is 1, as the reference writes it.
+// division happens once).  Alpha is 1, as the reference writes it.  Under
+// adaptive sampling the batch is folded instead.
 __device__ __forceinline__ void resolve_pixel(const JobDesc& J, uint32_t i) {
+    if (J.adapt_state != nullptr) {
+        fold_pixel(J, i);
+        return;
+    }
     const unsigned long long* a = J.accum + static_cast<size_t>(i) * kAccumWords;
     const uint64_t n = a[3];
     J.mean[i] = make_float4(accum_mean(a[0], a[4], n), accum_mean(a[1], a[5], n), accum_mean(a[2], a[6], n), 1.0f);
@@ -659,6 +707,10 @@ __global__ void k_resolve_job(const JobDesc* __restrict__ jobs, uint32_t j) {
 }
 
 __global__ void k_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
@@ -837,6 +889,10 @@ hipError_t vp_launch_resolve_job(const JobDesc* jobs, uint32_t j, uint32_t npix,
 
 hipError_t vp_launch_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(n, 64)), dim3(64), 0, s, p, n, v);
+    return hipGetLastError();
+}
+hipError_t vp_launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(cdiv(n, 256)), dim3(256), 0, s, p, n, v);
     return hipGetLastError();
 }
 

@@ -493,6 +493,9 @@ vp_status vp_server_submit(vp_server* S, const vp_job* job, uint64_t* ticket) {
     const uint64_t npix = static_cast<uint64_t>(job->params.width) * job->params.height;
     if (npix > S->max_npix) return fail(VP_ERR_INVALID, "request larger than the server's 1024x1024 pixel limit");
     if (job->params.measure_caps) return fail(VP_ERR_INVALID, "measure_caps is not available through the server");
+    if (job->params.adaptive_threshold > 0.0f)
+        return fail(VP_ERR_INVALID, "adaptive sampling is not available through the server (a request runs as one "
+                                    "slice; adaptive sampling needs its 16-sample batches)");
     if (job->params.active_cap > S->L.active_cap)
         return fail(VP_ERR_INVALID, "request asks for an active-set capacity above the server's (" +
                                         std::to_string(S->L.active_cap) + "): autotune with it first");
@@ -603,6 +606,7 @@ vp_status vp_autotune(vp_context* ctx, vp_job* jobs, uint32_t njobs, vp_launch_c
         if (st != VP_OK) return st;
         params[i] = jobs[i].params;
         params[i].measure_caps = 0;
+        params[i].adaptive_threshold = 0.0f;
         // long paths only lengthen each timing run's tail
         params[i].max_depth = std::min<uint32_t>(params[i].max_depth, 16u);
         const double want = static_cast<double>(params[i].spp) * f;  // samples per full-size pixel

@@ -72,9 +72,10 @@ T try_unwrap_or_exit(Result<T>&& res) {
 // types and defaults, so a caller that fills the reference's Config compiles
 // against this one.  The OptiX program and module names have no meaning here
 // (there is no SBT and no module); they are kept for source compatibility
-// and ignored.  active_cap_ and shadow_rr_ are this build's: the reference
-// fixes its active-set capacity at compile time
-// (device/core/constants.cuh:45-48) and has no shadow-ray roulette.
+// and ignored.  active_cap_, shadow_rr_ and the adaptive_ fields are this
+// build's: the reference fixes its active-set capacity and adaptive sampling
+// at compile time (device/core/constants.cuh:45-48,196-216) and has no
+// shadow-ray roulette.
 struct Config {
     std::string raygen_function_name_ = "__raygen__rg";
     std::string miss_function_name_ = "__miss__ms";
@@ -106,12 +107,19 @@ struct Config {
     // NEE shadow-ray roulette threshold (0 = off, the reference's estimator);
     // this build's option, --shadow-rr
     float shadow_rr_ = 0.0f;
+    // adaptive sampling (the reference's compile-time ENABLE_ADAPTIVE_SAMPLING,
+    // ADAPTIVE_THRESHOLD and ADAPTIVE_MIN_SAMPLES, device/core/constants.cuh:
+    // 196-216) as options: --adaptive-threshold (0 = off) and
+    // --adaptive-min-samples
+    float adaptive_threshold_ = 0.0f;
+    uint32_t adaptive_min_samples_ = 32;
 
     // Config::parse (src/thesis/host/app/config.cpp:11-69): the same options
     // (--output --height --width --samples_per_pixel --raygen --miss --anyhit
     // --launch_params --module_blob --env_map --seed --denoise --max-depth
     // --rr-depth --rr-max-survival --firefly-clamp --filter-width --phase-g
-    // --measure-caps, plus --ris --ris-candidates --active-cap --shadow-rr), the same
+    // --measure-caps, plus --ris --ris-candidates --active-cap --shadow-rr
+    // --adaptive-threshold --adaptive-min-samples), the same
     // range checks, an error Result on a bad argument, never an exception.
     [[nodiscard]] static utils::Result<Config> parse(int argc, char* argv[]) noexcept;
 };

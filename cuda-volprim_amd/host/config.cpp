@@ -19,7 +19,8 @@ const char* kUsage =
     "File paths:        --module_blob PATH (accepted, unused) --env_map PATH\n"
     "Runtime tweaks:    --seed N --denoise\n"
     "Render parameters: --max-depth N --rr-depth N --rr-max-survival X --firefly-clamp X --filter-width X\n"
-    "                   --phase-g X --measure-caps --ris --ris-candidates N --active-cap N --shadow-rr X\n";
+    "                   --phase-g X --measure-caps --ris --ris-candidates N --active-cap N --shadow-rr X\n"
+    "                   --adaptive-threshold X --adaptive-min-samples N\n";
 
 bool to_size(const std::string& v, size_t& out) {
     if (v.empty() || v[0] == '-') return false;
@@ -67,7 +68,7 @@ utils::Result<Config> Config::parse(int argc, char* argv[]) noexcept {
                 "--output", "--env_map", "--module_blob", "--raygen", "--miss", "--anyhit", "--launch_params",
                 "--height", "--width", "--samples_per_pixel", "--seed", "--max-depth", "--rr-depth",
                 "--ris-candidates", "--active-cap", "--rr-max-survival", "--firefly-clamp", "--filter-width",
-                "--phase-g", "--shadow-rr"};
+                "--phase-g", "--shadow-rr", "--adaptive-threshold", "--adaptive-min-samples"};
             bool known = false;
             for (const char* k : kValued) known = known || a == k;
             if (!known)
@@ -89,7 +90,8 @@ utils::Result<Config> Config::parse(int argc, char* argv[]) noexcept {
             else if (a == "--anyhit") c.anyhit_function_name_ = val;
             else if (a == "--launch_params") c.launch_params_variable_name_ = val;
             else if (a == "--height" || a == "--width" || a == "--samples_per_pixel" || a == "--seed" ||
-                     a == "--max-depth" || a == "--rr-depth" || a == "--ris-candidates" || a == "--active-cap") {
+                     a == "--max-depth" || a == "--rr-depth" || a == "--ris-candidates" || a == "--active-cap" ||
+                     a == "--adaptive-min-samples") {
                 if (!to_size(val, n)) return bad("not a non-negative integer");
                 if (a == "--height") c.image_height_ = n;
                 else if (a == "--width") c.image_width_ = n;
@@ -100,9 +102,10 @@ utils::Result<Config> Config::parse(int argc, char* argv[]) noexcept {
                     c.max_bounces_ = n;
                 } else if (a == "--rr-depth") c.rr_depth_ = n;
                 else if (a == "--ris-candidates") c.ris_num_candidates_ = static_cast<uint32_t>(n);
+                else if (a == "--adaptive-min-samples") c.adaptive_min_samples_ = static_cast<uint32_t>(n);
                 else c.active_cap_ = static_cast<uint32_t>(n);
             } else if (a == "--rr-max-survival" || a == "--firefly-clamp" || a == "--filter-width" ||
-                       a == "--phase-g" || a == "--shadow-rr") {
+                       a == "--phase-g" || a == "--shadow-rr" || a == "--adaptive-threshold") {
                 if (!to_float(val, f)) return bad("not a number");
                 if (a == "--rr-max-survival") {
                     if (!(f >= 0.0f && f <= 1.0f)) return bad("not in [0, 1]");  // CLI::Range(0, 1)
@@ -116,6 +119,9 @@ utils::Result<Config> Config::parse(int argc, char* argv[]) noexcept {
                 } else if (a == "--shadow-rr") {
                     if (!(f >= 0.0f)) return bad("must be non-negative");
                     c.shadow_rr_ = f;
+                } else if (a == "--adaptive-threshold") {
+                    if (!(f >= 0.0f)) return bad("must be non-negative");
+                    c.adaptive_threshold_ = f;
                 } else {
                     if (!(f >= -0.999f && f <= 0.999f)) return bad("not in [-0.999, 0.999]");  // CLI::Range
                     c.hg_g_ = f;

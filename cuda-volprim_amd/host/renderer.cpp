@@ -124,6 +124,8 @@ void Renderer::render() {
     p.measure_caps = config_.measure_caps_ ? 1u : 0u;
     p.active_cap = config_.active_cap_;
     p.shadow_rr = config_.shadow_rr_;
+    p.adaptive_threshold = config_.adaptive_threshold_;
+    p.adaptive_min_samples = config_.adaptive_min_samples_;
     std::printf("[info] Render params: max_depth=%u rr_depth=%u rr_max_survival=%.3f hg_g=%.3f%s firefly_clamp=%.3g "
                 "filter_stddev=%.3g%s seed=%u ris=%d\n",
                 p.max_depth, p.rr_depth, p.rr_max_survival, p.phase_g,

@@ -37,6 +37,8 @@ struct TestConfig {
     std::string env;
     uint32_t active_cap = 128;
     float shadow_rr = 0.0f;
+    float adaptive_threshold = 0.0f;
+    uint32_t adaptive_min_samples = 32;
 };
 
 void usage() {
@@ -61,7 +63,10 @@ void usage() {
         "  --measure-caps          measure max hits/ray and point overlap, print suggested caps\n"
         "  --env FILE.hdr          environment map override\n"
         "  --active-cap N          active-set capacity (128)\n"
-        "  --shadow-rr F           shadow-ray roulette threshold, 0 = off (unbiased when > 0)\n");
+        "  --shadow-rr F           shadow-ray roulette threshold, 0 = off (unbiased when > 0)\n"
+        "  --adaptive-threshold F  adaptive sampling: stop a pixel once its relative sample deviation is\n"
+        "                          below F (0 = off, the reference's compile-time default)\n"
+        "  --adaptive-min-samples N  samples before a pixel may stop (default 32)\n");
 }
 
 bool parse(int argc, char** argv, TestConfig& c, std::string& err) {
@@ -129,6 +134,11 @@ bool parse(int argc, char** argv, TestConfig& c, std::string& err) {
         else if (a == "--env") { const char* v = next("--env"); if (!v) return false; c.env = v; }
         else if (a == "--active-cap") { if (!num_u("--active-cap", c.active_cap, true)) return false; }
         else if (a == "--shadow-rr") { if (!num_f("--shadow-rr", c.shadow_rr, 0.0f, 3e38f)) return false; }
+        else if (a == "--adaptive-threshold") {
+            if (!num_f("--adaptive-threshold", c.adaptive_threshold, 0.0f, 3e38f)) return false;
+        } else if (a == "--adaptive-min-samples") {
+            if (!num_u("--adaptive-min-samples", c.adaptive_min_samples, false)) return false;
+        }
         else {
             err = "unknown argument '" + a + "'";
             return false;
@@ -158,6 +168,8 @@ vpapp::Config renderer_config(const TestConfig& t, uint32_t w, uint32_t h, const
     c.env_map_path_ = env;
     c.active_cap_ = t.active_cap;
     c.shadow_rr_ = t.shadow_rr;
+    c.adaptive_threshold_ = t.adaptive_threshold;
+    c.adaptive_min_samples_ = t.adaptive_min_samples;
     return c;
 }
 

@@ -49,7 +49,8 @@ class RenderParams(ctypes.Structure):
                 ("rr_max_survival", ctypes.c_float), ("firefly_clamp", ctypes.c_float),
                 ("filter_stddev", ctypes.c_float), ("phase_g", ctypes.c_float),
                 ("use_ris", ctypes.c_uint32), ("ris_candidates", ctypes.c_uint32),
-                ("measure_caps", ctypes.c_uint32), ("active_cap", ctypes.c_uint32), ("shadow_rr", ctypes.c_float)]
+                ("measure_caps", ctypes.c_uint32), ("active_cap", ctypes.c_uint32), ("shadow_rr", ctypes.c_float),
+                ("adaptive_threshold", ctypes.c_float), ("adaptive_min_samples", ctypes.c_uint32)]
 
 
 class RenderStats(ctypes.Structure):

@@ -89,6 +89,14 @@ typedef struct {
     float shadow_rr;          /* --shadow-rr: NEE shadow-ray roulette threshold eps (0 = off, the reference's
                                  estimator); a shadow ray whose term has fallen below eps survives with
                                  probability 1/4 and counts 4x (unbiased; no counterpart in the reference) */
+    float adaptive_threshold; /* adaptive sampling (device/entry/raygen.cuh:37-69,214-233, compile-time off
+                                 in the reference: ENABLE_ADAPTIVE_SAMPLING, constants.cuh:196-216); 0 = off.
+                                 On, the samples run in batches of 16 (renderer.cpp:32; a launch's batch_spp
+                                 overrides it) and a pixel stops once it has adaptive_min_samples or more
+                                 and max over channels of sqrt(M2/(n-1)) / max(mean, 1e-6) < threshold; the
+                                 image is then the per-pixel Welford mean.  Megakernel renders only; not
+                                 with per-sample output, the server or vp_render_device_noalloc. */
+    uint32_t adaptive_min_samples; /* ADAPTIVE_MIN_SAMPLES (default 32) */
 } vp_render_params;
 
 typedef struct {

@@ -1011,6 +1011,9 @@ typedef struct {
     uint32_t stride;                   /* render every stride-th row and column of the crop (0 = 1) */
     uint32_t use_bvh;                  /* queries through a CPU BVH instead of loops over all primitives */
     float shadow_rr;                   /* NEE shadow-ray roulette threshold (0 = off) */
+    float adaptive_threshold;          /* adaptive sampling (0 = off); needs samples [0, spp) */
+    uint32_t adaptive_min;             /* samples before a pixel may stop */
+    uint32_t adaptive_batch;           /* samples per batch (0 = 16) */
 } vpo_params;
 
 typedef struct {
@@ -1023,6 +1026,7 @@ typedef struct {
     hg_t hg;
     float* out_mean;
     float* out_samples;
+    uint32_t* out_counts;
     uint32_t next_row;
     pthread_mutex_t mu;
     counters_t total;
@@ -1339,6 +1343,51 @@ static inline float accum_mean(uint64_t lo, uint64_t hi, uint64_t n) {
     return (float)((s * (1.0 / 4294967296.0)) / (double)n);
 }
 
+/* Adaptive sampling of one pixel (raygen.cuh:37-69,214-233 with
+ * ENABLE_ADAPTIVE_SAMPLING on; batches as in renderer.cpp:341-372): the
+ * samples run in batches; before each batch after the first ones the pixel
+ * stops once n >= the minimum and max over channels of sqrt(M2/(n-1)) /
+ * max(mean, 1e-6) is below the threshold.  Each sample is folded into the
+ * running Welford mean and M2 in sample order.  The mean is the pixel's
+ * value and n its sample count. */
+static void adaptive_pixel(job_t* J, uint32_t x, uint32_t y, uint32_t* buf, uint32_t* cand, counters_t* k) {
+    const vpo_params* P = J->p;
+    const uint32_t batch = P->adaptive_batch ? P->adaptive_batch : 16u;
+    float mean[3] = {0.0f, 0.0f, 0.0f}, M2[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t n = 0;
+    for (uint32_t s0 = 0; s0 < P->spp; s0 += batch) {
+        if (n >= P->adaptive_min) {
+            const float r = 1.0f / (float)(n - 1u);
+            float err = 0.0f;
+            for (int c = 0; c < 3; ++c) {
+                const float e = sqrtf(M2[c] * r) / vpo_max(mean[c], 1e-6f);
+                err = c == 0 ? e : vpo_max(err, e);
+            }
+            if (err < P->adaptive_threshold) break;
+        }
+        const uint32_t cnt = P->spp - s0 < batch ? P->spp - s0 : batch;
+        for (uint32_t j = 0; j < cnt; ++j) {
+            v3 Ls = trace_sample(J, x, y, s0 + j, buf, cand, k);
+            const float xs[3] = {Ls.x, Ls.y, Ls.z};
+            const float n_inv = 1.0f / (float)(n + j + 1u);
+            for (int c = 0; c < 3; ++c) {
+                const float d1 = xs[c] - mean[c];
+                mean[c] = mean[c] + d1 * n_inv;
+                M2[c] = M2[c] + d1 * (xs[c] - mean[c]);
+            }
+        }
+        n += cnt;
+    }
+    const size_t idx = (size_t)(y - P->y0) * (P->x1 - P->x0) + (x - P->x0);
+    if (J->out_mean) {
+        J->out_mean[4 * idx] = mean[0];
+        J->out_mean[4 * idx + 1] = mean[1];
+        J->out_mean[4 * idx + 2] = mean[2];
+        J->out_mean[4 * idx + 3] = 1.0f;
+    }
+    if (J->out_counts) J->out_counts[idx] = n;
+}
+
 static void* worker(void* arg) {
     job_t* J = (job_t*)arg;
     const vpo_params* P = J->p;
@@ -1354,6 +1403,10 @@ static void* worker(void* arg) {
         const uint32_t st = P->stride ? P->stride : 1u;
         if ((y - P->y0) % st) continue;
         for (uint32_t x = P->x0; x < P->x1; x += st) {
+            if (P->adaptive_threshold > 0.0f) {
+                adaptive_pixel(J, x, y, buf, cand, &k);
+                continue;
+            }
             /* mean over the samples (raygen.cuh:214-219), as an exact
              * fixed-point sum */
             accum_t acc;
@@ -1393,9 +1446,14 @@ static void* worker(void* arg) {
  * uint64 in counters_t order.  Returns 0 on success. */
 int vpo_render(const float* prims16, long n, const float* env_rgba, const float* joint, const float* prob,
                const uint32_t* alias, float env_total, uint32_t ew, uint32_t eh, const float* cam_pod,
-               const vpo_params* p, float* out_mean, float* out_samples, uint64_t* counters, int threads) {
+               const vpo_params* p, float* out_mean, float* out_samples, uint64_t* counters, int threads,
+               uint32_t* out_counts) {
     if (p->active_cap == 0 || p->x1 > p->width || p->y1 > p->height || p->x0 >= p->x1 || p->y0 >= p->y1 ||
         p->sample_end > p->spp || p->sample_begin >= p->sample_end || ew == 0 || eh == 0)
+        return 1;
+    /* adaptive sampling covers all samples and writes no per-sample values */
+    if (p->adaptive_threshold > 0.0f &&
+        (p->sample_begin != 0 || p->sample_end != p->spp || out_samples || p->adaptive_min == 0))
         return 1;
     job_t J;
     memset(&J, 0, sizeof(J));
@@ -1413,6 +1471,7 @@ int vpo_render(const float* prims16, long n, const float* env_rgba, const float*
     J.hg = hg_make(p->phase_g);
     J.out_mean = out_mean;
     J.out_samples = out_samples;
+    J.out_counts = out_counts;
     J.next_row = p->y0;
     cbvh_t bvh = {NULL, 0};
     if (p->use_bvh) {

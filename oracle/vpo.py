@@ -38,7 +38,9 @@ class Params(ctypes.Structure):
                 ("x0", ctypes.c_uint32), ("y0", ctypes.c_uint32),
                 ("x1", ctypes.c_uint32), ("y1", ctypes.c_uint32),
                 ("sample_begin", ctypes.c_uint32), ("sample_end", ctypes.c_uint32),
-                ("stride", ctypes.c_uint32), ("use_bvh", ctypes.c_uint32), ("shadow_rr", ctypes.c_float)]
+                ("stride", ctypes.c_uint32), ("use_bvh", ctypes.c_uint32), ("shadow_rr", ctypes.c_float),
+                ("adaptive_threshold", ctypes.c_float), ("adaptive_min", ctypes.c_uint32),
+                ("adaptive_batch", ctypes.c_uint32)]
 
 
 def build():
@@ -75,7 +77,7 @@ def load():
     lib.vpo_render.argtypes = [_f32p, ctypes.c_long, _f32p, _f32p, ctypes.c_void_p,
                                ctypes.c_void_p, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
                                _f32p, ctypes.POINTER(Params), ctypes.c_void_p, ctypes.c_void_p,
-                               _u64p, ctypes.c_int]
+                               _u64p, ctypes.c_int, ctypes.c_void_p]
     lib.vpo_render.restype = ctypes.c_int
     lib.vpo_phase_eval.argtypes = [ctypes.c_float, _f32p, _f32p, ctypes.c_long, _f32p]
     lib.vpo_phase_sample.argtypes = [ctypes.c_float, _f32p, ctypes.c_uint64, ctypes.c_long, _f32p, _f32p]
@@ -227,11 +229,15 @@ def camera_orthographic(w, h, eye, target, up, ortho_h):
 def render(prims16, env_rgba, cam_pod, width, height, spp, seed=42, max_depth=128, rr_depth=12,
            rr_max_survival=0.99, firefly_clamp=0.0, filter_stddev=0.0, phase_g=0.85,
            use_ris=False, ris_candidates=6, active_cap=128, measure=False, crop=None,
-           samples=None, keep_samples=False, threads=None, env_tables_=None, stride=1, bvh=False, shadow_rr=0.0):
+           samples=None, keep_samples=False, threads=None, env_tables_=None, stride=1, bvh=False, shadow_rr=0.0,
+           adaptive_threshold=0.0, adaptive_min_samples=32, adaptive_batch=16):
     """CPU restatement of the estimator.  Returns (mean[h,w,4], samples or None,
     counters dict).  crop = (x0, y0, x1, y1); samples = (begin, end).
     bvh=True answers the ray and point queries through a CPU BVH instead of
-    loops over every primitive (identical images, faster at large N)."""
+    loops over every primitive (identical images, faster at large N).
+    adaptive_threshold > 0: adaptive sampling in batches of adaptive_batch;
+    the mean is the Welford mean and the counters dict gains
+    "samples_per_pixel" (uint32 [h, w])."""
     prims16 = f32(prims16).reshape(-1, 16)
     env_rgba = f32(env_rgba)
     eh, ew = env_rgba.shape[0], env_rgba.shape[1]
@@ -242,18 +248,24 @@ def render(prims16, env_rgba, cam_pod, width, height, spp, seed=42, max_depth=12
     s0, s1 = samples if samples is not None else (0, spp)
     p = Params(width, height, spp, seed, max_depth, rr_depth, rr_max_survival, firefly_clamp,
                filter_stddev, phase_g, int(use_ris), ris_candidates, active_cap, int(measure),
-               x0, y0, x1, y1, s0, s1, stride, int(bvh), float(shadow_rr))
+               x0, y0, x1, y1, s0, s1, stride, int(bvh), float(shadow_rr), float(adaptive_threshold),
+               int(adaptive_min_samples), int(adaptive_batch))
     mean = np.zeros((y1 - y0, x1 - x0, 4), np.float32)
     smp = np.zeros((s1 - s0, y1 - y0, x1 - x0, 4), np.float32) if keep_samples else None
     counters = np.zeros(len(COUNTER_NAMES), np.uint64)
+    counts = np.zeros((y1 - y0, x1 - x0), np.uint32) if adaptive_threshold > 0 else None
     threads = threads or os.cpu_count() or 1
     rc = lib().vpo_render(prims16, prims16.shape[0], env_rgba.reshape(-1), joint,
                           prob.ctypes.data, alias.ctypes.data, total, ew, eh, f32(cam_pod),
                           ctypes.byref(p), mean.ctypes.data,
-                          smp.ctypes.data if smp is not None else None, counters, threads)
+                          smp.ctypes.data if smp is not None else None, counters, threads,
+                          counts.ctypes.data if counts is not None else None)
     if rc != 0:
         raise ValueError("vpo_render rejected its arguments")
-    return mean, smp, dict(zip(COUNTER_NAMES, counters.tolist()))
+    k = dict(zip(COUNTER_NAMES, counters.tolist()))
+    if counts is not None:
+        k["samples_per_pixel"] = counts
+    return mean, smp, k
 
 
 def phase_eval(g, wi, wo):
