@@ -370,26 +370,34 @@ VP_HD float free_flight_tau(float xi) { return -logf_(maxf(1.0f - xi, 1e-10f)); 
 // Shadow-ray roulette, an estimator option of this build (the reference walks
 // every shadow ray to the tau = 88 saturation, sampling.cuh:548-583).  A NEE
 // shadow ray whose contribution is w * T(tau) (w: the largest channel of the
-// term T multiplies) plays one Russian roulette when its optical depth
-// reaches tau_c = log(w / eps), where w e^(-tau) has fallen below eps: it
-// survives with probability 1/4 and then counts 4 T, else it counts 0.  The
-// expectation is T, so the estimator stays unbiased, and the added variance
-// per ray is below 3 eps^2.  Whether the depth reaches tau_c does not depend
-// on the order chords are summed in, and the survival draw is a counter hash
-// of (path, bounce, ray), so a walk that does not survive stops at tau_c and
-// the result matches a full walk bit for bit.  eps = 0 turns it off.
-// Thresholds are 24.40 fixed-point depths, multiples of 2^16.
+// term T multiplies) plays a Russian roulette at each of the depths
+// theta_j = tau_c + j ln 4 (j = 0, 1, ...) it reaches, tau_c = log(w / eps)
+// being where w e^(-tau) falls below eps: it survives each with probability
+// 1/4, and a ray that reached L levels and survived them all counts
+// 4^L T, else 0.  E[4^L T 1(survived L)] = T, so the estimator stays
+// unbiased; each level holds the surviving contribution 4^L w e^(-tau) near
+// eps, so the variance added per ray stays of order eps^2.  The survival
+// draws are counter hashes of (path, bounce, ray, level), so the first level
+// a ray fails is known before its walk starts: the walk stops at that depth
+// (or at the tau = 88 saturation), and since whether a sum reaches a depth
+// does not depend on the order chords are summed in, the result matches a
+// full walk bit for bit.  eps = 0 turns it off.  Depths are 24.40
+// fixed-point; thresholds and the level step are multiples of 2^24.
 // ---------------------------------------------------------------------------
+constexpr uint64_t kRrGrain = 0xffffffull;                      // thresholds are multiples of 2^24
+constexpr uint64_t kRrStep = 1524246769571ull & ~kRrGrain;      // ln 4 * 2^40, rounded down
+constexpr uint32_t kRrNoFail = 127u;                             // no failing level below the saturation
 VP_HD uint64_t rr_threshold(float w, float eps) {
     if (!(eps > 0.0f)) return kTauLimitFixed;
     if (!(w > eps)) return 0ull;
     const float tc = logf_(w * rcpf(eps));
     if (!(tc > 0.0f)) return 0ull;
     if (tc >= kMaxTau) return kTauLimitFixed;
-    return static_cast<uint64_t>(tc * kTauScaleF) & ~0xffffull;
+    return static_cast<uint64_t>(tc * kTauScaleF) & ~kRrGrain;
 }
-VP_HD float rr_uniform(uint64_t key, uint32_t bounce, uint32_t ray) {
-    uint64_t x = (key ^ 0xd1b54a32d192ed03ull) + 0x9e3779b97f4a7c15ull * ((static_cast<uint64_t>(bounce) << 8) | ray);
+VP_HD float rr_uniform(uint64_t key, uint32_t bounce, uint32_t ray, uint32_t level) {
+    const uint64_t c = (static_cast<uint64_t>(bounce) << 16) | (static_cast<uint64_t>(ray) << 8) | level;
+    uint64_t x = (key ^ 0xd1b54a32d192ed03ull) + 0x9e3779b97f4a7c15ull * c;
     x ^= x >> 30;
     x *= 0xbf58476d1ce4e5b9ull;
     x ^= x >> 27;
@@ -397,23 +405,34 @@ VP_HD float rr_uniform(uint64_t key, uint32_t bounce, uint32_t ray) {
     x ^= x >> 31;
     return u32_to_unit(static_cast<uint32_t>(x >> 32));
 }
-// The roulette of one shadow ray: the depth at which its walk may stop
-// (kTauLimitFixed when it survives or the roulette is off) and the threshold.
+// The roulette of one shadow ray: its first level tau_c (kTauLimitFixed when
+// the roulette is off) and the first level it fails, fail (kRrNoFail when it
+// survives every level below the saturation); its walk may stop at limit().
 struct ShadowRr {
-    uint64_t thr, limit;
-    bool survive;
+    uint64_t thr;
+    uint32_t fail;
+    VP_HD uint64_t limit() const { return fail == kRrNoFail ? kTauLimitFixed : thr + fail * kRrStep; }
 };
 VP_HD ShadowRr shadow_rr(float w, float eps, uint64_t key, uint32_t bounce, uint32_t ray) {
     ShadowRr r;
     r.thr = rr_threshold(w, eps);
-    r.survive = r.thr < kTauLimitFixed && rr_uniform(key, bounce, ray) < 0.25f;
-    r.limit = r.survive ? kTauLimitFixed : r.thr;
+    r.fail = kRrNoFail;
+    for (uint32_t j = 0; r.thr + j * kRrStep < kTauLimitFixed; ++j)
+        if (!(rr_uniform(key, bounce, ray, j) < 0.25f)) {
+            r.fail = j;
+            break;
+        }
     return r;
 }
-// Transmittance from the summed depth acc under the roulette rr.
+// Transmittance from the summed depth acc (T = depth_T(acc)) under the
+// roulette rr: 0 past the failing level, else T times 4 per level reached
+// (two exact power-of-two scalings: at most 64 levels below tau = 88).
 VP_HD float shadow_rr_T(uint64_t acc, const ShadowRr& rr, float T) {
-    if (acc >= rr.thr) return rr.survive ? T * 4.0f : 0.0f;
-    return T;
+    if (acc >= rr.limit()) return 0.0f;
+    if (acc < rr.thr) return T;
+    const int e = 2 * static_cast<int>((acc - rr.thr) / kRrStep + 1u);
+    const int e1 = e >> 1;
+    return (T * exp2i_(e1)) * exp2i_(e - e1);
 }
 
 // ---------------------------------------------------------------------------

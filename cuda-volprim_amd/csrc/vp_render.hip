@@ -564,8 +564,8 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                         const f3 cy = base * (yenv * rcpf(lumy));
                         const ShadowRr rr = shadow_rr(max3(cy) * Wr, J.shadow_rr, st.key, b, 0u);
                         uint64_t acc, acc_unused;
-                        if constexpr (kCount) acc = shadow_depth(sc, x, ydir, act, nn, stride, lq, ev, rr.limit);
-                        else transmittance_pool(sc, x, ydir, ydir, act, nn, stride, wq, rr.limit, 0ull, acc, acc_unused,
+                        if constexpr (kCount) acc = shadow_depth(sc, x, ydir, act, nn, stride, lq, ev, rr.limit());
+                        else transmittance_pool(sc, x, ydir, ydir, act, nn, stride, wq, rr.limit(), 0ull, acc, acc_unused,
                                                 false);
                         const float T = shadow_rr_T(acc, rr, depth_T(acc));
                         st.L = st.L + (cy * T) * Wr;
@@ -595,10 +595,10 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                 // so its counts are per-ray work
                 uint64_t acc_a, acc_b;
                 if constexpr (kCount) {
-                    acc_a = shadow_depth(sc, x, awo, act, nn, stride, lq, ev, rra.limit);
-                    acc_b = shadow_depth(sc, x, bwo, act, nn, stride, lq, ev, rrb.limit);
+                    acc_a = shadow_depth(sc, x, awo, act, nn, stride, lq, ev, rra.limit());
+                    acc_b = shadow_depth(sc, x, bwo, act, nn, stride, lq, ev, rrb.limit());
                 } else {
-                    transmittance_pool(sc, x, awo, bwo, act, nn, stride, wq, rra.limit, rrb.limit, acc_a, acc_b);
+                    transmittance_pool(sc, x, awo, bwo, act, nn, stride, wq, rra.limit(), rrb.limit(), acc_a, acc_b);
                 }
                 const float Ta = shadow_rr_T(acc_a, rra, depth_T(acc_a));
                 const float Tb = shadow_rr_T(acc_b, rrb, depth_T(acc_b));

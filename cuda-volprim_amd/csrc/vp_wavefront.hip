@@ -341,15 +341,15 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
 
 // Pending radiance term k of slot i: contribution (((w*T)*v)*s1)*s2 once the
 // shadow kernel has written T.  Until then pd.w carries the ray's shadow
-// roulette: its threshold >> 16, and the survival flag in bit 31.
+// roulette: its first threshold >> 24 in bits 0..22 (at most 88 * 2^16) and
+// its failing level in bits 23..29.
 __device__ __forceinline__ uint32_t rr_bits(const ShadowRr& rr) {
-    return static_cast<uint32_t>(rr.thr >> 16) | (rr.survive ? 0x80000000u : 0u);
+    return static_cast<uint32_t>(rr.thr >> 24) | (rr.fail << 23);
 }
 __device__ __forceinline__ ShadowRr rr_unpack(uint32_t bits) {
     ShadowRr rr;
-    rr.thr = static_cast<uint64_t>(bits & 0x7fffffffu) << 16;
-    rr.survive = (bits >> 31) != 0u;
-    rr.limit = rr.survive ? kTauLimitFixed : rr.thr;
+    rr.thr = static_cast<uint64_t>(bits & 0x7fffffu) << 24;
+    rr.fail = bits >> 23;
     return rr;
 }
 __device__ __forceinline__ void put_pending(const WfPool& w, uint32_t i, uint32_t k, f3 org, uint32_t set,
@@ -362,7 +362,8 @@ __device__ __forceinline__ void put_pending(const WfPool& w, uint32_t i, uint32_
     w.pv[e] = make_float4(v.x, v.y, v.z, s2);
 }
 // the bounce-0 direct term has no roulette
-constexpr uint32_t kRrOff = static_cast<uint32_t>(kTauLimitFixed >> 16);
+constexpr uint32_t kRrOff = static_cast<uint32_t>(kTauLimitFixed >> 24) | (kRrNoFail << 23);
+static_assert((kTauLimitFixed >> 24) < (1u << 23) && (kTauLimitFixed & kRrGrain) == 0, "roulette bits fit pd.w");
 
 __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par,
                                       unsigned long long* __restrict__ counters) {
@@ -602,12 +603,13 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
             const uint32_t* act = w.slab + (static_cast<size_t>(i) * kSlabBuffers + set) * w.cap;
             rr = rr_unpack(__float_as_uint(pd.w));
             uint64_t acc = 0;
-            bool full = acc >= rr.limit;
+            const uint64_t lim = rr.limit();
+            bool full = acc >= lim;
             for (uint32_t c = 0; c < count && !full; ++c) {
                 const float tc = inside_chord(load_prim(sc.prims, act[c]), o, d).tau;
                 if (tc > 0.0f) {
                     if (tau_add(acc, tc)) acc = kTauLimitFixed;
-                    full = acc >= rr.limit;
+                    full = acc >= lim;
                 }
             }
             lds_store3(wq.ro + 3 * lane, o);
@@ -651,7 +653,7 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
         wave_sync();
         if (have) {
             const unsigned long long acc = wq.acc[lane];
-            if (acc >= rr.limit || w4w.n == kWalkEnd) {
+            if (acc >= rr.limit() || w4w.n == kWalkEnd) {
                 w.pd[e].w = shadow_rr_T(acc, rr, depth_T(acc));
                 have = false;
             }

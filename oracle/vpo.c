@@ -609,23 +609,29 @@ static float transmittance(const prim_t* prims, long n, v3 o, v3 d, const uint32
 
 /* Shadow-ray roulette (an option of this build; the reference walks every
  * shadow ray to the tau = 88 saturation).  A shadow ray whose contribution
- * is w * T(tau) plays a Russian roulette once its optical depth passes
- * tau_c = log(w / eps), where w e^(-tau) has fallen below eps: it survives
- * with probability 1/4 and then counts 4 T.  The expected value is T, so the
- * estimator stays unbiased; the added variance per ray is at most
- * 3 eps^2.  Whether tau passes tau_c does not depend on the order chords are
- * summed in, and the survival draw is a counter hash of (path, bounce, ray),
- * so the walk can stop at tau_c when the ray does not survive. */
+ * is w * T(tau) plays a Russian roulette at each depth theta_j = tau_c +
+ * j ln 4 it reaches (tau_c = log(w / eps), where w e^(-tau) falls below
+ * eps), surviving each with probability 1/4; a ray that reached L levels and
+ * survived them counts 4^L T, else 0.  The expectation is T, so the
+ * estimator stays unbiased.  Whether tau passes a depth does not depend on
+ * the order chords are summed in, and the survival draws are counter hashes
+ * of (path, bounce, ray, level), so the walk can stop at the first level the
+ * ray fails.  Depths are 24.40 fixed point; thresholds and the step are
+ * multiples of 2^24. */
+#define RR_GRAIN 0xffffffull
+#define RR_STEP (1524246769571ull & ~RR_GRAIN) /* ln 4 * 2^40 */
+#define RR_NO_FAIL 127u
 static inline uint64_t rr_threshold(float w, float eps) {
     if (!(eps > 0.0f)) return TAU_LIMIT_FIXED;
     if (!(w > eps)) return 0;
     float tc = vpo_logf(w * vpo_rcp(eps));
     if (!(tc > 0.0f)) return 0;
     if (tc >= 88.0f) return TAU_LIMIT_FIXED;
-    return ((uint64_t)(tc * 1099511627776.0f)) & ~(uint64_t)0xffff;
+    return ((uint64_t)(tc * 1099511627776.0f)) & ~RR_GRAIN;
 }
-static inline float rr_uniform(uint64_t key, uint32_t bounce, uint32_t ray) {
-    uint64_t x = (key ^ 0xd1b54a32d192ed03ull) + 0x9e3779b97f4a7c15ull * (((uint64_t)bounce << 8) | ray);
+static inline float rr_uniform(uint64_t key, uint32_t bounce, uint32_t ray, uint32_t level) {
+    const uint64_t c = ((uint64_t)bounce << 16) | ((uint64_t)ray << 8) | level;
+    uint64_t x = (key ^ 0xd1b54a32d192ed03ull) + 0x9e3779b97f4a7c15ull * c;
     x ^= x >> 30;
     x *= 0xbf58476d1ce4e5b9ull;
     x ^= x >> 27;
@@ -633,14 +639,47 @@ static inline float rr_uniform(uint64_t key, uint32_t bounce, uint32_t ray) {
     x ^= x >> 31;
     return u01((uint32_t)(x >> 32));
 }
+/* first failing level of a ray's roulette (RR_NO_FAIL: none below tau = 88) */
+static inline uint32_t rr_fail(uint64_t thr, uint64_t key, uint32_t bounce, uint32_t ray) {
+    for (uint32_t j = 0; thr + j * RR_STEP < TAU_LIMIT_FIXED; ++j)
+        if (!(rr_uniform(key, bounce, ray, j) < 0.25f)) return j;
+    return RR_NO_FAIL;
+}
+static inline uint64_t rr_limit(uint64_t thr, uint32_t fail) {
+    return fail == RR_NO_FAIL ? TAU_LIMIT_FIXED : thr + fail * RR_STEP;
+}
+/* T = depth_to_T(acc) under the roulette: 0 past the failing level, else
+ * T * 4^(levels reached) in two exact power-of-two scalings */
+static inline float rr_apply(uint64_t acc, uint64_t thr, uint32_t fail, float T) {
+    if (acc >= rr_limit(thr, fail)) return 0.0f;
+    if (acc < thr) return T;
+    const int e = 2 * (int)((acc - thr) / RR_STEP + 1u);
+    const int e1 = e >> 1;
+    return (T * vpo_exp2i(e1)) * vpo_exp2i(e - e1);
+}
 static float shadow_T(const prim_t* prims, long n, v3 o, v3 d, const uint32_t* set, int nset, counters_t* k,
                       const cbvh_t* B, float w, float eps, uint64_t key, uint32_t bounce, uint32_t ray) {
-    uint64_t thr = rr_threshold(w, eps);
-    int survive = thr < TAU_LIMIT_FIXED && rr_uniform(key, bounce, ray) < 0.25f;
-    uint64_t acc = shadow_depth(prims, n, o, d, set, nset, k, B, survive ? TAU_LIMIT_FIXED : thr);
-    float T = depth_to_T(acc);
-    if (acc >= thr) T = survive ? T * 4.0f : 0.0f;
-    return T;
+    const uint64_t thr = rr_threshold(w, eps);
+    const uint32_t fail = rr_fail(thr, key, bounce, ray);
+    const uint64_t acc = shadow_depth(prims, n, o, d, set, nset, k, B, rr_limit(thr, fail));
+    return rr_apply(acc, thr, fail, depth_to_T(acc));
+}
+
+/* Test access: the roulette's estimates of T for rays of weight w whose
+ * depth is tau (walked in full), one per key, and their first failing
+ * levels; the return value is the fixed-point threshold tau_c. */
+uint64_t vpo_shadow_rr_T(float w, float eps, const uint64_t* keys, long m, uint32_t bounce, uint32_t ray, double tau,
+                         float* T_out, uint32_t* fail_out) {
+    const uint64_t thr = rr_threshold(w, eps);
+    const double a = tau * 1099511627776.0;
+    const uint64_t acc = a >= (double)TAU_LIMIT_FIXED ? TAU_LIMIT_FIXED : (uint64_t)a;
+    const float T = depth_to_T(acc);
+    for (long i = 0; i < m; ++i) {
+        const uint32_t fail = rr_fail(thr, keys[i], bounce, ray);
+        fail_out[i] = fail;
+        T_out[i] = rr_apply(acc, thr, fail, T);
+    }
+    return thr;
 }
 
 /* Rays with origin-inside set = every primitive whose bound contains the

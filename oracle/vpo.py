@@ -79,6 +79,9 @@ def load():
                                _f32p, ctypes.POINTER(Params), ctypes.c_void_p, ctypes.c_void_p,
                                _u64p, ctypes.c_int, ctypes.c_void_p]
     lib.vpo_render.restype = ctypes.c_int
+    lib.vpo_shadow_rr_T.argtypes = [ctypes.c_float, ctypes.c_float, _u64p, ctypes.c_long, ctypes.c_uint32,
+                                    ctypes.c_uint32, ctypes.c_double, _f32p, _u32p]
+    lib.vpo_shadow_rr_T.restype = ctypes.c_uint64
     lib.vpo_phase_eval.argtypes = [ctypes.c_float, _f32p, _f32p, ctypes.c_long, _f32p]
     lib.vpo_phase_sample.argtypes = [ctypes.c_float, _f32p, ctypes.c_uint64, ctypes.c_long, _f32p, _f32p]
     lib.vpo_env_sample.argtypes = [_f32p, _f32p, _f32p, _u32p, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
@@ -266,6 +269,16 @@ def render(prims16, env_rgba, cam_pod, width, height, spp, seed=42, max_depth=12
     if counts is not None:
         k["samples_per_pixel"] = counts
     return mean, smp, k
+
+
+def shadow_rr_T(w, eps, keys, tau, bounce=0, ray=0):
+    """The shadow-ray roulette's estimates of T = e^-tau for one ray of
+    weight w per key, walked in full: (T[m], first failing level[m], tau_c)."""
+    keys = np.ascontiguousarray(keys, np.uint64)
+    T = np.empty(keys.size, np.float32)
+    fail = np.empty(keys.size, np.uint32)
+    thr = lib().vpo_shadow_rr_T(w, eps, keys, keys.size, bounce, ray, tau, T, fail)
+    return T, fail, thr / 2.0 ** 40
 
 
 def phase_eval(g, wi, wo):

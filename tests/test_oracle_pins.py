@@ -585,6 +585,39 @@ def test_shadow_roulette_is_unbiased_and_stops_walks():
     assert np.mean(np.abs(d) > 3 * s + 1e-6) < 0.02
 
 
+def test_shadow_roulette_levels_are_unbiased():
+    # The roulette's estimate of T = e^-tau for one ray, over many paths
+    # (keys): below tau_c = log(w / eps) it is T itself; past it the ray has
+    # reached L = floor((tau - tau_c) / ln 4) + 1 levels and the estimate is
+    # 4^L T with probability 4^-L (all L draws survive), else 0, so its mean
+    # is T.  The first failing level is geometric with P(fail) = 3/4.
+    rng = np.random.default_rng(5)
+    keys = rng.integers(0, 2**63, size=400_000, dtype=np.uint64)
+    w, eps = 2.0, 1e-3
+    T0, fail, tc = vpo.shadow_rr_T(w, eps, keys, 0.0)
+    assert abs(tc - math.log(w / eps)) < 2.0 ** -15
+    counts = np.bincount(np.minimum(fail, 6), minlength=7)[:4] / keys.size
+    assert np.allclose(counts, [0.75, 0.1875, 0.046875, 0.01171875], atol=4e-3), counts
+    step = math.log(4.0)
+    for tau in [0.3, tc - 0.01, tc + 0.01, tc + 0.5 * step, tc + 1.2 * step, tc + 2.7 * step, tc + 4.5 * step]:
+        T, _, _ = vpo.shadow_rr_T(w, eps, keys, tau)
+        exact = math.exp(-tau)
+        if tau < tc:
+            assert np.allclose(T, np.float32(exact), rtol=2e-6, atol=0), tau
+            continue
+        L = int((tau - tc) // step) + 1
+        alive = T > 0
+        assert np.allclose(T[alive], np.float32(exact) * 4.0 ** L, rtol=2e-6), (tau, L)
+        p = 4.0 ** -L
+        assert abs(alive.mean() - p) < 4.0 * math.sqrt(p * (1 - p) / keys.size), (tau, alive.mean(), p)
+        se = math.sqrt(exact * exact * (4.0 ** L - 1.0) / keys.size)
+        assert abs(float(T.astype(np.float64).mean()) - exact) < 4.0 * se + 1e-12, (tau, T.mean(), exact, se)
+    # the roulette off: T for every key
+    Toff, foff, tcoff = vpo.shadow_rr_T(w, 0.0, keys[:1000], 7.0)
+    assert (Toff == np.float32(math.exp(-7.0))).all() or np.allclose(Toff, math.exp(-7.0), rtol=2e-6)
+    assert tcoff == 88.0
+
+
 def test_shadow_roulette_keeps_the_furnace():
     rows = scenes.random_scene(24, 12, albedo=1.0)
     rows[:, 10:13] = 1.0
