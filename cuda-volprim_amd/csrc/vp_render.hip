@@ -69,13 +69,16 @@ struct PathState {
 // which leaf.  Candidates for the rebuild are recorded while t_entry is at
 // most the running minimum at the time, which covers every primitive entered
 // before the final one; their order is fixed by the sort in the rebuild.  At
-// bounce 0 (direct) the walk runs over the whole ray and also sums the direct
-// term's optical depth.  ts_origin is the minimum over the origin set; the
-// result keeps it unless a walk candidate is strictly smaller.
+// bounce 0 (direct) the walk also sums the direct term's optical depth into
+// dacc, and runs over the whole ray until that sum reaches dlimit (the
+// tau = 88 saturation or the direct term's roulette level, where T becomes
+// 0); from there on it prunes at the running minimum like any other ray.
+// ts_origin is the minimum over the origin set; the result keeps it unless a
+// walk candidate is strictly smaller.
 __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3 d, uint64_t key, uint32_t b,
                                                    bool direct, const uint32_t* act, uint32_t na, float* act_exit,
                                                    uint32_t* __restrict__ slab, uint32_t cap, const WaveQueue& wq,
-                                                   float& ts, uint32_t& nc, float& Tdirect) {
+                                                   float& ts, uint32_t& nc, uint64_t dlimit, uint64_t& dacc) {
     const uint32_t lane = __lane_id();
     lds_store3(wq.ro + 3 * lane, o);
     lds_store3(wq.rd + 3 * lane, d);
@@ -138,7 +141,7 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
     wave_sync();
     const float ts_origin = __uint_as_float(wq.ts[lane]);
     const f3 inv = safe_inv(d), oi = o * inv;
-    float tmax = direct ? kTMax : ts_origin;
+    float tmax = direct && wq.acc[lane] < dlimit ? kTMax : ts_origin;
     Walk4 w = walk4_begin();
     // the first thread of this wavefront, for the owners' candidate buffers
     const uint32_t gbase = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
@@ -217,14 +220,13 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
             atomicMin(&wq.ts[own], t == 0.0f ? 0u : __float_as_uint(t));
         }
         wave_sync();
-        if (!direct) tmax = __uint_as_float(wq.ts[lane]);
+        if (!direct || wq.acc[lane] >= dlimit) tmax = __uint_as_float(wq.ts[lane]);
     }
     wave_sync();
     const float wmin = __uint_as_float(wq.ts[lane]);
     ts = wmin < ts_origin ? wmin : ts_origin;
     nc = wq.nc[lane];
-    const unsigned long long acc = wq.acc[lane];
-    Tdirect = acc >= kTauLimitFixed ? 0.0f : tau_to_T(acc);
+    dacc = wq.acc[lane];
 }
 
 // launch_desc[0] = number of job slices, launch_desc[1] = end of the
@@ -376,9 +378,19 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
         // among them, because the walk only prunes beyond the running one)
         uint32_t* cnd = my + 2 * cap * stride;
         uint32_t nc = 0;
-        float Tdirect = 0.0f;
+        // the analytic direct term's factor and its roulette (a no-op unless
+        // J.shadow_rr > 0), known before the walk that integrates its depth
+        const EnvView env = env_view(J.env);
+        f3 envd = mk(0.0f);
+        ShadowRr drr{kTauLimitFixed, kRrNoFail};
+        if (b == 0) {
+            envd = env_lookup(env, d);
+            drr = shadow_rr(max3(st.beta * envd), J.shadow_rr, st.key, 0u, 2u);
+        }
+        uint64_t dwalk = 0;
         if (!kCount && !measure) {
-            argmin_walk_shared(sc, o, d, st.key, b, direct, act, na, act_exit, slab, cap, wq, ts, nc, Tdirect);
+            argmin_walk_shared(sc, o, d, st.key, b, direct, act, na, act_exit, slab, cap, wq, ts, nc, drr.limit(),
+                               dwalk);
         } else {
             const f3 inv = safe_inv(d);
             const bool prune = !measure && !direct;
@@ -418,14 +430,14 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
             if (measure) atomicMax(&counters[kCntMaxHits], static_cast<unsigned long long>(hits));
         }
 
-        const EnvView env = env_view(J.env);
         // analytic direct term at bounce 0 (raygen.cuh:148-160)
         if (b == 0) {
             ev.add(kCntShadowRays);
-            float T;
-            if constexpr (kCount) T = transmittance(sc, o, d, act, na, stride, lq, ev);
-            else T = measure ? (dfull ? 0.0f : tau_to_T(dacc)) : Tdirect;
-            st.L = st.L + (st.beta * T) * env_lookup(env, d);
+            uint64_t a;
+            if constexpr (kCount) a = shadow_depth(sc, o, d, act, na, stride, lq, ev, drr.limit());
+            else a = measure ? (dfull ? kTauLimitFixed : dacc) : dwalk;
+            const float T = shadow_rr_T(a, drr, depth_T(a));
+            st.L = st.L + (st.beta * T) * envd;
         }
 
         if (ts >= kTMax) {

@@ -361,8 +361,6 @@ __device__ __forceinline__ void put_pending(const WfPool& w, uint32_t i, uint32_
     w.pw[e] = make_float4(wt.x, wt.y, wt.z, s1);
     w.pv[e] = make_float4(v.x, v.y, v.z, s2);
 }
-// the bounce-0 direct term has no roulette
-constexpr uint32_t kRrOff = static_cast<uint32_t>(kTauLimitFixed >> 24) | (kRrNoFail << 23);
 static_assert((kTauLimitFixed >> 24) < (1u << 23) && (kTauLimitFixed & kRrGrain) == 0, "roulette bits fit pd.w");
 
 __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w, uint32_t par,
@@ -403,8 +401,12 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
             uint32_t nn = na;
 
             // analytic direct term at bounce 0 (raygen.cuh:148-160): T along the
-            // camera ray with the origin set
-            if (b == 0) put_pending(w, i, npend++, o, cur, na, d, beta, env_lookup(env, d), 1.0f, 1.0f, kRrOff);
+            // camera ray with the origin set, under its own roulette (ray 2)
+            if (b == 0) {
+                const f3 envd = env_lookup(env, d);
+                put_pending(w, i, npend++, o, cur, na, d, beta, envd, 1.0f, 1.0f,
+                            rr_bits(shadow_rr(max3(beta * envd), J.shadow_rr, key, 0u, 2u)));
+            }
 
             if (ts >= kTMax) {
                 finish = true;

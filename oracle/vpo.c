@@ -1194,11 +1194,12 @@ static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, 
         float ts_min = argmin_scatter(prims, n, o, d, act, na, key, bounce, &hits, k, B, !P->measure);
         if (P->measure && hits > k->max_hits) k->max_hits = hits;
 
-        /* analytic direct term at bounce 0 (raygen.cuh:148-160) */
+        /* analytic direct term at bounce 0 (raygen.cuh:148-160), under the
+         * shadow-ray roulette as ray 2 of bounce 0 (a no-op when it is off) */
         if (bounce == 0) {
             k->n_shadow_rays++;
-            float T = transmittance(prims, n, o, d, act, na, k, B);
             v3 env = env_lookup(&J->env, d);
+            float T = shadow_T(prims, n, o, d, act, na, k, B, vmax3(vmul(beta, env)), P->shadow_rr, key, 0, 2);
             L = vadd(L, vmul(vscale(beta, T), env));
         }
         if (ts_min >= INF_F) break; /* escape: nothing to add with NEE + analytic direct */
