@@ -315,7 +315,7 @@ vp_status vp_run_jobs(vp_context* ctx, std::vector<JobRun>& runs, const vp_launc
     const size_t lanes = static_cast<size_t>(L.grid) * L.block;
     vp_status st = VP_OK;
     if (L.pipeline == VP_PIPELINE_MEGAKERNEL)
-        st = vp_ensure_slab(ctx, vp::kMegaSlabBuffers * static_cast<size_t>(active_cap) * lanes);
+        st = vp_ensure_slab(ctx, vp::kMegaSlabBuffers * static_cast<size_t>(vp::mega_slab_stride(active_cap)) * lanes);
     if (st != VP_OK) return st;
     st = vp_ensure_jobs(ctx, static_cast<uint32_t>(runs.size()));
     if (st != VP_OK) return st;

@@ -123,6 +123,10 @@ constexpr uint32_t kSlabBuffers = 3;
 // The argmin computes the values of 3, 4 and 5 anyway, so the rebuild only
 // compares them with the scatter distance.
 constexpr uint32_t kMegaSlabBuffers = 6;
+// Words between the megakernel's slab buffers: the capacity rounded up to a
+// multiple of 4, so every buffer starts 16-byte aligned and can be read four
+// entries at a time.
+VP_HD uint32_t mega_slab_stride(uint32_t cap) { return (cap + 3u) & ~3u; }
 // Largest active-set capacity: the walks count set sizes in 16 bits.
 constexpr uint32_t kMaxActiveCap = 65535;
 

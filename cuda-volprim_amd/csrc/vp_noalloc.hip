@@ -120,7 +120,7 @@ extern "C" vp_status vp_render_device_noalloc(vp_context* ctx, vp_scene* sc, vp_
     if (ctx->have_tuned) c = ctx->tuned;
     c.pipeline = VP_PIPELINE_MEGAKERNEL;
     RenderLaunch L = vp_resolve_launch(ctx, &c, cap, 0, params->measure_caps ? 1 : 0);
-    const size_t lanes_max = kNoAllocSlabBytes / (sizeof(uint32_t) * kMegaSlabBuffers * static_cast<size_t>(cap));
+    const size_t lanes_max = kNoAllocSlabBytes / (sizeof(uint32_t) * kMegaSlabBuffers * static_cast<size_t>(mega_slab_stride(cap)));
     const size_t grid_max = lanes_max / L.block;
     if (grid_max == 0)
         return fail(VP_ERR_INVALID, "vp_render_device_noalloc: active_cap " + std::to_string(cap) +

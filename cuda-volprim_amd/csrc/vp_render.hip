@@ -191,10 +191,11 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
                     if (x.t0 <= __uint_as_float(wq.ts[own])) {
                         const uint32_t pos = atomicAdd(&wq.nc[own], 1u);
                         if (pos < cap) {
-                            uint32_t* mine = slab + static_cast<size_t>(gbase + own) * kMegaSlabBuffers * cap;
-                            mine[2 * cap + pos] = prim;
-                            mine[3 * cap + pos] = __float_as_uint(x.t0);
-                            mine[4 * cap + pos] = __float_as_uint(x.t1);
+                            const uint32_t bs = mega_slab_stride(cap);
+                            uint32_t* mine = slab + static_cast<size_t>(gbase + own) * kMegaSlabBuffers * bs;
+                            mine[2 * bs + pos] = prim;
+                            mine[3 * bs + pos] = __float_as_uint(x.t0);
+                            mine[4 * bs + pos] = __float_as_uint(x.t1);
                         }
                         tau = free_flight_tau(prim_uniform(wq.key[own], meta & 0x7fffffffu, 1u, prim));
                         keep = tau <= x.tau;
@@ -256,7 +257,8 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
     constexpr uint32_t stride = 1;
     // buffers 0 and 1: active sets (ping-pong); buffer 2: primitives the
     // argmin walk entered before the running minimum (rebuild candidates)
-    uint32_t* my = slab + static_cast<size_t>(gl) * kMegaSlabBuffers * cap;  // entry k of buffer b at my[b * cap + k]
+    const uint32_t bs = mega_slab_stride(cap);
+    uint32_t* my = slab + static_cast<size_t>(gl) * kMegaSlabBuffers * bs;  // entry k of buffer b at my[b * bs + k]
     const LeafQueue lq = leaf_queue(s_leafq);
     const WaveQueue wq = wave_queue(s_leafq);  // the same LDS, used by the MIS shadow walks
     PathState st;
@@ -339,8 +341,8 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
         const SceneView sc = J.scene;
         const f3 o = st.o, d = st.d;
         const uint32_t b = st.bounce;
-        uint32_t* act = my + (st.cur * cap) * stride;
-        uint32_t* nxt = my + ((st.cur ^ 1u) * cap) * stride;
+        uint32_t* act = my + (st.cur * bs) * stride;
+        uint32_t* nxt = my + ((st.cur ^ 1u) * bs) * stride;
         const uint32_t na = st.na;
         bool finish = false;
 
@@ -358,7 +360,7 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
         // exits are kept for the rebuild.  The timed kernel does this for the
         // whole wavefront inside argmin_walk_shared.
         float ts = kTMax;
-        float* act_exit = reinterpret_cast<float*>(my + 5 * cap * stride);
+        float* act_exit = reinterpret_cast<float*>(my + 5 * bs * stride);
         if (kCount || measure) {
             for (uint32_t k = 0; k < na; ++k) {
                 const uint32_t i = act[k * stride];
@@ -376,7 +378,7 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
         // primitives entered before the running minimum are kept as rebuild
         // candidates (every primitive entered before the final minimum is
         // among them, because the walk only prunes beyond the running one)
-        uint32_t* cnd = my + 2 * cap * stride;
+        uint32_t* cnd = my + 2 * bs * stride;
         uint32_t nc = 0;
         // the analytic direct term's factor and its roulette (a no-op unless
         // J.shadow_rr > 0), known before the walk that integrates its depth
@@ -408,8 +410,8 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                 if (x.t0 > ts) return true;
                 if (nc < cap) {
                     cnd[nc * stride] = prim;
-                    my[(3 * cap + nc) * stride] = __float_as_uint(x.t0);
-                    my[(4 * cap + nc) * stride] = __float_as_uint(x.t1);
+                    my[(3 * bs + nc) * stride] = __float_as_uint(x.t0);
+                    my[(4 * bs + nc) * stride] = __float_as_uint(x.t1);
                 }
                 ++nc;
                 const float tau = free_flight_tau(prim_uniform(st.key, b, 1u, prim));
@@ -487,8 +489,8 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
             };
             if (nc <= cap && !measure) {
                 // candidates from the argmin walk, in ascending index order
-                float* ctj = reinterpret_cast<float*>(my + 3 * cap * stride);
-                float* cte = reinterpret_cast<float*>(my + 4 * cap * stride);
+                float* ctj = reinterpret_cast<float*>(my + 3 * bs * stride);
+                float* cte = reinterpret_cast<float*>(my + 4 * bs * stride);
                 for (uint32_t i = 1; i < nc; ++i) {
                     const uint32_t v = cnd[i * stride];
                     const float vt = ctj[i * stride], ve = cte[i * stride];
@@ -503,11 +505,44 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                     ctj[j * stride] = vt;
                     cte[j * stride] = ve;
                 }
-                for (uint32_t i = 0; i < nc; ++i) {
-                    const uint32_t c = cnd[i * stride];
-                    drain_below(c);
-                    visit(c, true, ctj[i * stride], cte[i * stride]);
+                // Merge with the old members, read four at a time (ids and
+                // exits: the buffers are 16-byte aligned and the reads stay
+                // inside them) so the loop does not wait on a load per
+                // member.  Same tests and the same ascending order as
+                // drain_below/visit: an old member stays while the vertex is
+                // before its exit; a candidate that is also an old member is
+                // tested as a candidate only if the member test fails.
+                static_assert(stride == 1, "the merge reads the slab buffers four entries at a time");
+                uint32_t ic = 0;
+                auto candidate = [&](uint32_t c, float tj, float te) {
+                    ++rb_h;
+                    if (tj >= 0.0f && tj <= ts && ts <= te) push(c);
+                };
+                for (uint32_t base = 0; base < na; base += 4u) {
+                    const uint4 id4 = *reinterpret_cast<const uint4*>(act + base);
+                    const float4 ex4 = *reinterpret_cast<const float4*>(act_exit + base);
+                    const uint32_t id[4] = {id4.x, id4.y, id4.z, id4.w};
+                    const float ex[4] = {ex4.x, ex4.y, ex4.z, ex4.w};
+                    const uint32_t nk = na - base < 4u ? na - base : 4u;
+                    for (uint32_t k = 0; k < nk; ++k) {
+                        const uint32_t a = id[k];
+                        while (ic < nc && cnd[ic] < a) {
+                            candidate(cnd[ic], ctj[ic], cte[ic]);
+                            ++ic;
+                        }
+                        ++rb_a;
+                        const bool stay = ts <= ex[k];
+                        if (ic < nc && cnd[ic] == a) {
+                            if (stay) push(a);
+                            else candidate(a, ctj[ic], cte[ic]);
+                            ++ic;
+                        } else if (stay) {
+                            push(a);
+                        }
+                    }
                 }
+                ia = na;
+                for (; ic < nc; ++ic) candidate(cnd[ic], ctj[ic], cte[ic]);
             } else {
                 // more candidates than fit: one BVH point query at the vertex
                 walk_point_q(sc, x, visits, lq, [&](uint32_t c) {

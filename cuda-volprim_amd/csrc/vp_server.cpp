@@ -446,7 +446,7 @@ vp_status vp_server_create(vp_context* ctx, const vp_server_config* cfg, vp_serv
         else cu_mask[cu / 32] |= 1u << (cu % 32);
     }
     if (free_cus > 0 && S->L.grid >= cus) S->L.grid = S->L.grid / cus * (cus - free_cus);
-    const size_t slab_words = vp::kMegaSlabBuffers * static_cast<size_t>(cap) * S->L.grid * S->L.block;
+    const size_t slab_words = vp::kMegaSlabBuffers * static_cast<size_t>(vp::mega_slab_stride(cap)) * S->L.grid * S->L.block;
     bool ok = hipMalloc(&S->jobs_dev, sizeof(vp::JobDesc) * kServerTable) == hipSuccess &&
               hipMalloc(&S->begin_dev, sizeof(uint64_t) * kServerTable) == hipSuccess &&
               hipMalloc(&S->done_dev, sizeof(unsigned long long) * kServerTable) == hipSuccess &&
