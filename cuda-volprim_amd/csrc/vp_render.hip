@@ -513,11 +513,22 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                 // before its exit; a candidate that is also an old member is
                 // tested as a candidate only if the member test fails.
                 static_assert(stride == 1, "the merge reads the slab buffers four entries at a time");
-                uint32_t ic = 0;
-                auto candidate = [&](uint32_t c, float tj, float te) {
-                    ++rb_h;
-                    if (tj >= 0.0f && tj <= ts && ts <= te) push(c);
+                // the next candidate is held in registers (id ~0 once none is left)
+                uint32_t ic = 0, c_id = 0xffffffffu;
+                float c_tj = 0.0f, c_te = 0.0f;
+                auto next_candidate = [&]() {
+                    c_id = ic < nc ? cnd[ic] : 0xffffffffu;
+                    if (ic < nc) {
+                        c_tj = ctj[ic];
+                        c_te = cte[ic];
+                    }
+                    ++ic;
                 };
+                auto candidate = [&]() {
+                    ++rb_h;
+                    if (c_tj >= 0.0f && c_tj <= ts && ts <= c_te) push(c_id);
+                };
+                next_candidate();
                 for (uint32_t base = 0; base < na; base += 4u) {
                     const uint4 id4 = *reinterpret_cast<const uint4*>(act + base);
                     const float4 ex4 = *reinterpret_cast<const float4*>(act_exit + base);
@@ -526,23 +537,26 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                     const uint32_t nk = na - base < 4u ? na - base : 4u;
                     for (uint32_t k = 0; k < nk; ++k) {
                         const uint32_t a = id[k];
-                        while (ic < nc && cnd[ic] < a) {
-                            candidate(cnd[ic], ctj[ic], cte[ic]);
-                            ++ic;
+                        while (c_id < a) {
+                            candidate();
+                            next_candidate();
                         }
                         ++rb_a;
                         const bool stay = ts <= ex[k];
-                        if (ic < nc && cnd[ic] == a) {
+                        if (c_id == a) {
                             if (stay) push(a);
-                            else candidate(a, ctj[ic], cte[ic]);
-                            ++ic;
+                            else candidate();
+                            next_candidate();
                         } else if (stay) {
                             push(a);
                         }
                     }
                 }
                 ia = na;
-                for (; ic < nc; ++ic) candidate(cnd[ic], ctj[ic], cte[ic]);
+                while (c_id != 0xffffffffu) {
+                    candidate();
+                    next_candidate();
+                }
             } else {
                 // more candidates than fit: one BVH point query at the vertex
                 walk_point_q(sc, x, visits, lq, [&](uint32_t c) {
