@@ -381,18 +381,21 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
         uint32_t* cnd = my + 2 * bs * stride;
         uint32_t nc = 0;
         // the analytic direct term's factor and its roulette (a no-op unless
-        // J.shadow_rr > 0), known before the walk that integrates its depth
-        const EnvView env = env_view(J.env);
-        f3 envd = mk(0.0f);
-        ShadowRr drr{kTauLimitFixed, kRrNoFail};
-        if (b == 0) {
-            envd = env_lookup(env, d);
-            drr = shadow_rr(max3(st.beta * envd), J.shadow_rr, st.key, 0u, 2u);
-        }
+        // J.shadow_rr > 0): the walk that integrates its depth needs only the
+        // depth where the term becomes 0; both are formed again after the
+        // walk (the same values), so they hold no registers during it
+        auto direct_rr = [&](f3& envd) {
+            envd = env_lookup(env_view(J.env), d);
+            return shadow_rr(max3(st.beta * envd), J.shadow_rr, st.key, 0u, 2u);
+        };
         uint64_t dwalk = 0;
         if (!kCount && !measure) {
-            argmin_walk_shared(sc, o, d, st.key, b, direct, act, na, act_exit, slab, cap, wq, ts, nc, drr.limit(),
-                               dwalk);
+            uint64_t dlimit = kTauLimitFixed;
+            if (direct) {
+                f3 envd;
+                dlimit = direct_rr(envd).limit();
+            }
+            argmin_walk_shared(sc, o, d, st.key, b, direct, act, na, act_exit, slab, cap, wq, ts, nc, dlimit, dwalk);
         } else {
             const f3 inv = safe_inv(d);
             const bool prune = !measure && !direct;
@@ -432,9 +435,12 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
             if (measure) atomicMax(&counters[kCntMaxHits], static_cast<unsigned long long>(hits));
         }
 
+        const EnvView env = env_view(J.env);
         // analytic direct term at bounce 0 (raygen.cuh:148-160)
         if (b == 0) {
             ev.add(kCntShadowRays);
+            f3 envd;
+            const ShadowRr drr = direct_rr(envd);
             uint64_t a;
             if constexpr (kCount) a = shadow_depth(sc, o, d, act, na, stride, lq, ev, drr.limit());
             else a = measure ? (dfull ? kTauLimitFixed : dacc) : dwalk;
