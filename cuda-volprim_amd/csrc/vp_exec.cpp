@@ -69,7 +69,7 @@ vp_status vp_ensure_wf(vp_context* ctx, uint32_t P, uint32_t cap) {
     };
     const size_t o_ob = carve(16 * p), o_dj = carve(16 * p), o_bt = carve(16 * p), o_lr = carve(16 * p);
     const size_t o_rng = carve(16 * p), o_misc = carve(16 * p);
-    const size_t o_pw = carve(48 * p), o_pv = carve(48 * p), o_po = carve(48 * p), o_pd = carve(48 * p);
+    const size_t o_pw = carve(48 * p), o_po = carve(48 * p), o_pd = carve(48 * p);
     const size_t o_live = carve(4 * p), o_sh = carve(12 * p), o_free = carve(4 * p), o_qc = carve(64);
     const size_t o_cb = carve(16), o_fin = carve(8);
     const size_t o_slab = carve(static_cast<size_t>(vp::kSlabBuffers) * cap * p * 4);
@@ -87,7 +87,6 @@ vp_status vp_ensure_wf(vp_context* ctx, uint32_t P, uint32_t cap) {
     w.rng = reinterpret_cast<uint4*>(b + o_rng);
     w.misc = reinterpret_cast<uint4*>(b + o_misc);
     w.pw = reinterpret_cast<float4*>(b + o_pw);
-    w.pv = reinterpret_cast<float4*>(b + o_pv);
     w.po = reinterpret_cast<float4*>(b + o_po);
     w.pd = reinterpret_cast<float4*>(b + o_pd);
     w.live = reinterpret_cast<uint32_t*>(b + o_live);

@@ -138,8 +138,7 @@ struct WfPool {
     float4* lr;    // radiance.xyz, active-set size
     uint4* rng;    // PCG32 state, increment
     uint4* misc;   // key (2 words), pixel index, flags (set buffer, state, pending terms)
-    float4* pw;    // [3][P] pending term weight.xyz, s1
-    float4* pv;    // [3][P] pending term factor.xyz, s2
+    float4* pw;    // [3][P] pending term factor.xyz (the factor its transmittance multiplies)
     float4* po;    // [3][P] shadow-ray origin.xyz, set buffer | count << 1
     float4* pd;    // [3][P] shadow-ray direction.xyz, transmittance
     uint32_t* live;      // [P] live-slot queue

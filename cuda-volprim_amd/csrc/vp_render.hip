@@ -384,16 +384,16 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
         // J.shadow_rr > 0): the walk that integrates its depth needs only the
         // depth where the term becomes 0; both are formed again after the
         // walk (the same values), so they hold no registers during it
-        auto direct_rr = [&](f3& envd) {
-            envd = env_lookup(env_view(J.env), d);
-            return shadow_rr(max3(st.beta * envd), J.shadow_rr, st.key, 0u, 2u);
+        auto direct_rr = [&](f3& kd) {
+            kd = st.beta * env_lookup(env_view(J.env), d);
+            return shadow_rr(max3(kd), J.shadow_rr, st.key, 0u, 2u);
         };
         uint64_t dwalk = 0;
         if (!kCount && !measure) {
             uint64_t dlimit = kTauLimitFixed;
             if (direct) {
-                f3 envd;
-                dlimit = direct_rr(envd).limit();
+                f3 kd;
+                dlimit = direct_rr(kd).limit();
             }
             argmin_walk_shared(sc, o, d, st.key, b, direct, act, na, act_exit, slab, cap, wq, ts, nc, dlimit, dwalk);
         } else {
@@ -439,13 +439,13 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
         // analytic direct term at bounce 0 (raygen.cuh:148-160)
         if (b == 0) {
             ev.add(kCntShadowRays);
-            f3 envd;
-            const ShadowRr drr = direct_rr(envd);
+            f3 kd;
+            const ShadowRr drr = direct_rr(kd);
             uint64_t a;
             if constexpr (kCount) a = shadow_depth(sc, o, d, act, na, stride, lq, ev, drr.limit());
             else a = measure ? (dfull ? kTauLimitFixed : dacc) : dwalk;
             const float T = shadow_rr_T(a, drr, depth_T(a));
-            st.L = st.L + (st.beta * T) * envd;
+            st.L = st.L + kd * T;
         }
 
         if (ts >= kTMax) {
@@ -628,14 +628,19 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                     if (lumy > 0.0f) {
                         ev.add(kCntShadowRays);
                         const float Wr = wsum * rcpf(static_cast<float>(J.ris_candidates));
-                        const f3 cy = base * (yenv * rcpf(lumy));
-                        const ShadowRr rr = shadow_rr(max3(cy) * Wr, J.shadow_rr, st.key, b, 0u);
+                        // the factor T multiplies; the roulette is drawn again
+                        // after the walk (the same values), so it holds no
+                        // registers during it
+                        const f3 ky = (base * (yenv * rcpf(lumy))) * Wr;
                         uint64_t acc, acc_unused;
-                        if constexpr (kCount) acc = shadow_depth(sc, x, ydir, act, nn, stride, lq, ev, rr.limit());
-                        else transmittance_pool(sc, x, ydir, ydir, act, nn, stride, wq, rr.limit(), 0ull, acc, acc_unused,
-                                                false);
-                        const float T = shadow_rr_T(acc, rr, depth_T(acc));
-                        st.L = st.L + (cy * T) * Wr;
+                        {
+                            const uint64_t lim = shadow_rr(max3(ky), J.shadow_rr, st.key, b, 0u).limit();
+                            if constexpr (kCount) acc = shadow_depth(sc, x, ydir, act, nn, stride, lq, ev, lim);
+                            else transmittance_pool(sc, x, ydir, ydir, act, nn, stride, wq, lim, 0ull, acc, acc_unused,
+                                                    false);
+                        }
+                        const ShadowRr rr = shadow_rr(max3(ky), J.shadow_rr, st.key, b, 0u);
+                        st.L = st.L + ky * shadow_rr_T(acc, rr, depth_T(acc));
                     }
                 }
             } else {
@@ -653,24 +658,29 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                 const float wb = sb > 0.0f ? bpdf / sb : 0.0f;
                 const f3 envb = env_lookup(env, bwo);
                 ev.add(kCntShadowRays, 2);
-                // the terms the two transmittances multiply, and each ray's
-                // roulette (a no-op unless J.shadow_rr > 0)
-                const f3 ca = base * enva, cb = base * envb;
-                const ShadowRr rra = shadow_rr(max3(ca) * wa, J.shadow_rr, st.key, b, 0u);
-                const ShadowRr rrb = shadow_rr(((max3(cb) * wb) * ph_b) * rcpf(bpdf), J.shadow_rr, st.key, b, 1u);
+                // the factors the two transmittances multiply, and each ray's
+                // roulette (a no-op unless J.shadow_rr > 0); the roulettes
+                // are drawn again after the walks (the same values), so they
+                // hold no registers during them
+                const f3 ka = (base * enva) * wa;
+                const f3 kb = (((base * envb) * wb) * ph_b) * rcpf(bpdf);
                 // the counting build walks the two rays with separate calls,
                 // so its counts are per-ray work
                 uint64_t acc_a, acc_b;
-                if constexpr (kCount) {
-                    acc_a = shadow_depth(sc, x, awo, act, nn, stride, lq, ev, rra.limit());
-                    acc_b = shadow_depth(sc, x, bwo, act, nn, stride, lq, ev, rrb.limit());
-                } else {
-                    transmittance_pool(sc, x, awo, bwo, act, nn, stride, wq, rra.limit(), rrb.limit(), acc_a, acc_b);
+                {
+                    const uint64_t lim_a = shadow_rr(max3(ka), J.shadow_rr, st.key, b, 0u).limit();
+                    const uint64_t lim_b = shadow_rr(max3(kb), J.shadow_rr, st.key, b, 1u).limit();
+                    if constexpr (kCount) {
+                        acc_a = shadow_depth(sc, x, awo, act, nn, stride, lq, ev, lim_a);
+                        acc_b = shadow_depth(sc, x, bwo, act, nn, stride, lq, ev, lim_b);
+                    } else {
+                        transmittance_pool(sc, x, awo, bwo, act, nn, stride, wq, lim_a, lim_b, acc_a, acc_b);
+                    }
                 }
-                const float Ta = shadow_rr_T(acc_a, rra, depth_T(acc_a));
-                const float Tb = shadow_rr_T(acc_b, rrb, depth_T(acc_b));
-                st.L = st.L + (ca * Ta) * wa;
-                st.L = st.L + ((((cb * Tb) * wb) * ph_b) * rcpf(bpdf));
+                const float Ta = shadow_rr_T(acc_a, shadow_rr(max3(ka), J.shadow_rr, st.key, b, 0u), depth_T(acc_a));
+                const float Tb = shadow_rr_T(acc_b, shadow_rr(max3(kb), J.shadow_rr, st.key, b, 1u), depth_T(acc_b));
+                st.L = st.L + ka * Ta;
+                st.L = st.L + kb * Tb;
             }
             st.beta = st.beta * albedo;
 

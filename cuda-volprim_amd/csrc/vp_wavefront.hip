@@ -108,8 +108,8 @@ __global__ void VP_WF_LB k_wf_shade(const JobDesc* __restrict__ jobs, WfPool w, 
         f3 L = f3{lr.x, lr.y, lr.z};
         for (uint32_t k = 0; k < npend; ++k) {
             const float T = w.pd[k * P + i].w;
-            const float4 a = w.pw[k * P + i], b = w.pv[k * P + i];
-            L = L + ((((f3{a.x, a.y, a.z} * T) * f3{b.x, b.y, b.z}) * a.w) * b.w);
+            const float4 a = w.pw[k * P + i];
+            L = L + f3{a.x, a.y, a.z} * T;
         }
         if (state == kSlotDone) {
             job = __float_as_uint(w.dj[i].w);
@@ -339,8 +339,9 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
     }
 }
 
-// Pending radiance term k of slot i: contribution (((w*T)*v)*s1)*s2 once the
-// shadow kernel has written T.  Until then pd.w carries the ray's shadow
+// Pending radiance term k of slot i: contribution kf * T once the shadow
+// kernel has written T (kf: the factor T multiplies).  Until then pd.w
+// carries the ray's shadow
 // roulette: its first threshold >> 24 in bits 0..22 (at most 88 * 2^16) and
 // its failing level in bits 23..29.
 __device__ __forceinline__ uint32_t rr_bits(const ShadowRr& rr) {
@@ -353,13 +354,11 @@ __device__ __forceinline__ ShadowRr rr_unpack(uint32_t bits) {
     return rr;
 }
 __device__ __forceinline__ void put_pending(const WfPool& w, uint32_t i, uint32_t k, f3 org, uint32_t set,
-                                            uint32_t count, f3 dir, f3 wt, f3 v, float s1, float s2,
-                                            uint32_t rrbits) {
+                                            uint32_t count, f3 dir, f3 kf, uint32_t rrbits) {
     const uint32_t e = k * w.P + i;
     w.po[e] = make_float4(org.x, org.y, org.z, __uint_as_float(set | (count << 1)));
     w.pd[e] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(rrbits));
-    w.pw[e] = make_float4(wt.x, wt.y, wt.z, s1);
-    w.pv[e] = make_float4(v.x, v.y, v.z, s2);
+    w.pw[e] = make_float4(kf.x, kf.y, kf.z, 0.0f);
 }
 static_assert((kTauLimitFixed >> 24) < (1u << 23) && (kTauLimitFixed & kRrGrain) == 0, "roulette bits fit pd.w");
 
@@ -403,9 +402,8 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
             // analytic direct term at bounce 0 (raygen.cuh:148-160): T along the
             // camera ray with the origin set, under its own roulette (ray 2)
             if (b == 0) {
-                const f3 envd = env_lookup(env, d);
-                put_pending(w, i, npend++, o, cur, na, d, beta, envd, 1.0f, 1.0f,
-                            rr_bits(shadow_rr(max3(beta * envd), J.shadow_rr, key, 0u, 2u)));
+                const f3 kd = beta * env_lookup(env, d);
+                put_pending(w, i, npend++, o, cur, na, d, kd, rr_bits(shadow_rr(max3(kd), J.shadow_rr, key, 0u, 2u)));
             }
 
             if (ts >= kTMax) {
@@ -515,9 +513,9 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
                         const float lumy = dot(yenv, luma);
                         if (lumy > 0.0f) {
                             const float Wr = wsum * rcpf(static_cast<float>(J.ris_candidates));
-                            const f3 cy = base3 * (yenv * rcpf(lumy));
-                            put_pending(w, i, npend++, x, cur, nn, ydir, cy, mk(Wr), 1.0f, 1.0f,
-                                        rr_bits(shadow_rr(max3(cy) * Wr, J.shadow_rr, key, b, 0u)));
+                            const f3 ky = (base3 * (yenv * rcpf(lumy))) * Wr;
+                            put_pending(w, i, npend++, x, cur, nn, ydir, ky,
+                                        rr_bits(shadow_rr(max3(ky), J.shadow_rr, key, b, 0u)));
                         }
                     }
                 } else {
@@ -527,9 +525,8 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
                     const float sa = apdf + pb_a;
                     const float wa = sa > 0.0f ? apdf / sa : 0.0f;
                     const f3 enva = env_lookup(env, awo);
-                    const f3 ca = base3 * enva;
-                    put_pending(w, i, npend++, x, cur, nn, awo, ca, mk(wa), 1.0f, 1.0f,
-                                rr_bits(shadow_rr(max3(ca) * wa, J.shadow_rr, key, b, 0u)));
+                    const f3 ka = (base3 * enva) * wa;
+                    put_pending(w, i, npend++, x, cur, nn, awo, ka, rr_bits(shadow_rr(max3(ka), J.shadow_rr, key, b, 0u)));
 
                     float bpdf;
                     const f3 bwo = env_sample(env, rng, &bpdf);
@@ -537,9 +534,8 @@ __global__ void VP_WF_LB k_wf_scatter(const JobDesc* __restrict__ jobs, WfPool w
                     const float sb = bpdf + ph_b;
                     const float wb = sb > 0.0f ? bpdf / sb : 0.0f;
                     const f3 envb = env_lookup(env, bwo);
-                    const f3 cb = base3 * envb;
-                    put_pending(w, i, npend++, x, cur, nn, bwo, cb, mk(wb), ph_b, rcpf(bpdf),
-                                rr_bits(shadow_rr(((max3(cb) * wb) * ph_b) * rcpf(bpdf), J.shadow_rr, key, b, 1u)));
+                    const f3 kb = (((base3 * envb) * wb) * ph_b) * rcpf(bpdf);
+                    put_pending(w, i, npend++, x, cur, nn, bwo, kb, rr_bits(shadow_rr(max3(kb), J.shadow_rr, key, b, 1u)));
                 }
                 beta = beta * albedo;
 

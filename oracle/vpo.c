@@ -1198,9 +1198,9 @@ static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, 
          * shadow-ray roulette as ray 2 of bounce 0 (a no-op when it is off) */
         if (bounce == 0) {
             k->n_shadow_rays++;
-            v3 env = env_lookup(&J->env, d);
-            float T = shadow_T(prims, n, o, d, act, na, k, B, vmax3(vmul(beta, env)), P->shadow_rr, key, 0, 2);
-            L = vadd(L, vmul(vscale(beta, T), env));
+            v3 kd = vmul(beta, env_lookup(&J->env, d));
+            float T = shadow_T(prims, n, o, d, act, na, k, B, vmax3(kd), P->shadow_rr, key, 0, 2);
+            L = vadd(L, vscale(kd, T));
         }
         if (ts_min >= INF_F) break; /* escape: nothing to add with NEE + analytic direct */
 
@@ -1293,9 +1293,10 @@ static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, 
                 if (lumy > 0.0f) {
                     k->n_shadow_rays++;
                     float Wr = wsum * vpo_rcp((float)P->ris_candidates);
-                    v3 cy = vmul(base, vscale(yenv, vpo_rcp(lumy)));
-                    float T = shadow_T(prims, n, x, ydir, act, na, k, B, vmax3(cy) * Wr, P->shadow_rr, key, bounce, 0u);
-                    L = vadd(L, vscale(vscale(cy, T), Wr));
+                    /* the factor T multiplies, formed before the walk */
+                    v3 ky = vscale(vmul(base, vscale(yenv, vpo_rcp(lumy))), Wr);
+                    float T = shadow_T(prims, n, x, ydir, act, na, k, B, vmax3(ky), P->shadow_rr, key, bounce, 0u);
+                    L = vadd(L, vscale(ky, T));
                 }
             }
         } else {
@@ -1306,9 +1307,9 @@ static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, 
             float wa = sa > 0.0f ? apdf / sa : 0.0f;
             v3 enva = env_lookup(&J->env, awo);
             k->n_shadow_rays++;
-            v3 ca = vmul(base, enva);
-            float Ta = shadow_T(prims, n, x, awo, act, na, k, B, vmax3(ca) * wa, P->shadow_rr, key, bounce, 0u);
-            L = vadd(L, vscale(vscale(ca, Ta), wa));
+            v3 ka = vscale(vmul(base, enva), wa);
+            float Ta = shadow_T(prims, n, x, awo, act, na, k, B, vmax3(ka), P->shadow_rr, key, bounce, 0u);
+            L = vadd(L, vscale(ka, Ta));
 
             float bpdf;
             v3 bwo = env_sample(&J->env, &rng, &bpdf);
@@ -1317,10 +1318,9 @@ static v3 trace_sample(const job_t* J, uint32_t px, uint32_t py, uint32_t sidx, 
             float wb = sb > 0.0f ? bpdf / sb : 0.0f;
             v3 envb = env_lookup(&J->env, bwo);
             k->n_shadow_rays++;
-            v3 cb = vmul(base, envb);
-            float Tb = shadow_T(prims, n, x, bwo, act, na, k, B, ((vmax3(cb) * wb) * ph_b) * vpo_rcp(bpdf),
-                                P->shadow_rr, key, bounce, 1u);
-            L = vadd(L, vscale(vscale(vscale(vscale(cb, Tb), wb), ph_b), vpo_rcp(bpdf)));
+            v3 kb = vscale(vscale(vscale(vmul(base, envb), wb), ph_b), vpo_rcp(bpdf));
+            float Tb = shadow_T(prims, n, x, bwo, act, na, k, B, vmax3(kb), P->shadow_rr, key, bounce, 1u);
+            L = vadd(L, vscale(kb, Tb));
         }
         beta = vmul(beta, albedo);
 
