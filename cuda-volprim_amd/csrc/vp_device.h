@@ -862,8 +862,8 @@ __device__ __forceinline__ bool path_begin(const JobDesc& J, uint64_t q, f3& o, 
 // the total does not depend on which path finishes first), written to its
 // per-sample slot if the caller asked for per-sample output, and counted
 // against its job-table entry if the entry is tracked (the server).  The
-// release fence orders the sums before that count, so a host that sees the
-// entry complete and then launches the resolve reads finished sums.
+// count follows the completion of the sums' atomics, so a host that sees the
+// entry complete and then copies the sums reads finished ones.
 __device__ __forceinline__ void path_finish(const JobDesc& J, uint32_t pix, uint32_t sl, f3 L) {
     const float4 v = finalize_radiance(J, L);
     unsigned long long* a = J.accum + static_cast<size_t>(pix) * kAccumWords;
@@ -880,7 +880,14 @@ __device__ __forceinline__ void path_finish(const JobDesc& J, uint32_t pix, uint
     __hip_atomic_fetch_add(a + 3, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (J.samples != nullptr) J.samples[static_cast<size_t>(sl) * (static_cast<size_t>(J.width) * J.height) + pix] = v;
     if (J.done != nullptr) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        // The sums are agent-scope atomics, which the device performs where
+        // every compute unit sees them; the count only has to follow their
+        // completion, so the wave waits for all its outstanding memory
+        // operations (s_waitcnt with every counter at 0) before adding it.
+        // An agent-scope release fence would also write back the XCD's
+        // whole L2 for every finished path.  Entries with per-sample output
+        // are not counted (the server asks for none).
+        __builtin_amdgcn_s_waitcnt(0);
         __hip_atomic_fetch_add(J.done, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
