@@ -603,8 +603,11 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
             ev.add(kCntAlb, nn);
             const f3 albedo = aw <= 0.0f ? mk(0.0f) : aa * rcpf(aw);
 
-            // accumulate_nee_direct (path_stages.cuh:148-226)
+            // accumulate_nee_direct (path_stages.cuh:148-226); base is also
+            // the next bounce's throughput (raygen.cuh:189), set here so the
+            // albedo holds no registers during the shadow walks
             const f3 base = st.beta * albedo;
+            st.beta = base;
             const f3 luma = f3{kLumR, kLumG, kLumB};
             if (J.use_ris) {
                 float wsum = 0.0f;
@@ -682,7 +685,6 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
                 st.L = st.L + ka * Ta;
                 st.L = st.L + kb * Tb;
             }
-            st.beta = st.beta * albedo;
 
             // russian_roulette (path_stages.cuh:25-37) and guards (raygen.cuh:191-204)
             if (b >= J.rr_depth) {
