@@ -477,7 +477,12 @@ vp_status vp_build_scene_device(vp_context* ctx, const vp_primitive* host_prims,
     VP_HIP_TRY(hipMalloc(&d_idx2.p, sizeof(uint32_t) * n));
     VP_HIP_TRY(hipMalloc(&d_hist.p, sizeof(uint32_t) * 256 * nb_sort));
     VP_HIP_TRY(hipMalloc(&d_depth.p, sizeof(uint32_t)));
-    VP_HIP_TRY(hipMalloc(&sc->prims, sizeof(PrimRecord) * n));
+    // the records are padded to a multiple of the leaf block with zero records
+    // (zero inverse scales: no ray enters them), so a walk may hand every slot
+    // of a leaf block to the leaf tests
+    const size_t n_slots = (static_cast<size_t>(n) + kLeafBlock - 1u) / kLeafBlock * kLeafBlock;
+    VP_HIP_TRY(hipMalloc(&sc->prims, sizeof(PrimRecord) * n_slots));
+    if (n_slots > n) VP_HIP_TRY(hipMemsetAsync(sc->prims + n, 0, sizeof(PrimRecord) * (n_slots - n), s));
     VP_HIP_TRY(hipMalloc(&sc->nodes, sizeof(float4) * 2 * (2 * n - 1)));
     VP_HIP_TRY(hipMalloc(&sc->order, sizeof(uint32_t) * n));
     VP_HIP_TRY(hipMalloc(&sc->morton, sizeof(uint32_t) * n));

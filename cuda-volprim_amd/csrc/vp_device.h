@@ -346,13 +346,13 @@ __device__ __forceinline__ void lds_store3(float* p, f3 v) {
     p[2] = v.z;
 }
 
-// Node phase of the walks: a lane steps while its leaf queue has room for a
-// step (up to 4 leaves) and its walk has nodes left.  With VP_WALK_MIN > 0 the
-// loop runs for the whole wavefront, which leaves it early once fewer than
-// VP_WALK_MIN lanes are stepping and some lane's queue is full, so full lanes
-// do not wait for the last few; the stepping lanes resume after the leaf
-// phase.  A node phase that starts with every queue empty can only end early
-// after some lane has queued kLeafQ - 3 leaves, so walks always progress.
+// Node phase of the per-lane ray walks: a lane steps while its leaf queue has
+// room for a step (up to four leaf blocks) and its walk has nodes left.  With
+// VP_WALK_MIN > 0 the loop runs for the whole wavefront, which leaves it early
+// once fewer than VP_WALK_MIN lanes are stepping and some lane's queue is
+// full, so full lanes do not wait for the last few; the stepping lanes resume
+// after the leaf phase.  A node phase that starts with every queue empty can
+// only end early after some lane has queued leaves, so walks always progress.
 // Every query consumes leaves order-independently, so this changes only the
 // schedule.
 #ifndef VP_WALK_MIN
@@ -361,16 +361,19 @@ __device__ __forceinline__ void lds_store3(float* p, f3 v) {
 
 // room in a lane's queue for one more node step
 __device__ __forceinline__ bool queue_room(uint32_t cnt) { return cnt + 4u <= kLeafQ; }
+// ... for one more step of a walk of leaf blocks (up to four blocks)
+static_assert(4u * kLeafBlock <= kLeafQ, "a lane's queue must hold one step of leaf blocks");
+__device__ __forceinline__ bool block_room(uint32_t cnt) { return cnt + 4u * kLeafBlock <= kLeafQ; }
 
 // true while this lane (VP_WALK_MIN == 0) or the wavefront stays in the node
-// phase
-__device__ __forceinline__ bool node_phase(bool step, uint32_t cnt) {
+// phase (room: the lane's queue has room for its next step)
+__device__ __forceinline__ bool node_phase(bool step, bool room) {
 #if VP_WALK_MIN > 0
     const unsigned long long sm = __builtin_amdgcn_ballot_w64(step);
     if (sm == 0) return false;
-    return __popcll(sm) >= VP_WALK_MIN || __builtin_amdgcn_ballot_w64(!queue_room(cnt)) == 0;
+    return __popcll(sm) >= VP_WALK_MIN || __builtin_amdgcn_ballot_w64(!room) == 0;
 #else
-    (void)cnt;
+    (void)room;
     return step;
 #endif
 }
@@ -395,14 +398,19 @@ __device__ __forceinline__ void queue_leaves(const LeafQueue& q, uint32_t& cnt, 
     cnt += static_cast<uint32_t>(__popc(lm));
 }
 
-// One node step of a ray walk over [0, tmax]: crossed leaves are queued,
-// the walk moves on.
+// One node step of a ray walk of leaf blocks (walk4_block_levels) over
+// [0, tmax]: the leaves of the crossed blocks are queued, in ascending order,
+// and the walk moves on.  The same walk as the wave-shared ones, so the
+// counting build counts the node steps and leaf tests the timed kernel makes.
 __device__ __forceinline__ void ray_node_step(const SceneView& s, f3 inv, f3 oi, float tmax, Walk4& w, uint32_t& cnt,
                                               const LeafQueue& q) {
     const uint32_t m = ray_mask(s.w4, s.w4_half, w.n, inv, oi, tmax);
     uint32_t first;
-    const uint32_t lm = walk4_advance(w, s.w4_levels, m, first);
-    queue_leaves(q, cnt, lm, first);
+    const uint32_t bm = walk4_advance(w, walk4_block_levels(s.w4_levels), m, first);
+    for (uint32_t b = bm; b != 0u; b &= b - 1u) {
+        const uint32_t l0 = kLeafBlock * (first + static_cast<uint32_t>(__builtin_ctz(b)));
+        for (uint32_t k = 0; k < kLeafBlock; ++k) q.at(cnt++) = l0 + k;
+    }
 }
 
 template <typename Leaf>
@@ -413,8 +421,9 @@ __device__ __forceinline__ void walk_ray_q(const SceneView& s, f3 o, f3 inv, con
     while (true) {
         uint32_t cnt = 0;
         while (true) {
-            const bool step = w.n != kWalkEnd && queue_room(cnt);
-            if (!node_phase(step, cnt)) break;
+            const bool room = block_room(cnt);
+            const bool step = w.n != kWalkEnd && room;
+            if (!node_phase(step, room)) break;
             if (!step) continue;
             ++visits;
             ray_node_step(s, inv, oi, tmax, w, cnt, q);
@@ -443,31 +452,40 @@ __device__ __forceinline__ void walk_point_q(const SceneView& s, f3 x, uint32_t&
     }
 }
 
-// Appends the crossed leaves of the calling lanes (lm: a mask over the stored
-// primitives first .. first + 3) to the wave queue at count, in lane order,
-// as far as they fit in kWaveQ.  Returns false for a lane whose leaves did not
-// fit: its node step is to be repeated after the leaf phase.  The lanes that
-// fit form a prefix in lane order, so the queue has no gaps; count advances by
-// the leaves written.  Every lane of the wavefront that is active here must
-// call it.
-__device__ __forceinline__ bool wave_push(const WaveQueue& wq, uint32_t& count, uint32_t lm, uint32_t first,
-                                          uint32_t owner) {
-    const uint32_t np = static_cast<uint32_t>(__popc(lm));
-    const unsigned long long b0 = __builtin_amdgcn_ballot_w64(np > 0u);
-    if (b0 == 0ull) return true;  // no leaves this step (inner nodes)
-    const unsigned long long b1 = __builtin_amdgcn_ballot_w64(np > 1u);
-    const unsigned long long b2 = __builtin_amdgcn_ballot_w64(np > 2u), b3 = __builtin_amdgcn_ballot_w64(np > 3u);
-    const uint32_t pos = count + lanes_below(b0) + lanes_below(b1) + lanes_below(b2) + lanes_below(b3);
-    const bool fits = np == 0u || pos + np <= kWaveQ;
+// Appends the leaves of the calling lanes' crossed leaf blocks (bm: a mask
+// over the blocks first .. first + 3 of a walk of leaf blocks,
+// walk4_block_levels; block c holds the stored primitives kLeafBlock c ..
+// kLeafBlock (c + 1) - 1, records past the scene's last primitive being zero
+// records, which the leaf tests reject) to the wave queue at count, in lane
+// order, as far as they fit in kWaveQ.  Returns false for a lane whose leaves
+// did not fit: its node step is to be repeated after the leaf phase.  The
+// lanes that fit form a prefix in lane order, so the queue has no gaps; count
+// advances by the leaves written and stays a multiple of 4, so each group of
+// four leaves is one 16-byte LDS write of ids and one 4-byte write of owners.
+// Every lane of the wavefront that is active here must call it.
+__device__ __forceinline__ bool wave_push_blocks(const WaveQueue& wq, uint32_t& count, uint32_t bm, uint32_t first,
+                                                 uint32_t owner) {
+    static_assert(kWaveQ % 4 == 0, "blocks are queued four leaves at a time");
+    const uint32_t nb = static_cast<uint32_t>(__popc(bm));  // 0 .. 4 blocks
+    const unsigned long long b0 = __builtin_amdgcn_ballot_w64((nb & 1u) != 0u);
+    const unsigned long long b1 = __builtin_amdgcn_ballot_w64((nb & 2u) != 0u);
+    const unsigned long long b2 = __builtin_amdgcn_ballot_w64((nb & 4u) != 0u);
+    if ((b0 | b1 | b2) == 0ull) return true;  // no blocks this step
+    const uint32_t pos = count + kLeafBlock * (lanes_below(b0) + 2u * lanes_below(b1) + 4u * lanes_below(b2));
+    const bool fits = nb == 0u || pos + kLeafBlock * nb <= kWaveQ;
     const unsigned long long f = __builtin_amdgcn_ballot_w64(fits);
-    if (fits && np != 0u) {
+    if (fits && nb != 0u) {
         uint32_t p = pos;
-        for (uint32_t b = lm; b != 0u; b &= b - 1u, ++p) {
-            wq.prim[p] = first + static_cast<uint32_t>(__builtin_ctz(b));
-            wq.owner[p] = static_cast<uint8_t>(owner);
+        for (uint32_t b = bm; b != 0u; b &= b - 1u) {
+            const uint32_t l0 = kLeafBlock * (first + static_cast<uint32_t>(__builtin_ctz(b)));
+#pragma unroll
+            for (uint32_t k = 0; k < kLeafBlock; k += 4u, p += 4u) {
+                *reinterpret_cast<uint4*>(wq.prim + p) = make_uint4(l0 + k, l0 + k + 1u, l0 + k + 2u, l0 + k + 3u);
+                *reinterpret_cast<uint32_t*>(wq.owner + p) = owner * 0x01010101u;
+            }
         }
     }
-    count += static_cast<uint32_t>(__popcll(b0 & f) + __popcll(b1 & f) + __popcll(b2 & f) + __popcll(b3 & f));
+    count += kLeafBlock * static_cast<uint32_t>(__popcll(b0 & f) + 2 * __popcll(b1 & f) + 4 * __popcll(b2 & f));
     return fits;
 }
 
@@ -725,7 +743,7 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
         inv = safe_inv(rd);
         oi = ro * inv;
         w4 = wq.nodes[own];
-        levels = wq.nn[own] & 0xffu;
+        levels = walk4_block_levels(wq.nn[own] & 0xffu);
         half = wq.nn[own] >> 8;
     };
     setup();
@@ -751,7 +769,7 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
             uint32_t lm = 0, first = 0;
             Walk4 nw = w;
             if (step) lm = walk4_advance(nw, levels, mega_ray_mask(w4, half, w.n, inv, oi, kTMax), first);
-            const bool fits = wave_push(wq, count, lm, first, ray);
+            const bool fits = wave_push_blocks(wq, count, lm, first, ray);
             if (step && fits) w = nw;
             if (__builtin_amdgcn_ballot_w64(!fits) != 0) break;
         }

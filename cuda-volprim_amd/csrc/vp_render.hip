@@ -154,8 +154,10 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
             if (count >= kWaveQ / VP_SHARED_END_DIV && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
             uint32_t lm = 0, first = 0;
             Walk4 nw = w;
-            if (step) lm = walk4_advance(nw, sc.w4_levels, mega_ray_mask(sc.w4, sc.w4_half, w.n, inv, oi, tmax), first);
-            const bool fits = wave_push(wq, count, lm, first, lane);
+            if (step)
+                lm = walk4_advance(nw, walk4_block_levels(sc.w4_levels),
+                                   mega_ray_mask(sc.w4, sc.w4_half, w.n, inv, oi, tmax), first);
+            const bool fits = wave_push_blocks(wq, count, lm, first, lane);
             if (step && fits) w = nw;
             if (__builtin_amdgcn_ballot_w64(!fits) != 0) break;
         }

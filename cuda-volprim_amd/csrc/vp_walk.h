@@ -3,7 +3,8 @@
 // traversal (tests/walk_tree_check.cpp).
 //
 // The walk tree is an implicit complete 4-ary tree over the stored
-// primitives.  With L levels (the smallest L >= 1 with 4^L >= N), node k of
+// primitives.  With L levels (the smallest L > kLeafBlockLevels with
+// 4^L >= N), node k of
 // level l has heap index base(l) + k, base(l) = (4^l - 1) / 3, and its
 // children are nodes 4k .. 4k + 3 of level l + 1; below the last level
 // (l = L - 1) the four children of node k are the leaves 4k .. 4k + 3, i.e.
@@ -83,8 +84,17 @@ VP_HD float f16_value(uint16_t h) {
     return u2f(sign | ((e + 112u) << 23) | (f << 13));
 }
 
+// Levels of the tree the ray walks leave out (walk4_block_levels): their
+// leaves are handed to the leaf tests in blocks of 4^kLeafBlockLevels.
+#ifndef VP_BLOCK_LEVELS
+#define VP_BLOCK_LEVELS 1
+#endif
+constexpr uint32_t kLeafBlockLevels = VP_BLOCK_LEVELS;
+constexpr uint32_t kLeafBlock = 1u << (2u * kLeafBlockLevels);  // leaves per block
+// More levels than the ray walks leave out, so they always have a level to
+// walk.
 VP_HD uint32_t w4_levels(uint32_t n) {
-    uint32_t l = 1;
+    uint32_t l = kLeafBlockLevels + 1u;
     while (l < kW4MaxLevels && (1ull << (2 * l)) < n) ++l;
     return l;
 }
@@ -141,6 +151,16 @@ VP_HD uint32_t walk4_advance(Walk4& w, uint32_t levels, uint32_t m, uint32_t& fi
     }
     return last ? m : 0u;
 }
+
+// Levels a ray walk of leaf blocks runs over (the tree's L levels less the
+// last kLeafBlockLevels): passed to walk4_advance as its level count, it makes
+// the walk stop kLeafBlockLevels levels above the last and return the crossed
+// children of the nodes there, each the root of block c of leaves
+// kLeafBlock c .. kLeafBlock (c + 1) - 1.  The boxes below are then never
+// tested: a block is crossed when its box is, and its primitives go to the
+// leaf tests, which are exact.  That trades leaf tests, which run at the full
+// width of the wave-shared queue, for dependent node loads a walk would make.
+VP_HD uint32_t walk4_block_levels(uint32_t levels) { return levels - kLeafBlockLevels; }
 
 // Children still pending in walk w (trail bits of the levels above w.n).
 VP_HD uint64_t walk4_pending(const Walk4& w) {

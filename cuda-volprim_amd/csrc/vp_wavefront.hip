@@ -300,8 +300,10 @@ __global__ void VP_WF_LB k_wf_extend(const JobDesc* __restrict__ jobs, WfPool w,
             if (count >= kWaveQ / VP_SHARED_END_DIV && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
             uint32_t lm = 0, first = 0;
             Walk4 nw = w4w;
-            if (step) lm = walk4_advance(nw, sc.w4_levels, ray_mask_nf(sc.w4, sc.w4_half, w4w.n, inv, oi, tmax), first);
-            const bool fits = wave_push(wq, count, lm, first, lane);
+            if (step)
+                lm = walk4_advance(nw, walk4_block_levels(sc.w4_levels),
+                                   ray_mask_nf(sc.w4, sc.w4_half, w4w.n, inv, oi, tmax), first);
+            const bool fits = wave_push_blocks(wq, count, lm, first, lane);
             if (step && fits) w4w = nw;
             if (__builtin_amdgcn_ballot_w64(!fits) != 0) break;
         }
@@ -631,8 +633,10 @@ __global__ void VP_WF_LB k_wf_shadow(const JobDesc* __restrict__ jobs, WfPool w,
             if (count >= kWaveQ / VP_SHARED_END_DIV && static_cast<uint32_t>(__popcll(sm)) < VP_SHARED_STEP_MIN) break;
             uint32_t lm = 0, first = 0;
             Walk4 nw = w4w;
-            if (step) lm = walk4_advance(nw, sc.w4_levels, ray_mask_nf(sc.w4, sc.w4_half, w4w.n, inv, oi, kTMax), first);
-            const bool fits = wave_push(wq, count, lm, first, lane);
+            if (step)
+                lm = walk4_advance(nw, walk4_block_levels(sc.w4_levels),
+                                   ray_mask_nf(sc.w4, sc.w4_half, w4w.n, inv, oi, kTMax), first);
+            const bool fits = wave_push_blocks(wq, count, lm, first, lane);
             if (step && fits) w4w = nw;
             if (__builtin_amdgcn_ballot_w64(!fits) != 0) break;
         }
