@@ -150,7 +150,7 @@ vp_status vp_context_create(int device, vp_context** out) {
     ctx->device = device;
     if (hipGetDeviceProperties(&ctx->prop, device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
         hipMalloc(&ctx->counters, sizeof(unsigned long long) * vp::kNumCounters) != hipSuccess ||
-        hipMalloc(&ctx->path_counter, sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&ctx->path_counter, sizeof(unsigned long long) * vp::kClaimParts) != hipSuccess ||
         hipMalloc(&ctx->launch_desc, sizeof(uint64_t) * 4) != hipSuccess ||
         hipHostMalloc(&ctx->desc_pinned, sizeof(uint64_t) * 4, hipHostMallocDefault) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {

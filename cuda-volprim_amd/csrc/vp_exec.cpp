@@ -224,7 +224,7 @@ static vp_status vp_run_step_wf(vp_context* ctx, const RenderLaunch& L, uint64_t
     VP_HIP_TRY(hipMemcpyAsync(ctx->job_begin_dev, ctx->begin_pinned, sizeof(uint64_t) * max_jobs,
                               hipMemcpyHostToDevice, s));
     VP_HIP_TRY(hipMemcpyAsync(ctx->launch_desc, ctx->desc_pinned, sizeof(uint64_t) * 3, hipMemcpyHostToDevice, s));
-    VP_HIP_TRY(vp_launch_fill_u64(ctx->path_counter, 1u, 0ull, s));
+    VP_HIP_TRY(vp_launch_fill_u64(ctx->path_counter, vp::kClaimParts, 0ull, s));
     VP_HIP_TRY(vp_launch_fill_u64(ctx->job_done_dev, max_jobs, 0ull, s));
     VP_HIP_TRY(vp_wf_launch_init(ctx->wf, s));
     // Every replay advances each live path by two bounces or retires it, and

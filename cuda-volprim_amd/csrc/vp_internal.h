@@ -127,6 +127,11 @@ constexpr uint32_t kMegaSlabBuffers = 6;
 // multiple of 4, so every buffer starts 16-byte aligned and can be read four
 // entries at a time.
 VP_HD uint32_t mega_slab_stride(uint32_t cap) { return (cap + 3u) & ~3u; }
+// Path counters of one megakernel launch: its path range is split into this
+// many parts, one per XCD of the MI355X, and a wavefront claims paths from its
+// own XCD's part first (so each XCD's L2 serves a narrower band of the
+// image), then from the others once its part is used up.
+constexpr uint32_t kClaimParts = 8;
 // Largest active-set capacity: the walks count set sizes in 16 bits.
 constexpr uint32_t kMaxActiveCap = 65535;
 

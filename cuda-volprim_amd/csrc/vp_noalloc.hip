@@ -33,7 +33,8 @@ constexpr size_t kNoAllocAccumBytes = size_t(kNoAllocMaxPixels) * kAccumWords * 
 constexpr size_t kNoAllocTableBytes = 4096;
 constexpr size_t kNoAllocSlabBytes = size_t(112) << 20;
 constexpr size_t kNoAllocScratchBytes = kNoAllocAccumBytes + kNoAllocTableBytes + kNoAllocSlabBytes;
-static_assert(sizeof(JobDesc) + 8 * 8 + kNumCounters * 8 <= kNoAllocTableBytes, "tables fit their block");
+static_assert(sizeof(JobDesc) + 4 * 8 + kClaimParts * 8 + kNumCounters * 8 <= kNoAllocTableBytes,
+              "tables fit their block");
 
 __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
@@ -43,7 +44,7 @@ struct NoAllocLayout {
     JobDesc* job;
     uint64_t* job_begin;     // [1]
     uint64_t* launch_desc;   // [3]: job count, end and start of the path range
-    unsigned long long* path_counter;
+    unsigned long long* path_counter;  // [kClaimParts]
     unsigned long long* counters;  // [kNumCounters]
     uint32_t* slab;
 };
@@ -56,7 +57,7 @@ NoAllocLayout layout(void* base) {
     l.job_begin = reinterpret_cast<uint64_t*>(t + sizeof(JobDesc));
     l.launch_desc = l.job_begin + 1;
     l.path_counter = reinterpret_cast<unsigned long long*>(l.job_begin + 4);
-    l.counters = l.path_counter + 4;
+    l.counters = l.path_counter + kClaimParts;
     l.slab = reinterpret_cast<uint32_t*>(b + kNoAllocAccumBytes + kNoAllocTableBytes);
     return l;
 }
@@ -72,8 +73,8 @@ __global__ void k_noalloc_install(JobDesc J, JobDesc* job, uint64_t* job_begin, 
         launch_desc[0] = 1;
         launch_desc[1] = J.path_count;
         launch_desc[2] = 0;
-        *path_counter = 0ull;
     }
+    if (i < kClaimParts) path_counter[i] = 0ull;
     if (i < static_cast<uint32_t>(kNumCounters)) counters[i] = 0ull;
 }
 

@@ -236,12 +236,13 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
 // launch's path range, launch_desc[2] = its start; read from device memory so
 // one captured graph serves every step of a job mix.  A wavefront claims new
 // paths once at least regen_min of its lanes are idle (or none is busy), with
-// one fetch-and-add on the launch's own counter: path start + counter.  The
-// counter may run past the range (later claims then get nothing), and each
-// launch has its own counter and range, so launches that run at once (the
-// server starts a launch over the paths a longer job table adds while the
-// earlier ones drain) hand out every path exactly once, without the retries a
-// compare-and-swap on one shared counter costs when paths are short.
+// one fetch-and-add on the counter of a part of the launch's range (its XCD's
+// part first): part start + counter.  A counter may run past its part (later
+// claims there get nothing), and each launch has its own counters and range,
+// so launches that run at once (the server starts a launch over the paths a
+// longer job table adds while the earlier ones drain) hand out every path
+// exactly once, without the retries a compare-and-swap on one shared counter
+// costs when paths are short.
 template <bool kCount>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VP_RENDER_WAVES_PER_EU)))
 k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begin,
@@ -269,9 +270,18 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
     Ev<kCount> ev;
     ev.reset();
     uint32_t overflow = 0;
+    // The launch's paths in kClaimParts parts, part k claimed through
+    // path_counter[k]; a wavefront starts with the part of the XCD it runs on
+    // (the XCC_ID hardware register, bits 3:0), so each XCD's waves trace a
+    // narrower band of the image and its L2 holds more of what they read.
+    // Which lane traces which path changes no result.  parts_done: the parts
+    // this wavefront has found used up.
+    const uint32_t xcd = __builtin_amdgcn_s_getreg((3u << 11) | (0u << 6) | 20u) % kClaimParts;
+    const uint64_t part = (total_paths - path_lo + kClaimParts - 1u) / kClaimParts;
+    uint32_t parts_done = 0;
 
     while (true) {
-        // ---- claim new paths for idle lanes: one atomic per wavefront ----
+        // ---- claim new paths for idle lanes: one atomic per wavefront and part tried ----
         const bool want = !alive && !exhausted;
         unsigned long long wmask = __ballot(want);
         if (wmask && __popcll(wmask) < regen_min && __ballot(alive) != 0) wmask = 0;
@@ -280,22 +290,35 @@ k_render(const JobDesc* __restrict__ jobs, const uint64_t* __restrict__ job_begi
             unsigned long long base = 0, got = 0;
             if (lane == leader) {
                 const unsigned long long n_want = static_cast<unsigned long long>(__popcll(wmask));
-                const unsigned long long c =
-                    __hip_atomic_fetch_add(path_counter, n_want, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const uint64_t b = path_lo + c;
-                if (b < total_paths) {
-                    base = b;
-                    got = total_paths - b < n_want ? total_paths - b : n_want;
+                for (uint32_t t = 0; t < kClaimParts; ++t) {
+                    const uint32_t k = (xcd + t) % kClaimParts;
+                    if ((parts_done >> k) & 1u) continue;
+                    const uint64_t lo_k = path_lo + k * part;
+                    const uint64_t hi_k = lo_k + part < total_paths ? lo_k + part : total_paths;
+                    if (lo_k < hi_k) {
+                        const unsigned long long c =
+                            __hip_atomic_fetch_add(path_counter + k, n_want, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        const uint64_t b = lo_k + c;
+                        if (b < hi_k) {
+                            base = b;
+                            got = hi_k - b < n_want ? hi_k - b : n_want;
+                            break;
+                        }
+                    }
+                    parts_done |= 1u << k;
                 }
             }
             base = __shfl(base, leader);
             got = __shfl(got, leader);
+            parts_done = __builtin_amdgcn_readfirstlane(__shfl(parts_done, leader));
             if (want) {
                 const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
                 const uint64_t rank = static_cast<uint64_t>(__popcll(wmask & lt));
                 const uint64_t p = base + rank;
                 if (rank >= got) {
-                    exhausted = true;
+                    // every part used up; a lane beyond a part's last paths
+                    // claims again next round
+                    if (got == 0) exhausted = true;
                 } else {
                     // ---- path setup (raygen.cuh:72-113) ----
                     const uint32_t j = find_job(job_begin, njobs, p);
@@ -929,7 +952,9 @@ inline uint32_t cdiv(uint64_t a, uint32_t b) { return static_cast<uint32_t>((a +
 using namespace vp;
 
 // One persistent render launch over a job table (device pointers): the
-// kernel claims paths launch_desc[2] + *counter below launch_desc[1].
+// kernel claims the paths from launch_desc[2] below launch_desc[1], in
+// kClaimParts parts with one counter each (counter[0 .. kClaimParts), zero at
+// the start).
 hipError_t vp_launch_render(const RenderLaunch& L, const JobDesc* jobs, const uint64_t* job_begin,
                             const uint64_t* launch_desc, unsigned long long* counter, uint32_t* slab,
                             unsigned long long* counters, hipStream_t s) {
@@ -949,7 +974,7 @@ hipError_t vp_launch_render(const RenderLaunch& L, const JobDesc* jobs, const ui
 // and path total come from ctx->launch_desc.
 hipError_t vp_launch_step(vp_context* ctx, const RenderLaunch& L, uint32_t max_jobs, uint32_t max_npix,
                           hipStream_t s) {
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctx->path_counter, 1u, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, ctx->path_counter, kClaimParts, 0ull);
     const hipError_t e = vp_launch_render(L, ctx->jobs_dev, ctx->job_begin_dev, ctx->launch_desc, ctx->path_counter,
                                           ctx->slab, ctx->counters, s);
     if (e != hipSuccess) return e;

@@ -289,9 +289,11 @@ void server_loop(vp_server* S) {
                 if (hipStreamWaitEvent(S->ks[i], S->install_ev, 0) != hipSuccess ||
                     hipMemcpyAsync(S->desc_dev + 4 * i, d, 4 * sizeof(uint64_t), hipMemcpyHostToDevice, S->ks[i]) !=
                         hipSuccess ||
-                    hipMemcpyAsync(S->counter + i, S->zeros, sizeof(unsigned long long), hipMemcpyHostToDevice,
+                    hipMemcpyAsync(S->counter + vp::kClaimParts * i, S->zeros,
+                                   vp::kClaimParts * sizeof(unsigned long long), hipMemcpyHostToDevice,
                                    S->ks[i]) != hipSuccess ||
-                    vp_launch_render(S->L, S->jobs_dev, S->begin_dev, S->desc_dev + 4 * i, S->counter + i, S->slab[i],
+                    vp_launch_render(S->L, S->jobs_dev, S->begin_dev, S->desc_dev + 4 * i, S->counter + vp::kClaimParts * i,
+                                     S->slab[i],
                                      ctx->counters, S->ks[i]) != hipSuccess ||
                     hipEventRecord(S->kev[i], S->ks[i]) != hipSuccess) {
                     fail_all(S, "server: render launch failed");
@@ -450,7 +452,7 @@ vp_status vp_server_create(vp_context* ctx, const vp_server_config* cfg, vp_serv
     bool ok = hipMalloc(&S->jobs_dev, sizeof(vp::JobDesc) * kServerTable) == hipSuccess &&
               hipMalloc(&S->begin_dev, sizeof(uint64_t) * kServerTable) == hipSuccess &&
               hipMalloc(&S->done_dev, sizeof(unsigned long long) * kServerTable) == hipSuccess &&
-              hipMalloc(&S->counter, 2 * sizeof(unsigned long long)) == hipSuccess &&
+              hipMalloc(&S->counter, 2 * vp::kClaimParts * sizeof(unsigned long long)) == hipSuccess &&
               hipMalloc(&S->desc_dev, 8 * sizeof(uint64_t)) == hipSuccess &&
               hipMalloc(&S->slab[0], slab_words * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc(&S->slab[1], slab_words * sizeof(uint32_t)) == hipSuccess;
