@@ -850,9 +850,16 @@ __device__ __forceinline__ bool path_begin(const JobDesc& J, uint64_t q, f3& o, 
                                            uint32_t& pix_out, uint32_t& sl_out) {
     const uint32_t sl = static_cast<uint32_t>(q / J.npad);
     const uint32_t r = static_cast<uint32_t>(q - static_cast<uint64_t>(sl) * J.npad);
+    // 8x8 tiles, in bands of kTileBand tile rows, column by column within a
+    // band, so the paths in flight on one XCD (a run of consecutive path
+    // indices) cover a compact patch of the image
+    constexpr uint32_t kTileBand = 16;
     const uint32_t tile = r >> 6, w = r & 63u;
-    const uint32_t px = (tile % J.tiles_x) * 8u + (w & 7u);
-    const uint32_t py = (tile / J.tiles_x) * 8u + (w >> 3);
+    const uint32_t tiles_y = (J.height + 7u) / 8u;
+    const uint32_t band = tile / (kTileBand * J.tiles_x), c = tile - band * (kTileBand * J.tiles_x);
+    const uint32_t rows = tiles_y - band * kTileBand < kTileBand ? tiles_y - band * kTileBand : kTileBand;
+    const uint32_t px = (c / rows) * 8u + (w & 7u);
+    const uint32_t py = (band * kTileBand + c % rows) * 8u + (w >> 3);
     if (px >= J.width || py >= J.height) return false;
     const uint32_t sample = J.s_begin + sl;
     const uint64_t pix = static_cast<uint64_t>(py) * J.width + px;

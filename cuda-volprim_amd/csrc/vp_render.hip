@@ -28,13 +28,16 @@
 // device/entry/path_stages.cuh:25-226, device/core/sampling.cuh:257-583,
 // device/entry/anyhit.cuh:53-86, device/kernels/normalize_accumulator.cu:16-35.
 // The megakernel holds a whole path and a walk per lane, so it is tuned for
-// registers over occupancy: 4 waves per SIMD (128 VGPRs), a 448-entry shared
-// leaf queue in the LDS this frees, and the near/far node test.  On the
-// scattering workload (4096 Gaussians, 512x512, 64 spp): 26.5 Msamples/s,
-// against 23.6 at 6 waves with a 224-entry queue and the min/max test, 26.0
-// at 5 waves (320 entries) and 24.5 at 3 waves (640 entries).
+// registers over occupancy: 4 waves per SIMD (128 VGPRs), a 512-entry shared
+// leaf queue in the LDS this frees (16 waves x 9856 bytes of the CU's
+// 160 KiB), and the near/far node test.  On the scattering workload (4096
+// Gaussians, 512x512, 64 spp): 26.5 Msamples/s with 448 entries, against 23.6
+// at 6 waves with a 224-entry queue and the min/max test, 26.0 at 5 waves
+// (320 entries) and 24.5 at 3 waves (640 entries); with the leaf-block walks,
+// which fill the queue four leaves at a time, 512 entries gained 0.6-1.2 %
+// over 448.
 #ifndef VP_WAVE_Q
-#define VP_WAVE_Q 448
+#define VP_WAVE_Q 512
 #endif
 #ifndef VP_MEGA_NEAR_FAR
 #define VP_MEGA_NEAR_FAR 1
