@@ -848,8 +848,10 @@ __device__ __forceinline__ float4 finalize_radiance(const JobDesc& J, f3 L) {
 // tiles and for the paths of pixels that adaptive sampling has stopped.
 __device__ __forceinline__ bool path_begin(const JobDesc& J, uint64_t q, f3& o, f3& d, Pcg32& rng, uint64_t& key,
                                            uint32_t& pix_out, uint32_t& sl_out) {
-    const uint32_t sl = static_cast<uint32_t>(q / J.npad);
-    const uint32_t r = static_cast<uint32_t>(q - static_cast<uint64_t>(sl) * J.npad);
+    // the slice's samples of one pixel are consecutive paths, so a
+    // wavefront's lanes start on few pixels (nearly the same camera rays)
+    const uint32_t r = static_cast<uint32_t>(q / J.s_count);
+    const uint32_t sl = static_cast<uint32_t>(q - static_cast<uint64_t>(r) * J.s_count);
     // 8x8 tiles, in bands of kTileBand tile rows, column by column within a
     // band, so the paths in flight on one XCD (a run of consecutive path
     // indices) cover a compact patch of the image

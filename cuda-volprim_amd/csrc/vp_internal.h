@@ -74,9 +74,9 @@ struct EnvDev {
 };
 
 // One job slice of the persistent renderer: samples [s_begin, s_begin+s_count)
-// of one image.  Path q of the slice (0 <= q < path_count) is sample
-// s_begin + q / npad of the pixel at tile position q % npad (8x8 tiles, in
-// bands of tile rows, path_begin).
+// of one image.  Path q of the slice (0 <= q < path_count = npad s_count) is
+// sample s_begin + q % s_count of the pixel at tile position q / s_count
+// (8x8 tiles, in bands of tile rows, path_begin).
 // A finished path adds its radiance to its pixel's accumulator (kAccumWords
 // words per pixel, vp_core.h); the mean is resolved from it after the last
 // slice.  Entries are aligned to the 128-byte cache line so a render launch
