@@ -777,11 +777,20 @@ __device__ __forceinline__ void transmittance_pool(const SceneView& s, f3 o, f3 
         // leaf phase, shared by all lanes: each queued leaf's crossing (entry
         // test and whole-chord depth from one transform) goes into its ray's
         // fixed-point sum
-        for (uint32_t g = rank; g < count; g += nact) {
-            const uint32_t own = wq.owner[g];
+        // (two leaves per lane and round, both records loaded before either
+        // crossing, so each lane has two loads in flight)
+        for (uint32_t g = rank; g < count; g += 2u * nact) {
+            const uint32_t g2 = g + nact;
+            const bool two_leaves = g2 < count;
+            const uint32_t own = wq.owner[g], own2 = two_leaves ? wq.owner[g2] : own;
             const PrimRecord pr = load_prim(wq.prims[own >> 1], wq.prim[g]);
+            const PrimRecord pr2 = load_prim(wq.prims[own2 >> 1], two_leaves ? wq.prim[g2] : wq.prim[g]);
             const Crossing x = crossing(pr, lds_load3(wq.ro + 3 * (own >> 1)), lds_load3(wq.rd + 3 * own));
             if (x.tau > 0.0f) atomicAdd(&wq.acc[own], tau_fixed(x.tau));
+            if (two_leaves) {
+                const Crossing x2 = crossing(pr2, lds_load3(wq.ro + 3 * (own2 >> 1)), lds_load3(wq.rd + 3 * own2));
+                if (x2.tau > 0.0f) atomicAdd(&wq.acc[own2], tau_fixed(x2.tau));
+            }
         }
         wave_sync();
         // lanes whose ray is done (walked to the end, or saturated) take the
