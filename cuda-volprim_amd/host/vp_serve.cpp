@@ -316,7 +316,7 @@ class Service {
     bool ensure_context(std::string& err) {
         if (ctx_) return true;
         if (vp_context_create(0, &ctx_) != VP_OK) return err = vp_last_error(), false;
-        vp_server_config cfg{uint64_t(16) << 30, 64, 384};
+        vp_server_config cfg{uint64_t(16) << 30, 64, 384, 0};
         if (vp_server_create(ctx_, &cfg, &srv_) != VP_OK) return err = vp_last_error(), false;
         return true;
     }

@@ -80,7 +80,7 @@ class Job(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("mem_budget_bytes", ctypes.c_uint64), ("max_inflight", ctypes.c_uint32),
-                ("active_cap", ctypes.c_uint32)]
+                ("active_cap", ctypes.c_uint32), ("reserve_cus", ctypes.c_uint32)]
 
 
 class ServerStats(ctypes.Structure):
@@ -366,8 +366,8 @@ class Context:
         _check(load().vp_autotune(self.handle, arr, len(jobs), ctypes.byref(best)), "vp_autotune")
         return best
 
-    def server(self, mem_budget_bytes=8 << 30, max_inflight=64, active_cap=0):
-        return Server(self, mem_budget_bytes, max_inflight, active_cap)
+    def server(self, mem_budget_bytes=8 << 30, max_inflight=64, active_cap=0, reserve_cus=False):
+        return Server(self, mem_budget_bytes, max_inflight, active_cap, reserve_cus)
 
     def transmittance_rays(self, scene, rays6):
         rays6 = f32(rays6).reshape(-1, 6)
@@ -406,9 +406,9 @@ class Server:
     """Continuous-batching request loop (vp_server_*).  submit() returns a
     ticket and the host image the result lands in; poll() reports completion."""
 
-    def __init__(self, ctx, mem_budget_bytes, max_inflight, active_cap=0):
+    def __init__(self, ctx, mem_budget_bytes, max_inflight, active_cap=0, reserve_cus=False):
         self.ctx = ctx
-        cfg = ServerConfig(mem_budget_bytes, max_inflight, active_cap)
+        cfg = ServerConfig(mem_budget_bytes, max_inflight, active_cap, 1 if reserve_cus else 0)
         h = _vp()
         _check(load().vp_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)), "vp_server_create")
         self.handle = h

@@ -238,6 +238,12 @@ typedef struct {
     uint32_t max_inflight;     /* requests on the GPU at once (0 = 64) */
     uint32_t active_cap;       /* largest active-set capacity a request may ask for (0 = the context's
                                   autotuned capacity, else 128); sizes the server's slabs */
+    uint32_t reserve_cus;      /* 1: keep one CU of each XCD out of the render launches, so progress
+                                  polls and result copies (blit kernels) run while a launch is
+                                  resident and a request is returned as soon as its own paths finish;
+                                  0 (default): every CU renders, and those copies wait for CUs that
+                                  a launch frees (saturated throughput 11 % higher on the
+                                  request-stream workload) */
 } vp_server_config;
 typedef struct {
     uint64_t completed;

@@ -67,10 +67,11 @@ def test_batched_jobs_mixing_scenes_within_wavefronts(ctx):
         o.close()
 
 
-def test_server_result_same_alone_and_in_mix(ctx):
-    print("[case] server alone vs mix", flush=True)
+@pytest.mark.parametrize("reserve_cus", [False, True])
+def test_server_result_same_alone_and_in_mix(ctx, reserve_cus):
+    print(f"[case] server alone vs mix, reserve_cus={reserve_cus}", flush=True)
     jobs, scs, envs = _mix(ctx)
-    srv = ctx.server(max_inflight=4, mem_budget_bytes=1 << 20)
+    srv = ctx.server(max_inflight=4, mem_budget_bytes=1 << 20, reserve_cus=reserve_cus)
     t0, out0 = srv.submit(*jobs[0])
     srv.drain()
     lat0 = srv.poll(t0)
