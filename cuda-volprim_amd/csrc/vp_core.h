@@ -223,20 +223,32 @@ constexpr float kChordScale = 0x1.6c0504p-6f;
 struct Crossing {
     float t0, t1, tau;
 };
-VP_HD Crossing crossing(const PrimRecord& pr, f3 o, f3 d) {
-    Crossing r{-1.0f, -1.0f, 0.0f};
+// The first part of crossing(): false where crossing() finds no entry
+// before solving for it (a direction too short, an origin inside or on the
+// bound, a ray moving away, a miss or a tangent).  The same operations, so a
+// leaf pass can sort out beforehand the leaves crossing() would reject.
+struct CrossingSetup {
+    float a, b, c, a_inv, delta;
+};
+VP_HD bool crossing_setup(const PrimRecord& pr, f3 o, f3 d, CrossingSetup& s) {
     const f3 w = to_local_dir(pr, d);
     const f3 p = to_local_pos(pr, o);
-    const float a = length2(w);
-    if (a < kDirMin2) return r;
-    const float c = length2(p) - kExtent2;
-    if (!(c > 0.0f)) return r;
-    const float b = -dot(p, w);
-    if (!(b > 0.0f)) return r;  // moving away: both roots behind the origin
-    const float a_inv = rcpf(a);
-    const f3 q0 = fma3(b * a_inv, w, p);
-    const float delta = kExtent2 - length2(q0);
-    if (delta < kDiscEps) return r;
+    s.a = length2(w);
+    if (s.a < kDirMin2) return false;
+    s.c = length2(p) - kExtent2;
+    if (!(s.c > 0.0f)) return false;
+    s.b = -dot(p, w);
+    if (!(s.b > 0.0f)) return false;  // moving away: both roots behind the origin
+    s.a_inv = rcpf(s.a);
+    const f3 q0 = fma3(s.b * s.a_inv, w, p);
+    s.delta = kExtent2 - length2(q0);
+    return !(s.delta < kDiscEps);
+}
+VP_HD Crossing crossing(const PrimRecord& pr, f3 o, f3 d) {
+    Crossing r{-1.0f, -1.0f, 0.0f};
+    CrossingSetup s;
+    if (!crossing_setup(pr, o, d, s)) return r;
+    const float a = s.a, b = s.b, c = s.c, a_inv = s.a_inv, delta = s.delta;
     const float sq = sqrtf_(a * delta);
     const float q = b + sq;
     const float t = c * rcpf(q);

@@ -167,23 +167,46 @@ __device__ __forceinline__ void argmin_walk_shared(const SceneView& sc, f3 o, f3
         wave_sync();
         const unsigned long long am = __builtin_amdgcn_ballot_w64(true);
         const uint32_t nact = static_cast<uint32_t>(__popcll(am));
-        // leaf phase in two passes, each at full width.  First every queued
-        // leaf's crossing (entry, exit and whole-chord depth from one
-        // transform): on a direct ray its depth joins the ray's sum; if it is
-        // entered before the running minimum it is a rebuild candidate and
-        // its free flight is sampled, which ends inside the chord when the
-        // sampled depth is at most the chord's.  Those leaves move down to
-        // the front of the queue (an entry is only overwritten after it has
-        // been read) with their sampled depth, and the second pass inverts
-        // their flights into distances.
+        // leaf phase in three passes, each at full width.  First the entry
+        // tests' early rejects (crossing_setup): the leaves a ray may enter
+        // move down to the front of the queue (an entry is only overwritten
+        // after it has been read).  Then those leaves' crossings (entry, exit
+        // and whole-chord depth from one transform): on a direct ray the
+        // depth joins the ray's sum; if the leaf is entered before the
+        // running minimum it is a rebuild candidate and its free flight is
+        // sampled, which ends inside the chord when the sampled depth is at
+        // most the chord's.  Those leaves move down again with their sampled
+        // depth, and the third pass inverts their flights into distances.
         const uint32_t rk = lanes_below(am);
-        uint32_t nk = 0;
+        uint32_t ne = 0;
         for (uint32_t base = 0; base < count; base += nact) {
+            const uint32_t g = base + rk;
+            bool may = false;
+            uint32_t own = 0, prim = 0;
+            if (g < count) {
+                own = wq.owner[g];
+                prim = wq.prim[g];
+                CrossingSetup cs;
+                may = crossing_setup(load_prim(wq.prims[own], prim), lds_load3(wq.ro + 3 * own),
+                                     lds_load3(wq.rd + 3 * own), cs);
+            }
+            const unsigned long long mm = __builtin_amdgcn_ballot_w64(may);
+            wave_sync();
+            if (may) {
+                const uint32_t h = ne + lanes_below(mm);
+                wq.prim[h] = prim;
+                wq.owner[h] = static_cast<uint8_t>(own);
+            }
+            ne += static_cast<uint32_t>(__popcll(mm));
+        }
+        wave_sync();
+        uint32_t nk = 0;
+        for (uint32_t base = 0; base < ne; base += nact) {
             const uint32_t g = base + rk;
             bool keep = false;
             uint32_t own = 0, prim = 0;
             float tau = 0.0f;
-            if (g < count) {
+            if (g < ne) {
                 own = wq.owner[g];
                 prim = wq.prim[g];
                 const PrimRecord pr = load_prim(wq.prims[own], prim);
