@@ -20,6 +20,9 @@
 //        or {"id": "...", "status": "error", "message": "..."}
 // At end of input, after every request has finished:
 //   {"stats": {"completed": n, "samples": s, "p50_ms": .., "p99_ms": .., "busy_ms": ..}}
+// Option: --reserve-cus keeps one CU of each XCD out of the render launches
+// (vp_server_config.reserve_cus), so a request is answered as soon as its own
+// paths finish rather than when the launch it shares frees compute units.
 //
 // The scene and camera conventions are the reference's (PLY loading as
 // src/thesis/host/utils/io/ply.cpp:34-175, cameras as
@@ -311,12 +314,13 @@ class Service {
         return true;
     }
     vp_server* server() const { return srv_; }
+    void set_reserve_cus(bool on) { reserve_cus_ = on; }
 
    private:
     bool ensure_context(std::string& err) {
         if (ctx_) return true;
         if (vp_context_create(0, &ctx_) != VP_OK) return err = vp_last_error(), false;
-        vp_server_config cfg{uint64_t(16) << 30, 64, 384, 0};
+        vp_server_config cfg{uint64_t(16) << 30, 64, 384, reserve_cus_ ? 1u : 0u};
         if (vp_server_create(ctx_, &cfg, &srv_) != VP_OK) return err = vp_last_error(), false;
         return true;
     }
@@ -354,14 +358,23 @@ class Service {
     }
     vp_context* ctx_ = nullptr;
     vp_server* srv_ = nullptr;
+    bool reserve_cus_ = false;
     std::map<std::string, vp_scene*> scenes_;
     std::map<std::string, vp_env*> envs_;
 };
 
 }  // namespace
 
-int main() {
+int main(int argc, char** argv) {
     Service svc;
+    for (int i = 1; i < argc; ++i) {
+        if (std::string(argv[i]) == "--reserve-cus") {
+            svc.set_reserve_cus(true);
+        } else {
+            std::fprintf(stderr, "vp_serve: unknown option %s (known: --reserve-cus)\n", argv[i]);
+            return 2;
+        }
+    }
     std::mutex out_mu, pend_mu;
     std::vector<std::unique_ptr<Pending>> pending;
     bool input_done = false;

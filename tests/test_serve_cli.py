@@ -33,6 +33,18 @@ def test_malformed_requests_get_error_responses(tmp_path):
     assert "spp" in out[5]["message"] and "eye" in out[6]["message"]
 
 
+def test_options(tmp_path):
+    # an unknown option is refused before any input is read
+    r = subprocess.run([SERVE, "--bogus"], input="", cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "--reserve-cus" in r.stderr
+    # the CU reservation is accepted; malformed requests are answered without a GPU
+    r = subprocess.run([SERVE, "--reserve-cus"], input='{"id": "a"\n', cwd=tmp_path, capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    out = [json.loads(l) for l in r.stdout.splitlines() if l.strip()]
+    assert out and out[0]["status"] == "error"
+
+
 @pytest.mark.gpu
 def test_served_requests_match_direct_renders(tmp_path, ctx):
     print("case: vp_serve 3 requests", flush=True)
